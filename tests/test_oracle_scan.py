@@ -109,9 +109,10 @@ def test_scan_weights_unit_and_orientation():
     assert np.array_equal(pix0, pix1)       # same boresight
     np.testing.assert_allclose(w0[:, 1] ** 2 + w0[:, 2] ** 2, 1, atol=1e-6)
     assert np.all(w0[:, 0] == 1)
-    # rotating the detector by 45 deg rotates (cos2psi, sin2psi) by 90 deg
-    np.testing.assert_allclose(w1[:, 1], -w0[:, 2], atol=2e-6)
-    np.testing.assert_allclose(w1[:, 2], w0[:, 1], atol=2e-6)
+    # rotating the detector by 45 deg rotates (cos2psi, sin2psi) by 90 deg, with
+    # one sense of rotation for every sample (here psi = psi_scan - psi0)
+    np.testing.assert_allclose(w1[:, 1], w0[:, 2], atol=2e-6)
+    np.testing.assert_allclose(w1[:, 2], -w0[:, 1], atol=2e-6)
     # a ring scan repeats its pixel sequence after one spin (small precession)
     pix2, _ = oracle.expand_pointing(scan, dict(psi_turns=0.0), 64, 3000, 3000)
     assert np.mean(pix2 == pix0) > 0.9
