@@ -1,0 +1,86 @@
+/*
+ * libmadam.so — MI355X (gfx950) implementation of libmadam's destriping solve.
+ *
+ * The first three entry points keep the reference's C ABI unchanged; the
+ * madam_* entry points are additive (in-memory outputs, device-resident
+ * buffers, error codes).  Paths cite the reference source tree.
+ *
+ * Buffer types behind void* (src/smadam.F90:151-154, src/pointing.f90:13-14,
+ * src/tod_storage.f90:14-15): timestamps float64[nsamp], pix int32[ndet][nsamp],
+ * pixweights float32[ndet][nsamp][nnz], signal float64[ndet][nsamp].  As in
+ * the reference, flagged samples are left as 12*nside_max**2 in the caller's
+ * pix array, and the baselines are subtracted from signal when write_tod or a
+ * detset / survey is given.  Not re-entrant: one call at a time per process.
+ * FITS files are not written; use madam_get_maps for the products.
+ */
+#ifndef LIBMADAM_AMD_MADAM_H
+#define LIBMADAM_AMD_MADAM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Reference: src/madam.h:2, src/smadam.F90:64-590.  `comm` is a Fortran MPI
+ * handle in the reference; this single-GPU library records and ignores it. */
+void destripe(int comm, char *parstring, long ndet, char *detstring, double *detweights, long nsamp, long nnz,
+              void *timestamps, void *pix, void *pixweights, void *signal, long nperiod, long *periods, long *npsd,
+              long npsdtot, double *psdstarts, long npsdbin, double *psdfreqs, long npsdval, double *psdvals);
+
+/* Reference: src/smadam.F90:593-918 (bind(c) name destripe_with_cache; not in
+ * the reference madam.h).  Solves again on the cached configuration of a
+ * destripe(mcmode=T) call, computing the right-hand side from a zeroed map. */
+void destripe_with_cache(int comm, long ndet, long nsamp, long nnz, void *timestamps, void *pix, void *pixweights,
+                         void *signal, char *outpath);
+
+/* Reference: src/smadam.F90:921-951. */
+void clear_caches(void);
+
+/* ---------------------------------------------------------------- additive API
+ * Replaces the reference's module-global state (src/commonparam.f90:38-194)
+ * with an explicit handle.  All functions return 0 on success and 1 on error
+ * (message from madam_last_error).  on_device != 0 means the pointers are
+ * device pointers on the handle's GPU and are used in place.
+ */
+void *madam_create(int device);
+void madam_destroy(void *ctx);
+/* parameters, detectors, intervals, baselines: src/inputparam.f90:24-163,
+ * src/read_data.f90:203-323, src/parameter_control.f90:27-409, 752-977.
+ * subchunk_pass selects the pass of the subchunk loop (0 = full data). */
+int madam_configure(void *ctx, const char *parstring, long ndet, const char *detstring, const double *detweights,
+                    long nsamp, long nnz, const double *timestamps, long nperiod, const long *periods,
+                    const long *npsd, long npsdtot, const double *psdstarts, long npsdbin, const double *psdfreqs,
+                    long npsdval, const double *psdvals, int subchunk_pass);
+/* flagging and pointing setup (src/read_data.f90:241-272, src/pointing.f90:115-191) */
+int madam_set_pointing(void *ctx, int32_t *pix, float *pixweights, int on_device);
+/* binning, inversion, PCG, final maps (src/smadam.F90:316-433) */
+int madam_solve(void *ctx, double *signal, int on_device);
+/* write the flagged samples back into a host pix array as 12*nside_max**2 */
+int madam_restore_pixels(void *ctx, int32_t *host_pix);
+/* out6 = {nmap, npix_map, nbase (per detector), ndet, iterations, nside_map} */
+int madam_get_sizes(void *ctx, long *out6);
+/* maps [npix][nmap], cc [npix][nmap][nmap] inverse pixel matrices; NULL skips */
+int madam_get_maps(void *ctx, double *map, double *binmap, int32_t *hits, double *cc, int32_t *mask, double *crit);
+/* baselines [ndet][nbase] */
+int madam_get_baselines(void *ctx, double *aa, double *yba, double *nna);
+int madam_get_stats(void *ctx, double *out, int n);
+int madam_get_rr_history(void *ctx, double *out, int n);
+int madam_set_profiling(void *ctx, int on);
+/* pointing expansion of the synthetic ring scan: scan5 = {fsample, spin_period,
+ * prec_period, opening_turns, lon0_turns}, dets3[ndet][3] = {psi_turns,
+ * open_offset_turns, spin_offset_turns}; outputs pix[ndet][n], weights[ndet][n][3] */
+int madam_expand_pointing(const double *scan5, const double *dets3, int ndet, long nside, long i0, long n,
+                          int32_t *pix, float *weights, int on_device);
+const char *madam_last_error(void);
+/* status of the last destripe / destripe_with_cache call (0 ok); only
+ * meaningful when MADAM_RAISE is set, otherwise errors end the process like
+ * the reference's MPI_Abort */
+int madam_last_status(void);
+int madam_cache_handle(void **out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,239 @@
+"""MI355X-native destriping solve with the interface of libmadam.
+
+``libmadam_amd.libmadam_wrapper`` mirrors the reference's ctypes wrapper
+(python/libmadam_wrapper/__init__.py): ``destripe``, ``destripe_with_cache``,
+``clear_caches``.  ``Madam`` is the handle-based extension (in-memory maps,
+device-resident buffers).  The native library ``libmadam.so`` (HIP, gfx950)
+is required; there is no CPU fallback.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmadam.so")
+_lib = None
+
+TIMESTAMP_TYPE = np.float64
+SIGNAL_TYPE = np.float64
+PIXEL_TYPE = np.int32
+WEIGHT_TYPE = np.float32
+PSD_TYPE = np.float64
+
+# order of madam_get_stats
+STAT_NAMES = ["niter", "rrinit", "rzinit", "t_pointing", "t_bin", "t_init", "t_precond", "t_iter", "t_final",
+              "t_scatter", "t_gather", "t_prior", "t_apply_prec", "n_scatter", "n_gather", "precond_kind",
+              "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total"]
+
+
+class MadamError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmadam.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MadamError("libmadam.so not found at %s; build it with python -m libmadam_amd.build" % LIB_PATH)
+        L = ct.CDLL(LIB_PATH)
+        vp, cp, lg, i32 = ct.c_void_p, ct.c_char_p, ct.c_long, ct.c_int
+        L.madam_create.restype = vp
+        L.madam_create.argtypes = [i32]
+        L.madam_destroy.argtypes = [vp]
+        L.madam_configure.restype = i32
+        L.madam_configure.argtypes = [vp, cp, lg, cp, vp, lg, lg, vp, lg, vp, vp, lg, vp, lg, vp, lg, vp, i32]
+        L.madam_set_pointing.restype = i32
+        L.madam_set_pointing.argtypes = [vp, vp, vp, i32]
+        L.madam_solve.restype = i32
+        L.madam_solve.argtypes = [vp, vp, i32]
+        L.madam_restore_pixels.restype = i32
+        L.madam_restore_pixels.argtypes = [vp, vp]
+        L.madam_get_sizes.restype = i32
+        L.madam_get_sizes.argtypes = [vp, vp]
+        L.madam_get_maps.restype = i32
+        L.madam_get_maps.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.madam_get_baselines.restype = i32
+        L.madam_get_baselines.argtypes = [vp, vp, vp, vp]
+        L.madam_get_stats.restype = i32
+        L.madam_get_stats.argtypes = [vp, vp, i32]
+        L.madam_get_rr_history.restype = i32
+        L.madam_get_rr_history.argtypes = [vp, vp, i32]
+        L.madam_set_profiling.restype = i32
+        L.madam_set_profiling.argtypes = [vp, i32]
+        L.madam_expand_pointing.restype = i32
+        L.madam_expand_pointing.argtypes = [vp, vp, i32, lg, lg, lg, vp, vp, i32]
+        L.madam_last_error.restype = cp
+        L.madam_last_status.restype = i32
+        L.madam_cache_handle.restype = i32
+        L.madam_cache_handle.argtypes = [ct.POINTER(vp)]
+        L.destripe.restype = None
+        L.destripe_with_cache.restype = None
+        L.clear_caches.restype = None
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return ct.c_void_p(a)
+    if hasattr(a, "data_ptr"):          # torch tensor on the device
+        return ct.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise MadamError(lib().madam_last_error().decode())
+
+
+def dict2parstring(d):
+    """Parameter string in the reference format (python/libmadam_wrapper/__init__.py:73-84)."""
+    if isinstance(d, (str, bytes)):
+        return d.encode("ascii") if isinstance(d, str) else d
+    s = ""
+    for key, value in d.items():
+        if key in ("detset", "detset_nopol", "survey"):
+            for v in value:
+                s += "{} = {};".format(key, v)
+        else:
+            s += "{} = {};".format(key, value)
+    return s.encode("ascii")
+
+
+def dets2detstring(dets):
+    return "".join("{};".format(d) for d in dets).encode("ascii")
+
+
+class Madam:
+    """One destriping configuration on one GPU (handle over the C-ABI)."""
+
+    def __init__(self, device=0):
+        self._h = lib().madam_create(device)
+        if not self._h:
+            raise MadamError(lib().madam_last_error().decode())
+        self._keep = []
+
+    def close(self):
+        if self._h:
+            lib().madam_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def configure(self, pars, dets, weights, timestamps, periods, npsd, psdstarts, psdfreqs, psdvals, nnz,
+                  subchunk_pass=0):
+        self.ndet = len(dets)
+        self.nsamp = int(np.asarray(timestamps).size)
+        self.nnz = int(nnz)
+        arrs = [np.ascontiguousarray(weights, np.float64), np.ascontiguousarray(timestamps, np.float64),
+                np.ascontiguousarray(periods, np.int64), np.ascontiguousarray(npsd, np.int64),
+                np.ascontiguousarray(psdstarts, np.float64), np.ascontiguousarray(psdfreqs, np.float64),
+                np.ascontiguousarray(psdvals, np.float64).reshape(-1)]
+        w, ts, per, nps, pst, pfr, pv = arrs
+        npsdbin = pfr.size
+        npsdval = pv.size
+        npsdtot = npsdval // max(npsdbin, 1)
+        _check(lib().madam_configure(self._h, dict2parstring(pars), self.ndet, dets2detstring(dets), _ptr(w),
+                                     self.nsamp, self.nnz, _ptr(ts), per.size, _ptr(per), _ptr(nps), npsdtot,
+                                     _ptr(pst), npsdbin, _ptr(pfr), npsdval, _ptr(pv), int(subchunk_pass)))
+        return self
+
+    def set_pointing(self, pixels, pixweights, on_device=False):
+        if not on_device:
+            assert pixels.dtype == np.int32 and pixweights.dtype == np.float32
+            assert pixels.flags.c_contiguous and pixweights.flags.c_contiguous
+        self._pix = pixels
+        self._keep = [pixels, pixweights]
+        _check(lib().madam_set_pointing(self._h, _ptr(pixels), _ptr(pixweights), int(on_device)))
+
+    def solve(self, signal, on_device=False):
+        if not on_device:
+            assert signal.dtype == np.float64 and signal.flags.c_contiguous
+        self._keep.append(signal)
+        _check(lib().madam_solve(self._h, _ptr(signal), int(on_device)))
+
+    def restore_pixels(self, host_pix):
+        _check(lib().madam_restore_pixels(self._h, _ptr(host_pix)))
+
+    def sizes(self):
+        out = np.zeros(6, np.int64)
+        _check(lib().madam_get_sizes(self._h, _ptr(out)))
+        return dict(nmap=int(out[0]), npix=int(out[1]), nbase=int(out[2]), ndet=int(out[3]), niter=int(out[4]),
+                    nside_map=int(out[5]))
+
+    def maps(self):
+        sz = self.sizes()
+        npix, nmap = sz["npix"], sz["nmap"]
+        m = np.zeros((npix, nmap))
+        b = np.zeros((npix, nmap))
+        h = np.zeros(npix, np.int32)
+        cc = np.zeros((npix, nmap, nmap))
+        mk = np.zeros(npix, np.int32)
+        cr = np.zeros(npix)
+        _check(lib().madam_get_maps(self._h, _ptr(m), _ptr(b), _ptr(h), _ptr(cc), _ptr(mk), _ptr(cr)))
+        return dict(map=m, binmap=b, hits=h, cc=cc, mask=mk, crit=cr)
+
+    def baselines(self):
+        sz = self.sizes()
+        shape = (sz["ndet"], sz["nbase"])
+        aa, yba, nna = np.zeros(shape), np.zeros(shape), np.zeros(shape)
+        _check(lib().madam_get_baselines(self._h, _ptr(aa), _ptr(yba), _ptr(nna)))
+        return dict(aa=aa, yba=yba, nna=nna)
+
+    def stats(self):
+        out = np.zeros(len(STAT_NAMES))
+        _check(lib().madam_get_stats(self._h, _ptr(out), len(STAT_NAMES)))
+        return dict(zip(STAT_NAMES, out.tolist()))
+
+    def rr_history(self):
+        n = self.sizes()["niter"]
+        out = np.zeros(max(n, 1))
+        _check(lib().madam_get_rr_history(self._h, _ptr(out), n))
+        return out[:n]
+
+    def set_profiling(self, on=True):
+        _check(lib().madam_set_profiling(self._h, int(on)))
+
+
+def run(pars, dets, weights, timestamps, pixels, pixweights, signal, periods, npsd, psdstarts, psdfreqs, psdvals,
+        subchunk_pass=0, device=0):
+    """Full solve with in-memory outputs.  Mutates ``pixels`` and ``signal`` the
+    way the reference's destripe() mutates the caller's buffers."""
+    ndet = len(dets)
+    nsamp = timestamps.size
+    nnz = pixweights.size // max(nsamp * ndet, 1)
+    m = Madam(device)
+    try:
+        m.configure(pars, dets, weights, timestamps, periods, npsd, psdstarts, psdfreqs, psdvals, nnz,
+                    subchunk_pass)
+        m.set_pointing(pixels, pixweights)
+        m.solve(signal)
+        m.restore_pixels(pixels)
+        out = m.maps()
+        out.update(m.baselines())
+        out["stats"] = m.stats()
+        out["rr"] = m.rr_history()
+        out["niter"] = int(out["stats"]["niter"])
+        return out
+    finally:
+        m.close()
+
+
+def expand_pointing(scan, dets, nside, i0, n):
+    """GPU pointing expansion of the synthetic ring scan (host outputs)."""
+    scan5 = np.array([scan["fsample"], scan["spin_period"], scan["prec_period"], scan["opening_turns"],
+                      scan["lon0_turns"]], np.float64)
+    d3 = np.array([[d["psi_turns"], d.get("open_offset_turns", 0.0), d.get("spin_offset_turns", 0.0)]
+                   for d in dets], np.float64)
+    pix = np.zeros((len(dets), n), np.int32)
+    w = np.zeros((len(dets), n, 3), np.float32)
+    _check(lib().madam_expand_pointing(_ptr(scan5), _ptr(d3), len(dets), nside, i0, n, _ptr(pix), _ptr(w), 0))
+    return pix, w

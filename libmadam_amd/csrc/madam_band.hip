@@ -1,0 +1,397 @@
+// Band-Cholesky preconditioner for the PCG baseline solve.
+//
+// Reference: construct_preconditioner / get_cholesky_decomposition /
+// apply_cholesky_decomposition (src/noise_routines.f90:920-1153, 1259-1301):
+// per trimmed segment, the band matrix M with M(i, j) = invcov(|i - j|) for
+// |i - j| < nband plus nna on the diagonal is factored with DPBTRF('L') and
+// applied with DPBTRS.
+//
+// The factor here is the same global Cholesky factor L (one wave per segment,
+// right-looking, rolling (kd+1)^2 window in LDS).  The triangular solves are
+// parallelised exactly, without approximation, by partitioning each segment
+// into chunks of CH rows: a chunk's sweep is an affine function of the KD-wide
+// carry state entering it, s_out = s0_out + T s_in, with T precomputed per
+// chunk.  A solve is then: sweep every chunk with zero carry (parallel), chain
+// the carries (sequential over chunks, KD x KD mat-vec each), sweep again with
+// the true carries (parallel).  Segments with a smaller band are zero-padded
+// to the largest KD.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "madam_internal.hpp"
+
+namespace madam {
+
+static constexpr int BAND_CH = 512;
+
+// ---------------------------------------------------------------- factorization
+
+// one 64-thread block per segment; band output Lb[b * K + m] = L(b + m, b), with
+// the diagonal entry replaced by its reciprocal (the solve multiplies)
+__global__ void k_band_factor(const Segment *seg, int64_t nseg, const int *nband_seg, const double *invcov,
+                              int64_t nof, int K, const double *nna, double *Lb, int *info)
+{
+    extern __shared__ double W[];   // K * K window, plus K for the column
+    double *lv = W + K * K;
+    int64_t s = blockIdx.x;
+    if (s >= nseg) return;
+    Segment sg = seg[s];
+    int nb = nband_seg[s];
+    if (nb <= 0 || sg.skip) return;
+    const double *ic = invcov + (int64_t)sg.ipsd * nof;
+    int64_t n = sg.noba;
+    const double *nn = nna + sg.b0;
+    int kd = K - 1;
+    // initial window rows 0..kd: A(i, j) = ic[i - j] (|i-j| < nb) + nna on the diagonal
+    for (int t = threadIdx.x; t < K * K; t += blockDim.x) {
+        int a = t / K, b = t % K;
+        double v = 0;
+        if (a < n && b <= a && a - b < nb) v = ic[a - b] + (a == b ? nn[a] : 0.0);
+        W[a * K + b] = v;   // slot (i % K, j % K)
+    }
+    __syncthreads();
+    __shared__ int failed;
+    if (threadIdx.x == 0) failed = 0;
+    __syncthreads();
+    for (int64_t j = 0; j < n; ++j) {
+        int sj = (int)(j % K);
+        double ajj = W[sj * K + sj];
+        if (!(ajj > 0)) {
+            if (threadIdx.x == 0) { failed = 1; info[s] = (int)(j + 1); }
+            break;
+        }
+        double d = sqrt(ajj);
+        // column below the diagonal
+        for (int a = 1 + threadIdx.x; a <= kd; a += blockDim.x) {
+            double l = 0;
+            if (j + a < n) l = W[((j + a) % K) * K + sj] / d;
+            lv[a] = l;
+            Lb[(sg.b0 + j) * K + a] = l;
+        }
+        if (threadIdx.x == 0) Lb[(sg.b0 + j) * K] = 1.0 / d;
+        __syncthreads();
+        // trailing update W(j+a, j+b) -= l_a l_b for 1 <= b <= a <= kd
+        int npair = kd * (kd + 1) / 2;
+        for (int t = threadIdx.x; t < npair; t += blockDim.x) {
+            // decode t -> (a, b), a >= b >= 1
+            int a = (int)((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
+            while ((a + 1) * (a + 2) / 2 <= t) ++a;
+            while (a * (a + 1) / 2 > t) --a;
+            int b = t - a * (a + 1) / 2;
+            a += 1;
+            b += 1;
+            if (j + a < n) {
+                int sa = (int)((j + a) % K), sb = (int)((j + b) % K);
+                W[sa * K + sb] -= lv[a] * lv[b];
+            }
+        }
+        __syncthreads();
+        // load the new row i = j + K into slot sj
+        int64_t i = j + K;
+        for (int b = threadIdx.x; b < K; b += blockDim.x) {
+            // column i - kd + b' ... slot of column c = (c % K); entries A(i, c) for c in [i - kd, i]
+            int64_t c = i - kd + b;
+            double v = 0;
+            if (i < n && i - c < nb) v = ic[i - c] + (i == c ? nn[i] : 0.0);
+            W[sj * K + (int)(c % K)] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- partitioned solves
+
+struct ChunkInfo {
+    int64_t b0;    // first global baseline row of the chunk
+    int32_t n;     // rows
+    int32_t seg;   // segment
+    int32_t first, last;   // first / last chunk of its segment
+};
+
+// forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
+// next KD rows (acc[0] for the first row of the chunk).  Returns the outgoing
+// state.  If store, writes y.
+template <int KD>
+__device__ inline void fwd_sweep(const double *Lb, int64_t b0, int n, const double *r, double *acc, double *y,
+                                 bool store, bool zero_rhs)
+{
+    const int K = KD + 1;
+    for (int i = 0; i < n; ++i) {
+        const double *row = Lb + (b0 + i) * K;
+        double rhs = zero_rhs ? 0.0 : r[b0 + i];
+        double u = (rhs + acc[0]) * row[0];
+#pragma unroll
+        for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - row[m] * u;
+        acc[KD - 1] = -row[KD] * u;
+        if (store) y[b0 + i] = u;
+    }
+}
+
+// backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
+template <int KD>
+__device__ inline void bwd_sweep(const double *Lb, int64_t b0, int n, const double *y, double *xs, double *x,
+                                 bool store, bool zero_rhs)
+{
+    const int K = KD + 1;
+    for (int i = n - 1; i >= 0; --i) {
+        const double *row = Lb + (b0 + i) * K;
+        double s = zero_rhs ? 0.0 : y[b0 + i];
+#pragma unroll
+        for (int m = 1; m <= KD; ++m) s -= row[m] * xs[m - 1];
+        double xi = s * row[0];
+#pragma unroll
+        for (int m = KD - 1; m > 0; --m) xs[m] = xs[m - 1];
+        xs[0] = xi;
+        if (store) x[b0 + i] = xi;
+    }
+}
+
+// T matrices: response of the outgoing state to a unit incoming state
+template <int KD>
+__global__ void k_band_response(const ChunkInfo *ch, int64_t nch, const double *Lb, double *Tf, double *Tb)
+{
+    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nch * KD) return;
+    int64_t c = t / KD;
+    int j = (int)(t % KD);
+    ChunkInfo ci = ch[c];
+    double st[KD];
+#pragma unroll
+    for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
+    fwd_sweep<KD>(Lb, ci.b0, ci.n, nullptr, st, nullptr, false, true);
+#pragma unroll
+    for (int m = 0; m < KD; ++m) Tf[(c * KD + m) * KD + j] = st[m];
+#pragma unroll
+    for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
+    bwd_sweep<KD>(Lb, ci.b0, ci.n, nullptr, st, nullptr, false, true);
+#pragma unroll
+    for (int m = 0; m < KD; ++m) Tb[(c * KD + m) * KD + j] = st[m];
+}
+
+template <int KD>
+__global__ void k_band_fwd0(const ChunkInfo *ch, int64_t nch, const double *Lb, const double *r, double *s0)
+{
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    ChunkInfo ci = ch[c];
+    double acc[KD];
+#pragma unroll
+    for (int m = 0; m < KD; ++m) acc[m] = 0;
+    fwd_sweep<KD>(Lb, ci.b0, ci.n, r, acc, nullptr, false, false);
+#pragma unroll
+    for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
+}
+
+// chain carries inside each segment; one thread per segment (segments are few)
+template <int KD>
+__global__ void k_band_chain(const int64_t *seg_first_chunk, int64_t nseg, const double *s0, const double *T,
+                             double *sin, int dir)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    int64_t c0 = seg_first_chunk[s], c1 = seg_first_chunk[s + 1];
+    double st[KD];
+#pragma unroll
+    for (int m = 0; m < KD; ++m) st[m] = 0;
+    if (dir == 0) {
+        for (int64_t c = c0; c < c1; ++c) {
+#pragma unroll
+            for (int m = 0; m < KD; ++m) sin[c * KD + m] = st[m];
+            double nx[KD];
+#pragma unroll
+            for (int m = 0; m < KD; ++m) {
+                double v = s0[c * KD + m];
+                for (int k = 0; k < KD; ++k) v += T[(c * KD + m) * KD + k] * st[k];
+                nx[m] = v;
+            }
+#pragma unroll
+            for (int m = 0; m < KD; ++m) st[m] = nx[m];
+        }
+    } else {
+        for (int64_t c = c1 - 1; c >= c0; --c) {
+#pragma unroll
+            for (int m = 0; m < KD; ++m) sin[c * KD + m] = st[m];
+            double nx[KD];
+#pragma unroll
+            for (int m = 0; m < KD; ++m) {
+                double v = s0[c * KD + m];
+                for (int k = 0; k < KD; ++k) v += T[(c * KD + m) * KD + k] * st[k];
+                nx[m] = v;
+            }
+#pragma unroll
+            for (int m = 0; m < KD; ++m) st[m] = nx[m];
+        }
+    }
+}
+
+// final forward sweep with true carries writes y into z, then the zero-carry
+// backward sweep of the same chunk gives its backward partial state
+template <int KD>
+__global__ void k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, const double *Lb, const double *r,
+                                 const double *sinf, double *z, double *s0b)
+{
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    ChunkInfo ci = ch[c];
+    double acc[KD];
+#pragma unroll
+    for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
+    fwd_sweep<KD>(Lb, ci.b0, ci.n, r, acc, z, true, false);
+#pragma unroll
+    for (int m = 0; m < KD; ++m) acc[m] = 0;
+    bwd_sweep<KD>(Lb, ci.b0, ci.n, z, acc, nullptr, false, false);
+#pragma unroll
+    for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
+}
+
+template <int KD>
+__global__ void k_band_bwd1(const ChunkInfo *ch, int64_t nch, const double *Lb, const double *sinb, double *z)
+{
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nch) return;
+    ChunkInfo ci = ch[c];
+    double xs[KD];
+#pragma unroll
+    for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
+    bwd_sweep<KD>(Lb, ci.b0, ci.n, z, xs, z, true, false);
+}
+
+// ---------------------------------------------------------------- host side
+
+struct BandImpl {
+    ChunkInfo *ch = nullptr;
+    int64_t *seg_first = nullptr;
+    double *s0f = nullptr, *s0b = nullptr, *sinf = nullptr, *sinb = nullptr;
+    int64_t nseg_used = 0;
+};
+
+static int kd_class(int kd)
+{
+    static const int cls[] = {9, 19, 29, 39, 49};
+    for (int c : cls)
+        if (kd <= c) return c;
+    return -1;
+}
+
+#define BAND_DISPATCH(KDV, STMT)                         \
+    switch (KDV) {                                        \
+    case 9: { constexpr int KD = 9; STMT; } break;        \
+    case 19: { constexpr int KD = 19; STMT; } break;      \
+    case 29: { constexpr int KD = 29; STMT; } break;      \
+    case 39: { constexpr int KD = 39; STMT; } break;      \
+    default: { constexpr int KD = 49; STMT; } break;      \
+    }
+
+// Returns 0 on success; fills info_out[s] (0 ok, >0 failing column) for each segment.
+int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *dseg, int64_t nseg,
+                    const int *nband_seg_host, const double *invcov_dev, int64_t nof, const double *nna,
+                    int *info_out)
+{
+    int kdmax = 0;
+    for (int64_t i = 0; i < nseg; ++i)
+        if (!hseg[i].skip && nband_seg_host[i] - 1 > kdmax) kdmax = nband_seg_host[i] - 1;
+    int KD = kd_class(kdmax < 1 ? 1 : kdmax);
+    if (KD < 0) return -1;   // band wider than 50: not supported
+    int K = KD + 1;
+    bp.kd = KD;
+    bp.chunk = BAND_CH;
+    bp.nseg = nseg;
+    // storage for the factor over every baseline row of every segment
+    int64_t rows_max = 0;
+    for (int64_t i = 0; i < nseg; ++i) rows_max = std::max<int64_t>(rows_max, hseg[i].b0 + hseg[i].noba);
+    MADAM_CHECK(hipMalloc(&bp.L, sizeof(double) * (rows_max > 0 ? rows_max : 1) * K));
+    int *dnb = nullptr, *dinfo = nullptr;
+    MADAM_CHECK(hipMalloc(&dnb, sizeof(int) * (nseg > 0 ? nseg : 1)));
+    MADAM_CHECK(hipMalloc(&dinfo, sizeof(int) * (nseg > 0 ? nseg : 1)));
+    MADAM_CHECK(hipMemcpyAsync(dnb, nband_seg_host, sizeof(int) * nseg, hipMemcpyHostToDevice, s));
+    MADAM_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int) * (nseg > 0 ? nseg : 1), s));
+    size_t lds = sizeof(double) * (K * K + K);
+    if (nseg > 0) k_band_factor<<<(unsigned)nseg, 64, lds, s>>>(dseg, nseg, dnb, invcov_dev, nof, K, nna, bp.L, dinfo);
+    MADAM_CHECK(hipGetLastError());
+    MADAM_CHECK(hipMemcpyAsync(info_out, dinfo, sizeof(int) * nseg, hipMemcpyDeviceToHost, s));
+    MADAM_CHECK(hipStreamSynchronize(s));
+    (void)hipFree(dnb);
+    (void)hipFree(dinfo);
+    // chunk table over the segments that factored
+    std::vector<ChunkInfo> ch;
+    std::vector<int64_t> first;
+    for (int64_t i = 0; i < nseg; ++i) {
+        if (hseg[i].skip || nband_seg_host[i] <= 0 || info_out[i] != 0) continue;
+        first.push_back((int64_t)ch.size());
+        int64_t n = hseg[i].noba;
+        for (int64_t r = 0; r < n; r += BAND_CH) {
+            ChunkInfo ci;
+            ci.b0 = hseg[i].b0 + r;
+            ci.n = (int32_t)(n - r < BAND_CH ? n - r : BAND_CH);
+            ci.seg = (int32_t)i;
+            ci.first = r == 0;
+            ci.last = r + BAND_CH >= n;
+            ch.push_back(ci);
+        }
+    }
+    first.push_back((int64_t)ch.size());
+    bp.nchunk = (int64_t)ch.size();
+    BandImpl *im = new BandImpl();
+    im->nseg_used = (int64_t)first.size() - 1;
+    int64_t nc = bp.nchunk > 0 ? bp.nchunk : 1;
+    MADAM_CHECK(hipMalloc(&im->ch, sizeof(ChunkInfo) * nc));
+    MADAM_CHECK(hipMalloc(&im->seg_first, sizeof(int64_t) * first.size()));
+    MADAM_CHECK(hipMalloc(&bp.Tf, sizeof(double) * nc * KD * KD));
+    MADAM_CHECK(hipMalloc(&bp.Tb, sizeof(double) * nc * KD * KD));
+    MADAM_CHECK(hipMalloc(&im->s0f, sizeof(double) * nc * KD));
+    MADAM_CHECK(hipMalloc(&im->s0b, sizeof(double) * nc * KD));
+    MADAM_CHECK(hipMalloc(&im->sinf, sizeof(double) * nc * KD));
+    MADAM_CHECK(hipMalloc(&im->sinb, sizeof(double) * nc * KD));
+    if (bp.nchunk > 0)
+        MADAM_CHECK(hipMemcpyAsync(im->ch, ch.data(), sizeof(ChunkInfo) * ch.size(), hipMemcpyHostToDevice, s));
+    MADAM_CHECK(hipMemcpyAsync(im->seg_first, first.data(), sizeof(int64_t) * first.size(), hipMemcpyHostToDevice, s));
+    if (bp.nchunk > 0) {
+        int64_t nt = bp.nchunk * KD;
+        BAND_DISPATCH(KD, (k_band_response<KD><<<(unsigned)((nt + 63) / 64), 64, 0, s>>>(im->ch, bp.nchunk, bp.L,
+                                                                                         bp.Tf, bp.Tb)));
+        MADAM_CHECK(hipGetLastError());
+    }
+    bp.chunk_b0 = (int64_t *)im;   // opaque implementation pointer
+    MADAM_CHECK(hipStreamSynchronize(s));
+    return 0;
+}
+
+void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
+{
+    BandImpl *im = (BandImpl *)bp.chunk_b0;
+    if (!im || bp.nchunk == 0) return;
+    int KD = bp.kd;
+    unsigned g = (unsigned)((bp.nchunk + 63) / 64);
+    unsigned gs = (unsigned)((im->nseg_used + 63) / 64);
+    BAND_DISPATCH(KD, ({
+        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, bp.L, r, im->s0f);
+        k_band_chain<KD><<<gs, 64, 0, s>>>(im->seg_first, im->nseg_used, im->s0f, bp.Tf, im->sinf, 0);
+        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, bp.L, r, im->sinf, z, im->s0b);
+        k_band_chain<KD><<<gs, 64, 0, s>>>(im->seg_first, im->nseg_used, im->s0b, bp.Tb, im->sinb, 1);
+        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, bp.L, im->sinb, z);
+    }));
+}
+
+void band_free(BandPrec &bp)
+{
+    BandImpl *im = (BandImpl *)bp.chunk_b0;
+    if (im) {
+        (void)hipFree(im->ch);
+        (void)hipFree(im->seg_first);
+        (void)hipFree(im->s0f);
+        (void)hipFree(im->s0b);
+        (void)hipFree(im->sinf);
+        (void)hipFree(im->sinb);
+        delete im;
+    }
+    if (bp.L) (void)hipFree(bp.L);
+    if (bp.Tf) (void)hipFree(bp.Tf);
+    if (bp.Tb) (void)hipFree(bp.Tb);
+    bp = BandPrec();
+}
+
+}  // namespace madam

@@ -1,0 +1,444 @@
+// Batched FP64 real FFT convolution for the noise prior, and the noise filter.
+//
+// Reference: cinvmul -> convolve_pp -> convolve_interval
+// (src/noise_routines.f90:745-913) on FFTW r2c/c2r plans
+// (src/fourier_fftw_2003.f90:24-123), and build_filter / get_spectrum_interp
+// / interpolate_psd (src/noise_routines.f90:130-190, 367-739).
+//
+// A real signal of length nof is packed as N = nof/2 complex points and
+// transformed with a two-pass (four-step) Stockham-free Cooley-Tukey FFT,
+// N = L1 * L2 with L1, L2 <= 1024, every sub-FFT done in LDS:
+//   forward  pass A: per column n2, FFT over n1 (stride L2), twiddle W_N^(n2 k1)
+//            pass B: per row k1, FFT over n2 (contiguous)        -> X[k1 + L1 k2]
+//   spectral pass:   split into the real spectrum, multiply by the filter,
+//                    merge back into the half-length inverse input
+//   inverse  pass B': per row, inverse FFT; pass A': twiddle, inverse FFT per column
+// All passes run in place on a [batch][N] double2 buffer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "madam_internal.hpp"
+
+namespace madam {
+
+#define F32(x) ((double)(float)(x))
+
+static constexpr int FFT_TPB = 256;
+
+__device__ inline double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cmulc(double2 a, double2 b)   // a * conj(b)
+{
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+__device__ inline unsigned bitrev(unsigned v, int bits)
+{
+    return __brev(v) >> (32 - bits);
+}
+
+// In-place radix-2 FFT of `ncol` interleaved columns of length L held in LDS
+// as s[row * ncol + col]; input already in bit-reversed row order.
+// tw = exp(-2 pi i j / N) table, stride N / L gives W_L.
+__device__ void lds_fft(double2 *s, int L, int logL, int ncol, const double2 *tw, int64_t twstride, bool inv)
+{
+    int nbf = (L / 2) * ncol;
+    for (int st = 0; st < logL; ++st) {
+        int half = 1 << st;
+        for (int t = threadIdx.x; t < nbf; t += blockDim.x) {
+            int col = t % ncol;
+            int bf = t / ncol;
+            int grp = bf >> st, j = bf & (half - 1);
+            int i0 = grp * (2 * half) + j, i1 = i0 + half;
+            double2 w = tw[(int64_t)j * (L / (2 * half)) * twstride];
+            if (inv) w.y = -w.y;
+            double2 a = s[i0 * ncol + col], b = cmul(s[i1 * ncol + col], w);
+            s[i0 * ncol + col] = make_double2(a.x + b.x, a.y + b.y);
+            s[i1 * ncol + col] = make_double2(a.x - b.x, a.y - b.y);
+        }
+        __syncthreads();
+    }
+}
+
+// column pass; grid = (L2 / ncol, batch)
+__global__ void k_fft_col(double2 *data, int64_t N, int L1, int logL1, int L2, int ncol, const double2 *tw, int inv)
+{
+    extern __shared__ double2 s[];
+    int64_t b = blockIdx.y;
+    int c0 = blockIdx.x * ncol;
+    double2 *d = data + b * N;
+    int64_t twstride = N / L1;
+    int tot = L1 * ncol;
+    if (!inv) {
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+            int col = t % ncol, n1 = t / ncol;
+            s[bitrev(n1, logL1) * ncol + col] = d[(int64_t)n1 * L2 + c0 + col];
+        }
+        __syncthreads();
+        lds_fft(s, L1, logL1, ncol, tw, twstride, false);
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+            int col = t % ncol, k1 = t / ncol;
+            int64_t n2 = c0 + col;
+            double2 w = tw[(n2 * k1) % N];
+            d[(int64_t)k1 * L2 + n2] = cmul(s[k1 * ncol + col], w);
+        }
+    } else {
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+            int col = t % ncol, k1 = t / ncol;
+            int64_t n2 = c0 + col;
+            double2 w = tw[(n2 * k1) % N];
+            s[bitrev(k1, logL1) * ncol + col] = cmulc(d[(int64_t)k1 * L2 + n2], w);
+        }
+        __syncthreads();
+        lds_fft(s, L1, logL1, ncol, tw, twstride, true);
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+            int col = t % ncol, n1 = t / ncol;
+            d[(int64_t)n1 * L2 + c0 + col] = s[n1 * ncol + col];
+        }
+    }
+}
+
+// row pass; grid = (L1 / nrow, batch); rows are contiguous runs of L2
+__global__ void k_fft_row(double2 *data, int64_t N, int L1, int L2, int logL2, int nrow, const double2 *tw, int inv)
+{
+    extern __shared__ double2 s[];
+    int64_t b = blockIdx.y;
+    int r0 = blockIdx.x * nrow;
+    double2 *d = data + b * N;
+    int64_t twstride = N / L2;
+    int tot = L2 * nrow;
+    // s[n * nrow + row]
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int row = t / L2, n = t % L2;
+        s[bitrev(n, logL2) * nrow + row] = d[(int64_t)(r0 + row) * L2 + n];
+    }
+    __syncthreads();
+    lds_fft(s, L2, logL2, nrow, tw, twstride, inv != 0);
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int row = t / L2, n = t % L2;
+        d[(int64_t)(r0 + row) * L2 + n] = s[n * nrow + row];
+    }
+}
+
+__device__ inline int64_t kidx(int64_t k, int L1, int L2)
+{
+    // X[k] with k = k1 + L1 * k2 lives at k1 * L2 + k2
+    return (k % L1) * L2 + k / L1;
+}
+
+// real-spectrum split, filter multiply, merge; grid = (ceil((N/2+1)/TPB), batch)
+__global__ void k_spectral(double2 *data, int64_t N, int L1, int L2, const double2 *twnof, const double *fc,
+                           int64_t fc_stride, const int *ipsd_of_batch)
+{
+    int64_t b = blockIdx.y;
+    double2 *d = data + b * N;
+    const double *f = fc + (int64_t)ipsd_of_batch[b] * fc_stride;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k <= N / 2; k += (int64_t)gridDim.x * blockDim.x) {
+        if (k == 0) {
+            double2 a = d[0];
+            double x0 = a.x + a.y, xn = a.x - a.y;
+            double y0 = x0 * f[0], yn = xn * f[N];
+            d[0] = make_double2(y0 + yn, y0 - yn);
+            continue;
+        }
+        int64_t ik = kidx(k, L1, L2), im = kidx(N - k, L1, L2);
+        double2 a = d[ik], bc = d[im];
+        bc.y = -bc.y;   // conj(Z_{N-k})
+        double2 E = make_double2(0.5 * (a.x + bc.x), 0.5 * (a.y + bc.y));
+        double2 O = make_double2(0.5 * (a.y - bc.y), -0.5 * (a.x - bc.x));
+        double2 W = twnof[k];
+        double2 WO = cmul(W, O);
+        double2 Xk = make_double2(E.x + WO.x, E.y + WO.y);
+        double2 Xm = make_double2(E.x - WO.x, -(E.y - WO.y));   // X_{N-k} = conj(E - W O)
+        double fk = f[k], fm = f[N - k];
+        double2 Yk = make_double2(Xk.x * fk, Xk.y * fk);
+        double2 Ym = make_double2(Xm.x * fm, Xm.y * fm);
+        // z_k = (Yk + conj Ym) + i W^-k (Yk - conj Ym)
+        double2 s1 = make_double2(Yk.x + Ym.x, Yk.y - Ym.y);
+        double2 d1 = make_double2(Yk.x - Ym.x, Yk.y + Ym.y);
+        double2 t1 = cmulc(d1, W);   // d1 * W^-k
+        double2 zk = make_double2(s1.x - t1.y, s1.y + t1.x);
+        d[ik] = zk;
+        if (N - k != k) {
+            // z_{N-k} = (Ym + conj Yk) - i W^k (Ym - conj Yk)
+            double2 s2 = make_double2(Ym.x + Yk.x, Ym.y - Yk.y);
+            double2 d2 = make_double2(Ym.x - Yk.x, Ym.y + Yk.y);
+            double2 t2 = cmul(d2, W);
+            double2 zm = make_double2(s2.x + t2.y, s2.y - t2.x);
+            d[im] = zm;
+        }
+    }
+}
+
+// plain c2r of one filter row (no segment): imaginary parts ignored like FFTW
+__global__ void k_c2r_prep(double2 *data, int64_t N, int L1, int L2, const double2 *twnof, const double *fc,
+                           int64_t fc_stride)
+{
+    int64_t b = blockIdx.y;
+    double2 *d = data + b * N;
+    const double *Y = fc + b * fc_stride;   // real spectrum values
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < N; k += (int64_t)gridDim.x * blockDim.x) {
+        double2 W = twnof[k <= N / 2 ? k : N - k];
+        if (k > N / 2) { W.x = -W.x; }   // exp(-2 pi i k / nof) for k > N/2 from the mirror: -conj(W_{N-k})
+        // with purely real Y: z_k = (Y_k + Y_{N-k}) + i W^-k (Y_k - Y_{N-k}), Y_N at index N
+        double yk = Y[k], ym = Y[N - k];
+        if (k == 0) { d[0] = make_double2(Y[0] + Y[N], Y[0] - Y[N]); continue; }
+        double dr = yk - ym;
+        double2 t = make_double2(dr * W.x, -dr * W.y);   // dr * conj(W)
+        d[kidx(k, L1, L2)] = make_double2((yk + ym) - t.y, t.x);
+    }
+}
+
+// segment load: x - mean(x) over the segment, zero-padded to nof; one block per batch entry
+__global__ void k_seg_load(const double *x, const Segment *seg, int64_t s0, int64_t nof, double *work)
+{
+    Segment sg = seg[s0 + blockIdx.x];
+    const double *xs = x + sg.b0;
+    double *w = work + (int64_t)blockIdx.x * nof;
+    __shared__ double sh[FFT_TPB / 64];
+    double acc = 0;
+    for (int64_t j = threadIdx.x; j < sg.noba; j += blockDim.x) acc += xs[j];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    double tot = 0;
+    for (int k = 0; k < FFT_TPB / 64; ++k) tot += sh[k];
+    double mean = tot / sg.noba;
+    for (int64_t j = threadIdx.x; j < nof; j += blockDim.x) w[j] = j < sg.noba ? xs[j] - mean : 0.0;
+}
+
+__global__ void k_seg_store(const double *work, const Segment *seg, int64_t s0, int64_t nof, double *y)
+{
+    Segment sg = seg[s0 + blockIdx.x];
+    const double *w = work + (int64_t)blockIdx.x * nof;
+    double inv = 1 / (double)nof;
+    __shared__ double sh[FFT_TPB / 64];
+    double acc = 0;
+    for (int64_t j = threadIdx.x; j < sg.noba; j += blockDim.x) acc += w[j] * inv;
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    double tot = 0;
+    for (int k = 0; k < FFT_TPB / 64; ++k) tot += sh[k];
+    double mean = tot / sg.noba;
+    for (int64_t j = threadIdx.x; j < sg.noba; j += blockDim.x) y[sg.b0 + j] = w[j] * inv - mean;
+}
+
+__global__ void k_seg_ipsd(const Segment *seg, int64_t s0, int64_t nb, int *ipsd)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < nb) ipsd[i] = seg[s0 + i].ipsd;
+}
+
+static int ilog2(int64_t v)
+{
+    int r = 0;
+    while ((1LL << r) < v) ++r;
+    return r;
+}
+
+void fft_plan_init(FftPlan &p, int64_t nof)
+{
+    p.N = nof / 2;
+    int logN = ilog2(p.N);
+    int l1 = (logN + 1) / 2;
+    if (l1 > 10) l1 = 10;
+    int l2 = logN - l1;
+    if (l2 > 10) { l2 = 10; l1 = logN - 10; }
+    p.L1 = 1LL << l1;
+    p.L2 = 1LL << l2;
+    std::vector<double> tw(2 * p.N), twn(2 * (p.N / 2 + 1));
+    for (int64_t j = 0; j < p.N; ++j) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)p.N;
+        tw[2 * j] = (double)cosl(a);
+        tw[2 * j + 1] = (double)sinl(a);
+    }
+    for (int64_t k = 0; k <= p.N / 2; ++k) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)nof;
+        twn[2 * k] = (double)cosl(a);
+        twn[2 * k + 1] = (double)sinl(a);
+    }
+    MADAM_CHECK(hipMalloc(&p.tw_n, sizeof(double) * 2 * p.N));
+    MADAM_CHECK(hipMalloc(&p.tw_nof, sizeof(double) * 2 * (p.N / 2 + 1)));
+    MADAM_CHECK(hipMemcpy(p.tw_n, tw.data(), sizeof(double) * 2 * p.N, hipMemcpyHostToDevice));
+    MADAM_CHECK(hipMemcpy(p.tw_nof, twn.data(), sizeof(double) * 2 * (p.N / 2 + 1), hipMemcpyHostToDevice));
+}
+
+void fft_plan_free(FftPlan &p)
+{
+    if (p.tw_n) (void)hipFree(p.tw_n);
+    if (p.tw_nof) (void)hipFree(p.tw_nof);
+    p.tw_n = p.tw_nof = nullptr;
+}
+
+static void fft_forward(hipStream_t s, const FftPlan &p, double2 *d, int64_t batch)
+{
+    int logL1 = ilog2(p.L1), logL2 = ilog2(p.L2);
+    int ncol = (int)(p.L2 < 4 ? p.L2 : 4);
+    if (p.L1 * ncol > 4096) ncol = (int)(4096 / p.L1);
+    if (ncol < 1) ncol = 1;
+    dim3 gc((unsigned)(p.L2 / ncol), (unsigned)batch);
+    k_fft_col<<<gc, FFT_TPB, sizeof(double2) * p.L1 * ncol, s>>>(d, p.N, (int)p.L1, logL1, (int)p.L2, ncol,
+                                                                 (const double2 *)p.tw_n, 0);
+    if (p.L2 > 1) {
+        int nrow = (int)(4096 / p.L2);
+        if (nrow > p.L1) nrow = (int)p.L1;
+        if (nrow < 1) nrow = 1;
+        dim3 gr((unsigned)(p.L1 / nrow), (unsigned)batch);
+        k_fft_row<<<gr, FFT_TPB, sizeof(double2) * p.L2 * nrow, s>>>(d, p.N, (int)p.L1, (int)p.L2, logL2, nrow,
+                                                                     (const double2 *)p.tw_n, 0);
+    }
+}
+
+static void fft_inverse(hipStream_t s, const FftPlan &p, double2 *d, int64_t batch)
+{
+    int logL1 = ilog2(p.L1), logL2 = ilog2(p.L2);
+    if (p.L2 > 1) {
+        int nrow = (int)(4096 / p.L2);
+        if (nrow > p.L1) nrow = (int)p.L1;
+        if (nrow < 1) nrow = 1;
+        dim3 gr((unsigned)(p.L1 / nrow), (unsigned)batch);
+        k_fft_row<<<gr, FFT_TPB, sizeof(double2) * p.L2 * nrow, s>>>(d, p.N, (int)p.L1, (int)p.L2, logL2, nrow,
+                                                                     (const double2 *)p.tw_n, 1);
+    }
+    int ncol = (int)(p.L2 < 4 ? p.L2 : 4);
+    if (p.L1 * ncol > 4096) ncol = (int)(4096 / p.L1);
+    if (ncol < 1) ncol = 1;
+    dim3 gc((unsigned)(p.L2 / ncol), (unsigned)batch);
+    k_fft_col<<<gc, FFT_TPB, sizeof(double2) * p.L1 * ncol, s>>>(d, p.N, (int)p.L1, logL1, (int)p.L2, ncol,
+                                                                 (const double2 *)p.tw_n, 1);
+}
+
+void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg, const double *x,
+                           double *y, const double *fc, int64_t fc_stride, double *work, int64_t batch,
+                           const Segment *hseg)
+{
+    (void)hseg;
+    int64_t nof = 2 * p.N;
+    int *ipsd = (int *)(work + batch * nof);   // small scratch right after the batch buffer
+    for (int64_t s0 = 0; s0 < nseg; s0 += batch) {
+        int64_t nb = nseg - s0 < batch ? nseg - s0 : batch;
+        k_seg_load<<<(unsigned)nb, FFT_TPB, 0, s>>>(x, dseg, s0, nof, work);
+        k_seg_ipsd<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(dseg, s0, nb, ipsd);
+        fft_forward(s, p, (double2 *)work, nb);
+        dim3 gs((unsigned)((p.N / 2 + 1 + FFT_TPB - 1) / FFT_TPB), (unsigned)nb);
+        k_spectral<<<gs, FFT_TPB, 0, s>>>((double2 *)work, p.N, (int)p.L1, (int)p.L2, (const double2 *)p.tw_nof, fc,
+                                           fc_stride, ipsd);
+        fft_inverse(s, p, (double2 *)work, nb);
+        k_seg_store<<<(unsigned)nb, FFT_TPB, 0, s>>>(work, dseg, s0, nof, y);
+    }
+}
+
+__global__ void k_scale(double *x, double f, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] *= f;
+}
+
+void fft_c2r_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrows, double *out, double *work)
+{
+    (void)work;
+    // out[row][nof] = c2r(fc[row][N+1]) / nof
+    dim3 g((unsigned)((p.N + FFT_TPB - 1) / FFT_TPB), (unsigned)nrows);
+    k_c2r_prep<<<g, FFT_TPB, 0, s>>>((double2 *)out, p.N, (int)p.L1, (int)p.L2, (const double2 *)p.tw_nof, fc,
+                                      p.N + 1);
+    fft_inverse(s, p, (double2 *)out, nrows);
+    int64_t n = nrows * 2 * p.N;
+    k_scale<<<(unsigned)((n + 255) / 256 < 65535 ? (n + 255) / 256 : 65535), 256, 0, s>>>(out, 1 / (double)(2 * p.N), n);
+}
+
+// ---------------------------------------------------------------- noise filter
+
+// interpolate_psd for one target frequency (src/noise_routines.f90:130-190),
+// lf / lp are log(freq) / log(psd)
+__device__ double interp_psd(const double *freq, const double *psd, const double *lf, const double *lp, int64_t n_in,
+                             int64_t startbin, double tf)
+{
+    if (tf == 0) return 0.0;
+    double flog = log(tf);
+    // the reference walks j forward from startbin while lf[j+1] < flog, up to n_in - 2
+    int64_t lo = startbin, hi = n_in - 2;
+    if (lo > hi) lo = hi;
+    // smallest j in [startbin, n_in-2] with lf[j+1] >= flog, else n_in-2
+    int64_t a = lo, b = hi;
+    while (a < b) {
+        int64_t mid = (a + b) >> 1;
+        if (lf[mid + 1] < flog) a = mid + 1;
+        else b = mid;
+    }
+    int64_t j = a;
+    if (psd[j] > 0 && psd[j + 1] > 0) {
+        double r = (flog - lf[j]) / (lf[j + 1] - lf[j]);
+        return exp(lp[j] * (1.0 - r) + lp[j + 1] * r);
+    }
+    double r = (tf - freq[j]) / (freq[j + 1] - freq[j]);
+    return psd[j] * (1.0 - r) + psd[j + 1] * r;
+}
+
+__device__ double spec_value(const double *freq, const double *psd, const double *lf, const double *lp, int64_t n_in,
+                             int64_t startbin, double f, double plateau, double margin, int *bad)
+{
+    double v = interp_psd(freq, psd, lf, lp, n_in, startbin, f);
+    if (v > plateau * (1 + margin)) v = v - plateau;
+    else v = v * margin;
+    if (!(v > 0) && f != 0) atomicOr(bad, 1);
+    return v;
+}
+
+// aspec / fcov for one PSD; work holds [freq | psd | lf | lp] (4 * n_in)
+__global__ void k_build_filter(int64_t nof, double fsample, int nshort, double plateau, double margin, double sigma,
+                               int64_t n_in, int64_t startbin, const double *tab, double *fcov, int *bad)
+{
+    const double *freq = tab, *psd = tab + n_in, *lf = tab + 2 * n_in, *lp = tab + 3 * n_in;
+    double fa = fsample / nshort;
+    int64_t nh = nof / 2 + 1;
+    const double PI = 3.14159265358979323846;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nh; i += (int64_t)gridDim.x * blockDim.x) {
+        double asp = 0;
+        for (int k = 0; k <= 2; ++k) {
+            double fi = k * fa + ((double)i * fa) / nof;
+            double si = spec_value(freq, psd, lf, lp, n_in, startbin, fi, plateau, margin, bad);
+            if (k == 0 && i == 0) si = 0.0;   // f(1) == 0: mean left unconstrained
+            double x = PI * fi / fa;
+            double gi;
+            if (x < F32(1.e-30)) gi = 1;
+            else { double sx = sin(x); gi = (sx * sx) / (x * x); }
+            if (i == 0) {
+                if (k == 0) asp = si;
+                else asp = asp + 2 * si * gi;
+            } else {
+                int64_t m = nof - i;
+                double fm = k * fa + ((double)m * fa) / nof;
+                double sm = spec_value(freq, psd, lf, lp, n_in, startbin, fm, plateau, margin, bad);
+                double xm = PI * fm / fa;
+                double gm;
+                if (xm < F32(1.e-30)) gm = 1;
+                else { double sx = sin(xm); gm = (sx * sx) / (xm * xm); }
+                asp = asp + si * gi + sm * gm;
+            }
+        }
+        double v;
+        if (fabs(asp) > sigma * F32(1e-30)) v = 1.0 / (fa * asp);
+        else v = 0;
+        if (i == 0) v = 0;
+        fcov[i] = v;
+    }
+}
+
+void launch_build_filter(hipStream_t s, int64_t nof, double fsample, int nshort, int radiometers, double sigma,
+                         double plateau, int64_t n_in, int64_t startbin, const double *tab, double *fcov, int *bad)
+{
+    double margin = radiometers ? F32(1e-6) : F32(1e-3);
+    int64_t nh = nof / 2 + 1;
+    unsigned g = (unsigned)((nh + FFT_TPB - 1) / FFT_TPB);
+    if (g > 65535) g = 65535;
+    k_build_filter<<<g, FFT_TPB, 0, s>>>(nof, fsample, nshort, plateau, margin, sigma, n_in, startbin, tab, fcov,
+                                         bad);
+}
+
+}  // namespace madam

@@ -1,0 +1,1451 @@
+// Host orchestration and the C-ABI of libmadam.so.
+//
+// Reference entry points: destripe() (src/madam.h:2, src/smadam.F90:64-590),
+// destripe_with_cache() (src/smadam.F90:593-918) and clear_caches()
+// (src/smadam.F90:921-951).  The host side parses the parameter string
+// (src/inputparam.f90:24-371, defaults src/commonparam.f90:200-305), sets up
+// detectors (src/inputparam.f90:73-163), intervals and baselines
+// (src/read_data.f90:203-323, src/parameter_control.f90:27-409, 752-977),
+// and drives the HIP kernels; every per-sample, per-pixel and per-baseline
+// step runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "madam_internal.hpp"
+
+#define F32(x) ((double)(float)(x))
+
+namespace madam {
+
+static thread_local std::string g_err;
+
+struct Error {
+    std::string msg;
+};
+
+[[noreturn]] static void fail(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    throw Error{buf};
+}
+
+// ------------------------------------------------------------------ parameters
+
+struct Params {
+    int info = 2, nthreads = 1, nsubchunk = 0, isubchunk = 0;
+    double dnshort = -1;
+    int nside_map = 512, nside_cross = -1, nside_submap = 16;
+    double good_baseline_fraction = 0;
+    int pixmode_map = 2, pixmode_cross = 2;
+    double pixlim_map = F32(1e-6), pixlim_cross = F32(1e-3);
+    int incomplete_matrices = 0, allow_decoupling = 0, kfirst = 1, basis_order = 0;
+    int iter_min = 3, iter_max = 1000;
+    double cglimit = F32(1e-12), fsample = 1;
+    int mode_detweight = 0, temperature_only = 0, noise_weights_from_psd = 0, radiometers = 1;
+    int kfilter = 0;
+    double diagfilter = 0;
+    int precond_width_min = 10, precond_width_max = 100, use_fprecond = 0, use_cgprecond = 0;
+    int rm_monopole = 0, write_tod = 0, ndetset = 0, nsurvey = 0, mcmode = 0;
+    bool has_inmask = false, has_spectrum = false, has_covmat = false;
+    int nshort = 0;
+};
+
+static std::string strip(const std::string &s)
+{
+    size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+    if (a == std::string::npos) return "";
+    return s.substr(a, b - a + 1);
+}
+
+static std::string token(const std::string &v)
+{
+    std::string s = strip(v);
+    size_t e = s.find_first_of(" ,\t");
+    return e == std::string::npos ? s : s.substr(0, e);
+}
+
+static bool parse_int(const std::string &v, int &out)
+{
+    std::string t = token(v);
+    if (t.empty()) return false;
+    char *end = nullptr;
+    long x = strtol(t.c_str(), &end, 10);
+    if (*end) return false;
+    out = (int)x;
+    return true;
+}
+
+static bool parse_real(const std::string &v, double &out)
+{
+    std::string t = token(v);
+    if (t.empty()) return false;
+    for (auto &c : t)
+        if (c == 'd' || c == 'D') c = 'e';
+    char *end = nullptr;
+    double x = strtod(t.c_str(), &end);
+    if (*end) return false;
+    out = x;
+    return true;
+}
+
+static bool parse_bool(const std::string &v, int &out)
+{
+    std::string t = token(v);
+    size_t i = (!t.empty() && t[0] == '.') ? 1 : 0;
+    if (i >= t.size()) return false;
+    if (t[i] == 'T' || t[i] == 't') { out = 1; return true; }
+    if (t[i] == 'F' || t[i] == 'f') { out = 0; return true; }
+    return false;
+}
+
+static bool is_all(const std::string &s)
+{
+    return s.empty() || s == "all" || s == "All" || s == "ALL";
+}
+
+// src/inputparam.f90:169-371
+static void read_line(Params &p, std::string line)
+{
+    line = strip(line);
+    if (line.empty() || line[0] == '#') return;
+    size_t eq = line.find('=');
+    if (eq == std::string::npos) fail("Syntax error in parameter definition: %s", line.c_str());
+    std::string key = strip(line.substr(0, eq)), value = strip(line.substr(eq + 1));
+    size_t h = value.find('#');
+    if (h != std::string::npos) value = strip(value.substr(0, h));
+    if (key.compare(0, 8, "detector") == 0) key = "detector";
+    if (value.empty() && key.compare(0, 4, "file") != 0) return;
+    bool ok = true;
+    int ig;
+    double rg;
+    if (key == "info") ok = parse_int(value, p.info);
+    else if (key == "nthreads") ok = parse_int(value, p.nthreads);
+    else if (key == "nsubchunk") ok = parse_int(value, p.nsubchunk);
+    else if (key == "isubchunk") ok = parse_int(value, p.isubchunk);
+    else if (key == "time_unit") { if (value[0] != 'p' && value[0] != 'P') ok = parse_real(value, rg); }
+    else if (key == "base_first" || key == "nshort") ok = parse_real(value, p.dnshort);
+    else if (key == "nside_map") ok = parse_int(value, p.nside_map);
+    else if (key == "nside_cross") ok = parse_int(value, p.nside_cross);
+    else if (key == "nside_submap") ok = parse_int(value, p.nside_submap);
+    else if (key == "good_baseline_fraction") ok = parse_real(value, p.good_baseline_fraction);
+    else if (key == "concatenate_messages" || key == "allreduce" || key == "reassign_submaps" ||
+             key == "flag_by_horn" || key == "write_cut" || key == "checknan" || key == "sync_output" ||
+             key == "skip_existing" || key == "force_pol" || key == "unaligned_fft" || key == "write_map" ||
+             key == "write_matrix" || key == "write_wcov" || key == "write_base" || key == "write_leakmatrix" ||
+             key == "binary_output" || key == "bin_subsets") ok = parse_bool(value, ig);
+    else if (key == "pixmode_map") ok = parse_int(value, p.pixmode_map);
+    else if (key == "pixmode_cross") ok = parse_int(value, p.pixmode_cross);
+    else if (key == "pixlim_map") ok = parse_real(value, p.pixlim_map);
+    else if (key == "pixlim_cross") ok = parse_real(value, p.pixlim_cross);
+    else if (key == "incomplete_matrices") ok = parse_bool(value, p.incomplete_matrices);
+    else if (key == "allow_decoupling") ok = parse_bool(value, p.allow_decoupling);
+    else if (key == "kfirst") ok = parse_bool(value, p.kfirst);
+    else if (key == "basis_func") {
+        std::string v4 = value.substr(0, 4);
+        if (v4 != "poly" && v4 != "four" && v4 != "cheb" && v4 != "lege") fail("Unknown function basis : %s", value.c_str());
+    } else if (key == "basis_order") {
+        ok = parse_int(value, p.basis_order);
+        if (ok && p.basis_order < 0) fail("basis order must be nonnegative");
+    } else if (key == "iter_min") ok = parse_int(value, p.iter_min);
+    else if (key == "iter_max") ok = parse_int(value, p.iter_max);
+    else if (key == "cglimit") ok = parse_real(value, p.cglimit);
+    else if (key == "fsample") ok = parse_real(value, p.fsample);
+    else if (key == "mode_detweight") ok = parse_int(value, p.mode_detweight);
+    else if (key == "temperature_only") ok = parse_bool(value, p.temperature_only);
+    else if (key == "noise_weights_from_psd") ok = parse_bool(value, p.noise_weights_from_psd);
+    else if (key == "radiometers") ok = parse_bool(value, p.radiometers);
+    else if (key == "psdlen" || key == "psd_down" || key == "psd_downsample" || key == "nwrite_binary")
+        ok = parse_int(value, ig);
+    else if (key == "kfilter") ok = parse_bool(value, p.kfilter);
+    else if (key == "diagfilter") ok = parse_real(value, p.diagfilter);
+    else if (key == "precond_width_min") ok = parse_int(value, p.precond_width_min);
+    else if (key == "precond_width_max") ok = parse_int(value, p.precond_width_max);
+    else if (key == "use_fprecond") ok = parse_bool(value, p.use_fprecond);
+    else if (key == "use_cgprecond") ok = parse_bool(value, p.use_cgprecond);
+    else if (key == "rm_monopole") ok = parse_bool(value, p.rm_monopole);
+    else if (key == "path_output" || key == "file_root" || key == "unit_tod" || key == "file_gap_out" ||
+             key == "file_mc" || key == "file_gap") {
+    } else if (key == "write_binmap" || key == "write_hits" || key == "write_mask") ok = parse_bool(value, ig);
+    else if (key == "write_tod") ok = parse_bool(value, p.write_tod);
+    else if (key == "file_inmask") p.has_inmask = !value.empty();
+    else if (key == "file_spectrum") p.has_spectrum = !value.empty();
+    else if (key == "file_covmat") p.has_covmat = !value.empty();
+    else if (key == "detset" || key == "detset_nopol") {
+        size_t c = value.find(':');
+        if (c == std::string::npos) fail("Failed to parse: %s for valid detector set name", value.c_str());
+        if (!is_all(strip(value.substr(c + 1)))) p.ndetset++;
+    } else if (key == "survey") {
+        size_t c = value.find(':');
+        if (c == std::string::npos) fail("Failed to parse: %s for valid survey name", value.c_str());
+        std::string rest = strip(value.substr(c + 1));
+        if (rest.find('-') == std::string::npos) {
+            if (!is_all(rest)) fail("Failed to parse: %s for valid survey span marker (-)", rest.c_str());
+        } else p.nsurvey++;
+    } else if (key == "mcmode") ok = parse_bool(value, p.mcmode);
+    else fail("Unknown parameter: %s", key.c_str());
+    if (!ok) fail("Failed to parse %s from %s", key.c_str(), value.c_str());
+}
+
+// src/inputparam.f90:24-67, 500-505
+static Params read_parameters(const char *parstring)
+{
+    Params p;
+    std::string s(parstring ? parstring : "");
+    size_t pos = 0;
+    while (pos < s.size()) {
+        size_t e = s.find(';', pos);
+        std::string line = s.substr(pos, e == std::string::npos ? std::string::npos : e - pos);
+        read_line(p, line);
+        if (e == std::string::npos) break;
+        pos = e + 1;
+    }
+    p.dnshort = (double)llround(p.dnshort * p.fsample);
+    if (p.kfirst && p.dnshort < 0) p.dnshort = F32(1e10);
+    return p;
+}
+
+// ------------------------------------------------------------------ detectors and plan
+
+struct Detector {
+    int npsd = 1;
+    bool has_psd = false;
+    std::vector<double> weights, sigmas, psdstarts, plateaus, psds;   // psds [npsd][npsdbin]
+};
+
+static void interpolate_psd(const std::vector<double> &freq, const double *psd, const double *tf, int64_t n_out,
+                            double *tp)
+{
+    // src/noise_routines.f90:130-190 (host copy for the plateau and noise weights)
+    int64_t n_in = (int64_t)freq.size();
+    std::vector<double> lf(n_in), lp(n_in);
+    int64_t startbin = 0;
+    while (startbin < n_in && freq[startbin] == 0) startbin++;
+    for (int64_t i = 0; i < n_in; ++i) { lf[i] = std::log(freq[i]); lp[i] = std::log(psd[i]); }
+    int64_t j = startbin;
+    for (int64_t i = 0; i < n_out; ++i) {
+        if (tf[i] == 0) { tp[i] = 0; continue; }
+        double flog = std::log(tf[i]);
+        if (j < n_in - 2) {
+            while (lf[j + 1] < flog) {
+                j++;
+                if (j == n_in - 2) break;
+            }
+        }
+        if (psd[j] > 0 && psd[j + 1] > 0) {
+            double r = (flog - lf[j]) / (lf[j + 1] - lf[j]);
+            tp[i] = std::exp(lp[j] * (1.0 - r) + lp[j + 1] * r);
+        } else {
+            double r = (tf[i] - freq[j]) / (freq[j + 1] - freq[j]);
+            tp[i] = psd[j] * (1.0 - r) + psd[j + 1] * r;
+        }
+    }
+}
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        if (count <= n && p) return;
+        release();
+        MADAM_CHECK(hipMalloc(&p, sizeof(T) * (count > 0 ? count : 1)));
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+};
+
+enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_PRECOND, ST_T_ITER, ST_T_FINAL,
+       ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
+       ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_COUNT };
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool configured = false, pointed = false, solved = false;
+    Params p;
+    int ndet = 0, nnz = 0, nmap = 0, nside_max = 0;
+    int64_t nsamp = 0, npix_map = 0, npix_cross = 0, ndeg = 1;
+    int pass = 0;
+    std::vector<Detector> det;
+    std::vector<double> psdfreqs;
+    // intervals
+    int nival = 0;
+    std::vector<int64_t> ivs;          // [nival + 1] starts
+    std::vector<int64_t> noba_pp, kstart_pp;
+    int64_t nbase = 0, nbtot = 0, noba_pp_max = 0;
+    std::vector<int64_t> bstart;       // [nbase + 1]
+    std::vector<double> btime;
+    std::vector<double> dw_ival;       // [ndet][nival]
+    std::vector<int> ipsd_det_ival, ipsd_glob_ival;
+    // filter
+    int64_t nof = 0;
+    int npsdtot = 0;
+    std::vector<int> psddet;
+    std::vector<double> psdstart, psdstop, psd_w;   // per global PSD
+    // device: plan
+    DBuf<int64_t> d_ivs, d_bstart, d_kstart_pp, d_noba_pp;
+    DBuf<double> d_dw_ival, d_dwb, d_sdwb, d_wbinv;
+    DBuf<int8_t> d_haspsd;
+    // device: pointing
+    DBuf<int32_t> d_pix_own;
+    DBuf<float> d_w_own;
+    int32_t *d_pix = nullptr;
+    float *d_w = nullptr;
+    DBuf<PixW> d_pixw;
+    DBuf<int32_t> d_hits;
+    DBuf<int64_t> d_off, d_cursor, d_scan_tmp;
+    DBuf<CsrEnt> d_ent;
+    DBuf<uint32_t> d_src;
+    int64_t nactive = 0;
+    bool pix_host_dirty = false;
+    // device: maps
+    DBuf<double> d_cc6, d_rhs4, d_rhsd4, d_cc6x, d_rhs4x, d_ccinv6x, d_m4x, d_ccinv6, d_crit;
+    DBuf<int32_t> d_mask;
+    DBuf<double> d_map4, d_binmap4;
+    // device: baselines
+    DBuf<double> d_tod_own;
+    double *d_tod = nullptr;
+    DBuf<double> d_yba, d_nna, d_aa, d_r, d_ro, d_z, d_p, d_pw, d_ap, d_prec, d_scal, d_part;
+    DBuf<int8_t> d_rmask;
+    // segments / prior
+    std::vector<Segment> seg;
+    DBuf<Segment> d_seg;
+    std::vector<Segment> segconv;      // segments the prior convolves (PSD found)
+    DBuf<Segment> d_segconv;
+    DBuf<unsigned long long> d_cnt;
+    DBuf<double> d_fcov, d_fprec, d_invcov, d_fftwork, d_tab;
+    FftPlan fft{};
+    bool fft_ready = false;
+    int64_t fft_batch = 0;
+    int precond_kind = 0;
+    BandPrec band;
+    bool band_ready = false;
+    // results
+    double stats[ST_COUNT] = {0};
+    std::vector<double> rr_hist;
+    int profile = 0;
+
+    ~Context()
+    {
+        if (band_ready) band_free(band);
+        if (fft_ready) fft_plan_free(fft);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+static double now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int psd_index_det(const Detector &d, double t)
+{
+    // src/noise_routines.f90:211-236
+    if (!d.has_psd && d.npsd == 1) return 0;
+    int ipsd = 0;
+    for (;;) {
+        if (ipsd == d.npsd - 1) return d.psdstarts[ipsd] > t ? -1 : ipsd;
+        if (d.psdstarts[ipsd + 1] > t) return ipsd;
+        ipsd++;
+    }
+}
+
+static int psd_index(const Context &c, int idet, double t)
+{
+    // src/noise_routines.f90:193-208
+    for (int i = 0; i < c.npsdtot; ++i)
+        if (c.psddet[i] == idet && c.psdstart[i] <= t && c.psdstop[i] >= t) return i;
+    return -1;
+}
+
+static void ensure_stream(Context &c)
+{
+    if (!c.stream) {
+        MADAM_CHECK(hipSetDevice(c.device));
+        MADAM_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    }
+}
+
+template <typename T>
+static void h2d(T *d, const T *h, size_t n, hipStream_t s)
+{
+    if (n) MADAM_CHECK(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
+}
+template <typename T>
+static void d2h(T *h, const T *d, size_t n, hipStream_t s)
+{
+    if (n) MADAM_CHECK(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+}
+
+// ------------------------------------------------------------------ configure
+
+static void configure(Context &c, const char *parstring, long ndet, const char *detstring, const double *detweights,
+                      long nsamp, long nnz, const double *timestamps, long nperiod, const long *periods,
+                      const long *npsd, long npsdtot, const double *psdstarts, long npsdbin, const double *psdfreqs,
+                      long npsdval, const double *psdvals, int pass)
+{
+    (void)detstring;
+    (void)npsdval;
+    ensure_stream(c);
+    Params p = read_parameters(parstring);
+    // src/smadam.F90:138-144
+    if (p.mcmode && p.nsubchunk > 1) fail("MCMode is not compatible with nsubchunk > 1.");
+    if (p.temperature_only && nnz != 1) fail("temperature_only=T but pointing weights are polarized");
+    if (ndet > 10000) fail("ndet > NDETMAX");
+    if (nnz != 1 && nnz != 3) fail("nnz=%ld is not supported (nmap must be 1 or 3)", nnz);
+    if (p.has_inmask || p.has_spectrum || p.has_covmat) fail("file_inmask / file_spectrum / file_covmat are not supported");
+    if ((uint64_t)ndet * (uint64_t)nsamp >= (1ULL << 32)) fail("ndet * nsamp must be below 2^32");
+    c.ndet = (int)ndet;
+    c.nsamp = nsamp;
+    c.nnz = (int)nnz;
+    c.nmap = (int)nnz;
+    c.pass = pass;
+    c.psdfreqs.assign(psdfreqs, psdfreqs + npsdbin);
+    // read_detectors
+    c.det.assign(ndet, Detector());
+    long ipsd = 0;
+    for (long d = 0; d < ndet; ++d) {
+        Detector &D = c.det[d];
+        int nint = (int)npsd[d];
+        D.npsd = nint == 0 ? 1 : nint;
+        double w = detweights[d];
+        D.weights.assign(D.npsd, w);
+        D.sigmas.assign(D.npsd, w > 0 ? 1.0 / std::sqrt(w) : 0.0);
+        D.plateaus.resize(D.npsd);
+        for (int j = 0; j < D.npsd; ++j) D.plateaus[j] = D.sigmas[j] * D.sigmas[j] / p.fsample;
+        D.psdstarts.assign(D.npsd, 0.0);
+        if (nint > 0) {
+            D.has_psd = true;
+            D.psds.resize((size_t)nint * npsdbin);
+            for (int j = 0; j < nint; ++j) {
+                D.psdstarts[j] = psdstarts[ipsd];
+                std::copy(psdvals + ipsd * npsdbin, psdvals + (ipsd + 1) * npsdbin, D.psds.begin() + (size_t)j * npsdbin);
+                ipsd++;
+            }
+        }
+    }
+    (void)npsdtot;
+    // intervals (check_files) and parameters (init_parameters)
+    c.nival = (int)nperiod;
+    c.ivs.assign(nperiod + 1, 0);
+    std::vector<int64_t> len(nperiod);
+    for (long i = 0; i < nperiod; ++i) {
+        len[i] = i < nperiod - 1 ? periods[i + 1] - periods[i] : nsamp - periods[i];
+        c.ivs[i + 1] = c.ivs[i] + len[i];
+    }
+    if (nperiod < 1) fail("ERROR: intervals not known.");
+    if (!p.radiometers)
+        for (auto &D : c.det) {
+            if (!D.has_psd) continue;
+            for (int j = 0; j < D.npsd; ++j) {
+                double *psd = D.psds.data() + (size_t)j * npsdbin;
+                long im = 0;
+                for (long i = 1; i < npsdbin; ++i) if (psd[i] > psd[im]) im = i;
+                for (long i = 0; i < im; ++i) psd[i] = psd[im];
+            }
+        }
+    if (p.noise_weights_from_psd) {
+        for (auto &D : c.det) {
+            if (!D.has_psd) fail("noise_weights_from_psd needs PSDs for every detector");
+            for (int j = 0; j < D.npsd; ++j) {
+                const double *psd = D.psds.data() + (size_t)j * npsdbin;
+                if (p.radiometers) {
+                    double fr = p.fsample / 2, pl;
+                    interpolate_psd(c.psdfreqs, psd, &fr, 1, &pl);
+                    D.sigmas[j] = std::sqrt(pl * p.fsample);
+                    D.plateaus[j] = pl;
+                } else {
+                    int nbin = 10000;
+                    double fstep = p.fsample * .5 / nbin;
+                    std::vector<double> fr(nbin), dd(nbin);
+                    for (int i = 0; i < nbin; ++i) fr[i] = fstep * (i + 1);
+                    double fbase = 1 / (p.dnshort / p.fsample);
+                    int ibase = (int)(fbase / fstep), ilower = (int)(1 / fstep), iupper = (int)(10 / fstep);
+                    interpolate_psd(c.psdfreqs, psd, fr.data(), nbin, dd.data());
+                    double pl = dd[ilower - 1];
+                    for (int i = ilower; i < iupper && i < nbin; ++i) pl = std::min(pl, dd[i]);
+                    double rms = ibase * pl;
+                    for (int i = ibase; i < nbin; ++i) rms += dd[i];
+                    D.sigmas[j] = std::sqrt(rms / nbin * p.fsample);
+                    D.plateaus[j] = pl;
+                    D.weights[j] = 1 / (pl * p.fsample);
+                }
+            }
+        }
+    }
+    if (p.nsubchunk < 2) { p.nsubchunk = 0; p.isubchunk = 0; }
+    if (p.isubchunk > p.nsubchunk) fail("ERROR: isubchunk > nsubchunk");
+    if (pass < 0 || pass > p.nsubchunk) fail("subchunk pass %d out of range", pass);
+    if (p.good_baseline_fraction < 0 || p.good_baseline_fraction > 1) fail("ERROR: good_baseline_fraction must be in 0..1");
+    if (p.nside_cross < 0) p.nside_cross = p.nside_map;
+    for (int ns : {p.nside_map, p.nside_submap, p.nside_cross})
+        if (ns < 1 || (ns & (ns - 1))) fail("nside must be a power of two");
+    if (p.nside_map < p.nside_cross) fail("nside_cross > nside_map not supported by Madam/TOAST");
+    if (p.pixlim_cross < 0) p.pixlim_cross = p.pixlim_map;
+    if (p.pixmode_map < 0 || p.pixmode_map > 2) fail("Pixmode_map must be in range [0-2]");
+    if (p.pixmode_cross < 0 || p.pixmode_cross > 4) fail("Pixmode_cross must be in range [0-4]");
+    if (p.basis_order != 0) fail("basis_order > 0 is not supported");
+    if (c.nmap == 1) p.temperature_only = 1;
+    for (auto &D : c.det)
+        for (int j = 0; j < D.npsd; ++j) {
+            if (p.mode_detweight == 0) D.weights[j] = D.sigmas[j] == 0 ? 0 : 1.0 / D.plateaus[j] / p.fsample;
+            else if (p.mode_detweight == 1) D.weights[j] = 1;
+            else fail("mode_detweight=%d is not supported", p.mode_detweight);
+        }
+    c.nside_max = std::max(p.nside_map, p.nside_cross);
+    c.npix_map = 12LL * p.nside_map * p.nside_map;
+    c.npix_cross = 12LL * p.nside_cross * p.nside_cross;
+    c.ndeg = c.npix_map / c.npix_cross;
+    p.nshort = (int)(p.dnshort + .5);
+    if (!p.kfirst) {
+        p.dnshort = 1e6;
+        p.nshort = (int)(p.dnshort + .5);
+        p.precond_width_min = 0;
+        p.precond_width_max = 0;
+        p.kfilter = 0;
+    } else if (p.precond_width_max < p.precond_width_min) p.precond_width_max = p.precond_width_min;
+    c.noba_pp.resize(nperiod);
+    c.kstart_pp.resize(nperiod);
+    c.nbase = 0;
+    c.noba_pp_max = 0;
+    for (long i = 0; i < nperiod; ++i) {
+        c.noba_pp[i] = (int64_t)((len[i] - 1) / p.dnshort + 1);
+        c.kstart_pp[i] = c.nbase;
+        c.nbase += c.noba_pp[i];
+        c.noba_pp_max = std::max(c.noba_pp_max, c.noba_pp[i]);
+    }
+    // start_timeloop: baseline lengths (src/parameter_control.f90:787-818)
+    c.bstart.assign(c.nbase + 1, 0);
+    c.btime.assign(c.nbase, 0.0);
+    {
+        std::vector<int64_t> bl(c.nbase);
+        int64_t m = 0;
+        for (long i = 0; i < nperiod; ++i) {
+            double dn = 0.0;
+            int64_t n = 0;
+            for (int64_t k = 0; k < c.noba_pp[i] - 1; ++k) {
+                int64_t n0 = n;
+                dn = dn + p.dnshort;
+                n = (int64_t)(dn + .5);
+                bl[m++] = n - n0;
+            }
+            bl[m++] = len[i] - n;
+            if (p.kfirst && (bl[m - 1] < 0 || bl[m - 1] > (int64_t)p.dnshort + 1))
+                fail("ERROR in start_timeloop: Lengths do not match.");
+        }
+        for (int64_t k = 0; k < c.nbase; ++k) {
+            c.bstart[k + 1] = c.bstart[k] + bl[k];
+            c.btime[k] = (c.bstart[k] >= 0 && c.bstart[k] < nsamp) ? timestamps[c.bstart[k]] : 0.0;
+        }
+    }
+    c.nbtot = (int64_t)c.ndet * c.nbase;
+    // per (det, interval) PSD indices and weights
+    c.dw_ival.assign((size_t)ndet * nperiod, 0.0);
+    c.ipsd_det_ival.assign((size_t)ndet * nperiod, -1);
+    for (long d = 0; d < ndet; ++d)
+        for (long i = 0; i < nperiod; ++i) {
+            int ip = c.noba_pp[i] > 0 ? psd_index_det(c.det[d], c.btime[c.kstart_pp[i]]) : -1;
+            c.ipsd_det_ival[d * nperiod + i] = ip;
+            c.dw_ival[d * nperiod + i] = ip < 0 ? 0.0 : c.det[d].weights[ip];
+        }
+    // filter tables (init_filter / build_filter bookkeeping)
+    c.npsdtot = 0;
+    c.psddet.clear();
+    c.psdstart.clear();
+    c.psdstop.clear();
+    c.psd_w.clear();
+    c.nof = 0;
+    c.p = p;
+    if (p.kfilter) {
+        int64_t nof_min = 2 * c.noba_pp_max;
+        if (p.precond_width_max > nof_min - 1) nof_min = p.precond_width_max + 1;
+        int64_t nof = 1;
+        while (nof < nof_min) nof *= 2;
+        if (nof < 4) nof = 4;
+        if (nof > (1LL << 21)) fail("FFT length %lld exceeds the supported 2^21", (long long)nof);
+        c.nof = nof;
+        for (long d = 0; d < ndet; ++d) {
+            if (!c.det[d].has_psd) fail("All detectors must have at least one PSD to build the filter");
+            for (int j = 0; j < c.det[d].npsd; ++j) {
+                c.psddet.push_back((int)d);
+                c.psdstart.push_back(c.det[d].psdstarts[j]);
+                c.psdstop.push_back(j < c.det[d].npsd - 1 ? c.det[d].psdstarts[j + 1] : F32(1e30));
+                c.psd_w.push_back(c.det[d].weights[j]);
+            }
+        }
+        c.npsdtot = (int)c.psddet.size();
+    }
+    c.ipsd_glob_ival.assign((size_t)ndet * nperiod, -1);
+    if (p.kfilter)
+        for (long d = 0; d < ndet; ++d)
+            for (long i = 0; i < nperiod; ++i)
+                if (c.noba_pp[i] > 0) c.ipsd_glob_ival[d * nperiod + i] = psd_index(c, (int)d, c.btime[c.kstart_pp[i]]);
+    // device plan arrays
+    hipStream_t s = c.stream;
+    c.d_ivs.alloc(c.ivs.size());
+    h2d(c.d_ivs.p, c.ivs.data(), c.ivs.size(), s);
+    c.d_bstart.alloc(c.bstart.size());
+    h2d(c.d_bstart.p, c.bstart.data(), c.bstart.size(), s);
+    c.d_kstart_pp.alloc(nperiod);
+    h2d(c.d_kstart_pp.p, c.kstart_pp.data(), nperiod, s);
+    c.d_noba_pp.alloc(nperiod);
+    h2d(c.d_noba_pp.p, c.noba_pp.data(), nperiod, s);
+    c.d_dw_ival.alloc(c.dw_ival.size());
+    h2d(c.d_dw_ival.p, c.dw_ival.data(), c.dw_ival.size(), s);
+    {
+        std::vector<double> dwb(c.nbtot), sdwb(c.nbtot), wbinv(c.nbtot);
+        std::vector<int8_t> has(c.nbtot);
+        for (int d = 0; d < c.ndet; ++d)
+            for (int i = 0; i < c.nival; ++i) {
+                double w = c.dw_ival[(size_t)d * c.nival + i];
+                int ip = c.ipsd_det_ival[(size_t)d * c.nival + i];
+                for (int64_t k = c.kstart_pp[i]; k < c.kstart_pp[i] + c.noba_pp[i]; ++k) {
+                    int64_t b = (int64_t)d * c.nbase + k;
+                    dwb[b] = w;
+                    sdwb[b] = std::sqrt(w);
+                    has[b] = ip >= 0;
+                    wbinv[b] = ip >= 0 ? 1. / c.det[d].weights[ip] : 1.0;
+                }
+            }
+        c.d_dwb.alloc(c.nbtot);
+        c.d_sdwb.alloc(c.nbtot);
+        c.d_wbinv.alloc(c.nbtot);
+        c.d_haspsd.alloc(c.nbtot);
+        h2d(c.d_dwb.p, dwb.data(), c.nbtot, s);
+        h2d(c.d_sdwb.p, sdwb.data(), c.nbtot, s);
+        h2d(c.d_wbinv.p, wbinv.data(), c.nbtot, s);
+        h2d(c.d_haspsd.p, has.data(), c.nbtot, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+    }
+    c.configured = true;
+    c.pointed = false;
+    c.solved = false;
+}
+
+// ------------------------------------------------------------------ noise filter on the device
+
+static void build_filters(Context &c)
+{
+    hipStream_t s = c.stream;
+    const Params &p = c.p;
+    int64_t nh = c.nof / 2 + 1;
+    int64_t n_in = (int64_t)c.psdfreqs.size();
+    c.d_fcov.alloc((size_t)nh * c.npsdtot);
+    c.d_tab.alloc(4 * n_in);
+    DBuf<int> bad;
+    bad.alloc(1);
+    MADAM_CHECK(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    int64_t startbin = 0;
+    while (startbin < n_in && c.psdfreqs[startbin] == 0) startbin++;
+    std::vector<double> tab(4 * n_in);
+    int ip = 0;
+    for (int d = 0; d < c.ndet; ++d) {
+        Detector &D = c.det[d];
+        for (int j = 0; j < D.npsd; ++j, ++ip) {
+            const double *psd = D.psds.data() + (size_t)j * n_in;
+            double sigma = D.sigmas[j];
+            double plateau;
+            if (p.noise_weights_from_psd) plateau = D.plateaus[j];
+            else {
+                if (p.radiometers) {
+                    double fr = p.fsample / 2;
+                    interpolate_psd(c.psdfreqs, psd, &fr, 1, &plateau);
+                } else {
+                    std::vector<double> fr(1000), dd(1000);
+                    for (int i = 0; i < 1000; ++i) fr[i] = 1 + (double)(i * 10) / 1000;
+                    interpolate_psd(c.psdfreqs, psd, fr.data(), 1000, dd.data());
+                    plateau = *std::min_element(dd.begin(), dd.end());
+                }
+                D.sigmas[j] = std::sqrt(plateau * p.fsample);
+            }
+            for (int64_t i = 0; i < n_in; ++i) {
+                tab[i] = c.psdfreqs[i];
+                tab[n_in + i] = psd[i];
+                tab[2 * n_in + i] = std::log(c.psdfreqs[i]);
+                tab[3 * n_in + i] = std::log(psd[i]);
+            }
+            h2d(c.d_tab.p, tab.data(), tab.size(), s);
+            launch_build_filter(s, c.nof, p.fsample, p.nshort, p.radiometers, sigma, plateau, n_in, startbin, c.d_tab.p,
+                                c.d_fcov.p + (size_t)ip * nh, bad.p);
+            MADAM_CHECK(hipGetLastError());
+            MADAM_CHECK(hipStreamSynchronize(s));   // tab is reused
+        }
+    }
+    int hbad = 0;
+    d2h(&hbad, bad.p, 1, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    if (hbad) fail("non-positive PSD values are not supported by the noise filter");
+    if (!c.fft_ready || c.fft.N != c.nof / 2) {
+        if (c.fft_ready) fft_plan_free(c.fft);
+        fft_plan_init(c.fft, c.nof);
+        c.fft_ready = true;
+    }
+}
+
+// ------------------------------------------------------------------ pointing
+
+static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
+{
+    if (!c.configured) fail("madam_set_pointing called before configure");
+    hipStream_t s = c.stream;
+    double t0 = now();
+    int64_t n = (int64_t)c.ndet * c.nsamp;
+    if (on_device) {
+        c.d_pix = pix;
+        c.d_w = w;
+    } else {
+        c.d_pix_own.alloc(n);
+        c.d_w_own.alloc((size_t)n * c.nnz);
+        h2d(c.d_pix_own.p, pix, n, s);
+        h2d(c.d_w_own.p, w, (size_t)n * c.nnz, s);
+        c.d_pix = c.d_pix_own.p;
+        c.d_w = c.d_w_own.p;
+    }
+    // pixel range check (src/smadam.F90:158-171)
+    if (n > 0) {
+        DBuf<int> mm;
+        mm.alloc(2);
+        int init[2] = {INT32_MAX, INT32_MIN};
+        h2d(mm.p, init, 2, s);
+        launch_minmax(s, c.d_pix, n, mm.p, mm.p + 1);
+        int hm[2];
+        d2h(hm, mm.p, 2, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        if (hm[0] < -1) fail("Too small pixel number %d", hm[0]);
+        if ((int64_t)hm[1] > 12LL * c.nside_max * c.nside_max - 1) fail("Too large pixel number %d", hm[1]);
+    }
+    // check_files flagging (src/read_data.f90:241-272)
+    DBuf<unsigned long long> cnt;
+    cnt.alloc((size_t)c.ndet * c.nival + 1);
+    MADAM_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * ((size_t)c.ndet * c.nival + 1), s));
+    launch_flag_and_count(s, c.d_pix, c.d_w, c.nnz, c.ndet, c.nsamp, c.d_ivs.p, c.nival, cnt.p,
+                          cnt.p + (size_t)c.ndet * c.nival);
+    std::vector<unsigned long long> valid((size_t)c.ndet * c.nival + 1);
+    d2h(valid.data(), cnt.p, valid.size(), s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    std::vector<int8_t> flag((size_t)c.ndet * c.nival, 0);
+    bool anyflag = false;
+    for (int d = 0; d < c.ndet; ++d)
+        for (int i = 0; i < c.nival; ++i) {
+            unsigned long long ng = valid[(size_t)d * c.nival + i];
+            int64_t len = c.ivs[i + 1] - c.ivs[i];
+            if (ng == 0 || len <= 0) continue;
+            double frac = (double)ng / len;
+            if (frac < F32(1e-3)) { flag[(size_t)d * c.nival + i] = 1; anyflag = true; }
+        }
+    if (anyflag) {
+        DBuf<int8_t> df;
+        df.alloc(flag.size());
+        h2d(df.p, flag.data(), flag.size(), s);
+        launch_flag_intervals(s, c.d_pix, c.nsamp, c.d_ivs.p, c.nival, c.ndet, df.p);
+        MADAM_CHECK(hipStreamSynchronize(s));
+    }
+    c.pix_host_dirty = true;
+    // pack and build the CSR by map pixel
+    c.d_pixw.alloc(n);
+    launch_pack(s, c.d_pix, c.d_w, c.nnz, c.nmap, c.ndet, c.nsamp, c.d_ivs.p, c.nival, c.d_dw_ival.p, c.p.nsubchunk,
+                c.pass, c.d_pixw.p);
+    c.d_hits.alloc(c.npix_map);
+    MADAM_CHECK(hipMemsetAsync(c.d_hits.p, 0, sizeof(int32_t) * c.npix_map, s));
+    launch_hist(s, c.d_pixw.p, (uint64_t)n, c.d_hits.p);
+    c.d_off.alloc(c.npix_map + 1);
+    int64_t nchunks = (c.npix_map + 1 + 4095) / 4096;
+    c.d_scan_tmp.alloc(nchunks);
+    launch_exclusive_scan_i32_to_i64(s, c.d_hits.p, c.npix_map, c.d_off.p, c.d_scan_tmp.p, nchunks);
+    int64_t total = 0;
+    d2h(&total, c.d_off.p + c.npix_map, 1, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    c.nactive = total;
+    c.d_cursor.alloc(c.npix_map + 1);
+    MADAM_CHECK(hipMemcpyAsync(c.d_cursor.p, c.d_off.p, sizeof(int64_t) * (c.npix_map + 1), hipMemcpyDeviceToDevice, s));
+    c.d_ent.alloc(total > 0 ? total : 1);
+    c.d_src.alloc(total > 0 ? total : 1);
+    launch_place(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.d_cursor.p, c.d_ent.p, c.d_src.p);
+    MADAM_CHECK(hipGetLastError());
+    MADAM_CHECK(hipStreamSynchronize(s));
+    c.stats[ST_T_POINTING] = now() - t0;
+    c.stats[ST_NACTIVE] = (double)total;
+    c.pointed = true;
+}
+
+// ------------------------------------------------------------------ solve
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer()
+    {
+        MADAM_CHECK(hipEventCreate(&a));
+        MADAM_CHECK(hipEventCreate(&b));
+    }
+    ~Timer()
+    {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    void start(hipStream_t s) { MADAM_CHECK(hipEventRecord(a, s)); }
+    void stop(hipStream_t s) { MADAM_CHECK(hipEventRecord(b, s)); }
+    double ms()
+    {
+        MADAM_CHECK(hipEventSynchronize(b));
+        float v = 0;
+        MADAM_CHECK(hipEventElapsedTime(&v, a, b));
+        return v;
+    }
+};
+
+static int group_for(const Context &c)
+{
+    int64_t L = (int64_t)c.p.dnshort;
+    int g = 4;
+    while (g < 64 && g < L) g *= 2;
+    return g;
+}
+
+static void apply_prior(Context &c, const double *x, double *y)
+{
+    // cinvmul: y = x outside segments, convolution inside (src/noise_routines.f90:745-913);
+    // intervals without a PSD are left as y = x (convolve_interval returns early)
+    launch_copy(c.stream, y, x, c.nbtot);
+    if (c.segconv.empty()) return;
+    fft_convolve_segments(c.stream, c.fft, c.d_segconv.p, (int64_t)c.segconv.size(), x, y, c.d_fcov.p, c.nof / 2 + 1,
+                          c.d_fftwork.p, c.fft_batch, c.segconv.data());
+}
+
+static void apply_precond(Context &c, const double *r, double *z)
+{
+    hipStream_t s = c.stream;
+    switch (c.precond_kind) {
+    case 0: launch_copy(s, z, r, c.nbtot); break;
+    case 1: launch_diag_precond(s, r, c.d_prec.p, z, c.nbtot); break;
+    case 2:
+        launch_copy(s, z, r, c.nbtot);
+        if (!c.segconv.empty())
+            fft_convolve_segments(s, c.fft, c.d_segconv.p, (int64_t)c.segconv.size(), r, z, c.d_fprec.p,
+                                  c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, c.segconv.data());
+        break;
+    default:
+        launch_copy(s, z, r, c.nbtot);
+        band_apply(s, c.band, r, z);
+        break;
+    }
+}
+
+static void build_segments(Context &c)
+{
+    hipStream_t s = c.stream;
+    int64_t cap = (int64_t)c.ndet * c.nival * 4 + 1024;
+    for (;;) {
+        c.d_seg.alloc(cap);
+        c.d_cnt.alloc(1);
+        MADAM_CHECK(hipMemsetAsync(c.d_cnt.p, 0, sizeof(unsigned long long), s));
+        int64_t gapmin = (int64_t)(60 * c.p.fsample / c.p.dnshort + 1);
+        launch_find_segments(s, c.d_nna.p, c.ndet, c.nbase, c.d_kstart_pp.p, c.d_noba_pp.p, c.nival, gapmin, c.d_seg.p,
+                             c.d_cnt.p, cap);
+        unsigned long long ns = 0;
+        d2h(&ns, c.d_cnt.p, 1, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        if ((int64_t)ns > cap) { cap = (int64_t)ns; continue; }
+        c.seg.resize(ns);
+        d2h(c.seg.data(), c.d_seg.p, ns, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        break;
+    }
+    std::sort(c.seg.begin(), c.seg.end(), [](const Segment &a, const Segment &b) { return a.b0 < b.b0; });
+    for (auto &g : c.seg) {
+        size_t di = (size_t)g.det * c.nival + g.ival;
+        int64_t k0 = g.b0 - (int64_t)g.det * c.nbase;
+        // convolve_interval looks the PSD up at the interval's first baseline
+        g.ipsd = c.p.kfilter ? c.ipsd_glob_ival[di] : -1;
+        int ipd = psd_index_det(c.det[g.det], c.btime[k0]);
+        g.skip = (ipd < 0 || c.det[g.det].weights[ipd] == 0) ? 1 : 0;
+        if (c.p.kfilter && psd_index(c, g.det, c.btime[k0]) < 0) g.skip = 1;
+    }
+    if (!c.seg.empty()) h2d(c.d_seg.p, c.seg.data(), c.seg.size(), s);
+    c.segconv.clear();
+    for (auto &g : c.seg)
+        if (g.ipsd >= 0) c.segconv.push_back(g);
+    if (!c.segconv.empty()) {
+        c.d_segconv.alloc(c.segconv.size());
+        h2d(c.d_segconv.p, c.segconv.data(), c.segconv.size(), s);
+    }
+    c.d_rmask.alloc(c.nbtot);
+    MADAM_CHECK(hipMemsetAsync(c.d_rmask.p, 0, c.nbtot, s));
+    launch_set_segments_mask(s, c.d_seg.p, (int64_t)c.seg.size(), c.d_rmask.p);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    c.stats[ST_NSEG] = (double)c.seg.size();
+}
+
+static void build_preconditioner(Context &c)
+{
+    hipStream_t s = c.stream;
+    const Params &p = c.p;
+    if (c.band_ready) { band_free(c.band); c.band_ready = false; }
+    if (p.precond_width_max < 1) { c.precond_kind = 0; return; }
+    if (!p.kfilter || p.precond_width_max == 1) {
+        c.precond_kind = 1;
+        c.d_prec.alloc(c.nbtot);
+        launch_prec_diag_build(s, c.d_nna.p, c.d_wbinv.p, c.d_haspsd.p, c.d_prec.p, c.nbtot);
+        return;
+    }
+    int64_t nh = c.nof / 2 + 1;
+    c.d_fprec.alloc((size_t)nh * c.npsdtot);
+    {
+        DBuf<double> wp;
+        wp.alloc(c.npsdtot);
+        h2d(wp.p, c.psd_w.data(), c.npsdtot, s);
+        launch_fprecond(s, c.d_fcov.p, nh, wp.p, p.nshort, c.d_fprec.p, c.npsdtot);
+        MADAM_CHECK(hipStreamSynchronize(s));
+    }
+    if (p.use_fprecond) { c.precond_kind = 2; return; }
+    if (p.use_cgprecond) fail("use_cgprecond (inner-CG preconditioner) is not supported on the GPU path");
+    c.precond_kind = 3;
+    // invcov = c2r(fcov) / nof per PSD
+    c.d_invcov.alloc((size_t)c.nof * c.npsdtot);
+    fft_c2r_rows(s, c.fft, c.d_fcov.p, c.npsdtot, c.d_invcov.p, nullptr);
+    std::vector<double> ic((size_t)c.nof * c.npsdtot);
+    d2h(ic.data(), c.d_invcov.p, ic.size(), s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    // band width per PSD and segment (src/noise_routines.f90:1071-1090)
+    const double plim = F32(1 - 1e-5f);
+    std::vector<int> nband(c.seg.size(), 0);
+    std::vector<double> ptot(c.npsdtot, 0.0);
+    for (int ip = 0; ip < c.npsdtot; ++ip)
+        for (int64_t i = 0; i < c.nof / 2; ++i) ptot[ip] += ic[(size_t)ip * c.nof + i] * ic[(size_t)ip * c.nof + i];
+    std::vector<int> tries(c.seg.size(), 1);
+    for (size_t is = 0; is < c.seg.size(); ++is) {
+        const Segment &g = c.seg[is];
+        if (g.skip || g.ipsd < 0) continue;
+        const double *v = ic.data() + (size_t)g.ipsd * c.nof;
+        int tr = 1, nb = 0, ok = 0;
+        for (;;) {
+            nb = (int)std::min<int64_t>(g.noba, (int64_t)tr * p.precond_width_min);
+            nb = std::min(nb, p.precond_width_max);
+            double pp = 0;
+            for (int i = 0; i < nb; ++i) pp += v[i] * v[i];
+            if (pp > plim * ptot[g.ipsd]) { ok = 1; break; }
+            if (nb == g.noba || nb == p.precond_width_max) break;
+            if (tr == 1000) break;
+            tr++;
+        }
+        if (!ok) fail("band preconditioner width criterion not met (reference falls back to an inner CG, not supported on the GPU path)");
+        nband[is] = nb;
+        tries[is] = tr;
+    }
+    for (;;) {
+        std::vector<int> info(c.seg.size(), 0);
+        if (band_build(s, c.band, c.seg.data(), c.d_seg.p, (int64_t)c.seg.size(), nband.data(), c.d_invcov.p, c.nof,
+                       c.d_nna.p, info.data()) != 0)
+            fail("band preconditioner wider than 50 baselines is not supported on the GPU path");
+        c.band_ready = true;
+        bool retry = false;
+        for (size_t is = 0; is < c.seg.size(); ++is) {
+            if (info[is] == 0 || nband[is] == 0) continue;
+            const Segment &g = c.seg[is];
+            if (tries[is] == 1000 || nband[is] == g.noba || nband[is] == p.precond_width_max)
+                fail("band Cholesky failed (reference falls back to an inner CG, not supported on the GPU path)");
+            tries[is]++;
+            nband[is] = std::min(nband[is] + p.precond_width_min, p.precond_width_max);
+            retry = true;
+        }
+        if (!retry) break;
+        band_free(c.band);
+        c.band_ready = false;
+    }
+    int nbmax = 0;
+    for (int v : nband) nbmax = std::max(nbmax, v);
+    c.stats[ST_NBAND] = nbmax;
+}
+
+static void solve(Context &c, double *signal, int on_device)
+{
+    if (!c.pointed) fail("madam_solve called before the pointing was set");
+    hipStream_t s = c.stream;
+    const Params &p = c.p;
+    double tstart = now();
+    int64_t n = (int64_t)c.ndet * c.nsamp;
+    if (on_device) c.d_tod = signal;
+    else {
+        c.d_tod_own.alloc(n);
+        h2d(c.d_tod_own.p, signal, n, s);
+        c.d_tod = c.d_tod_own.p;
+    }
+    Timer tm;
+    // binning
+    tm.start(s);
+    c.d_cc6.alloc(c.npix_map * 6);
+    c.d_rhs4.alloc(c.npix_map * 4);
+    launch_bin(s, c.nmap, c.npix_map, c.d_off.p, c.d_ent.p, c.d_src.p, c.d_tod, c.d_dwb.p, c.d_sdwb.p, c.d_cc6.p,
+               c.d_rhs4.p);
+    double *cc6x = c.d_cc6.p, *rhs4x = c.d_rhs4.p;
+    if (c.ndeg > 1) {
+        c.d_cc6x.alloc(c.npix_cross * 6);
+        c.d_rhs4x.alloc(c.npix_cross * 4);
+        launch_degrade(s, c.nmap, c.npix_cross, c.ndeg, c.d_cc6.p, c.d_rhs4.p, c.d_cc6x.p, c.d_rhs4x.p);
+        cc6x = c.d_cc6x.p;
+        rhs4x = c.d_rhs4x.p;
+    }
+    tm.stop(s);
+    c.stats[ST_T_BIN] = tm.ms() * 1e-3;
+    c.d_aa.alloc(c.nbtot);
+    launch_fill(s, c.d_aa.p, 0.0, c.nbtot);
+    c.stats[ST_NITER] = 0;
+    c.rr_hist.clear();
+    if (p.kfirst) {
+        tm.start(s);
+        c.d_ccinv6x.alloc(c.npix_cross * 6);
+        launch_invert(s, c.nmap, c.npix_cross, cc6x, c.d_ccinv6x.p, nullptr, nullptr, p.pixmode_cross, p.pixlim_cross,
+                      p.incomplete_matrices, p.allow_decoupling, nullptr);
+        c.d_m4x.alloc(c.npix_cross * 4);
+        launch_ccmultiply(s, c.nmap, c.npix_cross, c.d_ccinv6x.p, rhs4x, c.d_m4x.p);
+        c.d_yba.alloc(c.nbtot);
+        c.d_nna.alloc(c.nbtot);
+        int grp = group_for(c);
+        launch_init_a(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_tod, c.d_m4x.p, c.ndeg,
+                      c.d_dwb.p, c.d_yba.p, c.d_nna.p, grp);
+        MADAM_CHECK(hipGetLastError());
+        build_segments(c);
+        tm.stop(s);
+        c.stats[ST_T_INIT] = tm.ms() * 1e-3;
+        double tp = now();
+        if (p.kfilter) {
+            build_filters(c);
+            c.fft_batch = std::min<int64_t>(std::max<int64_t>((int64_t)c.segconv.size(), 1), 64);
+            c.d_fftwork.alloc((size_t)c.fft_batch * c.nof + c.fft_batch);
+        }
+        build_preconditioner(c);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        c.stats[ST_T_PRECOND] = now() - tp;
+        c.stats[ST_PRECOND_KIND] = c.precond_kind;
+        c.stats[ST_NOF] = (double)c.nof;
+        // PCG (src/smadam_routines.f90:677-1483)
+        double ti = now();
+        int64_t nb = c.nbtot;
+        for (auto *b : {&c.d_r, &c.d_ro, &c.d_z, &c.d_p, &c.d_pw, &c.d_ap}) b->alloc(nb);
+        c.d_scal.alloc(8);
+        int nblk = dots_nblocks(nb);
+        c.d_part.alloc((size_t)nblk * 3);
+        launch_copy(s, c.d_r.p, c.d_yba.p, nb);
+        apply_precond(c, c.d_r.p, c.d_z.p);
+        launch_copy(s, c.d_p.p, c.d_z.p, nb);
+        launch_mul(s, c.d_p.p, c.d_dwb.p, c.d_pw.p, nb);
+        launch_dots(s, 1, c.d_r.p, c.d_z.p, c.d_r.p, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
+        launch_finalize(s, c.d_part.p, nblk, 3, c.d_scal.p + 2);
+        double hs[8];
+        d2h(hs, c.d_scal.p, 8, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        double rz = hs[2], rr = hs[4];
+        double rzinit = rz, rrinit = rr;
+        c.stats[ST_RZINIT] = rzinit;
+        c.stats[ST_RRINIT] = rrinit;
+        h2d(c.d_scal.p, &rz, 1, s);   // scal[0] = rz
+        int istep = 0;
+        Timer t_sc, t_ga, t_pr, t_pc;
+        double ms_sc = 0, ms_ga = 0, ms_pr = 0, ms_pc = 0;
+        int prof = c.profile;
+        if (!(rzinit < F32(1.e-30)) && !std::isnan(rz)) {
+            for (;;) {
+                if (istep >= p.iter_max) break;
+                istep++;
+                if (prof) t_sc.start(s);
+                launch_scatter_pixels(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
+                                      c.d_m4x.p);
+                if (prof) { t_sc.stop(s); t_pr.start(s); }
+                if (p.kfilter) apply_prior(c, c.d_p.p, c.d_ap.p);
+                else if (p.diagfilter != 0) launch_axpby_nna(s, c.d_ap.p, c.d_nna.p, c.d_p.p, p.diagfilter, nb);
+                else launch_fill(s, c.d_ap.p, 0.0, nb);
+                if (prof) { t_pr.stop(s); t_ga.start(s); }
+                launch_gather_baselines(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
+                                        c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p, grp);
+                if (prof) t_ga.stop(s);
+                launch_dots(s, 0, c.d_p.p, c.d_ap.p, nullptr, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
+                launch_finalize(s, c.d_part.p, nblk, 1, c.d_scal.p + 1);
+                launch_pcg_update_r(s, c.d_scal.p, c.d_aa.p, c.d_r.p, c.d_ro.p, c.d_p.p, c.d_ap.p, nb);
+                if (prof) t_pc.start(s);
+                apply_precond(c, c.d_r.p, c.d_z.p);
+                if (prof) t_pc.stop(s);
+                launch_dots(s, 1, c.d_r.p, c.d_z.p, c.d_ro.p, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
+                launch_finalize(s, c.d_part.p, nblk, 3, c.d_scal.p + 2);
+                launch_scalars(s, c.d_scal.p);
+                d2h(hs, c.d_scal.p, 8, s);
+                MADAM_CHECK(hipStreamSynchronize(s));
+                if (prof) {
+                    ms_sc += t_sc.ms();
+                    ms_ga += t_ga.ms();
+                    ms_pr += t_pr.ms();
+                    ms_pc += t_pc.ms();
+                }
+                rz = hs[0];
+                rr = hs[4];
+                c.rr_hist.push_back(rr / rrinit);
+                if (rr / rrinit > 1e3) { c.stats[ST_NITER] = istep; fail("CG is diverging"); }
+                if (rr / rrinit < p.cglimit && istep > p.iter_min) break;
+                if (rz == 0) break;
+                launch_pcg_update_p(s, c.d_scal.p, c.d_p.p, c.d_z.p, c.d_pw.p, c.d_dwb.p, nb);
+            }
+        }
+        MADAM_CHECK(hipStreamSynchronize(s));
+        c.stats[ST_NITER] = istep;
+        c.stats[ST_T_ITER] = now() - ti;
+        c.stats[ST_T_SCATTER] = ms_sc * 1e-3;
+        c.stats[ST_T_GATHER] = ms_ga * 1e-3;
+        c.stats[ST_T_PRIOR] = ms_pr * 1e-3;
+        c.stats[ST_T_APPLY_PREC] = ms_pc * 1e-3;
+        c.stats[ST_N_SCATTER] = prof ? istep : 0;
+        c.stats[ST_N_GATHER] = prof ? istep : 0;
+        c.stats[ST_RR_FINAL] = c.rr_hist.empty() ? 0 : c.rr_hist.back();
+    }
+    // final maps
+    tm.start(s);
+    c.d_rhsd4.alloc(c.npix_map * 4);
+    MADAM_CHECK(hipMemcpyAsync(c.d_rhsd4.p, c.d_rhs4.p, sizeof(double) * c.npix_map * 4, hipMemcpyDeviceToDevice, s));
+    if (p.kfirst)
+        launch_subtract_baselines(s, c.nmap, c.npix_map, c.d_off.p, c.d_ent.p, c.d_aa.p, c.d_dwb.p, c.d_rhsd4.p);
+    c.d_ccinv6.alloc(c.npix_map * 6);
+    c.d_mask.alloc(c.npix_map);
+    c.d_crit.alloc(c.npix_map);
+    MADAM_CHECK(hipMemsetAsync(c.d_crit.p, 0, sizeof(double) * c.npix_map, s));
+    launch_invert(s, c.nmap, c.npix_map, c.d_cc6.p, c.d_ccinv6.p, c.d_crit.p, c.d_mask.p, p.pixmode_map, p.pixlim_map,
+                  p.incomplete_matrices, p.allow_decoupling, nullptr);
+    c.d_map4.alloc(c.npix_map * 4);
+    c.d_binmap4.alloc(c.npix_map * 4);
+    launch_ccmultiply(s, c.nmap, c.npix_map, c.d_ccinv6.p, c.d_rhs4.p, c.d_binmap4.p);
+    if (p.kfirst) launch_ccmultiply(s, c.nmap, c.npix_map, c.d_ccinv6.p, c.d_rhsd4.p, c.d_map4.p);
+    else MADAM_CHECK(hipMemsetAsync(c.d_map4.p, 0, sizeof(double) * c.npix_map * 4, s));
+    // clean_tod (src/smadam_routines.f90:1545-1585), final full-data pass only
+    bool clean = c.pass == 0 && p.kfirst && (p.write_tod || p.ndetset > 0 || p.nsurvey > 0);
+    if (clean) {
+        launch_clean_tod(s, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_aa.p, c.d_tod);
+        if (!on_device) d2h(signal, c.d_tod, n, s);
+    }
+    tm.stop(s);
+    MADAM_CHECK(hipGetLastError());
+    c.stats[ST_T_FINAL] = tm.ms() * 1e-3;
+    MADAM_CHECK(hipStreamSynchronize(s));
+    c.stats[ST_T_TOTAL] = now() - tstart;
+    c.solved = true;
+}
+
+// restore_pixels for host arrays: flagged samples -> 12 * nside_max**2
+static void write_back_pixels(Context &c, int32_t *host_pix)
+{
+    if (!c.pix_host_dirty) return;
+    int64_t n = (int64_t)c.ndet * c.nsamp;
+    launch_restore_pixels(c.stream, c.d_pix, n, (int32_t)(12LL * c.nside_max * c.nside_max));
+    d2h(host_pix, c.d_pix, n, c.stream);
+    MADAM_CHECK(hipStreamSynchronize(c.stream));
+    c.pix_host_dirty = false;
+}
+
+static void get_maps(Context &c, double *map, double *binmap, int32_t *hits, double *cc, int32_t *mask, double *crit)
+{
+    if (!c.solved) fail("no solution available");
+    hipStream_t s = c.stream;
+    int64_t np = c.npix_map;
+    int nm = c.nmap;
+    std::vector<double> m4(np * 4), b4(np * 4), ci(np * 6);
+    std::vector<int32_t> mk(np);
+    d2h(m4.data(), c.d_map4.p, np * 4, s);
+    d2h(b4.data(), c.d_binmap4.p, np * 4, s);
+    d2h(ci.data(), c.d_ccinv6.p, np * 6, s);
+    d2h(mk.data(), c.d_mask.p, np, s);
+    if (hits) d2h(hits, c.d_hits.p, np, s);
+    if (crit) d2h(crit, c.d_crit.p, np, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    if (c.p.rm_monopole) {
+        // makemaps monopole removal (src/map_routines.f90:341-353)
+        int64_t cover = 0;
+        for (int64_t q = 0; q < np; ++q) cover += mk[q] > 0;
+        if (cover > 0) {
+            for (auto *v : {&m4, &b4}) {
+                double mono = 0;
+                for (int64_t q = 0; q < np; ++q) mono += (*v)[q * 4];
+                mono = mono / cover;
+                for (int64_t q = 0; q < np; ++q) if (mk[q] > 0) (*v)[q * 4] -= mono;
+            }
+        }
+    }
+    for (int64_t q = 0; q < np; ++q)
+        for (int a = 0; a < nm; ++a) {
+            if (map) map[q * nm + a] = m4[q * 4 + a];
+            if (binmap) binmap[q * nm + a] = b4[q * 4 + a];
+        }
+    if (cc) {
+        for (int64_t q = 0; q < np; ++q) {
+            const double *u = ci.data() + q * 6;
+            if (nm == 1) cc[q] = u[0];
+            else {
+                double full[9] = {u[0], u[1], u[2], u[1], u[3], u[4], u[2], u[4], u[5]};
+                for (int a = 0; a < 9; ++a) cc[q * 9 + a] = full[a];
+            }
+        }
+    }
+    if (mask) std::copy(mk.begin(), mk.end(), mask);
+}
+
+static Context *g_cache = nullptr;   // destripe(mcmode=T) keeps its context here
+
+}  // namespace madam
+
+using namespace madam;
+
+// ============================================================================ C ABI
+
+#define MADAM_API extern "C" __attribute__((visibility("default")))
+
+template <typename F>
+static int guarded(F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const madam::Error &e) {
+        madam::g_err = e.msg;
+    } catch (const madam::HipError &e) {
+        char buf[512];
+        snprintf(buf, sizeof buf, "HIP error %d (%s) at %s:%d: %s", (int)e.err, hipGetErrorString(e.err), e.file, e.line,
+                 e.what);
+        madam::g_err = buf;
+    } catch (const std::exception &e) {
+        madam::g_err = e.what();
+    }
+    return 1;
+}
+
+static int g_last_status = 0;
+
+// The reference aborts the MPI job on any error (src/mpi_wrappers.f90:120-134).
+// The same happens here unless MADAM_RAISE is set in the environment, in which
+// case the error is kept for madam_last_status() / madam_last_error().
+static void abort_like_reference(const char *where)
+{
+    g_last_status = 1;
+    fprintf(stderr, "libmadam: %s: %s\n", where, madam::g_err.c_str());
+    fflush(stderr);
+    if (getenv("MADAM_RAISE")) return;
+    exit(1);
+}
+
+MADAM_API int madam_last_status(void) { return g_last_status; }
+
+MADAM_API const char *madam_last_error(void) { return madam::g_err.c_str(); }
+
+MADAM_API void *madam_create(int device)
+{
+    Context *c = nullptr;
+    int rc = guarded([&] {
+        c = new Context();
+        c->device = device;
+        MADAM_CHECK(hipSetDevice(device));
+        ensure_stream(*c);
+    });
+    if (rc) { delete c; return nullptr; }
+    return c;
+}
+
+MADAM_API void madam_destroy(void *ctx) { delete (Context *)ctx; }
+
+MADAM_API int madam_set_profiling(void *ctx, int on)
+{
+    return guarded([&] { ((Context *)ctx)->profile = on; });
+}
+
+MADAM_API int madam_configure(void *ctx, const char *parstring, long ndet, const char *detstring,
+                              const double *detweights, long nsamp, long nnz, const double *timestamps, long nperiod,
+                              const long *periods, const long *npsd, long npsdtot, const double *psdstarts,
+                              long npsdbin, const double *psdfreqs, long npsdval, const double *psdvals,
+                              int subchunk_pass)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        MADAM_CHECK(hipSetDevice(c.device));
+        configure(c, parstring, ndet, detstring, detweights, nsamp, nnz, timestamps, nperiod, periods, npsd, npsdtot,
+                  psdstarts, npsdbin, psdfreqs, npsdval, psdvals, subchunk_pass);
+    });
+}
+
+MADAM_API int madam_set_pointing(void *ctx, int32_t *pix, float *pixweights, int on_device)
+{
+    return guarded([&] { set_pointing(*(Context *)ctx, pix, pixweights, on_device); });
+}
+
+MADAM_API int madam_solve(void *ctx, double *signal, int on_device)
+{
+    return guarded([&] { solve(*(Context *)ctx, signal, on_device); });
+}
+
+MADAM_API int madam_restore_pixels(void *ctx, int32_t *host_pix)
+{
+    return guarded([&] { write_back_pixels(*(Context *)ctx, host_pix); });
+}
+
+MADAM_API int madam_get_sizes(void *ctx, long *out6)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        out6[0] = c.nmap;
+        out6[1] = (long)c.npix_map;
+        out6[2] = (long)c.nbase;
+        out6[3] = c.ndet;
+        out6[4] = (long)c.stats[ST_NITER];
+        out6[5] = c.p.nside_map;
+    });
+}
+
+MADAM_API int madam_get_maps(void *ctx, double *map, double *binmap, int32_t *hits, double *cc, int32_t *mask,
+                             double *crit)
+{
+    return guarded([&] { get_maps(*(Context *)ctx, map, binmap, hits, cc, mask, crit); });
+}
+
+MADAM_API int madam_get_baselines(void *ctx, double *aa, double *yba, double *nna)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        if (!c.solved) fail("no solution available");
+        int64_t nb = c.nbtot;
+        if (aa) d2h(aa, c.d_aa.p, nb, c.stream);
+        if (c.p.kfirst) {
+            if (yba) d2h(yba, c.d_yba.p, nb, c.stream);
+            if (nna) d2h(nna, c.d_nna.p, nb, c.stream);
+        } else {
+            if (yba) std::fill(yba, yba + nb, 0.0);
+            if (nna) std::fill(nna, nna + nb, 0.0);
+        }
+        MADAM_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+MADAM_API int madam_get_stats(void *ctx, double *out, int n)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        for (int i = 0; i < n && i < ST_COUNT; ++i) out[i] = c.stats[i];
+    });
+}
+
+MADAM_API int madam_get_rr_history(void *ctx, double *out, int n)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        for (int i = 0; i < n && i < (int)c.rr_hist.size(); ++i) out[i] = c.rr_hist[i];
+    });
+}
+
+MADAM_API int madam_expand_pointing(const double *scan5, const double *dets3, int ndet, long nside, long i0, long n,
+                                    int32_t *pix, float *weights, int on_device)
+{
+    return guarded([&] {
+        ScanParams sp{scan5[0], scan5[1], scan5[2], scan5[3], scan5[4]};
+        hipStream_t s = nullptr;
+        MADAM_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        DBuf<ScanDet> dd;
+        dd.alloc(ndet);
+        h2d(dd.p, (const ScanDet *)dets3, ndet, s);
+        int32_t *dp = pix;
+        float *dw = weights;
+        DBuf<int32_t> op;
+        DBuf<float> ow;
+        if (!on_device) {
+            op.alloc((size_t)ndet * n);
+            ow.alloc((size_t)ndet * n * 3);
+            dp = op.p;
+            dw = ow.p;
+        }
+        launch_expand_pointing(s, sp, dd.p, ndet, nside, i0, n, n, dp, dw);
+        MADAM_CHECK(hipGetLastError());
+        if (!on_device) {
+            d2h(pix, dp, (size_t)ndet * n, s);
+            d2h(weights, dw, (size_t)ndet * n * 3, s);
+        }
+        MADAM_CHECK(hipStreamSynchronize(s));
+        (void)hipStreamDestroy(s);
+    });
+}
+
+// ---------------------------------------------------------------- reference ABI
+
+MADAM_API void destripe(int comm, char *parstring, long ndet, char *detstring, double *detweights, long nsamp,
+                        long nnz, void *timestamps, void *pix, void *pixweights, void *signal, long nperiod,
+                        long *periods, long *npsd, long npsdtot, double *psdstarts, long npsdbin, double *psdfreqs,
+                        long npsdval, double *psdvals)
+{
+    (void)comm;   // single rank: the communicator is recorded, not used
+    g_last_status = 0;
+    if (g_cache) {
+        g_err = "Destripe called while caches are not empty.";
+        abort_like_reference("destripe");
+        return;
+    }
+    Context *c = nullptr;
+    int rc = guarded([&] {
+        c = new Context();
+        MADAM_CHECK(hipGetDevice(&c->device));
+        configure(*c, parstring, ndet, detstring, detweights, nsamp, nnz, (const double *)timestamps, nperiod, periods,
+                  npsd, npsdtot, psdstarts, npsdbin, psdfreqs, npsdval, psdvals, 0);
+        set_pointing(*c, (int32_t *)pix, (float *)pixweights, 0);
+        solve(*c, (double *)signal, 0);
+        write_back_pixels(*c, (int32_t *)pix);
+    });
+    if (rc) {
+        delete c;
+        abort_like_reference("destripe");
+        return;
+    }
+    if (c->p.mcmode) g_cache = c;
+    else delete c;
+}
+
+MADAM_API void destripe_with_cache(int comm, long ndet, long nsamp, long nnz, void *timestamps, void *pix,
+                                   void *pixweights, void *signal, char *outpath)
+{
+    (void)comm;
+    (void)outpath;   // FITS output is not produced by this library
+    (void)timestamps;
+    g_last_status = 0;
+    int rc = guarded([&] {
+        if (!g_cache) fail("destripe_with_cache called without cached data (run destripe with mcmode=T first)");
+        Context &c = *g_cache;
+        if (ndet != c.ndet || nsamp != c.nsamp || nnz != c.nnz) fail("destripe_with_cache: data shape differs from the cached run");
+        set_pointing(c, (int32_t *)pix, (float *)pixweights, 0);
+        solve(c, (double *)signal, 0);
+        write_back_pixels(c, (int32_t *)pix);
+    });
+    if (rc) abort_like_reference("destripe_with_cache");
+}
+
+MADAM_API void clear_caches(void)
+{
+    if (!g_cache) {
+        printf("WARNING: clear_caches called while caches are already empty\n");
+        return;
+    }
+    delete g_cache;
+    g_cache = nullptr;
+}
+
+MADAM_API int madam_cache_handle(void **out)
+{
+    *out = g_cache;
+    return 0;
+}

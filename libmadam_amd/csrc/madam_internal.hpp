@@ -1,0 +1,178 @@
+// Internal interfaces between the host orchestration (madam_host.cpp) and the
+// HIP kernels (madam_kernels.hip, madam_fft.hip, madam_band.hip).
+//
+// Device data model for one scan (pointing) on one GPU:
+//   samples      detector-major, index g = det * nsamp + i (uint32_t; the
+//                library rejects ndet * nsamp >= 2^32)
+//   pixw         16-byte record per sample {pix, w0, w1, w2}; pix = -1 for
+//                every sample that takes no part in this pass (flagged, zero
+//                detector weight, no PSD period, other subchunk)
+//   csr          samples regrouped by map pixel: off[npix_map + 1], and per
+//                entry {bidx, w0, w1, w2} plus the source sample index; the
+//                cross pixel of map pixel q is q / ndeg, so a cross pixel's
+//                entries are a contiguous range of the map-pixel CSR
+//   baselines    global baseline index b = det * nbase + k; vectors of
+//                length nbtot = ndet * nbase
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace madam {
+
+struct PixW {
+    int32_t pix;
+    float w0, w1, w2;
+};
+
+struct CsrEnt {
+    int32_t b;   // global baseline index
+    float w0, w1, w2;
+};
+
+// Segment of consecutive baselines with nonzero nna inside one (det, interval),
+// as produced by the reference's trim_interval (src/noise_routines.f90:764-813).
+struct Segment {
+    int64_t b0;      // first global baseline index
+    int32_t noba;    // number of baselines
+    int32_t det;
+    int32_t ival;
+    int32_t ipsd;    // global PSD index for the filter, -1 = none
+    int32_t skip;    // preconditioner leaves z = r on this segment
+    int32_t pad;
+};
+
+#define MADAM_CHECK(x)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) throw madam::HipError(e_, #x, __FILE__, __LINE__);           \
+    } while (0)
+
+struct HipError {
+    hipError_t err;
+    const char *what;
+    const char *file;
+    int line;
+    HipError(hipError_t e, const char *w, const char *f, int l) : err(e), what(w), file(f), line(l) {}
+};
+
+// ------------------------------------------------------------------ kernels.hip
+
+void launch_flag_and_count(hipStream_t s, int32_t *pix, const float *w, int nnz, int ndet, int64_t nsamp,
+                           const int64_t *ival_start, int ninterval, unsigned long long *valid_counts,
+                           unsigned long long *nzero);
+void launch_flag_intervals(hipStream_t s, int32_t *pix, int64_t nsamp, const int64_t *ival_start,
+                           int ninterval, int ndet, const int8_t *flag_ival);
+void launch_pack(hipStream_t s, const int32_t *pix, const float *w, int nnz, int nmap, int ndet, int64_t nsamp,
+                 const int64_t *ival_start, int ninterval, const double *dw_ival, int nsubchunk, int pass,
+                 PixW *pixw);
+void launch_hist(hipStream_t s, const PixW *pixw, uint64_t n, int32_t *counts);
+void launch_exclusive_scan_i32_to_i64(hipStream_t s, const int32_t *in, int64_t n, int64_t *out, int64_t *tmp,
+                                      int64_t tmp_len);
+void launch_place(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
+                  int64_t *cursor, CsrEnt *ent, uint32_t *src);
+void launch_bin(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent, const uint32_t *src,
+                const double *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4);
+void launch_degrade(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const double *cc6, const double *rhs4,
+                    double *cc6x, double *rhs4x);
+void launch_invert(hipStream_t s, int nmap, int64_t npix, const double *cc6, double *ccinv6, double *crit,
+                   int32_t *mask, int pixmode, double pixlim, int incomplete, int allow_decoupling,
+                   unsigned long long *ngood);
+void launch_ccmultiply(hipStream_t s, int nmap, int64_t npix, const double *ccinv6, const double *rhs4, double *out4);
+void launch_init_a(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
+                   const PixW *pixw, const double *tod, const double *m4, int64_t ndeg, const double *dwb,
+                   double *yba, double *nna, int group);
+void launch_scatter_pixels(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
+                           const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4);
+void launch_gather_baselines(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase,
+                             const int64_t *bstart, const PixW *pixw, const double *m4, int64_t ndeg,
+                             const double *dwb, const double *nna, const double *p, double *ap, int group);
+void launch_subtract_baselines(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent,
+                               const double *aa, const double *dwb, double *rhs4);
+void launch_clean_tod(hipStream_t s, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
+                      const double *aa, double *tod);
+void launch_set_segments_mask(hipStream_t s, const Segment *seg, int64_t nseg, int8_t *rmask);
+void launch_find_segments(hipStream_t s, const double *nna, int ndet, int64_t nbase, const int64_t *kstart_pp,
+                          const int64_t *noba_pp, int ninterval, int64_t gapmin, Segment *out,
+                          unsigned long long *nseg, int64_t cap);
+void launch_fill(hipStream_t s, double *p, double v, int64_t n);
+void launch_copy(hipStream_t s, double *dst, const double *src, int64_t n);
+// elementwise: out = a * x
+void launch_diag_precond(hipStream_t s, const double *r, const double *pd, double *z, int64_t n);
+void launch_prec_diag_build(hipStream_t s, const double *nna, const double *wb_inv, const int8_t *haspsd,
+                            double *pd, int64_t n);
+// masked dot products; partial sums per block -> finalize
+void launch_dots(hipStream_t s, int which, const double *a, const double *b, const double *c, const double *d,
+                 const int8_t *mask, int64_t n, double *partials, int nblocks);
+int dots_nblocks(int64_t n);
+void launch_finalize(hipStream_t s, const double *partials, int nblocks, int nval, double *out);
+// PCG vector steps, scalars read from device memory
+void launch_pcg_update_r(hipStream_t s, const double *scal, double *aa, double *r, double *ro, const double *p,
+                         const double *ap, int64_t n);
+void launch_pcg_update_p(hipStream_t s, const double *scal, double *p, const double *z, double *pw,
+                         const double *dwb, int64_t n);
+void launch_mul(hipStream_t s, const double *a, const double *b, double *out, int64_t n);
+void launch_axpby_nna(hipStream_t s, double *ap, const double *nna, const double *p, double factor, int64_t n);
+
+void launch_minmax(hipStream_t s, const int32_t *pix, int64_t n, int *mn, int *mx);
+void launch_restore_pixels(hipStream_t s, int32_t *pix, int64_t n, int32_t dummy);
+void launch_scalars(hipStream_t s, double *scal);
+void launch_fprecond(hipStream_t s, const double *fcov, int64_t nh, const double *wpsd, int nshort, double *out,
+                     int64_t npsd);
+
+// ------------------------------------------------------------------ fft.hip
+
+struct FftPlan {
+    int64_t N;       // complex length (= nof / 2)
+    int64_t L1, L2;  // N = L1 * L2, column pass length L1, row pass length L2
+    double *tw_n;    // [N] complex exp(-2 pi i j / N)
+    double *tw_nof;  // [N/2 + 1] complex exp(-2 pi i k / nof)
+};
+
+void fft_plan_init(FftPlan &p, int64_t nof);
+void fft_plan_free(FftPlan &p);
+// convolve every segment: y = x outside segments is the caller's job (copy first)
+void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg_dev,
+                           const double *x, double *y, const double *fc, int64_t fc_stride, double *work,
+                           int64_t batch, const Segment *hseg);
+// inverse real FFT of fcov rows (invcov = c2r(fcov) / nof)
+void fft_c2r_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrows, double *out, double *work);
+// noise filter (build_filter) on the device
+void launch_build_filter(hipStream_t s, int64_t nof, double fsample, int nshort, int radiometers, double sigma,
+                         double plateau, int64_t n_in, int64_t startbin, const double *tab, double *fcov, int *bad);
+
+// ------------------------------------------------------------------ band.hip
+
+struct BandPrec {
+    int64_t nseg = 0;
+    int kd = 0;             // max bandwidth - 1
+    int chunk = 0;          // rows per partition
+    int64_t nchunk = 0;     // total partitions over all segments
+    double *L = nullptr;    // interleaved factor [row_in_chunk][kd+1][chunk_id]
+    double *Tf = nullptr;   // forward carry response [chunk][kd][kd]
+    double *Tb = nullptr;   // backward carry response
+    int64_t *chunk_b0 = nullptr;  // first global baseline of each partition
+    int32_t *chunk_n = nullptr;   // rows in each partition
+    int32_t *chunk_first = nullptr;  // 1 if first partition of its segment
+    int32_t *chunk_nband = nullptr;  // bandwidth of the segment
+    double *carry = nullptr;      // [nchunk][kd] carries
+    double *u = nullptr;          // scratch per baseline row
+};
+
+int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *dseg, int64_t nseg,
+               const int *nband_seg_host, const double *invcov_dev, int64_t nof, const double *nna, int *info_out);
+void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z);
+void band_free(BandPrec &bp);
+
+// ------------------------------------------------------------------ pointing.hip
+
+struct ScanParams {
+    double fsample, spin_period, prec_period, opening_turns, lon0_turns;
+};
+struct ScanDet {
+    double psi_turns, open_offset_turns, spin_offset_turns;
+};
+void launch_expand_pointing(hipStream_t s, ScanParams sp, const ScanDet *dets, int ndet, int64_t nside,
+                            int64_t i0, int64_t n, int64_t ld, int32_t *pix, float *w);
+
+}  // namespace madam

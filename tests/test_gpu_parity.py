@@ -1,0 +1,159 @@
+"""GPU parity: the HIP path (libmadam.so through its C-ABI) against the FP64
+CPU oracle on the same seeded inputs.
+
+Bars (stated here, checked below):
+  * hits and flagged pixels: bit-exact;
+  * pointing expansion (P0): bit-exact pixels and weights;
+  * FP64 binned maps, destriped maps, baselines, RHS and nna: relative rms
+    error <= 1e-10 against the oracle (FP64 accumulations in a different
+    order; PCG iterates inherit those rounding differences).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import madam_cases as mc
+
+pytestmark = pytest.mark.gpu
+
+RTOL_RMS = 1e-10
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.sqrt(np.mean(b ** 2))
+    if den == 0:
+        return float(np.sqrt(np.mean(a ** 2)))
+    return float(np.sqrt(np.mean((a - b) ** 2)) / den)
+
+
+def both(case, **kw):
+    import libmadam_amd
+    c1 = mc.copy_case(case)
+    c2 = mc.copy_case(case)
+    g = libmadam_amd.run(*mc.call_args(c1), **kw)
+    o = oracle.destripe(*mc.call_args(c2), **kw)
+    return g, o, c1, c2
+
+
+def check_parity(g, o, c1, c2, maps=True, tol=RTOL_RMS):
+    print("niter gpu/oracle", g["niter"], o["niter"], "stats", {k: g["stats"][k] for k in ("precond_kind", "nseg", "nband")})
+    assert np.array_equal(g["hits"], o["hits"])
+    assert np.array_equal(c1["pixels"], c2["pixels"])
+    assert np.array_equal(g["mask"], o["mask"])
+    errs = dict(binmap=rel_rms(g["binmap"], o["binmap"]), yba=rel_rms(g["yba"], o["yba"]),
+                nna=rel_rms(g["nna"], o["nna"]), cc=rel_rms(g["cc"], o["cc"]))
+    if maps:
+        errs.update(map=rel_rms(g["map"], o["map"]), aa=rel_rms(g["aa"], o["aa"]),
+                    signal=rel_rms(c1["signal"], c2["signal"]))
+    print("relative rms errors", errs)
+    for k, v in errs.items():
+        assert v <= tol, (k, v)
+    assert abs(g["niter"] - o["niter"]) <= 1
+
+
+SCAN_SMALL = dict(fsample=50.0, spin_period=2.0, prec_period=400.0, opening_turns=85.0 / 360.0, lon0_turns=0.0)
+
+
+def pointing_fn(nside, fs=50.0, spin=2.0, prec=400.0):
+    scan = dict(SCAN_SMALL, fsample=fs, spin_period=spin, prec_period=prec)
+
+    def f(d, n):
+        return oracle.expand_pointing(scan, dict(psi_turns=(0.0, 0.125, 0.25, 0.375)[d % 4],
+                                                 open_offset_turns=0.004 * (d // 4)), nside, 0, n)
+    return f
+
+
+def ring(nside=8, ndet=4, nsamp=20000, prior=False, nnz=3, nperiod=1, extra=None, base=1.0, seed=11, **kw):
+    return mc.ring_case(ndet=ndet, nsamp=nsamp, nside=nside, fsample=50.0, base_first=base, nnz=nnz, seed=seed,
+                        prior=prior, nperiod=nperiod, pointing=pointing_fn(nside), extra_pars=extra, **kw)
+
+
+def test_pointing_expansion_bit_exact():
+    import libmadam_amd
+    scan = dict(fsample=100.0, spin_period=60.0, prec_period=360 * 3600.0, opening_turns=85 / 360, lon0_turns=0.01)
+    dets = [dict(psi_turns=a, open_offset_turns=o, spin_offset_turns=s)
+            for a, o, s in ((0.0, 0.0, 0.0), (0.125, 0.001, 0.0003), (0.25, -0.002, 0.0), (0.375, 0.0, -0.01))]
+    for nside in (64, 512, 4096):
+        n = 200000
+        i0 = 123456789 if nside == 4096 else 0
+        gp, gw = libmadam_amd.expand_pointing(scan, dets, nside, i0, n)
+        for d, det in enumerate(dets):
+            op, ow = oracle.expand_pointing(scan, det, nside, i0, n)
+            assert np.array_equal(gp[d], op), (nside, d, np.sum(gp[d] != op))
+            assert np.array_equal(gw[d].view(np.uint32), ow.view(np.uint32)), (nside, d)
+
+
+def test_reference_known_answers_on_gpu():
+    """python/tests/test_libmadam.py:155-174 known answers, through the GPU path."""
+    import libmadam_amd
+    case, gold = mc.reference_test_case()
+    case = mc.copy_case(case)
+    case["pars"] = dict(case["pars"], kfilter=False)
+    g = libmadam_amd.run(*mc.call_args(case))
+    assert np.array_equal(g["hits"], gold["hmap"])
+    np.testing.assert_allclose(g["binmap"][:, 0], gold["bmap"], rtol=0, atol=1e-12)
+    assert np.array_equal(g["binmap"][:, 0].astype(np.float32), gold["bmap32"])
+    case1, _ = mc.reference_test_case()
+    case1["pars"] = dict(case1["pars"], kfilter=False)
+    h1 = libmadam_amd.run(*mc.call_args(mc.copy_case(case1)), subchunk_pass=1)["hits"]
+    h2 = libmadam_amd.run(*mc.call_args(mc.copy_case(case1)), subchunk_pass=2)["hits"]
+    assert np.array_equal(h1 + h2, g["hits"])
+
+
+@pytest.mark.parametrize("nnz", [1, 3])
+def test_no_prior_diagonal_preconditioner(nnz):
+    g, o, c1, c2 = both(ring(nnz=nnz))
+    assert g["stats"]["precond_kind"] == 1
+    check_parity(g, o, c1, c2)
+
+
+def test_noise_prior_band_preconditioner():
+    g, o, c1, c2 = both(ring(prior=True, nsamp=30000, nperiod=2))
+    assert g["stats"]["precond_kind"] == 3
+    check_parity(g, o, c1, c2)
+
+
+def test_noise_prior_fourier_preconditioner():
+    g, o, c1, c2 = both(ring(prior=True, nsamp=30000, nperiod=2, extra=dict(use_fprecond=True, iter_max=30)))
+    assert g["stats"]["precond_kind"] == 2
+    check_parity(g, o, c1, c2, tol=1e-8)
+
+
+def test_flags_degrade_and_write_tod():
+    case = ring(prior=True, nside=16, nsamp=40000, nperiod=3,
+                extra=dict(nside_cross=8, write_tod=True, nside_submap=4))
+    case["pixels"][500:900] = -1
+    case["pixweights"][3 * 7000:3 * 7100] = 0
+    g, o, c1, c2 = both(case)
+    assert np.sum(c1["pixels"] == 12 * 16 ** 2) >= 500
+    check_parity(g, o, c1, c2)
+
+
+def test_subchunk_pass():
+    case = ring(nsamp=24000, nperiod=2, extra=dict(nsubchunk=3))
+    for sub in (1, 3):
+        g, o, c1, c2 = both(case, subchunk_pass=sub)
+        check_parity(g, o, c1, c2)
+
+
+def test_wrapper_drop_in_mutates_like_reference():
+    import os
+    from libmadam_amd import libmadam_wrapper as madam
+    os.environ["MADAM_RAISE"] = "1"
+    case = ring(prior=True, nsamp=20000, extra=dict(write_tod=True))
+    case["pixels"][100:200] = -1
+    c1 = mc.copy_case(case)
+    c2 = mc.copy_case(case)
+    madam.destripe(None, c1["pars"], c1["dets"], c1["weights"], c1["timestamps"], c1["pixels"], c1["pixweights"],
+                   c1["signal"], c1["periods"], c1["npsd"], c1["psdstarts"], c1["psdfreqs"], c1["psdvals"])
+    oracle.destripe(*mc.call_args(c2))
+    assert np.array_equal(c1["pixels"], c2["pixels"])
+    assert rel_rms(c1["signal"], c2["signal"]) < RTOL_RMS
+    # unknown parameters are rejected
+    c3 = mc.copy_case(case)
+    with pytest.raises(Exception, match="Unknown parameter"):
+        madam.destripe(None, dict(c3["pars"], bogus=1), c3["dets"], c3["weights"], c3["timestamps"], c3["pixels"],
+                       c3["pixweights"], c3["signal"], c3["periods"], c3["npsd"], c3["psdstarts"], c3["psdfreqs"],
+                       c3["psdvals"])
