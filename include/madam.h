@@ -67,6 +67,11 @@ int madam_get_baselines(void *ctx, double *aa, double *yba, double *nna);
 int madam_get_stats(void *ctx, double *out, int n);
 int madam_get_rr_history(void *ctx, double *out, int n);
 int madam_set_profiling(void *ctx, int on);
+/* benchmark / continuation: run exactly niter PCG iterations without the exit
+ * tests (restart != 0 starts again from a = 0); products of the last
+ * madam_solve are not refreshed */
+int madam_iterate(void *ctx, int niter, int restart);
+int madam_reset_timers(void *ctx);
 /* pointing expansion of the synthetic ring scan: scan5 = {fsample, spin_period,
  * prec_period, opening_turns, lon0_turns}, dets3[ndet][3] = {psi_turns,
  * open_offset_turns, spin_offset_turns}; outputs pix[ndet][n], weights[ndet][n][3] */

@@ -24,7 +24,7 @@ PSD_TYPE = np.float64
 # order of madam_get_stats
 STAT_NAMES = ["niter", "rrinit", "rzinit", "t_pointing", "t_bin", "t_init", "t_precond", "t_iter", "t_final",
               "t_scatter", "t_gather", "t_prior", "t_apply_prec", "n_scatter", "n_gather", "precond_kind",
-              "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total"]
+              "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total", "t_step"]
 
 
 class MadamError(RuntimeError):
@@ -60,6 +60,10 @@ def lib():
         L.madam_get_stats.argtypes = [vp, vp, i32]
         L.madam_get_rr_history.restype = i32
         L.madam_get_rr_history.argtypes = [vp, vp, i32]
+        L.madam_iterate.restype = i32
+        L.madam_iterate.argtypes = [vp, i32, i32]
+        L.madam_reset_timers.restype = i32
+        L.madam_reset_timers.argtypes = [vp]
         L.madam_set_profiling.restype = i32
         L.madam_set_profiling.argtypes = [vp, i32]
         L.madam_expand_pointing.restype = i32
@@ -199,6 +203,12 @@ class Madam:
         _check(lib().madam_get_rr_history(self._h, _ptr(out), n))
         return out[:n]
 
+    def iterate(self, niter, restart=False):
+        _check(lib().madam_iterate(self._h, int(niter), int(restart)))
+
+    def reset_timers(self):
+        _check(lib().madam_reset_timers(self._h))
+
     def set_profiling(self, on=True):
         _check(lib().madam_set_profiling(self._h, int(on)))
 
@@ -237,3 +247,13 @@ def expand_pointing(scan, dets, nside, i0, n):
     w = np.zeros((len(dets), n, 3), np.float32)
     _check(lib().madam_expand_pointing(_ptr(scan5), _ptr(d3), len(dets), nside, i0, n, _ptr(pix), _ptr(w), 0))
     return pix, w
+
+
+def expand_pointing_device(scan, dets, nside, i0, n, pix_ptr, w_ptr):
+    """GPU pointing expansion written straight into device buffers
+    pix int32[ndet][n] and w float32[ndet][n][3] (device pointers or tensors)."""
+    scan5 = np.array([scan["fsample"], scan["spin_period"], scan["prec_period"], scan["opening_turns"],
+                      scan["lon0_turns"]], np.float64)
+    d3 = np.array([[d["psi_turns"], d.get("open_offset_turns", 0.0), d.get("spin_offset_turns", 0.0)]
+                   for d in dets], np.float64)
+    _check(lib().madam_expand_pointing(_ptr(scan5), _ptr(d3), len(dets), nside, i0, n, _ptr(pix_ptr), _ptr(w_ptr), 1))

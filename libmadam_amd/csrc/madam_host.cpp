@@ -277,7 +277,7 @@ struct DBuf {
 
 enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_PRECOND, ST_T_ITER, ST_T_FINAL,
        ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
-       ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_COUNT };
+       ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_COUNT };
 
 struct Context {
     int device = 0;
@@ -345,6 +345,7 @@ struct Context {
     double stats[ST_COUNT] = {0};
     std::vector<double> rr_hist;
     int profile = 0;
+    int pcg_steps_done = 0;
 
     ~Context()
     {
@@ -977,6 +978,99 @@ static void build_preconditioner(Context &c)
     c.stats[ST_NBAND] = nbmax;
 }
 
+// r = yba, z = M r, p = z (src/smadam_routines.f90:732-748)
+static void pcg_init(Context &c)
+{
+    hipStream_t s = c.stream;
+    int64_t nb = c.nbtot;
+    for (auto *b : {&c.d_r, &c.d_ro, &c.d_z, &c.d_p, &c.d_pw, &c.d_ap}) b->alloc(nb);
+    c.d_scal.alloc(8);
+    int nblk = dots_nblocks(nb);
+    c.d_part.alloc((size_t)nblk * 3);
+    launch_fill(s, c.d_aa.p, 0.0, nb);
+    launch_copy(s, c.d_r.p, c.d_yba.p, nb);
+    apply_precond(c, c.d_r.p, c.d_z.p);
+    launch_copy(s, c.d_p.p, c.d_z.p, nb);
+    launch_mul(s, c.d_p.p, c.d_dwb.p, c.d_pw.p, nb);
+    launch_dots(s, 1, c.d_r.p, c.d_z.p, c.d_r.p, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
+    launch_finalize(s, c.d_part.p, nblk, 3, c.d_scal.p + 2);
+    double hs[8];
+    d2h(hs, c.d_scal.p, 8, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    c.stats[ST_RZINIT] = hs[2];
+    c.stats[ST_RRINIT] = hs[4];
+    h2d(c.d_scal.p, &hs[2], 1, s);   // scal[0] = rz
+    MADAM_CHECK(hipStreamSynchronize(s));
+    c.rr_hist.clear();
+    c.pcg_steps_done = 0;
+}
+
+// PCG iterations (src/smadam_routines.f90:785-924).  With converge = false the
+// exit tests are skipped and exactly nmax iterations run (benchmark mode).
+static int pcg_steps(Context &c, int nmax, bool converge)
+{
+    hipStream_t s = c.stream;
+    const Params &p = c.p;
+    int64_t nb = c.nbtot;
+    int nblk = dots_nblocks(nb);
+    int grp = group_for(c);
+    double rrinit = c.stats[ST_RRINIT];
+    double hs[8];
+    Timer t_sc, t_ga, t_pr, t_pc, t_it;
+    int prof = c.profile;
+    int done = 0;
+    for (;;) {
+        if (done >= nmax) break;
+        done++;
+        c.pcg_steps_done++;
+        if (prof) { t_it.start(s); t_sc.start(s); }
+        launch_scatter_pixels(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
+                              c.d_m4x.p);
+        if (prof) { t_sc.stop(s); t_pr.start(s); }
+        if (p.kfilter) apply_prior(c, c.d_p.p, c.d_ap.p);
+        else if (p.diagfilter != 0) launch_axpby_nna(s, c.d_ap.p, c.d_nna.p, c.d_p.p, p.diagfilter, nb);
+        else launch_fill(s, c.d_ap.p, 0.0, nb);
+        if (prof) { t_pr.stop(s); t_ga.start(s); }
+        launch_gather_baselines(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
+                                c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p, grp);
+        if (prof) t_ga.stop(s);
+        launch_dots(s, 0, c.d_p.p, c.d_ap.p, nullptr, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
+        launch_finalize(s, c.d_part.p, nblk, 1, c.d_scal.p + 1);
+        launch_pcg_update_r(s, c.d_scal.p, c.d_aa.p, c.d_r.p, c.d_ro.p, c.d_p.p, c.d_ap.p, nb);
+        if (prof) t_pc.start(s);
+        apply_precond(c, c.d_r.p, c.d_z.p);
+        if (prof) t_pc.stop(s);
+        launch_dots(s, 1, c.d_r.p, c.d_z.p, c.d_ro.p, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
+        launch_finalize(s, c.d_part.p, nblk, 3, c.d_scal.p + 2);
+        launch_scalars(s, c.d_scal.p);
+        bool last = false;
+        if (converge) {
+            d2h(hs, c.d_scal.p, 8, s);
+            MADAM_CHECK(hipStreamSynchronize(s));
+            double rz = hs[0], rr = hs[4];
+            c.rr_hist.push_back(rr / rrinit);
+            if (rr / rrinit > 1e3) { c.stats[ST_NITER] = c.pcg_steps_done; fail("CG is diverging"); }
+            if (rr / rrinit < p.cglimit && c.pcg_steps_done > p.iter_min) last = true;
+            if (rz == 0) last = true;
+        }
+        if (!last) launch_pcg_update_p(s, c.d_scal.p, c.d_p.p, c.d_z.p, c.d_pw.p, c.d_dwb.p, nb);
+        if (prof) {
+            t_it.stop(s);
+            c.stats[ST_T_SCATTER] += t_sc.ms() * 1e-3;
+            c.stats[ST_T_PRIOR] += t_pr.ms() * 1e-3;
+            c.stats[ST_T_GATHER] += t_ga.ms() * 1e-3;
+            c.stats[ST_T_APPLY_PREC] += t_pc.ms() * 1e-3;
+            c.stats[ST_T_STEP] += t_it.ms() * 1e-3;
+            c.stats[ST_N_SCATTER] += 1;
+            c.stats[ST_N_GATHER] += 1;
+        }
+        if (last) break;
+    }
+    MADAM_CHECK(hipStreamSynchronize(s));
+    c.stats[ST_NITER] = c.pcg_steps_done;
+    return done;
+}
+
 static void solve(Context &c, double *signal, int on_device)
 {
     if (!c.pointed) fail("madam_solve called before the pointing was set");
@@ -991,6 +1085,8 @@ static void solve(Context &c, double *signal, int on_device)
         c.d_tod = c.d_tod_own.p;
     }
     Timer tm;
+    for (int i : {ST_T_SCATTER, ST_T_PRIOR, ST_T_GATHER, ST_T_APPLY_PREC, ST_T_STEP, ST_N_SCATTER, ST_N_GATHER})
+        c.stats[i] = 0;
     // binning
     tm.start(s);
     c.d_cc6.alloc(c.npix_map * 6);
@@ -1040,79 +1136,9 @@ static void solve(Context &c, double *signal, int on_device)
         c.stats[ST_NOF] = (double)c.nof;
         // PCG (src/smadam_routines.f90:677-1483)
         double ti = now();
-        int64_t nb = c.nbtot;
-        for (auto *b : {&c.d_r, &c.d_ro, &c.d_z, &c.d_p, &c.d_pw, &c.d_ap}) b->alloc(nb);
-        c.d_scal.alloc(8);
-        int nblk = dots_nblocks(nb);
-        c.d_part.alloc((size_t)nblk * 3);
-        launch_copy(s, c.d_r.p, c.d_yba.p, nb);
-        apply_precond(c, c.d_r.p, c.d_z.p);
-        launch_copy(s, c.d_p.p, c.d_z.p, nb);
-        launch_mul(s, c.d_p.p, c.d_dwb.p, c.d_pw.p, nb);
-        launch_dots(s, 1, c.d_r.p, c.d_z.p, c.d_r.p, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
-        launch_finalize(s, c.d_part.p, nblk, 3, c.d_scal.p + 2);
-        double hs[8];
-        d2h(hs, c.d_scal.p, 8, s);
-        MADAM_CHECK(hipStreamSynchronize(s));
-        double rz = hs[2], rr = hs[4];
-        double rzinit = rz, rrinit = rr;
-        c.stats[ST_RZINIT] = rzinit;
-        c.stats[ST_RRINIT] = rrinit;
-        h2d(c.d_scal.p, &rz, 1, s);   // scal[0] = rz
-        int istep = 0;
-        Timer t_sc, t_ga, t_pr, t_pc;
-        double ms_sc = 0, ms_ga = 0, ms_pr = 0, ms_pc = 0;
-        int prof = c.profile;
-        if (!(rzinit < F32(1.e-30)) && !std::isnan(rz)) {
-            for (;;) {
-                if (istep >= p.iter_max) break;
-                istep++;
-                if (prof) t_sc.start(s);
-                launch_scatter_pixels(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
-                                      c.d_m4x.p);
-                if (prof) { t_sc.stop(s); t_pr.start(s); }
-                if (p.kfilter) apply_prior(c, c.d_p.p, c.d_ap.p);
-                else if (p.diagfilter != 0) launch_axpby_nna(s, c.d_ap.p, c.d_nna.p, c.d_p.p, p.diagfilter, nb);
-                else launch_fill(s, c.d_ap.p, 0.0, nb);
-                if (prof) { t_pr.stop(s); t_ga.start(s); }
-                launch_gather_baselines(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
-                                        c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p, grp);
-                if (prof) t_ga.stop(s);
-                launch_dots(s, 0, c.d_p.p, c.d_ap.p, nullptr, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
-                launch_finalize(s, c.d_part.p, nblk, 1, c.d_scal.p + 1);
-                launch_pcg_update_r(s, c.d_scal.p, c.d_aa.p, c.d_r.p, c.d_ro.p, c.d_p.p, c.d_ap.p, nb);
-                if (prof) t_pc.start(s);
-                apply_precond(c, c.d_r.p, c.d_z.p);
-                if (prof) t_pc.stop(s);
-                launch_dots(s, 1, c.d_r.p, c.d_z.p, c.d_ro.p, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
-                launch_finalize(s, c.d_part.p, nblk, 3, c.d_scal.p + 2);
-                launch_scalars(s, c.d_scal.p);
-                d2h(hs, c.d_scal.p, 8, s);
-                MADAM_CHECK(hipStreamSynchronize(s));
-                if (prof) {
-                    ms_sc += t_sc.ms();
-                    ms_ga += t_ga.ms();
-                    ms_pr += t_pr.ms();
-                    ms_pc += t_pc.ms();
-                }
-                rz = hs[0];
-                rr = hs[4];
-                c.rr_hist.push_back(rr / rrinit);
-                if (rr / rrinit > 1e3) { c.stats[ST_NITER] = istep; fail("CG is diverging"); }
-                if (rr / rrinit < p.cglimit && istep > p.iter_min) break;
-                if (rz == 0) break;
-                launch_pcg_update_p(s, c.d_scal.p, c.d_p.p, c.d_z.p, c.d_pw.p, c.d_dwb.p, nb);
-            }
-        }
-        MADAM_CHECK(hipStreamSynchronize(s));
-        c.stats[ST_NITER] = istep;
+        pcg_init(c);
+        if (!(c.stats[ST_RZINIT] < F32(1.e-30)) && !std::isnan(c.stats[ST_RZINIT])) pcg_steps(c, p.iter_max, true);
         c.stats[ST_T_ITER] = now() - ti;
-        c.stats[ST_T_SCATTER] = ms_sc * 1e-3;
-        c.stats[ST_T_GATHER] = ms_ga * 1e-3;
-        c.stats[ST_T_PRIOR] = ms_pr * 1e-3;
-        c.stats[ST_T_APPLY_PREC] = ms_pc * 1e-3;
-        c.stats[ST_N_SCATTER] = prof ? istep : 0;
-        c.stats[ST_N_GATHER] = prof ? istep : 0;
         c.stats[ST_RR_FINAL] = c.rr_hist.empty() ? 0 : c.rr_hist.back();
     }
     // final maps
@@ -1292,6 +1318,25 @@ MADAM_API int madam_set_pointing(void *ctx, int32_t *pix, float *pixweights, int
 MADAM_API int madam_solve(void *ctx, double *signal, int on_device)
 {
     return guarded([&] { solve(*(Context *)ctx, signal, on_device); });
+}
+
+MADAM_API int madam_iterate(void *ctx, int niter, int restart)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        if (!c.solved || !c.p.kfirst) fail("madam_iterate needs a completed madam_solve with destriping");
+        if (restart) pcg_init(c);
+        pcg_steps(c, niter, false);
+    });
+}
+
+MADAM_API int madam_reset_timers(void *ctx)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        for (int i : {ST_T_SCATTER, ST_T_PRIOR, ST_T_GATHER, ST_T_APPLY_PREC, ST_T_STEP, ST_N_SCATTER, ST_N_GATHER})
+            c.stats[i] = 0;
+    });
 }
 
 MADAM_API int madam_restore_pixels(void *ctx, int32_t *host_pix)
