@@ -72,6 +72,9 @@ int madam_set_profiling(void *ctx, int on);
  * madam_solve are not refreshed */
 int madam_iterate(void *ctx, int niter, int restart);
 int madam_reset_timers(void *ctx);
+/* projection kernels: 0 = flat tiles with LDS segmented reduction (default),
+ * 1 = one lane group per baseline / wave per pixel */
+int madam_set_kernel_variant(void *ctx, int variant);
 /* pointing expansion of the synthetic ring scan: scan5 = {fsample, spin_period,
  * prec_period, opening_turns, lon0_turns}, dets3[ndet][3] = {psi_turns,
  * open_offset_turns, spin_offset_turns}; outputs pix[ndet][n], weights[ndet][n][3] */

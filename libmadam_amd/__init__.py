@@ -24,7 +24,7 @@ PSD_TYPE = np.float64
 # order of madam_get_stats
 STAT_NAMES = ["niter", "rrinit", "rzinit", "t_pointing", "t_bin", "t_init", "t_precond", "t_iter", "t_final",
               "t_scatter", "t_gather", "t_prior", "t_apply_prec", "n_scatter", "n_gather", "precond_kind",
-              "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total", "t_step"]
+              "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total", "t_step", "ncg"]
 
 
 class MadamError(RuntimeError):
@@ -62,6 +62,8 @@ def lib():
         L.madam_get_rr_history.argtypes = [vp, vp, i32]
         L.madam_iterate.restype = i32
         L.madam_iterate.argtypes = [vp, i32, i32]
+        L.madam_set_kernel_variant.restype = i32
+        L.madam_set_kernel_variant.argtypes = [vp, i32]
         L.madam_reset_timers.restype = i32
         L.madam_reset_timers.argtypes = [vp]
         L.madam_set_profiling.restype = i32
@@ -206,6 +208,9 @@ class Madam:
     def iterate(self, niter, restart=False):
         _check(lib().madam_iterate(self._h, int(niter), int(restart)))
 
+    def set_kernel_variant(self, v):
+        _check(lib().madam_set_kernel_variant(self._h, int(v)))
+
     def reset_timers(self):
         _check(lib().madam_reset_timers(self._h))
 
@@ -214,7 +219,7 @@ class Madam:
 
 
 def run(pars, dets, weights, timestamps, pixels, pixweights, signal, periods, npsd, psdstarts, psdfreqs, psdvals,
-        subchunk_pass=0, device=0):
+        subchunk_pass=0, device=0, kernel_variant=0):
     """Full solve with in-memory outputs.  Mutates ``pixels`` and ``signal`` the
     way the reference's destripe() mutates the caller's buffers."""
     ndet = len(dets)
@@ -224,6 +229,7 @@ def run(pars, dets, weights, timestamps, pixels, pixweights, signal, periods, np
     try:
         m.configure(pars, dets, weights, timestamps, periods, npsd, psdstarts, psdfreqs, psdvals, nnz,
                     subchunk_pass)
+        m.set_kernel_variant(kernel_variant)
         m.set_pointing(pixels, pixweights)
         m.solve(signal)
         m.restore_pixels(pixels)

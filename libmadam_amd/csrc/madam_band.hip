@@ -28,83 +28,6 @@ namespace madam {
 
 static constexpr int BAND_CH = 512;
 
-// ---------------------------------------------------------------- factorization
-
-// one 64-thread block per segment; band output Lb[b * K + m] = L(b + m, b), with
-// the diagonal entry replaced by its reciprocal (the solve multiplies)
-__global__ void k_band_factor(const Segment *seg, int64_t nseg, const int *nband_seg, const double *invcov,
-                              int64_t nof, int K, const double *nna, double *Lb, int *info)
-{
-    extern __shared__ double W[];   // K * K window, plus K for the column
-    double *lv = W + K * K;
-    int64_t s = blockIdx.x;
-    if (s >= nseg) return;
-    Segment sg = seg[s];
-    int nb = nband_seg[s];
-    if (nb <= 0 || sg.skip) return;
-    const double *ic = invcov + (int64_t)sg.ipsd * nof;
-    int64_t n = sg.noba;
-    const double *nn = nna + sg.b0;
-    int kd = K - 1;
-    // initial window rows 0..kd: A(i, j) = ic[i - j] (|i-j| < nb) + nna on the diagonal
-    for (int t = threadIdx.x; t < K * K; t += blockDim.x) {
-        int a = t / K, b = t % K;
-        double v = 0;
-        if (a < n && b <= a && a - b < nb) v = ic[a - b] + (a == b ? nn[a] : 0.0);
-        W[a * K + b] = v;   // slot (i % K, j % K)
-    }
-    __syncthreads();
-    __shared__ int failed;
-    if (threadIdx.x == 0) failed = 0;
-    __syncthreads();
-    for (int64_t j = 0; j < n; ++j) {
-        int sj = (int)(j % K);
-        double ajj = W[sj * K + sj];
-        if (!(ajj > 0)) {
-            if (threadIdx.x == 0) { failed = 1; info[s] = (int)(j + 1); }
-            break;
-        }
-        double d = sqrt(ajj);
-        // column below the diagonal
-        for (int a = 1 + threadIdx.x; a <= kd; a += blockDim.x) {
-            double l = 0;
-            if (j + a < n) l = W[((j + a) % K) * K + sj] / d;
-            lv[a] = l;
-            Lb[(sg.b0 + j) * K + a] = l;
-        }
-        if (threadIdx.x == 0) Lb[(sg.b0 + j) * K] = 1.0 / d;
-        __syncthreads();
-        // trailing update W(j+a, j+b) -= l_a l_b for 1 <= b <= a <= kd
-        int npair = kd * (kd + 1) / 2;
-        for (int t = threadIdx.x; t < npair; t += blockDim.x) {
-            // decode t -> (a, b), a >= b >= 1
-            int a = (int)((sqrt(8.0 * t + 1.0) - 1.0) / 2.0);
-            while ((a + 1) * (a + 2) / 2 <= t) ++a;
-            while (a * (a + 1) / 2 > t) --a;
-            int b = t - a * (a + 1) / 2;
-            a += 1;
-            b += 1;
-            if (j + a < n) {
-                int sa = (int)((j + a) % K), sb = (int)((j + b) % K);
-                W[sa * K + sb] -= lv[a] * lv[b];
-            }
-        }
-        __syncthreads();
-        // load the new row i = j + K into slot sj
-        int64_t i = j + K;
-        for (int b = threadIdx.x; b < K; b += blockDim.x) {
-            // column i - kd + b' ... slot of column c = (c % K); entries A(i, c) for c in [i - kd, i]
-            int64_t c = i - kd + b;
-            double v = 0;
-            if (i < n && i - c < nb) v = ic[i - c] + (i == c ? nn[i] : 0.0);
-            W[sj * K + (int)(c % K)] = v;
-        }
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------- partitioned solves
-
 struct ChunkInfo {
     int64_t b0;    // first global baseline row of the chunk
     int32_t n;     // rows
@@ -112,36 +35,133 @@ struct ChunkInfo {
     int32_t first, last;   // first / last chunk of its segment
 };
 
-// forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
-// next KD rows (acc[0] for the first row of the chunk).  Returns the outgoing
-// state.  If store, writes y.
+
+// ---------------------------------------------------------------- factorization
+
+// Register-window variant: one wave per segment, lane a holds row j + a of the
+// active (K x K) window, W[b] = A(j + a, j + b).  Per column: broadcast the
+// pivot, scale the column, rank-1 update, shift the window up by one row.
 template <int KD>
-__device__ inline void fwd_sweep(const double *Lb, int64_t b0, int n, const double *r, double *acc, double *y,
-                                 bool store, bool zero_rhs)
+__global__ void k_band_factor_w(const Segment *seg, int64_t nseg, const int *nband_seg, const double *invcov,
+                                int64_t nof, const double *nna, double *Lb, int *info)
+{
+    constexpr int K = KD + 1;
+    int lane = threadIdx.x & 63;
+    int64_t s = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    if (s >= nseg) return;
+    Segment sg = seg[s];
+    int nb = nband_seg[s];
+    if (nb <= 0 || sg.skip) return;
+    const double *ic = invcov + (int64_t)sg.ipsd * nof;
+    const double *nn = nna + sg.b0;
+    int64_t n = sg.noba;
+    double icv[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) icv[m] = m < nb ? ic[m] : 0.0;
+    double W[K];
+#pragma unroll
+    for (int bb = 0; bb < K; ++bb) {
+        // A(a, bb) for a = lane, lower triangle
+        double v = 0.0;
+        if (lane < K && bb <= lane && lane < n) {
+            int d = lane - bb;
+#pragma unroll
+            for (int m = 0; m < K; ++m) if (m == d) v = icv[m];
+            if (bb == lane) v += nn[lane];
+        }
+        W[bb] = v;
+    }
+    for (int64_t j = 0; j < n; ++j) {
+        double ajj = __shfl(W[0], 0, 64);
+        if (!(ajj > 0)) {
+            if (lane == 0) info[s] = (int)(j + 1);
+            return;
+        }
+        double d = sqrt(ajj);
+        double l = (lane >= 1 && lane < K && j + lane < n) ? W[0] / d : 0.0;
+        if (lane == 0) Lb[(sg.b0 + j) * K] = 1.0 / d;
+        else if (lane < K) Lb[(sg.b0 + j) * K + lane] = l;
+        // rank-1 update of rows 1..KD: W[bb] -= l_a l_bb for 1 <= bb <= a
+#pragma unroll
+        for (int bb = 1; bb < K; ++bb) {
+            double lb = __shfl(l, bb, 64);
+            if (bb <= lane) W[bb] -= l * lb;
+        }
+        // shift: new row a = old row a + 1, columns shifted left by one
+        double Wn[K];
+#pragma unroll
+        for (int bb = 0; bb < K - 1; ++bb) Wn[bb] = __shfl(W[bb + 1], lane + 1, 64);
+        Wn[K - 1] = 0.0;
+        // new last row: i = j + K, entries A(i, j + 1 + bb) = ic[K - 1 - bb] (+ nna on the diagonal)
+        if (lane == K - 1) {
+            int64_t i = j + K;
+#pragma unroll
+            for (int bb = 0; bb < K; ++bb) {
+                double v = 0.0;
+                if (i < n) {
+                    v = icv[K - 1 - bb];
+                    if (bb == K - 1) v += nn[i];
+                }
+                Wn[bb] = v;
+            }
+        }
+#pragma unroll
+        for (int bb = 0; bb < K; ++bb) W[bb] = Wn[bb];
+    }
+}
+
+// ---------------------------------------------------------------- partitioned solves
+
+
+__global__ void k_band_relayout(const ChunkInfo *ch, int64_t nch, int64_t NCP, int K, int CH, const double *Lb,
+                                double *Li)
+{
+    int64_t tot = NCP * (int64_t)CH * K;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c = t % NCP, im = t / NCP;
+        int64_t i = im / K, m = im % K;
+        double v = 0.0;
+        if (c < nch) {
+            ChunkInfo ci = ch[c];
+            if (i < ci.n) v = Lb[(ci.b0 + i) * K + m];
+        }
+        Li[t] = v;
+    }
+}
+
+// The factor is re-laid out per chunk for the sweeps: Li[(i * K + m) * NCP + c]
+// is L(b0_c + i + m, b0_c + i) (m = 0 holds 1 / L(i, i)), so that the lanes of
+// a wave (consecutive chunks c) read consecutive addresses.
+
+// forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
+// next KD rows (acc[0] for the first row of the chunk).  If store, writes y.
+template <int KD>
+__device__ inline void fwd_sweep(const double *Li, int64_t NCP, int64_t c, int64_t b0, int n, const double *r,
+                                 double *acc, double *y, bool store, bool zero_rhs)
 {
     const int K = KD + 1;
     for (int i = 0; i < n; ++i) {
-        const double *row = Lb + (b0 + i) * K;
+        const double *row = Li + (int64_t)i * K * NCP + c;
         double rhs = zero_rhs ? 0.0 : r[b0 + i];
         double u = (rhs + acc[0]) * row[0];
 #pragma unroll
-        for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - row[m] * u;
-        acc[KD - 1] = -row[KD] * u;
+        for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - row[m * NCP] * u;
+        acc[KD - 1] = -row[KD * NCP] * u;
         if (store) y[b0 + i] = u;
     }
 }
 
 // backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
 template <int KD>
-__device__ inline void bwd_sweep(const double *Lb, int64_t b0, int n, const double *y, double *xs, double *x,
-                                 bool store, bool zero_rhs)
+__device__ inline void bwd_sweep(const double *Li, int64_t NCP, int64_t c, int64_t b0, int n, const double *y,
+                                 double *xs, double *x, bool store, bool zero_rhs)
 {
     const int K = KD + 1;
     for (int i = n - 1; i >= 0; --i) {
-        const double *row = Lb + (b0 + i) * K;
+        const double *row = Li + (int64_t)i * K * NCP + c;
         double s = zero_rhs ? 0.0 : y[b0 + i];
 #pragma unroll
-        for (int m = 1; m <= KD; ++m) s -= row[m] * xs[m - 1];
+        for (int m = 1; m <= KD; ++m) s -= row[m * NCP] * xs[m - 1];
         double xi = s * row[0];
 #pragma unroll
         for (int m = KD - 1; m > 0; --m) xs[m] = xs[m - 1];
@@ -152,7 +172,8 @@ __device__ inline void bwd_sweep(const double *Lb, int64_t b0, int n, const doub
 
 // T matrices: response of the outgoing state to a unit incoming state
 template <int KD>
-__global__ void k_band_response(const ChunkInfo *ch, int64_t nch, const double *Lb, double *Tf, double *Tb)
+__global__ void k_band_response(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, double *Tf,
+                                double *Tb)
 {
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= nch * KD) return;
@@ -162,18 +183,19 @@ __global__ void k_band_response(const ChunkInfo *ch, int64_t nch, const double *
     double st[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    fwd_sweep<KD>(Lb, ci.b0, ci.n, nullptr, st, nullptr, false, true);
+    fwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tf[(c * KD + m) * KD + j] = st[m];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    bwd_sweep<KD>(Lb, ci.b0, ci.n, nullptr, st, nullptr, false, true);
+    bwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tb[(c * KD + m) * KD + j] = st[m];
 }
 
 template <int KD>
-__global__ void k_band_fwd0(const ChunkInfo *ch, int64_t nch, const double *Lb, const double *r, double *s0)
+__global__ void k_band_fwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *r,
+                            double *s0)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
@@ -181,57 +203,41 @@ __global__ void k_band_fwd0(const ChunkInfo *ch, int64_t nch, const double *Lb, 
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    fwd_sweep<KD>(Lb, ci.b0, ci.n, r, acc, nullptr, false, false);
+    fwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, r, acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
 }
 
-// chain carries inside each segment; one thread per segment (segments are few)
+// chain carries inside each segment: one wave per segment, lane m < KD holds
+// state component m; s_next = s0[c] + T[c] s, sequential over the chunks
 template <int KD>
 __global__ void k_band_chain(const int64_t *seg_first_chunk, int64_t nseg, const double *s0, const double *T,
                              double *sin, int dir)
 {
-    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int lane = threadIdx.x & 63;
+    int64_t s = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     if (s >= nseg) return;
     int64_t c0 = seg_first_chunk[s], c1 = seg_first_chunk[s + 1];
-    double st[KD];
+    bool on = lane < KD;
+    double st = 0.0;
+    int64_t n = c1 - c0;
+    for (int64_t t = 0; t < n; ++t) {
+        int64_t c = dir == 0 ? c0 + t : c1 - 1 - t;
+        if (on) sin[c * KD + lane] = st;
+        double v = on ? s0[c * KD + lane] : 0.0;
+        double tr[KD];
 #pragma unroll
-    for (int m = 0; m < KD; ++m) st[m] = 0;
-    if (dir == 0) {
-        for (int64_t c = c0; c < c1; ++c) {
+        for (int k = 0; k < KD; ++k) tr[k] = on ? T[(c * KD + lane) * KD + k] : 0.0;
 #pragma unroll
-            for (int m = 0; m < KD; ++m) sin[c * KD + m] = st[m];
-            double nx[KD];
-#pragma unroll
-            for (int m = 0; m < KD; ++m) {
-                double v = s0[c * KD + m];
-                for (int k = 0; k < KD; ++k) v += T[(c * KD + m) * KD + k] * st[k];
-                nx[m] = v;
-            }
-#pragma unroll
-            for (int m = 0; m < KD; ++m) st[m] = nx[m];
-        }
-    } else {
-        for (int64_t c = c1 - 1; c >= c0; --c) {
-#pragma unroll
-            for (int m = 0; m < KD; ++m) sin[c * KD + m] = st[m];
-            double nx[KD];
-#pragma unroll
-            for (int m = 0; m < KD; ++m) {
-                double v = s0[c * KD + m];
-                for (int k = 0; k < KD; ++k) v += T[(c * KD + m) * KD + k] * st[k];
-                nx[m] = v;
-            }
-#pragma unroll
-            for (int m = 0; m < KD; ++m) st[m] = nx[m];
-        }
+        for (int k = 0; k < KD; ++k) v += tr[k] * __shfl(st, k, 64);
+        st = v;
     }
 }
 
 // final forward sweep with true carries writes y into z, then the zero-carry
 // backward sweep of the same chunk gives its backward partial state
 template <int KD>
-__global__ void k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, const double *Lb, const double *r,
+__global__ void k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *r,
                                  const double *sinf, double *z, double *s0b)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -240,16 +246,17 @@ __global__ void k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, const double 
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
-    fwd_sweep<KD>(Lb, ci.b0, ci.n, r, acc, z, true, false);
+    fwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, r, acc, z, true, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    bwd_sweep<KD>(Lb, ci.b0, ci.n, z, acc, nullptr, false, false);
+    bwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, z, acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
 }
 
 template <int KD>
-__global__ void k_band_bwd1(const ChunkInfo *ch, int64_t nch, const double *Lb, const double *sinb, double *z)
+__global__ void k_band_bwd1(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *sinb,
+                            double *z)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
@@ -257,12 +264,14 @@ __global__ void k_band_bwd1(const ChunkInfo *ch, int64_t nch, const double *Lb, 
     double xs[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
-    bwd_sweep<KD>(Lb, ci.b0, ci.n, z, xs, z, true, false);
+    bwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, z, xs, z, true, false);
 }
 
 // ---------------------------------------------------------------- host side
 
 struct BandImpl {
+    double *Li = nullptr;   // interleaved factor
+    int64_t NCP = 0;
     ChunkInfo *ch = nullptr;
     int64_t *seg_first = nullptr;
     double *s0f = nullptr, *s0b = nullptr, *sinf = nullptr, *sinb = nullptr;
@@ -309,8 +318,9 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
     MADAM_CHECK(hipMalloc(&dinfo, sizeof(int) * (nseg > 0 ? nseg : 1)));
     MADAM_CHECK(hipMemcpyAsync(dnb, nband_seg_host, sizeof(int) * nseg, hipMemcpyHostToDevice, s));
     MADAM_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int) * (nseg > 0 ? nseg : 1), s));
-    size_t lds = sizeof(double) * (K * K + K);
-    if (nseg > 0) k_band_factor<<<(unsigned)nseg, 64, lds, s>>>(dseg, nseg, dnb, invcov_dev, nof, K, nna, bp.L, dinfo);
+    if (nseg > 0)
+        BAND_DISPATCH(KD, (k_band_factor_w<KD><<<(unsigned)nseg, 64, 0, s>>>(dseg, nseg, dnb, invcov_dev, nof, nna, bp.L,
+                                                                            dinfo)));
     MADAM_CHECK(hipGetLastError());
     MADAM_CHECK(hipMemcpyAsync(info_out, dinfo, sizeof(int) * nseg, hipMemcpyDeviceToHost, s));
     MADAM_CHECK(hipStreamSynchronize(s));
@@ -350,9 +360,18 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
         MADAM_CHECK(hipMemcpyAsync(im->ch, ch.data(), sizeof(ChunkInfo) * ch.size(), hipMemcpyHostToDevice, s));
     MADAM_CHECK(hipMemcpyAsync(im->seg_first, first.data(), sizeof(int64_t) * first.size(), hipMemcpyHostToDevice, s));
     if (bp.nchunk > 0) {
+        im->NCP = (bp.nchunk + 63) / 64 * 64;
+        MADAM_CHECK(hipMalloc(&im->Li, sizeof(double) * im->NCP * BAND_CH * K));
+        int64_t tot = im->NCP * BAND_CH * K;
+        unsigned g = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535 * 16);
+        k_band_relayout<<<g, 256, 0, s>>>(im->ch, bp.nchunk, im->NCP, K, BAND_CH, bp.L, im->Li);
+        MADAM_CHECK(hipGetLastError());
+        MADAM_CHECK(hipStreamSynchronize(s));
+        (void)hipFree(bp.L);   // the sweeps only read the interleaved copy
+        bp.L = nullptr;
         int64_t nt = bp.nchunk * KD;
-        BAND_DISPATCH(KD, (k_band_response<KD><<<(unsigned)((nt + 63) / 64), 64, 0, s>>>(im->ch, bp.nchunk, bp.L,
-                                                                                         bp.Tf, bp.Tb)));
+        BAND_DISPATCH(KD, (k_band_response<KD><<<(unsigned)((nt + 63) / 64), 64, 0, s>>>(im->ch, bp.nchunk, im->NCP,
+                                                                                         im->Li, bp.Tf, bp.Tb)));
         MADAM_CHECK(hipGetLastError());
     }
     bp.chunk_b0 = (int64_t *)im;   // opaque implementation pointer
@@ -366,13 +385,13 @@ void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
     if (!im || bp.nchunk == 0) return;
     int KD = bp.kd;
     unsigned g = (unsigned)((bp.nchunk + 63) / 64);
-    unsigned gs = (unsigned)((im->nseg_used + 63) / 64);
+    unsigned gs = (unsigned)im->nseg_used;   // one 64-lane wave per segment
     BAND_DISPATCH(KD, ({
-        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, bp.L, r, im->s0f);
+        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, r, im->s0f);
         k_band_chain<KD><<<gs, 64, 0, s>>>(im->seg_first, im->nseg_used, im->s0f, bp.Tf, im->sinf, 0);
-        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, bp.L, r, im->sinf, z, im->s0b);
+        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, r, im->sinf, z, im->s0b);
         k_band_chain<KD><<<gs, 64, 0, s>>>(im->seg_first, im->nseg_used, im->s0b, bp.Tb, im->sinb, 1);
-        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, bp.L, im->sinb, z);
+        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, im->sinb, z);
     }));
 }
 
@@ -380,6 +399,7 @@ void band_free(BandPrec &bp)
 {
     BandImpl *im = (BandImpl *)bp.chunk_b0;
     if (im) {
+        if (im->Li) (void)hipFree(im->Li);
         (void)hipFree(im->ch);
         (void)hipFree(im->seg_first);
         (void)hipFree(im->s0f);

@@ -195,39 +195,62 @@ __global__ void k_c2r_prep(double2 *data, int64_t N, int L1, int L2, const doubl
     }
 }
 
-// segment load: x - mean(x) over the segment, zero-padded to nof; one block per batch entry
-__global__ void k_seg_load(const double *x, const Segment *seg, int64_t s0, int64_t nof, double *work)
+// Segment mean removal around the convolution (src/noise_routines.f90:840-850).
+// Sums are deterministic: SEG_NB partial sums per segment, combined in order.
+static constexpr int SEG_NB = 64;
+
+// partial[i * SEG_NB + blk] = sum over the block's stride of scale * v[j], j < noba
+__global__ void k_seg_partial(const double *v, int64_t vstride, int use_b0, const Segment *seg, int64_t s0,
+                              double scale, double *partial)
 {
-    Segment sg = seg[s0 + blockIdx.x];
-    const double *xs = x + sg.b0;
-    double *w = work + (int64_t)blockIdx.x * nof;
-    __shared__ double sh[FFT_TPB / 64];
+    int64_t i = blockIdx.y;
+    Segment sg = seg[s0 + i];
+    const double *x = use_b0 ? v + sg.b0 : v + i * vstride;
     double acc = 0;
-    for (int64_t j = threadIdx.x; j < sg.noba; j += blockDim.x) acc += xs[j];
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < sg.noba; j += (int64_t)SEG_NB * blockDim.x)
+        acc += x[j] * scale;
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ double sh[FFT_TPB / 64];
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
-    double tot = 0;
-    for (int k = 0; k < FFT_TPB / 64; ++k) tot += sh[k];
-    double mean = tot / sg.noba;
-    for (int64_t j = threadIdx.x; j < nof; j += blockDim.x) w[j] = j < sg.noba ? xs[j] - mean : 0.0;
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < FFT_TPB / 64; ++k) t += sh[k];
+        partial[i * SEG_NB + blockIdx.x] = t;
+    }
 }
 
-__global__ void k_seg_store(const double *work, const Segment *seg, int64_t s0, int64_t nof, double *y)
+__device__ inline double seg_mean(const double *partial, int64_t i, int64_t noba)
 {
-    Segment sg = seg[s0 + blockIdx.x];
-    const double *w = work + (int64_t)blockIdx.x * nof;
+    double t = 0;
+    for (int k = 0; k < SEG_NB; ++k) t += partial[i * SEG_NB + k];
+    return t / noba;
+}
+
+// work[i][j] = x[j] - mean for j < noba, 0 up to nof; grid (blocks, batch)
+__global__ void k_seg_load(const double *x, const Segment *seg, int64_t s0, int64_t nof, double *work,
+                           const double *partial)
+{
+    int64_t i = blockIdx.y;
+    Segment sg = seg[s0 + i];
+    double mean = seg_mean(partial, i, sg.noba);
+    const double *xs = x + sg.b0;
+    double *w = work + i * nof;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nof; j += (int64_t)gridDim.x * blockDim.x)
+        w[j] = j < sg.noba ? xs[j] - mean : 0.0;
+}
+
+// y[b0 + j] = work[i][j] / nof - mean(work / nof), j < noba
+__global__ void k_seg_store(const double *work, const Segment *seg, int64_t s0, int64_t nof, double *y,
+                            const double *partial)
+{
+    int64_t i = blockIdx.y;
+    Segment sg = seg[s0 + i];
     double inv = 1 / (double)nof;
-    __shared__ double sh[FFT_TPB / 64];
-    double acc = 0;
-    for (int64_t j = threadIdx.x; j < sg.noba; j += blockDim.x) acc += w[j] * inv;
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    double tot = 0;
-    for (int k = 0; k < FFT_TPB / 64; ++k) tot += sh[k];
-    double mean = tot / sg.noba;
-    for (int64_t j = threadIdx.x; j < sg.noba; j += blockDim.x) y[sg.b0 + j] = w[j] * inv - mean;
+    double mean = seg_mean(partial, i, sg.noba);
+    const double *w = work + i * nof;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < sg.noba; j += (int64_t)gridDim.x * blockDim.x)
+        y[sg.b0 + j] = w[j] * inv - mean;
 }
 
 __global__ void k_seg_ipsd(const Segment *seg, int64_t s0, int64_t nb, int *ipsd)
@@ -321,17 +344,23 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
 {
     (void)hseg;
     int64_t nof = 2 * p.N;
-    int *ipsd = (int *)(work + batch * nof);   // small scratch right after the batch buffer
+    int *ipsd = (int *)(work + batch * nof);                  // batch ints after the batch buffer
+    double *partial = work + batch * nof + (batch + 1) / 2 + 1; // batch * SEG_NB doubles
     for (int64_t s0 = 0; s0 < nseg; s0 += batch) {
         int64_t nb = nseg - s0 < batch ? nseg - s0 : batch;
-        k_seg_load<<<(unsigned)nb, FFT_TPB, 0, s>>>(x, dseg, s0, nof, work);
+        dim3 gp(SEG_NB, (unsigned)nb);
+        k_seg_partial<<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
+        unsigned gl = (unsigned)((nof + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
+        if (gl > 1024) gl = 1024;
+        k_seg_load<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(x, dseg, s0, nof, work, partial);
         k_seg_ipsd<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(dseg, s0, nb, ipsd);
         fft_forward(s, p, (double2 *)work, nb);
         dim3 gs((unsigned)((p.N / 2 + 1 + FFT_TPB - 1) / FFT_TPB), (unsigned)nb);
         k_spectral<<<gs, FFT_TPB, 0, s>>>((double2 *)work, p.N, (int)p.L1, (int)p.L2, (const double2 *)p.tw_nof, fc,
                                            fc_stride, ipsd);
         fft_inverse(s, p, (double2 *)work, nb);
-        k_seg_store<<<(unsigned)nb, FFT_TPB, 0, s>>>(work, dseg, s0, nof, y);
+        k_seg_partial<<<gp, FFT_TPB, 0, s>>>(work, nof, 0, dseg, s0, 1 / (double)nof, partial);
+        k_seg_store<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(work, dseg, s0, nof, y, partial);
     }
 }
 

@@ -277,7 +277,7 @@ struct DBuf {
 
 enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_PRECOND, ST_T_ITER, ST_T_FINAL,
        ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
-       ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_COUNT };
+       ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_NCG, ST_COUNT };
 
 struct Context {
     int device = 0;
@@ -339,6 +339,12 @@ struct Context {
     bool fft_ready = false;
     int64_t fft_batch = 0;
     int precond_kind = 0;
+    // inner-CG preconditioner segments (band width criterion or Cholesky failed)
+    std::vector<Segment> segcg;
+    std::vector<double> ic0cg;
+    DBuf<Segment> d_segcg, d_segact;
+    DBuf<double> d_cg_resid, d_cg_prec, d_cg_zr, d_cg_p, d_cg_ap, d_cg_part, d_cg_ic0, d_cg_sc;
+    DBuf<int> d_cg_act;
     BandPrec band;
     bool band_ready = false;
     // results
@@ -346,6 +352,7 @@ struct Context {
     std::vector<double> rr_hist;
     int profile = 0;
     int pcg_steps_done = 0;
+    int kernel_variant = 0;   // 0: flat-tile projections, 1: group-per-item projections
 
     ~Context()
     {
@@ -833,6 +840,75 @@ static void apply_prior(Context &c, const double *x, double *y)
                           c.d_fftwork.p, c.fft_batch, c.segconv.data());
 }
 
+// apply_CG_preconditioner (src/noise_routines.f90:1305-1409) on every segment
+// listed in c.segcg; x is written into z on those segments
+static void cg_apply(Context &c, const double *r, double *z)
+{
+    int64_t ns = (int64_t)c.segcg.size();
+    if (ns == 0) return;
+    hipStream_t s = c.stream;
+    int64_t nb = c.nbtot;
+    for (auto *b : {&c.d_cg_resid, &c.d_cg_prec, &c.d_cg_zr, &c.d_cg_p, &c.d_cg_ap}) b->alloc(nb);
+    c.d_cg_part.alloc((size_t)ns * 32);
+    c.d_cg_sc.alloc((size_t)ns * 4);
+    c.d_cg_act.alloc(ns);
+    c.d_segact.alloc(ns);
+    std::vector<int> act(ns, 1);
+    h2d(c.d_cg_act.p, act.data(), ns, s);
+    launch_cg_init(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_ic0.p, r, c.d_nna.p, z, c.d_cg_resid.p, c.d_cg_prec.p,
+                   c.d_cg_zr.p, c.d_cg_p.p);
+    double *rr_d = c.d_cg_sc.p, *rz_d = c.d_cg_sc.p + ns, *an_d = c.d_cg_sc.p + 2 * ns, *ab_d = c.d_cg_sc.p + 3 * ns;
+    launch_cg_dot(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_resid.p, c.d_cg_resid.p, c.d_cg_part.p, rr_d);
+    launch_cg_dot(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_resid.p, c.d_cg_zr.p, c.d_cg_part.p, rz_d);
+    std::vector<double> rr(ns), rr0(ns), rz(ns), an(ns), ab(ns);
+    d2h(rr.data(), rr_d, ns, s);
+    d2h(rz.data(), rz_d, ns, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < ns; ++i) {
+        rr0[i] = rr[i];
+        if (rr[i] == 0) act[i] = 0;   // x = 0
+    }
+    const double lim = F32(1e-6);
+    for (int iter = 1;; ++iter) {
+        std::vector<Segment> live;
+        for (int64_t i = 0; i < ns; ++i)
+            if (act[i]) live.push_back(c.segcg[i]);
+        if (live.empty()) break;
+        h2d(c.d_cg_act.p, act.data(), ns, s);
+        h2d(c.d_segact.p, live.data(), live.size(), s);
+        // ap = convolve(p) + nna p
+        fft_convolve_segments(s, c.fft, c.d_segact.p, (int64_t)live.size(), c.d_cg_p.p, c.d_cg_ap.p, c.d_fcov.p,
+                              c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, live.data());
+        launch_cg_nna(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_nna.p, c.d_cg_p.p, c.d_cg_ap.p);
+        launch_cg_dot(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_p.p, c.d_cg_ap.p, c.d_cg_part.p, an_d);
+        d2h(an.data(), an_d, ns, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < ns; ++i) ab[i] = act[i] ? rz[i] / an[i] : 0.0;
+        h2d(ab_d, ab.data(), ns, s);
+        launch_cg_axpy(s, c.d_segcg.p, ns, c.d_cg_act.p, ab_d, z, c.d_cg_p.p, c.d_cg_resid.p, c.d_cg_ap.p);
+        launch_cg_dot(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_resid.p, c.d_cg_resid.p, c.d_cg_part.p, rr_d);
+        d2h(rr.data(), rr_d, ns, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        bool any = false;
+        for (int64_t i = 0; i < ns; ++i) {
+            if (!act[i]) continue;
+            if (rr[i] / rr0[i] < lim || iter == 100) act[i] = 0;
+            else any = true;
+        }
+        if (!any) break;
+        h2d(c.d_cg_act.p, act.data(), ns, s);
+        launch_cg_prec(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_prec.p, c.d_cg_resid.p, c.d_cg_zr.p);
+        std::vector<double> rzo = rz;
+        launch_cg_dot(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_resid.p, c.d_cg_zr.p, c.d_cg_part.p, rz_d);
+        d2h(rz.data(), rz_d, ns, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < ns; ++i) ab[i] = act[i] ? rz[i] / rzo[i] : 0.0;
+        h2d(ab_d, ab.data(), ns, s);
+        launch_cg_p(s, c.d_segcg.p, ns, c.d_cg_act.p, ab_d, c.d_cg_zr.p, c.d_cg_p.p);
+    }
+    MADAM_CHECK(hipStreamSynchronize(s));
+}
+
 static void apply_precond(Context &c, const double *r, double *z)
 {
     hipStream_t s = c.stream;
@@ -848,6 +924,7 @@ static void apply_precond(Context &c, const double *r, double *z)
     default:
         launch_copy(s, z, r, c.nbtot);
         band_apply(s, c.band, r, z);
+        cg_apply(c, r, z);
         break;
     }
 }
@@ -919,7 +996,6 @@ static void build_preconditioner(Context &c)
         MADAM_CHECK(hipStreamSynchronize(s));
     }
     if (p.use_fprecond) { c.precond_kind = 2; return; }
-    if (p.use_cgprecond) fail("use_cgprecond (inner-CG preconditioner) is not supported on the GPU path");
     c.precond_kind = 3;
     // invcov = c2r(fcov) / nof per PSD
     c.d_invcov.alloc((size_t)c.nof * c.npsdtot);
@@ -934,9 +1010,11 @@ static void build_preconditioner(Context &c)
     for (int ip = 0; ip < c.npsdtot; ++ip)
         for (int64_t i = 0; i < c.nof / 2; ++i) ptot[ip] += ic[(size_t)ip * c.nof + i] * ic[(size_t)ip * c.nof + i];
     std::vector<int> tries(c.seg.size(), 1);
+    std::vector<char> use_cg(c.seg.size(), 0);
     for (size_t is = 0; is < c.seg.size(); ++is) {
         const Segment &g = c.seg[is];
         if (g.skip || g.ipsd < 0) continue;
+        if (p.use_cgprecond) { use_cg[is] = 1; continue; }
         const double *v = ic.data() + (size_t)g.ipsd * c.nof;
         int tr = 1, nb = 0, ok = 0;
         for (;;) {
@@ -949,7 +1027,7 @@ static void build_preconditioner(Context &c)
             if (tr == 1000) break;
             tr++;
         }
-        if (!ok) fail("band preconditioner width criterion not met (reference falls back to an inner CG, not supported on the GPU path)");
+        if (!ok) { use_cg[is] = 1; continue; }   // src/noise_routines.f90:1113-1120
         nband[is] = nb;
         tries[is] = tr;
     }
@@ -963,8 +1041,11 @@ static void build_preconditioner(Context &c)
         for (size_t is = 0; is < c.seg.size(); ++is) {
             if (info[is] == 0 || nband[is] == 0) continue;
             const Segment &g = c.seg[is];
-            if (tries[is] == 1000 || nband[is] == g.noba || nband[is] == p.precond_width_max)
-                fail("band Cholesky failed (reference falls back to an inner CG, not supported on the GPU path)");
+            if (tries[is] == 1000 || nband[is] == g.noba || nband[is] == p.precond_width_max) {
+                use_cg[is] = 1;   // Cholesky failed at the widest band: inner CG
+                nband[is] = 0;
+                continue;
+            }
             tries[is]++;
             nband[is] = std::min(nband[is] + p.precond_width_min, p.precond_width_max);
             retry = true;
@@ -976,6 +1057,21 @@ static void build_preconditioner(Context &c)
     int nbmax = 0;
     for (int v : nband) nbmax = std::max(nbmax, v);
     c.stats[ST_NBAND] = nbmax;
+    c.segcg.clear();
+    c.ic0cg.clear();
+    for (size_t is = 0; is < c.seg.size(); ++is)
+        if (use_cg[is]) {
+            c.segcg.push_back(c.seg[is]);
+            c.ic0cg.push_back(ic[(size_t)c.seg[is].ipsd * c.nof]);
+        }
+    if (!c.segcg.empty()) {
+        c.d_segcg.alloc(c.segcg.size());
+        h2d(c.d_segcg.p, c.segcg.data(), c.segcg.size(), s);
+        c.d_cg_ic0.alloc(c.ic0cg.size());
+        h2d(c.d_cg_ic0.p, c.ic0cg.data(), c.ic0cg.size(), s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+    }
+    c.stats[ST_NCG] = (double)c.segcg.size();
 }
 
 // r = yba, z = M r, p = z (src/smadam_routines.f90:732-748)
@@ -1024,15 +1120,23 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         done++;
         c.pcg_steps_done++;
         if (prof) { t_it.start(s); t_sc.start(s); }
-        launch_scatter_pixels(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
-                              c.d_m4x.p);
+        if (c.kernel_variant == 0)
+            launch_scatter_flat(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
+                                c.d_m4x.p);
+        else
+            launch_scatter_pixels(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
+                                  c.d_m4x.p);
         if (prof) { t_sc.stop(s); t_pr.start(s); }
         if (p.kfilter) apply_prior(c, c.d_p.p, c.d_ap.p);
         else if (p.diagfilter != 0) launch_axpby_nna(s, c.d_ap.p, c.d_nna.p, c.d_p.p, p.diagfilter, nb);
         else launch_fill(s, c.d_ap.p, 0.0, nb);
         if (prof) { t_pr.stop(s); t_ga.start(s); }
-        launch_gather_baselines(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
-                                c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p, grp);
+        if (c.kernel_variant == 0)
+            launch_gather_flat(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
+                               c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p);
+        else
+            launch_gather_baselines(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
+                                    c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p, grp);
         if (prof) t_ga.stop(s);
         launch_dots(s, 0, c.d_p.p, c.d_ap.p, nullptr, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
         launch_finalize(s, c.d_part.p, nblk, 1, c.d_scal.p + 1);
@@ -1127,7 +1231,8 @@ static void solve(Context &c, double *signal, int on_device)
         if (p.kfilter) {
             build_filters(c);
             c.fft_batch = std::min<int64_t>(std::max<int64_t>((int64_t)c.segconv.size(), 1), 64);
-            c.d_fftwork.alloc((size_t)c.fft_batch * c.nof + c.fft_batch);
+            // batch buffer, ipsd ints and the per-segment partial sums
+            c.d_fftwork.alloc((size_t)c.fft_batch * c.nof + c.fft_batch + 2 + (size_t)c.fft_batch * 64);
         }
         build_preconditioner(c);
         MADAM_CHECK(hipStreamSynchronize(s));
@@ -1328,6 +1433,11 @@ MADAM_API int madam_iterate(void *ctx, int niter, int restart)
         if (restart) pcg_init(c);
         pcg_steps(c, niter, false);
     });
+}
+
+MADAM_API int madam_set_kernel_variant(void *ctx, int v)
+{
+    return guarded([&] { ((Context *)ctx)->kernel_variant = v; });
 }
 
 MADAM_API int madam_reset_timers(void *ctx)

@@ -87,6 +87,11 @@ void launch_scatter_pixels(hipStream_t s, int nmap, int64_t npix_cross, int64_t 
 void launch_gather_baselines(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase,
                              const int64_t *bstart, const PixW *pixw, const double *m4, int64_t ndeg,
                              const double *dwb, const double *nna, const double *p, double *ap, int group);
+void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
+                        const PixW *pixw, const double *m4, int64_t ndeg, const double *dwb, const double *nna,
+                        const double *p, double *ap);
+void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
+                         const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4);
 void launch_subtract_baselines(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent,
                                const double *aa, const double *dwb, double *rhs4);
 void launch_clean_tod(hipStream_t s, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
@@ -119,6 +124,20 @@ void launch_restore_pixels(hipStream_t s, int32_t *pix, int64_t n, int32_t dummy
 void launch_scalars(hipStream_t s, double *scal);
 void launch_fprecond(hipStream_t s, const double *fcov, int64_t nh, const double *wpsd, int nshort, double *out,
                      int64_t npsd);
+
+// inner-CG preconditioner pieces (segments x per-segment scalars; act[s] != 0 = active)
+void launch_cg_init(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *ic0, const double *r,
+                    const double *nna, double *x, double *resid, double *prec, double *zr, double *p);
+void launch_cg_dot(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *a, const double *b,
+                   double *partial, double *out);
+void launch_cg_nna(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *nna, const double *p,
+                   double *ap);
+void launch_cg_axpy(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *alpha, double *x,
+                    const double *p, double *resid, const double *ap);
+void launch_cg_prec(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *prec,
+                    const double *resid, double *zr);
+void launch_cg_p(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *beta, const double *zr,
+                 double *p);
 
 // ------------------------------------------------------------------ fft.hip
 

@@ -64,6 +64,20 @@ __device__ inline int64_t find_baseline(const int64_t *bstart, int64_t nbase, in
     return lo;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Remap so that XCD x owns a contiguous range of logical blocks: consecutive
+// work (the same ring of the scan, or neighbouring nested pixels) then shares
+// one L2 instead of being spread over eight.
+__device__ inline int64_t xcd_block(int64_t bid, int64_t nblk)
+{
+    const int64_t NX = 8;
+    int64_t per = nblk / NX;
+    if (bid >= per * NX) return bid;
+    return (bid % NX) * per + bid / NX;
+}
+
+static constexpr int ITEMS_PER_BLOCK = 64;
+
 template <typename T>
 __device__ inline T wave_sum(T v)
 {
@@ -665,10 +679,11 @@ __global__ void k_init_a(int nmap, int ndet, int64_t nsamp, int64_t nbase, const
                          double *nna)
 {
     int lane = threadIdx.x & (G - 1);
-    int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
-    int64_t ngrp = ((int64_t)gridDim.x * blockDim.x) / G;
+    int gib = threadIdx.x / G, ngib = blockDim.x / G;
     int64_t nbt = (int64_t)ndet * nbase;
-    for (int64_t b = grp; b < nbt; b += ngrp) {
+    int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    int64_t b1 = (lb + 1) * ITEMS_PER_BLOCK < nbt ? (lb + 1) * ITEMS_PER_BLOCK : nbt;
+    for (int64_t b = lb * ITEMS_PER_BLOCK + gib; b < b1; b += ngib) {
         int det = (int)(b / nbase);
         int64_t k = b - (int64_t)det * nbase;
         int64_t i0 = bstart[k], i1 = bstart[k + 1];
@@ -710,7 +725,7 @@ void launch_init_a(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nba
                    const PixW *pixw, const double *tod, const double *m4, int64_t ndeg, const double *dwb,
                    double *yba, double *nna, int group)
 {
-    int grid = grid_for((int64_t)ndet * nbase, TPB / group, 1 << 20);
+    int grid = (int)(((int64_t)ndet * nbase + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
     DISPATCH_G(group, k_init_a, nmap, ndet, nsamp, nbase, bstart, pixw, tod, m4, ndeg, dwb, yba, nna);
 }
 
@@ -721,10 +736,11 @@ __global__ void k_gather_baselines(int nmap, int ndet, int64_t nsamp, int64_t nb
                                    const double *nna, const double *p, double *ap)
 {
     int lane = threadIdx.x & (G - 1);
-    int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
-    int64_t ngrp = ((int64_t)gridDim.x * blockDim.x) / G;
+    int gib = threadIdx.x / G, ngib = blockDim.x / G;
     int64_t nbt = (int64_t)ndet * nbase;
-    for (int64_t b = grp; b < nbt; b += ngrp) {
+    int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    int64_t b1 = (lb + 1) * ITEMS_PER_BLOCK < nbt ? (lb + 1) * ITEMS_PER_BLOCK : nbt;
+    for (int64_t b = lb * ITEMS_PER_BLOCK + gib; b < b1; b += ngib) {
         int det = (int)(b / nbase);
         int64_t k = b - (int64_t)det * nbase;
         int64_t i0 = bstart[k], i1 = bstart[k + 1];
@@ -749,7 +765,7 @@ void launch_gather_baselines(hipStream_t s, int nmap, int ndet, int64_t nsamp, i
                              const PixW *pixw, const double *m4, int64_t ndeg, const double *dwb, const double *nna,
                              const double *p, double *ap, int group)
 {
-    int grid = grid_for((int64_t)ndet * nbase, TPB / group, 1 << 20);
+    int grid = (int)(((int64_t)ndet * nbase + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
     DISPATCH_G(group, k_gather_baselines, nmap, ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p, ap);
 }
 
@@ -758,9 +774,10 @@ __global__ void k_scatter_pixels(int nmap, int64_t npc, int64_t ndeg, const int6
                                  const double *pw, const double *ccinv6, double *m4)
 {
     int lane = threadIdx.x & 63;
-    int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-    int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t c = wave; c < npc; c += nwaves) {
+    int wib = threadIdx.x >> 6, nwib = blockDim.x >> 6;
+    int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    int64_t c1 = (lb + 1) * ITEMS_PER_BLOCK < npc ? (lb + 1) * ITEMS_PER_BLOCK : npc;
+    for (int64_t c = lb * ITEMS_PER_BLOCK + wib; c < c1; c += nwib) {
         int64_t j0 = off[c * ndeg], j1 = off[(c + 1) * ndeg];
         double a0 = 0, a1 = 0, a2 = 0;
         for (int64_t j = j0 + lane; j < j1; j += 64) {
@@ -788,8 +805,139 @@ __global__ void k_scatter_pixels(int nmap, int64_t npc, int64_t ndeg, const int6
 void launch_scatter_pixels(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
                            const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4)
 {
-    k_scatter_pixels<<<grid_for(npix_cross, TPB / 64, 1 << 20), TPB, 0, s>>>(nmap, npix_cross, ndeg, off, ent, pw,
-                                                                            ccinv6, m4);
+    int grid = (int)((npix_cross + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
+    k_scatter_pixels<<<grid, TPB, 0, s>>>(nmap, npix_cross, ndeg, off, ent, pw, ccinv6, m4);
+}
+
+// ---------------------------------------------------------------- flat tiles
+// Flat variants of the two per-iteration projections: every thread of a block
+// issues coalesced loads for a tile of the stream (maximum memory-level
+// parallelism), the per-sample products go to LDS, and a segmented reduction
+// from LDS produces one value per baseline (or pixel).  Deterministic for a
+// fixed CSR.  A block owns whole baselines (pixels), so no atomics are needed.
+
+static constexpr int FLAT_TILE = 2048;
+static constexpr int FLAT_ITEMS = 64;    // baselines / pixels per block
+static constexpr int FLAT_LANES = 4;     // lanes per item in the reduction phase
+
+// global sample index of the start of global baseline b
+__device__ inline int64_t base_gstart(const int64_t *bstart, int64_t nbase, int64_t nsamp, int64_t b)
+{
+    int64_t det = b / nbase, k = b - det * nbase;
+    return det * nsamp + bstart[k];
+}
+
+__global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t nsamp, int64_t nbase,
+                                                     const int64_t *bstart, const PixW *pixw, const double *m4,
+                                                     int64_t ndeg, const double *dwb, const double *nna,
+                                                     const double *p, double *ap)
+{
+    __shared__ double vals[FLAT_TILE];
+    __shared__ int64_t bs[FLAT_ITEMS + 1];   // global sample starts of the block's baselines, bs[n] = end
+    int64_t nbt = (int64_t)ndet * nbase;
+    int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    int64_t b0 = lb * FLAT_ITEMS, b1 = b0 + FLAT_ITEMS < nbt ? b0 + FLAT_ITEMS : nbt;
+    int nb = (int)(b1 - b0);
+    if (nb <= 0) return;
+    for (int t = threadIdx.x; t <= nb; t += blockDim.x) {
+        int64_t b = b0 + t;
+        bs[t] = t < nb ? base_gstart(bstart, nbase, nsamp, b)
+                       : base_gstart(bstart, nbase, nsamp, b - 1) + (bstart[(b - 1) % nbase + 1] - bstart[(b - 1) % nbase]);
+    }
+    __syncthreads();
+    int item = threadIdx.x / FLAT_LANES, sub = threadIdx.x % FLAT_LANES;
+    double acc = 0.0;
+    int64_t S0 = bs[0], S1 = bs[nb];
+    for (int64_t t0 = S0; t0 < S1; t0 += FLAT_TILE) {
+        int64_t t1 = t0 + FLAT_TILE < S1 ? t0 + FLAT_TILE : S1;
+        for (int64_t g = t0 + threadIdx.x; g < t1; g += blockDim.x) {
+            PixW r = pixw[g];
+            double v = 0.0;
+            if (r.pix >= 0) {
+                double4 m = *(const double4 *)(m4 + (r.pix / ndeg) * 4);
+                v = nmap == 1 ? m.x : (double)r.w0 * m.x + (double)r.w1 * m.y + (double)r.w2 * m.z;
+            }
+            vals[g - t0] = v;
+        }
+        __syncthreads();
+        if (item < nb) {
+            int64_t a = bs[item] > t0 ? bs[item] : t0, e = bs[item + 1] < t1 ? bs[item + 1] : t1;
+            for (int64_t g = a + sub; g < e; g += FLAT_LANES) acc += vals[g - t0];
+        }
+        __syncthreads();
+    }
+    acc = group_sum<FLAT_LANES>(acc);
+    if (item < nb && sub == 0) {
+        int64_t b = b0 + item;
+        double v = nna[b] * p[b] - acc * dwb[b];
+        ap[b] = ap[b] + v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scatter_flat(int nmap, int64_t npc, int64_t ndeg, const int64_t *off,
+                                                      const CsrEnt *ent, const double *pw, const double *ccinv6,
+                                                      double *m4)
+{
+    __shared__ double vals[3][FLAT_TILE];
+    __shared__ int64_t qs[FLAT_ITEMS + 1];
+    int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    int64_t c0 = lb * FLAT_ITEMS, c1 = c0 + FLAT_ITEMS < npc ? c0 + FLAT_ITEMS : npc;
+    int nc = (int)(c1 - c0);
+    if (nc <= 0) return;
+    for (int t = threadIdx.x; t <= nc; t += blockDim.x) qs[t] = off[(c0 + t) * ndeg];
+    __syncthreads();
+    int item = threadIdx.x / FLAT_LANES, sub = threadIdx.x % FLAT_LANES;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    int64_t S0 = qs[0], S1 = qs[nc];
+    for (int64_t t0 = S0; t0 < S1; t0 += FLAT_TILE) {
+        int64_t t1 = t0 + FLAT_TILE < S1 ? t0 + FLAT_TILE : S1;
+        for (int64_t j = t0 + threadIdx.x; j < t1; j += blockDim.x) {
+            CsrEnt e = ent[j];
+            double v = pw[e.b];
+            int64_t o = j - t0;
+            if (nmap == 1) vals[0][o] = v;
+            else {
+                vals[0][o] = (double)e.w0 * v;
+                vals[1][o] = (double)e.w1 * v;
+                vals[2][o] = (double)e.w2 * v;
+            }
+        }
+        __syncthreads();
+        if (item < nc) {
+            int64_t a = qs[item] > t0 ? qs[item] : t0, e = qs[item + 1] < t1 ? qs[item + 1] : t1;
+            for (int64_t j = a + sub; j < e; j += FLAT_LANES) {
+                a0 += vals[0][j - t0];
+                if (nmap == 3) { a1 += vals[1][j - t0]; a2 += vals[2][j - t0]; }
+            }
+        }
+        __syncthreads();
+    }
+    a0 = group_sum<FLAT_LANES>(a0);
+    a1 = group_sum<FLAT_LANES>(a1);
+    a2 = group_sum<FLAT_LANES>(a2);
+    if (item < nc && sub == 0) {
+        int64_t c = c0 + item;
+        double m[3] = {a0, a1, a2}, o[3];
+        ccmul(nmap, ccinv6 + c * 6, m, o);
+        double4 r;
+        r.x = o[0]; r.y = o[1]; r.z = o[2]; r.w = 0.0;
+        *(double4 *)(m4 + c * 4) = r;
+    }
+}
+
+void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
+                        const PixW *pixw, const double *m4, int64_t ndeg, const double *dwb, const double *nna,
+                        const double *p, double *ap)
+{
+    int64_t grid = ((int64_t)ndet * nbase + FLAT_ITEMS - 1) / FLAT_ITEMS;
+    k_gather_flat<<<(unsigned)grid, 256, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p, ap);
+}
+
+void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
+                         const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4)
+{
+    int64_t grid = (npix_cross + FLAT_ITEMS - 1) / FLAT_ITEMS;
+    k_scatter_flat<<<(unsigned)grid, 256, 0, s>>>(nmap, npix_cross, ndeg, off, ent, pw, ccinv6, m4);
 }
 
 // subtract_baselines_a: rhs4 (binned P^T W d at map resolution) += -P^T W F a
@@ -869,6 +1017,27 @@ __global__ void k_find_segments(const double *nna, int ndet, int64_t nbase, cons
             int64_t k = base + lane;
             bool nz = k < k1 && nn[k] != 0.0;
             unsigned long long m = __ballot(nz);
+            if (m == 0) continue;
+            if (gapmin > 63 && start >= 0) {
+                // no gap of gapmin zeros fits inside one 64-wide word: only the
+                // run from the previous nonzero to the first set bit can split
+                int first = __ffsll((long long)m) - 1;
+                int64_t kk = base + first;
+                if (kk - last - 1 >= gapmin) {
+                    if (lane == 0) {
+                        unsigned long long pos = atomicAdd(nseg, 1ull);
+                        if ((int64_t)pos < cap) {
+                            Segment sg;
+                            sg.b0 = (int64_t)det * nbase + start; sg.noba = (int32_t)(last - start + 1);
+                            sg.det = det; sg.ival = iv; sg.ipsd = -1; sg.skip = 0; sg.pad = 0;
+                            out[pos] = sg;
+                        }
+                    }
+                    start = kk;
+                }
+                last = base + 63 - __clzll((long long)m);
+                continue;
+            }
             while (m) {
                 int bit = __ffsll((long long)m) - 1;
                 m &= m - 1;
@@ -1167,6 +1336,149 @@ void launch_fprecond(hipStream_t s, const double *fcov, int64_t nh, const double
                      int64_t npsd)
 {
     k_fprecond<<<grid_for(nh * npsd, TPB), TPB, 0, s>>>(fcov, nh, wpsd, nshort, out, npsd);
+}
+
+}  // namespace madam
+
+// ---------------------------------------------------------------- inner-CG preconditioner
+// apply_CG_preconditioner (src/noise_routines.f90:1305-1409), batched over the
+// segments whose band Cholesky was not available; per-segment scalars.
+namespace madam {
+
+static constexpr int CG_NB = 32;
+
+// x = 0, resid = r, prec = 1 / (nna + invcov(0)) where nonzero, zr = prec resid, p = zr
+__global__ void k_cg_init(const Segment *seg, const int *act, const double *ic0, const double *r, const double *nna,
+                          double *x, double *resid, double *prec, double *zr, double *p)
+{
+    int64_t s = blockIdx.y;
+    if (!act[s]) return;
+    Segment g = seg[s];
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < g.noba; j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t b = g.b0 + j;
+        x[b] = 0.0;
+        double rs = r[b] - 0.0;
+        resid[b] = rs;
+        double pc = nna[b] + ic0[s];
+        if (pc != 0) pc = 1 / pc;
+        prec[b] = pc;
+        double z = pc * rs;
+        zr[b] = z;
+        p[b] = z;
+    }
+}
+
+// partial[s * CG_NB + blk] = sum a * b over segment s
+__global__ void k_cg_dot(const Segment *seg, const int *act, const double *a, const double *bv, double *partial)
+{
+    int64_t s = blockIdx.y;
+    if (!act[s]) return;
+    Segment g = seg[s];
+    double acc = 0;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < g.noba; j += (int64_t)CG_NB * blockDim.x)
+        acc += a[g.b0 + j] * bv[g.b0 + j];
+    acc = wave_sum(acc);
+    __shared__ double sh[TPB / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < TPB / 64; ++k) t += sh[k];
+        partial[s * CG_NB + blockIdx.x] = t;
+    }
+}
+
+__global__ void k_cg_dot_final(const double *partial, const int *act, int64_t nseg, double *out)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nseg || !act[s]) return;
+    double t = 0;
+    for (int k = 0; k < CG_NB; ++k) t += partial[s * CG_NB + k];
+    out[s] = t;
+}
+
+// ap += nna * p
+__global__ void k_cg_nna(const Segment *seg, const int *act, const double *nna, const double *p, double *ap)
+{
+    int64_t s = blockIdx.y;
+    if (!act[s]) return;
+    Segment g = seg[s];
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < g.noba; j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t b = g.b0 + j;
+        ap[b] = ap[b] + nna[b] * p[b];
+    }
+}
+
+// x += alpha p; resid -= alpha ap
+__global__ void k_cg_axpy(const Segment *seg, const int *act, const double *alpha, double *x, const double *p,
+                          double *resid, const double *ap)
+{
+    int64_t s = blockIdx.y;
+    if (!act[s]) return;
+    Segment g = seg[s];
+    double al = alpha[s];
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < g.noba; j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t b = g.b0 + j;
+        x[b] = x[b] + al * p[b];
+        resid[b] = resid[b] - al * ap[b];
+    }
+}
+
+// zr = prec * resid
+__global__ void k_cg_prec(const Segment *seg, const int *act, const double *prec, const double *resid, double *zr)
+{
+    int64_t s = blockIdx.y;
+    if (!act[s]) return;
+    Segment g = seg[s];
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < g.noba; j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t b = g.b0 + j;
+        zr[b] = prec[b] * resid[b];
+    }
+}
+
+// p = zr + beta p
+__global__ void k_cg_p(const Segment *seg, const int *act, const double *beta, const double *zr, double *p)
+{
+    int64_t s = blockIdx.y;
+    if (!act[s]) return;
+    Segment g = seg[s];
+    double bt = beta[s];
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < g.noba; j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t b = g.b0 + j;
+        p[b] = zr[b] + bt * p[b];
+    }
+}
+
+void launch_cg_init(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *ic0, const double *r,
+                    const double *nna, double *x, double *resid, double *prec, double *zr, double *p)
+{
+    k_cg_init<<<dim3(64, (unsigned)nseg), TPB, 0, s>>>(seg, act, ic0, r, nna, x, resid, prec, zr, p);
+}
+void launch_cg_dot(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *a, const double *b,
+                   double *partial, double *out)
+{
+    k_cg_dot<<<dim3(CG_NB, (unsigned)nseg), TPB, 0, s>>>(seg, act, a, b, partial);
+    k_cg_dot_final<<<(unsigned)((nseg + 63) / 64), 64, 0, s>>>(partial, act, nseg, out);
+}
+void launch_cg_nna(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *nna, const double *p,
+                   double *ap)
+{
+    k_cg_nna<<<dim3(64, (unsigned)nseg), TPB, 0, s>>>(seg, act, nna, p, ap);
+}
+void launch_cg_axpy(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *alpha, double *x,
+                    const double *p, double *resid, const double *ap)
+{
+    k_cg_axpy<<<dim3(64, (unsigned)nseg), TPB, 0, s>>>(seg, act, alpha, x, p, resid, ap);
+}
+void launch_cg_prec(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *prec,
+                    const double *resid, double *zr)
+{
+    k_cg_prec<<<dim3(64, (unsigned)nseg), TPB, 0, s>>>(seg, act, prec, resid, zr);
+}
+void launch_cg_p(hipStream_t s, const Segment *seg, int64_t nseg, const int *act, const double *beta, const double *zr,
+                 double *p)
+{
+    k_cg_p<<<dim3(64, (unsigned)nseg), TPB, 0, s>>>(seg, act, beta, zr, p);
 }
 
 }  // namespace madam

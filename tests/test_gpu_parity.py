@@ -115,6 +115,16 @@ def test_noise_prior_band_preconditioner():
     check_parity(g, o, c1, c2)
 
 
+def test_noise_prior_band_preconditioner_group_kernels():
+    """the alternative projection kernels (lane group per baseline / wave per pixel)"""
+    import libmadam_amd
+    case = ring(prior=True, nsamp=30000, nperiod=2)
+    c1, c2 = mc.copy_case(case), mc.copy_case(case)
+    g = libmadam_amd.run(*mc.call_args(c1), kernel_variant=1)
+    o = oracle.destripe(*mc.call_args(c2))
+    check_parity(g, o, c1, c2)
+
+
 def test_noise_prior_fourier_preconditioner():
     g, o, c1, c2 = both(ring(prior=True, nsamp=30000, nperiod=2, extra=dict(use_fprecond=True, iter_max=30)))
     assert g["stats"]["precond_kind"] == 2
@@ -157,3 +167,46 @@ def test_wrapper_drop_in_mutates_like_reference():
         madam.destripe(None, dict(c3["pars"], bogus=1), c3["dets"], c3["weights"], c3["timestamps"], c3["pixels"],
                        c3["pixweights"], c3["signal"], c3["periods"], c3["npsd"], c3["psdstarts"], c3["psdfreqs"],
                        c3["psdvals"])
+
+
+def test_reference_case_with_noise_filter():
+    """The reference's own test configuration (kfilter=T, flat PSDs, nside_cross <
+    nside_map, nsubchunk=2, surveys): the band criterion fails and the inner-CG
+    preconditioner is used (src/noise_routines.f90:1113-1120, 1305-1409)."""
+    case, gold = mc.reference_test_case()
+    g, o, c1, c2 = both(case)
+    assert g["stats"]["ncg"] > 0
+    assert np.array_equal(g["hits"], gold["hmap"])
+    check_parity(g, o, c1, c2)
+
+
+def test_inner_cg_preconditioner_forced():
+    g, o, c1, c2 = both(ring(prior=True, nsamp=20000, nperiod=2, extra=dict(use_cgprecond=True)))
+    assert g["stats"]["ncg"] > 0 and g["stats"]["nband"] == 0
+    check_parity(g, o, c1, c2)
+
+
+def test_mc_mode_cache_cycle():
+    """destripe(mcmode=T) -> destripe_with_cache(new signal) -> clear_caches, as in
+    python/tests/test_libmadam_mc.py:132-191; the cached solve equals a fresh
+    solve of the new signal (clean right-hand side)."""
+    import os
+    from libmadam_amd import libmadam_wrapper as madam
+    os.environ["MADAM_RAISE"] = "1"
+    case = ring(prior=True, nsamp=20000, extra=dict(mcmode=True, write_tod=True))
+    c1 = mc.copy_case(case)
+    madam.destripe(None, c1["pars"], c1["dets"], c1["weights"], c1["timestamps"], c1["pixels"], c1["pixweights"],
+                   c1["signal"], c1["periods"], c1["npsd"], c1["psdstarts"], c1["psdfreqs"], c1["psdvals"])
+    rng = np.random.default_rng(3)
+    sig2 = case["signal"] + rng.standard_normal(case["signal"].size) * 1e-4
+    c2 = mc.copy_case(case)
+    c2["signal"] = sig2.copy()
+    madam.destripe_with_cache(None, c2["timestamps"], c2["pixels"], c2["pixweights"], c2["signal"], "unused")
+    madam.clear_caches()
+    madam.clear_caches()   # a second call only warns
+    c3 = mc.copy_case(case)
+    c3["signal"] = sig2.copy()
+    c3["pars"] = dict(case["pars"], mcmode=False)
+    oracle.destripe(*mc.call_args(c3))
+    assert rel_rms(c2["signal"], c3["signal"]) < RTOL_RMS
+    assert np.array_equal(c2["pixels"], c3["pixels"])
