@@ -328,6 +328,7 @@ struct Context {
     double *d_tod = nullptr;
     DBuf<double> d_yba, d_nna, d_aa, d_r, d_ro, d_z, d_p, d_pw, d_ap, d_prec, d_scal, d_part;
     DBuf<int8_t> d_rmask;
+    DBuf<char> d_tilescratch;
     // segments / prior
     std::vector<Segment> seg;
     DBuf<Segment> d_seg;
@@ -1121,8 +1122,8 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         c.pcg_steps_done++;
         if (prof) { t_it.start(s); t_sc.start(s); }
         if (c.kernel_variant == 0)
-            launch_scatter_flat(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
-                                c.d_m4x.p);
+            launch_scatter_tiles(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_ent.p, c.d_pw.p,
+                                 c.d_ccinv6x.p, c.d_m4x.p, c.d_tilescratch.p);
         else
             launch_scatter_pixels(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
                                   c.d_m4x.p);
@@ -1218,6 +1219,7 @@ static void solve(Context &c, double *signal, int on_device)
                       p.incomplete_matrices, p.allow_decoupling, nullptr);
         c.d_m4x.alloc(c.npix_cross * 4);
         launch_ccmultiply(s, c.nmap, c.npix_cross, c.d_ccinv6x.p, rhs4x, c.d_m4x.p);
+        c.d_tilescratch.alloc(scatter_tiles_scratch_bytes(c.nactive));
         c.d_yba.alloc(c.nbtot);
         c.d_nna.alloc(c.nbtot);
         int grp = group_for(c);

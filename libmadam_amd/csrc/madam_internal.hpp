@@ -92,6 +92,9 @@ void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_
                         const double *p, double *ap);
 void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4);
+void launch_scatter_tiles(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off, int64_t nent,
+                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4, void *scratch);
+size_t scatter_tiles_scratch_bytes(int64_t nent);
 void launch_subtract_baselines(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent,
                                const double *aa, const double *dwb, double *rhs4);
 void launch_clean_tod(hipStream_t s, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,

@@ -925,6 +925,126 @@ __global__ void __launch_bounds__(256) k_scatter_flat(int nmap, int64_t npc, int
     }
 }
 
+// Entry-balanced scatter: block j reduces CSR entries [j T, (j+1) T) (T =
+// FLAT_TILE).  Pixels wholly inside the tile are finished in place; the
+// partial sums of the first / last pixel of the tile, when they continue into a
+// neighbouring tile, go to part[j][0] / part[j][1] and are combined in tile
+// order by k_scatter_fixup.  Every block does the same amount of work however
+// uneven the hit counts are (the scan's pole pixels hold tens of thousands of
+// samples).
+struct TileInfo {
+    int64_t qa, qb;   // first and last cross pixel touching the tile
+};
+
+__device__ inline int64_t cross_of_entry(const int64_t *off, int64_t ndeg, int64_t npc, int64_t j)
+{
+    // largest c with off[c * ndeg] <= j
+    int64_t lo = 0, hi = npc - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi + 1) >> 1;
+        if (off[mid * ndeg] <= j) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_scatter_tiles(int nmap, int64_t npc, int64_t ndeg, const int64_t *off,
+                                                       int64_t nent, const CsrEnt *ent, const double *pw,
+                                                       const double *ccinv6, double *m4, double4 *part,
+                                                       TileInfo *tinfo)
+{
+    __shared__ double vals[3][FLAT_TILE];
+    __shared__ int64_t s_qa, s_qb;
+    int64_t j = xcd_block(blockIdx.x, gridDim.x);
+    int64_t t0 = j * FLAT_TILE, t1 = t0 + FLAT_TILE < nent ? t0 + FLAT_TILE : nent;
+    if (t0 >= nent) return;
+    if (threadIdx.x == 0) s_qa = cross_of_entry(off, ndeg, npc, t0);
+    if (threadIdx.x == 1) s_qb = cross_of_entry(off, ndeg, npc, t1 - 1);
+    for (int64_t jj = t0 + threadIdx.x; jj < t1; jj += blockDim.x) {
+        CsrEnt e = ent[jj];
+        double v = pw[e.b];
+        int64_t o = jj - t0;
+        if (nmap == 1) vals[0][o] = v;
+        else {
+            vals[0][o] = (double)e.w0 * v;
+            vals[1][o] = (double)e.w1 * v;
+            vals[2][o] = (double)e.w2 * v;
+        }
+    }
+    __syncthreads();
+    int64_t qa = s_qa, qb = s_qb;
+    if (threadIdx.x == 0) { tinfo[j].qa = qa; tinfo[j].qb = qb; }
+    int item = threadIdx.x / FLAT_LANES, sub = threadIdx.x % FLAT_LANES;
+    for (int64_t q = qa + item; q <= qb; q += blockDim.x / FLAT_LANES) {
+        int64_t a = off[q * ndeg], e = off[(q + 1) * ndeg];
+        int64_t lo = a > t0 ? a : t0, hi = e < t1 ? e : t1;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int64_t jj = lo + sub; jj < hi; jj += FLAT_LANES) {
+            a0 += vals[0][jj - t0];
+            if (nmap == 3) { a1 += vals[1][jj - t0]; a2 += vals[2][jj - t0]; }
+        }
+        a0 = group_sum<FLAT_LANES>(a0);
+        a1 = group_sum<FLAT_LANES>(a1);
+        a2 = group_sum<FLAT_LANES>(a2);
+        if (sub == 0) {
+            if (a >= t0 && e <= t1) {
+                double m[3] = {a0, a1, a2}, o[3];
+                ccmul(nmap, ccinv6 + q * 6, m, o);
+                double4 r;
+                r.x = o[0]; r.y = o[1]; r.z = o[2]; r.w = 0.0;
+                *(double4 *)(m4 + q * 4) = r;
+            } else {
+                double4 r;
+                r.x = a0; r.y = a1; r.z = a2; r.w = 0.0;
+                part[j * 2 + (q == qa ? 0 : 1)] = r;
+            }
+        }
+    }
+}
+
+// one thread per tile: finish the pixel that starts in tile j and continues past it
+__global__ void k_scatter_fixup(int nmap, int64_t ntiles, int64_t ndeg, const int64_t *off, const double4 *part,
+                                const TileInfo *tinfo, const double *ccinv6, double *m4)
+{
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= ntiles) return;
+    int64_t t0 = j * FLAT_TILE, t1 = t0 + FLAT_TILE;
+    TileInfo ti = tinfo[j];
+    int64_t q = ti.qb;
+    int64_t a = off[q * ndeg], e = off[(q + 1) * ndeg];
+    if (!(a >= t0 && e > t1)) return;   // q must start in this tile and continue
+    double4 acc = part[j * 2 + (q == ti.qa ? 0 : 1)];
+    for (int64_t jj = j + 1; jj < ntiles; ++jj) {
+        double4 v = part[jj * 2 + 0];   // q is the first pixel of every later tile it reaches
+        acc.x += v.x; acc.y += v.y; acc.z += v.z;
+        if (e <= (jj + 1) * FLAT_TILE) break;
+    }
+    double m[3] = {acc.x, acc.y, acc.z}, o[3];
+    ccmul(nmap, ccinv6 + q * 6, m, o);
+    double4 r;
+    r.x = o[0]; r.y = o[1]; r.z = o[2]; r.w = 0.0;
+    *(double4 *)(m4 + q * 4) = r;
+}
+
+void launch_scatter_tiles(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off, int64_t nent,
+                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4, void *scratch)
+{
+    int64_t ntiles = (nent + FLAT_TILE - 1) / FLAT_TILE;
+    if (ntiles == 0) return;
+    double4 *part = (double4 *)scratch;
+    TileInfo *tinfo = (TileInfo *)(part + 2 * ntiles);
+    k_scatter_tiles<<<(unsigned)ntiles, 256, 0, s>>>(nmap, npix_cross, ndeg, off, nent, ent, pw, ccinv6, m4, part,
+                                                     tinfo);
+    k_scatter_fixup<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(nmap, ntiles, ndeg, off, part, tinfo, ccinv6,
+                                                                      m4);
+}
+
+size_t scatter_tiles_scratch_bytes(int64_t nent)
+{
+    int64_t ntiles = (nent + FLAT_TILE - 1) / FLAT_TILE;
+    return (size_t)ntiles * (2 * sizeof(double4) + sizeof(TileInfo)) + 256;
+}
+
 void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
                         const PixW *pixw, const double *m4, int64_t ndeg, const double *dwb, const double *nna,
                         const double *p, double *ap)
