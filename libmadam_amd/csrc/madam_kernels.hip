@@ -850,14 +850,24 @@ __global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t
     int64_t S0 = bs[0], S1 = bs[nb];
     for (int64_t t0 = S0; t0 < S1; t0 += FLAT_TILE) {
         int64_t t1 = t0 + FLAT_TILE < S1 ? t0 + FLAT_TILE : S1;
-        for (int64_t g = t0 + threadIdx.x; g < t1; g += blockDim.x) {
-            PixW r = pixw[g];
+        constexpr int PER = FLAT_TILE / 256;
+        PixW r[PER];
+        double4 m[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            int64_t g = t0 + u * 256 + threadIdx.x;
+            if (g < t1) r[u] = pixw[g];
+            else r[u].pix = -1;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u)
+            if (r[u].pix >= 0) m[u] = *(const double4 *)(m4 + (r[u].pix / ndeg) * 4);
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
             double v = 0.0;
-            if (r.pix >= 0) {
-                double4 m = *(const double4 *)(m4 + (r.pix / ndeg) * 4);
-                v = nmap == 1 ? m.x : (double)r.w0 * m.x + (double)r.w1 * m.y + (double)r.w2 * m.z;
-            }
-            vals[g - t0] = v;
+            if (r[u].pix >= 0)
+                v = nmap == 1 ? m[u].x : (double)r[u].w0 * m[u].x + (double)r[u].w1 * m[u].y + (double)r[u].w2 * m[u].z;
+            vals[u * 256 + threadIdx.x] = v;
         }
         __syncthreads();
         if (item < nb) {
@@ -960,15 +970,28 @@ __global__ void __launch_bounds__(256) k_scatter_tiles(int nmap, int64_t npc, in
     if (t0 >= nent) return;
     if (threadIdx.x == 0) s_qa = cross_of_entry(off, ndeg, npc, t0);
     if (threadIdx.x == 1) s_qb = cross_of_entry(off, ndeg, npc, t1 - 1);
-    for (int64_t jj = t0 + threadIdx.x; jj < t1; jj += blockDim.x) {
-        CsrEnt e = ent[jj];
-        double v = pw[e.b];
-        int64_t o = jj - t0;
-        if (nmap == 1) vals[0][o] = v;
-        else {
-            vals[0][o] = (double)e.w0 * v;
-            vals[1][o] = (double)e.w1 * v;
-            vals[2][o] = (double)e.w2 * v;
+    {
+        // all stream loads first, then all gathers: FLAT_TILE / 256 independent requests per thread
+        constexpr int PER = FLAT_TILE / 256;
+        CsrEnt e[PER];
+        double v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            int64_t jj = t0 + u * 256 + threadIdx.x;
+            if (jj < t1) e[u] = ent[jj];
+            else { e[u].b = -1; e[u].w0 = e[u].w1 = e[u].w2 = 0.0f; }
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) v[u] = e[u].b >= 0 ? pw[e[u].b] : 0.0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            int o = u * 256 + threadIdx.x;
+            if (nmap == 1) vals[0][o] = v[u];
+            else {
+                vals[0][o] = (double)e[u].w0 * v[u];
+                vals[1][o] = (double)e[u].w1 * v[u];
+                vals[2][o] = (double)e[u].w2 * v[u];
+            }
         }
     }
     __syncthreads();
