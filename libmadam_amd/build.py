@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmadam.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["madam_kernels.hip", "madam_fft.hip", "madam_band.hip", "madam_pointing.hip", "madam_host.cpp"]
+SOURCES = ["madam_kernels.hip", "madam_fft.hip", "madam_band.hip", "madam_pointing.hip", "madam_csr.hip",
+           "madam_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MADAM_OFFLOAD_ARCH", "gfx950")
 

@@ -314,7 +314,7 @@ struct Context {
     float *d_w = nullptr;
     DBuf<PixW> d_pixw;
     DBuf<int32_t> d_hits;
-    DBuf<int64_t> d_off, d_cursor, d_scan_tmp;
+    DBuf<int64_t> d_off, d_scan_tmp;
     DBuf<CsrEnt> d_ent;
     DBuf<uint32_t> d_src;
     int64_t nactive = 0;
@@ -328,7 +328,7 @@ struct Context {
     double *d_tod = nullptr;
     DBuf<double> d_yba, d_nna, d_aa, d_r, d_ro, d_z, d_p, d_pw, d_ap, d_prec, d_scal, d_part;
     DBuf<int8_t> d_rmask;
-    DBuf<char> d_tilescratch;
+    DBuf<char> d_tilescratch, d_tileinfo;
     // segments / prior
     std::vector<Segment> seg;
     DBuf<Segment> d_seg;
@@ -786,11 +786,9 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
     d2h(&total, c.d_off.p + c.npix_map, 1, s);
     MADAM_CHECK(hipStreamSynchronize(s));
     c.nactive = total;
-    c.d_cursor.alloc(c.npix_map + 1);
-    MADAM_CHECK(hipMemcpyAsync(c.d_cursor.p, c.d_off.p, sizeof(int64_t) * (c.npix_map + 1), hipMemcpyDeviceToDevice, s));
     c.d_ent.alloc(total > 0 ? total : 1);
     c.d_src.alloc(total > 0 ? total : 1);
-    launch_place(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.d_cursor.p, c.d_ent.p, c.d_src.p);
+    csr_build_sorted(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.npix_map, total, c.d_ent.p, c.d_src.p);
     MADAM_CHECK(hipGetLastError());
     MADAM_CHECK(hipStreamSynchronize(s));
     c.stats[ST_T_POINTING] = now() - t0;
@@ -1123,7 +1121,7 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         if (prof) { t_it.start(s); t_sc.start(s); }
         if (c.kernel_variant == 0)
             launch_scatter_tiles(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_ent.p, c.d_pw.p,
-                                 c.d_ccinv6x.p, c.d_m4x.p, c.d_tilescratch.p);
+                                 c.d_ccinv6x.p, c.d_m4x.p, c.d_tileinfo.p, c.d_tilescratch.p);
         else
             launch_scatter_pixels(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.d_ent.p, c.d_pw.p, c.d_ccinv6x.p,
                                   c.d_m4x.p);
@@ -1220,6 +1218,8 @@ static void solve(Context &c, double *signal, int on_device)
         c.d_m4x.alloc(c.npix_cross * 4);
         launch_ccmultiply(s, c.nmap, c.npix_cross, c.d_ccinv6x.p, rhs4x, c.d_m4x.p);
         c.d_tilescratch.alloc(scatter_tiles_scratch_bytes(c.nactive));
+        c.d_tileinfo.alloc(scatter_tiles_info_bytes(c.nactive));
+        launch_tile_info(s, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_tileinfo.p);
         c.d_yba.alloc(c.nbtot);
         c.d_nna.alloc(c.nbtot);
         int grp = group_for(c);

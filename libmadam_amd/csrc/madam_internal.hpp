@@ -69,8 +69,6 @@ void launch_pack(hipStream_t s, const int32_t *pix, const float *w, int nnz, int
 void launch_hist(hipStream_t s, const PixW *pixw, uint64_t n, int32_t *counts);
 void launch_exclusive_scan_i32_to_i64(hipStream_t s, const int32_t *in, int64_t n, int64_t *out, int64_t *tmp,
                                       int64_t tmp_len);
-void launch_place(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
-                  int64_t *cursor, CsrEnt *ent, uint32_t *src);
 void launch_bin(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent, const uint32_t *src,
                 const double *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4);
 void launch_degrade(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const double *cc6, const double *rhs4,
@@ -92,9 +90,17 @@ void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_
                         const double *p, double *ap);
 void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4);
+// entry-balanced pixel-side projection: tinfo from launch_tile_info (once per
+// CSR), scratch of scatter_tiles_scratch_bytes for the cross-tile partial sums
 void launch_scatter_tiles(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off, int64_t nent,
-                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4, void *scratch);
+                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4, const void *tinfo,
+                          void *scratch);
 size_t scatter_tiles_scratch_bytes(int64_t nent);
+size_t scatter_tiles_info_bytes(int64_t nent);
+void launch_tile_info(hipStream_t s, int64_t npix_cross, int64_t ndeg, const int64_t *off, int64_t nent, void *tinfo);
+// stable regrouping of the samples by map pixel (off from the hit histogram)
+void csr_build_sorted(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
+                      int64_t npix_map, int64_t total, CsrEnt *ent, uint32_t *src);
 void launch_subtract_baselines(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent,
                                const double *aa, const double *dwb, double *rhs4);
 void launch_clean_tod(hipStream_t s, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,

@@ -303,32 +303,6 @@ void launch_exclusive_scan_i32_to_i64(hipStream_t s, const int32_t *in, int64_t 
     k_scan_final<<<(unsigned)nb, TPB, 0, s>>>(in, n, tmp, out);
 }
 
-__global__ void k_place(const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
-                        int64_t *cursor, CsrEnt *ent, uint32_t *src)
-{
-    int64_t n = (int64_t)ndet * nsamp;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
-        PixW r = pixw[g];
-        if (r.pix < 0) continue;
-        int det = (int)(g / nsamp);
-        int64_t k = find_baseline(bstart, nbase, g % nsamp);
-        unsigned long long pos = atomicAdd((unsigned long long *)&cursor[r.pix], 1ull);
-        CsrEnt e;
-        e.b = (int32_t)((int64_t)det * nbase + k);
-        e.w0 = r.w0;
-        e.w1 = r.w1;
-        e.w2 = r.w2;
-        ent[pos] = e;
-        src[pos] = (uint32_t)g;
-    }
-}
-
-void launch_place(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
-                  int64_t *cursor, CsrEnt *ent, uint32_t *src)
-{
-    k_place<<<grid_for((int64_t)ndet * nsamp, TPB), TPB, 0, s>>>(pixw, ndet, nsamp, bstart, nbase, cursor, ent, src);
-}
-
 // ---------------------------------------------------------------- binning
 
 // pixel_matrix + bin_tod, one wave per map pixel over its CSR range.
@@ -850,24 +824,14 @@ __global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t
     int64_t S0 = bs[0], S1 = bs[nb];
     for (int64_t t0 = S0; t0 < S1; t0 += FLAT_TILE) {
         int64_t t1 = t0 + FLAT_TILE < S1 ? t0 + FLAT_TILE : S1;
-        constexpr int PER = FLAT_TILE / 256;
-        PixW r[PER];
-        double4 m[PER];
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            int64_t g = t0 + u * 256 + threadIdx.x;
-            if (g < t1) r[u] = pixw[g];
-            else r[u].pix = -1;
-        }
-#pragma unroll
-        for (int u = 0; u < PER; ++u)
-            if (r[u].pix >= 0) m[u] = *(const double4 *)(m4 + (r[u].pix / ndeg) * 4);
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
+        for (int64_t g = t0 + threadIdx.x; g < t1; g += blockDim.x) {
+            PixW r = pixw[g];
             double v = 0.0;
-            if (r[u].pix >= 0)
-                v = nmap == 1 ? m[u].x : (double)r[u].w0 * m[u].x + (double)r[u].w1 * m[u].y + (double)r[u].w2 * m[u].z;
-            vals[u * 256 + threadIdx.x] = v;
+            if (r.pix >= 0) {
+                double4 m = *(const double4 *)(m4 + (r.pix / ndeg) * 4);
+                v = nmap == 1 ? m.x : (double)r.w0 * m.x + (double)r.w1 * m.y + (double)r.w2 * m.z;
+            }
+            vals[g - t0] = v;
         }
         __syncthreads();
         if (item < nb) {
@@ -958,45 +922,28 @@ __device__ inline int64_t cross_of_entry(const int64_t *off, int64_t ndeg, int64
     return lo;
 }
 
-__global__ void __launch_bounds__(256) k_scatter_tiles(int nmap, int64_t npc, int64_t ndeg, const int64_t *off,
-                                                       int64_t nent, const CsrEnt *ent, const double *pw,
-                                                       const double *ccinv6, double *m4, double4 *part,
-                                                       TileInfo *tinfo)
+__global__ void __launch_bounds__(256) k_scatter_tiles(int nmap, int64_t ndeg, const int64_t *off, int64_t nent,
+                                                       const CsrEnt *ent, const double *pw, const double *ccinv6,
+                                                       double *m4, double4 *part, const TileInfo *tinfo)
 {
     __shared__ double vals[3][FLAT_TILE];
-    __shared__ int64_t s_qa, s_qb;
     int64_t j = xcd_block(blockIdx.x, gridDim.x);
     int64_t t0 = j * FLAT_TILE, t1 = t0 + FLAT_TILE < nent ? t0 + FLAT_TILE : nent;
     if (t0 >= nent) return;
-    if (threadIdx.x == 0) s_qa = cross_of_entry(off, ndeg, npc, t0);
-    if (threadIdx.x == 1) s_qb = cross_of_entry(off, ndeg, npc, t1 - 1);
-    {
-        // all stream loads first, then all gathers: FLAT_TILE / 256 independent requests per thread
-        constexpr int PER = FLAT_TILE / 256;
-        CsrEnt e[PER];
-        double v[PER];
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            int64_t jj = t0 + u * 256 + threadIdx.x;
-            if (jj < t1) e[u] = ent[jj];
-            else { e[u].b = -1; e[u].w0 = e[u].w1 = e[u].w2 = 0.0f; }
-        }
-#pragma unroll
-        for (int u = 0; u < PER; ++u) v[u] = e[u].b >= 0 ? pw[e[u].b] : 0.0;
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            int o = u * 256 + threadIdx.x;
-            if (nmap == 1) vals[0][o] = v[u];
-            else {
-                vals[0][o] = (double)e[u].w0 * v[u];
-                vals[1][o] = (double)e[u].w1 * v[u];
-                vals[2][o] = (double)e[u].w2 * v[u];
-            }
+    TileInfo ti = tinfo[j];
+    for (int64_t jj = t0 + threadIdx.x; jj < t1; jj += blockDim.x) {
+        CsrEnt e = ent[jj];
+        double v = pw[e.b];
+        int64_t o = jj - t0;
+        if (nmap == 1) vals[0][o] = v;
+        else {
+            vals[0][o] = (double)e.w0 * v;
+            vals[1][o] = (double)e.w1 * v;
+            vals[2][o] = (double)e.w2 * v;
         }
     }
     __syncthreads();
-    int64_t qa = s_qa, qb = s_qb;
-    if (threadIdx.x == 0) { tinfo[j].qa = qa; tinfo[j].qb = qb; }
+    int64_t qa = ti.qa, qb = ti.qb;
     int item = threadIdx.x / FLAT_LANES, sub = threadIdx.x % FLAT_LANES;
     for (int64_t q = qa + item; q <= qb; q += blockDim.x / FLAT_LANES) {
         int64_t a = off[q * ndeg], e = off[(q + 1) * ndeg];
@@ -1049,15 +996,40 @@ __global__ void k_scatter_fixup(int nmap, int64_t ntiles, int64_t ndeg, const in
     *(double4 *)(m4 + q * 4) = r;
 }
 
-void launch_scatter_tiles(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off, int64_t nent,
-                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4, void *scratch)
+// tile -> first / last cross pixel; depends on the CSR only, computed once per pointing
+__global__ void k_tile_info(int64_t npc, int64_t ndeg, const int64_t *off, int64_t nent, TileInfo *tinfo)
+{
+    int64_t ntiles = (nent + FLAT_TILE - 1) / FLAT_TILE;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < ntiles; j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t0 = j * FLAT_TILE, t1 = t0 + FLAT_TILE < nent ? t0 + FLAT_TILE : nent;
+        tinfo[j].qa = cross_of_entry(off, ndeg, npc, t0);
+        tinfo[j].qb = cross_of_entry(off, ndeg, npc, t1 - 1);
+    }
+}
+
+size_t scatter_tiles_info_bytes(int64_t nent)
+{
+    int64_t ntiles = (nent + FLAT_TILE - 1) / FLAT_TILE;
+    return (size_t)(ntiles > 0 ? ntiles : 1) * sizeof(TileInfo);
+}
+
+void launch_tile_info(hipStream_t s, int64_t npix_cross, int64_t ndeg, const int64_t *off, int64_t nent, void *tinfo)
 {
     int64_t ntiles = (nent + FLAT_TILE - 1) / FLAT_TILE;
     if (ntiles == 0) return;
+    k_tile_info<<<grid_for(ntiles, TPB), TPB, 0, s>>>(npix_cross, ndeg, off, nent, (TileInfo *)tinfo);
+}
+
+void launch_scatter_tiles(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off, int64_t nent,
+                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4, const void *tinfo_v,
+                          void *scratch)
+{
+    (void)npix_cross;
+    int64_t ntiles = (nent + FLAT_TILE - 1) / FLAT_TILE;
+    if (ntiles == 0) return;
     double4 *part = (double4 *)scratch;
-    TileInfo *tinfo = (TileInfo *)(part + 2 * ntiles);
-    k_scatter_tiles<<<(unsigned)ntiles, 256, 0, s>>>(nmap, npix_cross, ndeg, off, nent, ent, pw, ccinv6, m4, part,
-                                                     tinfo);
+    const TileInfo *tinfo = (const TileInfo *)tinfo_v;
+    k_scatter_tiles<<<(unsigned)ntiles, 256, 0, s>>>(nmap, ndeg, off, nent, ent, pw, ccinv6, m4, part, tinfo);
     k_scatter_fixup<<<(unsigned)((ntiles + 255) / 256), 256, 0, s>>>(nmap, ntiles, ndeg, off, part, tinfo, ccinv6,
                                                                       m4);
 }
@@ -1065,7 +1037,7 @@ void launch_scatter_tiles(hipStream_t s, int nmap, int64_t npix_cross, int64_t n
 size_t scatter_tiles_scratch_bytes(int64_t nent)
 {
     int64_t ntiles = (nent + FLAT_TILE - 1) / FLAT_TILE;
-    return (size_t)ntiles * (2 * sizeof(double4) + sizeof(TileInfo)) + 256;
+    return (size_t)ntiles * 2 * sizeof(double4) + 256;
 }
 
 void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
