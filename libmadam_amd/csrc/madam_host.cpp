@@ -329,6 +329,15 @@ struct Context {
     DBuf<double> d_yba, d_nna, d_aa, d_r, d_ro, d_z, d_p, d_pw, d_ap, d_prec, d_scal, d_part;
     DBuf<int8_t> d_rmask;
     DBuf<char> d_tilescratch, d_tileinfo;
+    // segmented-tile projections: pixel side (s) and baseline side (g)
+    struct SegTiles {
+        DBuf<uint8_t> heads;
+        DBuf<int64_t> k0, fix_end, fix_ord;
+        int64_t nent = 0;
+        bool ok = false;
+    } sr_s, sr_g;
+    DBuf<int32_t> d_nzq;
+    DBuf<double4> d_sr_part;
     // segments / prior
     std::vector<Segment> seg;
     DBuf<Segment> d_seg;
@@ -353,7 +362,9 @@ struct Context {
     std::vector<double> rr_hist;
     int profile = 0;
     int pcg_steps_done = 0;
-    int kernel_variant = 0;   // 0: flat-tile projections, 1: group-per-item projections
+    // projections: 0 segmented tiles (both sides), 1 lane group per baseline / wave per pixel,
+    // 2 LDS tiles, 3 segmented pixel side + LDS baseline side
+    int kernel_variant = 0;
 
     ~Context()
     {
@@ -798,6 +809,65 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
 
 // ------------------------------------------------------------------ solve
 
+static void segtiles_alloc(Context::SegTiles &t, int64_t nent)
+{
+    int64_t nt = segred_ntiles(nent);
+    t.nent = nent;
+    t.heads.alloc(segred_heads_bytes(nent));
+    t.k0.alloc(nt > 0 ? nt : 1);
+    t.fix_end.alloc(nt > 0 ? nt : 1);
+    t.fix_ord.alloc(nt > 0 ? nt : 1);
+}
+
+// segment tables of the two projections; they depend on the pointing only
+static void build_segtiles(Context &c)
+{
+    hipStream_t s = c.stream;
+    // pixel side: non-empty cross pixels of the CSR
+    {
+        int64_t npc = c.npix_cross;
+        DBuf<int32_t> flag;
+        DBuf<int64_t> pos, tmp, segstart;
+        flag.alloc(npc);
+        pos.alloc(npc + 1);
+        int64_t nchunks = (npc + 1 + 4095) / 4096;
+        tmp.alloc(nchunks);
+        c.d_nzq.alloc(npc);
+        segstart.alloc(npc + 1);
+        segred_pixel_segments(s, c.d_off.p, npc, c.ndeg, flag.p, pos.p, tmp.p, nchunks, c.d_nzq.p, segstart.p);
+        int64_t last_pos = 0;
+        int32_t last_flag = 0;
+        d2h(&last_pos, pos.p + npc - 1, 1, s);
+        d2h(&last_flag, flag.p + npc - 1, 1, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        int64_t nseg = last_pos + last_flag;
+        segtiles_alloc(c.sr_s, c.nactive);
+        segred_build(s, segstart.p, nseg, c.nactive, c.sr_s.heads.p, c.sr_s.k0.p, c.sr_s.fix_end.p,
+                     c.sr_s.fix_ord.p);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        c.sr_s.ok = true;
+    }
+    // baseline side: every baseline is a non-empty sample range covering the stream
+    {
+        bool ok = c.bstart.front() == 0 && c.bstart.back() == c.nsamp;
+        for (int64_t k = 0; k < c.nbase && ok; ++k) ok = c.bstart[k + 1] > c.bstart[k];
+        c.sr_g.ok = false;
+        if (ok) {
+            int64_t nent = (int64_t)c.ndet * c.nsamp;
+            DBuf<int64_t> segstart;
+            segstart.alloc(c.nbtot + 1);
+            segred_baseline_segments(s, c.d_bstart.p, c.nbase, c.ndet, c.nsamp, segstart.p);
+            segtiles_alloc(c.sr_g, nent);
+            segred_build(s, segstart.p, c.nbtot, nent, c.sr_g.heads.p, c.sr_g.k0.p, c.sr_g.fix_end.p,
+                         c.sr_g.fix_ord.p);
+            MADAM_CHECK(hipStreamSynchronize(s));
+            c.sr_g.ok = true;
+        }
+    }
+    int64_t nt = std::max(segred_ntiles(c.nactive), segred_ntiles((int64_t)c.ndet * c.nsamp));
+    c.d_sr_part.alloc(2 * (nt > 0 ? nt : 1));
+}
+
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
     Timer()
@@ -1119,7 +1189,11 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         done++;
         c.pcg_steps_done++;
         if (prof) { t_it.start(s); t_sc.start(s); }
-        if (c.kernel_variant == 0)
+        if (c.kernel_variant == 0 || c.kernel_variant == 3)
+            launch_scatter_seg(s, c.nmap, c.nactive, c.d_ent.p, c.d_pw.p, c.sr_s.heads.p, c.sr_s.k0.p,
+                               c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.d_ccinv6x.p, c.d_m4x.p,
+                               c.d_sr_part.p);
+        else if (c.kernel_variant == 2)
             launch_scatter_tiles(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_ent.p, c.d_pw.p,
                                  c.d_ccinv6x.p, c.d_m4x.p, c.d_tileinfo.p, c.d_tilescratch.p);
         else
@@ -1130,7 +1204,11 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         else if (p.diagfilter != 0) launch_axpby_nna(s, c.d_ap.p, c.d_nna.p, c.d_p.p, p.diagfilter, nb);
         else launch_fill(s, c.d_ap.p, 0.0, nb);
         if (prof) { t_pr.stop(s); t_ga.start(s); }
-        if (c.kernel_variant == 0)
+        if (c.kernel_variant == 0 && c.sr_g.ok)
+            launch_gather_seg(s, c.nmap, (int64_t)c.ndet * c.nsamp, c.d_pixw.p, c.d_m4x.p, c.ndeg, c.sr_g.heads.p,
+                              c.sr_g.k0.p, c.sr_g.fix_end.p, c.sr_g.fix_ord.p, c.d_dwb.p, c.d_nna.p, c.d_p.p,
+                              c.d_ap.p, c.d_sr_part.p);
+        else if (c.kernel_variant != 1)
             launch_gather_flat(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
                                c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p);
         else
@@ -1220,6 +1298,7 @@ static void solve(Context &c, double *signal, int on_device)
         c.d_tilescratch.alloc(scatter_tiles_scratch_bytes(c.nactive));
         c.d_tileinfo.alloc(scatter_tiles_info_bytes(c.nactive));
         launch_tile_info(s, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_tileinfo.p);
+        build_segtiles(c);
         c.d_yba.alloc(c.nbtot);
         c.d_nna.alloc(c.nbtot);
         int grp = group_for(c);

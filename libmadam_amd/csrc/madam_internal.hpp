@@ -98,6 +98,21 @@ void launch_scatter_tiles(hipStream_t s, int nmap, int64_t npix_cross, int64_t n
 size_t scatter_tiles_scratch_bytes(int64_t nent);
 size_t scatter_tiles_info_bytes(int64_t nent);
 void launch_tile_info(hipStream_t s, int64_t npix_cross, int64_t ndeg, const int64_t *off, int64_t nent, void *tinfo);
+// segmented tile reductions (one wave per 512-entry tile, no LDS)
+int64_t segred_ntiles(int64_t nent);
+size_t segred_heads_bytes(int64_t nent);
+void segred_build(hipStream_t s, const int64_t *segstart, int64_t nseg, int64_t nent, uint8_t *heads,
+                  int64_t *tile_k0, int64_t *fix_end, int64_t *fix_ord);
+void segred_pixel_segments(hipStream_t s, const int64_t *off, int64_t npc, int64_t ndeg, int32_t *flag, int64_t *pos,
+                           int64_t *scan_tmp, int64_t nchunks, int32_t *nzq, int64_t *segstart);
+void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbase, int ndet, int64_t nsamp,
+                              int64_t *segstart);
+void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const CsrEnt *ent, const double *pw,
+                        const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
+                        const int32_t *nzq, const double *ccinv6, double *m4, double4 *part);
+void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,
+                       const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
+                       const double *dwb, const double *nna, const double *p, double *ap, double4 *part);
 // stable regrouping of the samples by map pixel (off from the hit histogram)
 void csr_build_sorted(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
                       int64_t npix_map, int64_t total, CsrEnt *ent, uint32_t *src);

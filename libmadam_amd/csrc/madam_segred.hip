@@ -1,0 +1,421 @@
+// Segmented reductions of a sorted entry stream, one wave per 512-entry tile,
+// without LDS: the two per-iteration pointing projections of the PCG.
+//
+//   pixel side    m_q = ccinv_q . sum_{j in pixel q} w_j p[b_j]
+//                 (baseline_to_map_order0 + ccmultiply,
+//                  src/smadam_routines.f90:1038-1148, src/map_routines.f90:24-77)
+//   baseline side ap_b += nna_b p_b - dwb_b sum_{i in baseline b} w_i . m[pix_i]
+//                 (map_to_baseline_order0, src/smadam_routines.f90:1235-1376)
+//
+// Entries are consumed in stream order; a segment (a cross pixel's CSR range,
+// or a baseline's sample range) starts where the head bitmask has a bit.  Lane
+// l of the wave owns entries t0 + 8 l .. t0 + 8 l + 7: it sums its runs
+// sequentially, finishes segments that begin and end inside it, and one
+// segmented inclusive scan over the 64 lanes joins runs that cross lanes.  A
+// segment that crosses a tile edge leaves a partial sum per tile (slot 1 in
+// the tile where it starts, slot 0 in every later tile) and is finished by the
+// fix-up pass in tile order.  Every sum is taken in one fixed order, so the
+// results are bitwise reproducible; every wave does the same work whatever the
+// segment lengths are (pole pixels hold tens of thousands of entries).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "madam_internal.hpp"
+
+namespace madam {
+
+static constexpr int SR_PER_LANE = 8;
+static constexpr int SR_TILE = 64 * SR_PER_LANE;
+static constexpr int SR_TPB = 256;
+static constexpr int SR_WAVES = SR_TPB / 64;
+
+__device__ inline int64_t sr_xcd_block(int64_t bid, int64_t nblk)
+{
+    // workgroups go round-robin over the 8 XCDs; give each XCD a contiguous range
+    const int64_t NX = 8;
+    int64_t per = nblk / NX;
+    if (bid >= per * NX) return bid;
+    return (bid % NX) * per + bid / NX;
+}
+
+template <int NC>
+__device__ inline void sr_seg_scan(double (&v)[NC], bool head, int lane)
+{
+    // inclusive scan with the operator (f1, v1) + (f2, v2) = (f1 | f2, f2 ? v2 : v1 + v2)
+    int f = head ? 1 : 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        double o[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) o[c] = __shfl_up(v[c], d, 64);
+        int of = __shfl_up(f, d, 64);
+        if (lane >= d) {
+            if (!f) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) v[c] += o[c];
+            }
+            f |= of;
+        }
+    }
+}
+
+// Reduce one tile.  prod[u] are the lane's entry contributions (zero past the
+// end), hb the head bits of its 8 entries, k0 the ordinal of the segment that
+// holds the tile's first entry, cont_last whether the segment that starts last
+// in the tile continues into the next tile.  fin(ordinal, sum) finishes a
+// segment; part(slot, sum) stores a partial sum of this tile.
+template <int NC, typename Fin, typename Part>
+__device__ inline void sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsigned hb, int lane, int64_t k0,
+                                      bool head0, bool cont_last, Fin fin, Part part)
+{
+    int nh = __popc(hb);
+    int H = nh;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int o = __shfl_up(H, d, 64);
+        if (lane >= d) H += o;
+    }
+    int Hex = H - nh;                    // heads in the lanes before this one
+    int Htot = __shfl(H, 63, 64);
+    int64_t kb = k0 - (head0 ? 1 : 0);   // ordinal of a head h = kb + #heads in [t0, h]
+    double pre[NC], cur[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) pre[c] = cur[c] = 0.0;
+    int hk = 0;
+#pragma unroll
+    for (int u = 0; u < SR_PER_LANE; ++u) {
+        if ((hb >> u) & 1u) {
+            if (hk == 0) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) pre[c] = cur[c];
+            } else {
+                fin(kb + Hex + hk, cur);   // began and ended inside this lane
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) cur[c] = 0.0;
+            ++hk;
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) cur[c] += prod[u][c];
+    }
+    if (hk == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) pre[c] = cur[c];
+    }
+    double v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = hk ? cur[c] : pre[c];
+    sr_seg_scan<NC>(v, hk > 0, lane);
+    double sprev[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        sprev[c] = __shfl_up(v[c], 1, 64);
+        if (lane == 0) sprev[c] = 0.0;
+    }
+    if (hk > 0) {
+        int firstu = __ffs(hb) - 1;
+        if (lane > 0 || firstu > 0) {   // the segment that ends before this lane's first head
+            double tot[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) tot[c] = pre[c] + sprev[c];
+            if (Hex == 0) part(0, tot);   // began in an earlier tile
+            else fin(kb + Hex, tot);
+        }
+    }
+    if (lane == 63) {
+        if (Htot == 0) part(0, v);
+        else if (cont_last) part(1, v);
+        else fin(kb + Htot, v);
+    }
+}
+
+__device__ inline void sr_part_store(double4 *part, int64_t j, int slot, const double *v, int nc)
+{
+    double4 r;
+    r.x = v[0];
+    r.y = nc > 1 ? v[1] : 0.0;
+    r.z = nc > 2 ? v[2] : 0.0;
+    r.w = 0.0;
+    part[j * 2 + slot] = r;
+}
+
+__device__ inline void sr_ccmul_store(int nmap, const double *cc, const double *m, double *out4)
+{
+    // lower-triangle storage [c11, c21, c31, c22, c32, c33] of the symmetric inverse
+    double4 r;
+    if (nmap == 1) {
+        r.x = cc[0] * m[0];
+        r.y = r.z = 0.0;
+    } else {
+        r.x = cc[0] * m[0] + cc[1] * m[1] + cc[2] * m[2];
+        r.y = cc[1] * m[0] + cc[3] * m[1] + cc[4] * m[2];
+        r.z = cc[2] * m[0] + cc[4] * m[1] + cc[5] * m[2];
+    }
+    r.w = 0.0;
+    *(double4 *)out4 = r;
+}
+
+// ------------------------------------------------------------ pixel side
+
+template <int NC>
+__global__ void __launch_bounds__(SR_TPB) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
+                                                       const uint8_t *heads, const int64_t *tile_k0,
+                                                       const int64_t *fix_end, const int32_t *nzq,
+                                                       const double *ccinv6, double *m4, double4 *part)
+{
+    int lane = threadIdx.x & 63;
+    int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * SR_WAVES + (threadIdx.x >> 6);
+    int64_t t0 = j * SR_TILE;
+    if (t0 >= nent) return;   // uniform over the wave
+    int64_t e0 = t0 + lane * SR_PER_LANE;
+    int64_t rem = nent - e0;
+    int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
+    unsigned hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
+    CsrEnt e[SR_PER_LANE];
+#pragma unroll
+    for (int u = 0; u < SR_PER_LANE; ++u)
+        if (u < nv) e[u] = ent[e0 + u];
+    double x[SR_PER_LANE];
+#pragma unroll
+    for (int u = 0; u < SR_PER_LANE; ++u) x[u] = u < nv ? pw[e[u].b] : 0.0;
+    double prod[SR_PER_LANE][NC];
+#pragma unroll
+    for (int u = 0; u < SR_PER_LANE; ++u) {
+        if (NC == 1) prod[u][0] = x[u];
+        else {
+            bool ok = u < nv;
+            prod[u][0] = ok ? (double)e[u].w0 * x[u] : 0.0;
+            prod[u][NC > 1 ? 1 : 0] = ok ? (double)e[u].w1 * x[u] : 0.0;
+            prod[u][NC > 2 ? 2 : 0] = ok ? (double)e[u].w2 * x[u] : 0.0;
+        }
+    }
+    int64_t k0 = tile_k0[j];
+    bool head0 = (heads[t0 / 8] & 1u) != 0;
+    bool cont_last = fix_end[j] >= 0;
+    const int nmap = NC == 1 ? 1 : 3;
+    sr_reduce_tile<NC>(
+        prod, hb, lane, k0, head0, cont_last,
+        [&](int64_t k, const double *s) {
+            int64_t q = nzq[k];
+            sr_ccmul_store(nmap, ccinv6 + q * 6, s, m4 + q * 4);
+        },
+        [&](int slot, const double *s) { sr_part_store(part, j, slot, s, NC); });
+}
+
+// one thread per tile: finish the segment that starts in tile j and continues
+__global__ void k_sr_scatter_fix(int nmap, int64_t ntiles, const int64_t *fix_end, const int64_t *fix_ord,
+                                 const int32_t *nzq, const double4 *part, const double *ccinv6, double *m4)
+{
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= ntiles) return;
+    int64_t jl = fix_end[j];
+    if (jl < 0) return;
+    double4 a = part[j * 2 + 1];
+    for (int64_t jj = j + 1; jj <= jl; ++jj) {
+        double4 v = part[jj * 2];
+        a.x += v.x;
+        a.y += v.y;
+        a.z += v.z;
+    }
+    int64_t q = nzq[fix_ord[j]];
+    double m[3] = {a.x, a.y, a.z};
+    sr_ccmul_store(nmap, ccinv6 + q * 6, m, m4 + q * 4);
+}
+
+// ------------------------------------------------------------ baseline side
+
+template <int NC>
+__global__ void __launch_bounds__(SR_TPB) k_sr_gather(int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,
+                                                      const uint8_t *heads, const int64_t *tile_k0,
+                                                      const int64_t *fix_end, const double *dwb, const double *nna,
+                                                      const double *p, double *ap, double4 *part)
+{
+    int lane = threadIdx.x & 63;
+    int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * SR_WAVES + (threadIdx.x >> 6);
+    int64_t t0 = j * SR_TILE;
+    if (t0 >= nent) return;
+    int64_t e0 = t0 + lane * SR_PER_LANE;
+    int64_t rem = nent - e0;
+    int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
+    unsigned hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
+    PixW r[SR_PER_LANE];
+#pragma unroll
+    for (int u = 0; u < SR_PER_LANE; ++u) {
+        if (u < nv) r[u] = pixw[e0 + u];
+        else r[u].pix = -1;
+    }
+    double4 m[SR_PER_LANE];
+#pragma unroll
+    for (int u = 0; u < SR_PER_LANE; ++u)
+        if (r[u].pix >= 0) m[u] = *(const double4 *)(m4 + (r[u].pix / ndeg) * 4);
+    double prod[SR_PER_LANE][1];
+#pragma unroll
+    for (int u = 0; u < SR_PER_LANE; ++u) {
+        double v = 0.0;
+        if (r[u].pix >= 0)
+            v = NC == 1 ? m[u].x : (double)r[u].w0 * m[u].x + (double)r[u].w1 * m[u].y + (double)r[u].w2 * m[u].z;
+        prod[u][0] = v;
+    }
+    int64_t k0 = tile_k0[j];
+    bool head0 = (heads[t0 / 8] & 1u) != 0;
+    bool cont_last = fix_end[j] >= 0;
+    sr_reduce_tile<1>(
+        prod, hb, lane, k0, head0, cont_last,
+        [&](int64_t b, const double *s) { ap[b] = ap[b] + (nna[b] * p[b] - s[0] * dwb[b]); },
+        [&](int slot, const double *s) { sr_part_store(part, j, slot, s, 1); });
+}
+
+__global__ void k_sr_gather_fix(int64_t ntiles, const int64_t *fix_end, const int64_t *fix_ord, const double4 *part,
+                                const double *dwb, const double *nna, const double *p, double *ap)
+{
+    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= ntiles) return;
+    int64_t jl = fix_end[j];
+    if (jl < 0) return;
+    double a = part[j * 2 + 1].x;
+    for (int64_t jj = j + 1; jj <= jl; ++jj) a += part[jj * 2].x;
+    int64_t b = fix_ord[j];
+    ap[b] = ap[b] + (nna[b] * p[b] - a * dwb[b]);
+}
+
+// ------------------------------------------------------------ set-up
+
+// segstart[k] for k < nseg: first entry of segment k (strictly increasing), segstart[nseg] = nent
+__global__ void k_sr_heads(const int64_t *segstart, int64_t nseg, uint32_t *heads32)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nseg; k += (int64_t)gridDim.x * blockDim.x) {
+        int64_t s = segstart[k];
+        atomicOr(&heads32[s >> 5], 1u << (s & 31));
+    }
+}
+
+__global__ void k_sr_tiles(const int64_t *segstart, int64_t nseg, int64_t *tile_k0, int64_t *fix_end,
+                           int64_t *fix_ord)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nseg; k += (int64_t)gridDim.x * blockDim.x) {
+        int64_t a = segstart[k], e = segstart[k + 1];
+        int64_t ja = a / SR_TILE, jl = (e - 1) / SR_TILE;
+        // tiles whose first entry lies in [a, e)
+        for (int64_t t = (a + SR_TILE - 1) / SR_TILE; t * SR_TILE < e; ++t) tile_k0[t] = k;
+        if (jl > ja) {
+            fix_end[ja] = jl;
+            fix_ord[ja] = k;
+        }
+    }
+}
+
+__global__ void k_sr_fill_i64(int64_t *x, int64_t v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+static int sr_grid(int64_t n, int cap = 65535 * 8)
+{
+    int64_t g = (n + SR_TPB - 1) / SR_TPB;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+int64_t segred_ntiles(int64_t nent) { return (nent + SR_TILE - 1) / SR_TILE; }
+
+size_t segred_heads_bytes(int64_t nent)
+{
+    // lanes read the byte of their first entry; pad to whole 32-bit words of whole tiles
+    return (size_t)(segred_ntiles(nent) + 1) * (SR_TILE / 8);
+}
+
+void segred_build(hipStream_t s, const int64_t *segstart, int64_t nseg, int64_t nent, uint8_t *heads,
+                  int64_t *tile_k0, int64_t *fix_end, int64_t *fix_ord)
+{
+    int64_t nt = segred_ntiles(nent);
+    MADAM_CHECK(hipMemsetAsync(heads, 0, segred_heads_bytes(nent), s));
+    if (nt == 0) return;
+    k_sr_fill_i64<<<sr_grid(nt), SR_TPB, 0, s>>>(tile_k0, 0, nt);
+    k_sr_fill_i64<<<sr_grid(nt), SR_TPB, 0, s>>>(fix_end, -1, nt);
+    k_sr_fill_i64<<<sr_grid(nt), SR_TPB, 0, s>>>(fix_ord, -1, nt);
+    if (nseg > 0) {
+        k_sr_heads<<<sr_grid(nseg), SR_TPB, 0, s>>>(segstart, nseg, (uint32_t *)heads);
+        k_sr_tiles<<<sr_grid(nseg), SR_TPB, 0, s>>>(segstart, nseg, tile_k0, fix_end, fix_ord);
+    }
+    MADAM_CHECK(hipGetLastError());
+}
+
+// segments of the pixel side: the non-empty cross pixels, in order
+__global__ void k_sr_nonempty(const int64_t *off, int64_t npc, int64_t ndeg, int32_t *flag)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < npc; c += (int64_t)gridDim.x * blockDim.x)
+        flag[c] = off[(c + 1) * ndeg] > off[c * ndeg] ? 1 : 0;
+}
+
+__global__ void k_sr_compact(const int64_t *off, int64_t npc, int64_t ndeg, const int32_t *flag, const int64_t *pos,
+                             int32_t *nzq, int64_t *segstart)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < npc; c += (int64_t)gridDim.x * blockDim.x) {
+        if (flag[c]) {
+            nzq[pos[c]] = (int32_t)c;
+            segstart[pos[c]] = off[c * ndeg];
+        }
+        if (c == npc - 1) segstart[pos[c] + flag[c]] = off[npc * ndeg];
+    }
+}
+
+void segred_pixel_segments(hipStream_t s, const int64_t *off, int64_t npc, int64_t ndeg, int32_t *flag, int64_t *pos,
+                           int64_t *scan_tmp, int64_t nchunks, int32_t *nzq, int64_t *segstart)
+{
+    k_sr_nonempty<<<sr_grid(npc), SR_TPB, 0, s>>>(off, npc, ndeg, flag);
+    launch_exclusive_scan_i32_to_i64(s, flag, npc, pos, scan_tmp, nchunks);
+    k_sr_compact<<<sr_grid(npc), SR_TPB, 0, s>>>(off, npc, ndeg, flag, pos, nzq, segstart);
+    MADAM_CHECK(hipGetLastError());
+}
+
+// segments of the baseline side: every baseline of every detector
+__global__ void k_sr_base_starts(const int64_t *bstart, int64_t nbase, int ndet, int64_t nsamp, int64_t *segstart)
+{
+    int64_t nb = (int64_t)ndet * nbase;
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b <= nb; b += (int64_t)gridDim.x * blockDim.x) {
+        if (b == nb) segstart[b] = (int64_t)ndet * nsamp;
+        else segstart[b] = (b / nbase) * nsamp + bstart[b % nbase];
+    }
+}
+
+void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbase, int ndet, int64_t nsamp,
+                              int64_t *segstart)
+{
+    k_sr_base_starts<<<sr_grid((int64_t)ndet * nbase + 1), SR_TPB, 0, s>>>(bstart, nbase, ndet, nsamp, segstart);
+    MADAM_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------ launchers
+
+void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const CsrEnt *ent, const double *pw,
+                        const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
+                        const int32_t *nzq, const double *ccinv6, double *m4, double4 *part)
+{
+    int64_t nt = segred_ntiles(nent);
+    if (nt == 0) return;
+    unsigned grid = (unsigned)((nt + SR_WAVES - 1) / SR_WAVES);
+    if (nmap == 1)
+        k_sr_scatter<1><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, ccinv6, m4, part);
+    else
+        k_sr_scatter<3><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, ccinv6, m4, part);
+    k_sr_scatter_fix<<<sr_grid(nt, 1 << 30), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part, ccinv6, m4);
+}
+
+void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,
+                       const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
+                       const double *dwb, const double *nna, const double *p, double *ap, double4 *part)
+{
+    int64_t nt = segred_ntiles(nent);
+    if (nt == 0) return;
+    unsigned grid = (unsigned)((nt + SR_WAVES - 1) / SR_WAVES);
+    if (nmap == 1)
+        k_sr_gather<1><<<grid, SR_TPB, 0, s>>>(nent, pixw, m4, ndeg, heads, tile_k0, fix_end, dwb, nna, p, ap, part);
+    else
+        k_sr_gather<3><<<grid, SR_TPB, 0, s>>>(nent, pixw, m4, ndeg, heads, tile_k0, fix_end, dwb, nna, p, ap, part);
+    k_sr_gather_fix<<<sr_grid(nt, 1 << 30), SR_TPB, 0, s>>>(nt, fix_end, fix_ord, part, dwb, nna, p, ap);
+}
+
+}  // namespace madam

@@ -115,12 +115,14 @@ def test_noise_prior_band_preconditioner():
     check_parity(g, o, c1, c2)
 
 
-def test_noise_prior_band_preconditioner_group_kernels():
-    """the alternative projection kernels (lane group per baseline / wave per pixel)"""
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_noise_prior_band_preconditioner_other_kernels(variant):
+    """the alternative projection kernels: 1 lane group per baseline / wave per
+    pixel, 2 LDS tiles, 3 segmented pixel side with the LDS baseline side"""
     import libmadam_amd
     case = ring(prior=True, nsamp=30000, nperiod=2)
     c1, c2 = mc.copy_case(case), mc.copy_case(case)
-    g = libmadam_amd.run(*mc.call_args(c1), kernel_variant=1)
+    g = libmadam_amd.run(*mc.call_args(c1), kernel_variant=variant)
     o = oracle.destripe(*mc.call_args(c2))
     check_parity(g, o, c1, c2)
 
