@@ -1,5 +1,5 @@
-// Segmented reductions of a sorted entry stream, one wave per 512-entry tile,
-// without LDS: the two per-iteration pointing projections of the PCG.
+// Segmented reductions of a sorted entry stream, one wave per 512-entry tile:
+// the two per-iteration pointing projections of the PCG.
 //
 //   pixel side    m_q = ccinv_q . sum_{j in pixel q} w_j p[b_j]
 //                 (baseline_to_map_order0 + ccmultiply,
@@ -8,8 +8,9 @@
 //                 (map_to_baseline_order0, src/smadam_routines.f90:1235-1376)
 //
 // Entries are consumed in stream order; a segment (a cross pixel's CSR range,
-// or a baseline's sample range) starts where the head bitmask has a bit.  Lane
-// l of the wave owns entries t0 + 8 l .. t0 + 8 l + 7: it sums its runs
+// or a baseline's sample range) starts where the head bitmask has a bit.  The
+// wave loads its tile coalesced and transposes the per-entry products through
+// LDS so that lane l holds entries t0 + 8 l .. t0 + 8 l + 7: it sums its runs
 // sequentially, finishes segments that begin and end inside it, and one
 // segmented inclusive scan over the 64 lanes joins runs that cross lanes.  A
 // segment that crosses a tile edge leaves a partial sum per tile (slot 1 in
@@ -158,38 +159,58 @@ __device__ inline void sr_ccmul_store(int nmap, const double *cc, const double *
 
 // ------------------------------------------------------------ pixel side
 
-template <int NC>
-__global__ void __launch_bounds__(SR_TPB) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
-                                                       const uint8_t *heads, const int64_t *tile_k0,
-                                                       const int64_t *fix_end, const int32_t *nzq,
-                                                       const double *ccinv6, double *m4, double4 *part)
+// Entries are loaded coalesced (lane l takes t0 + 64 u + l) and their products
+// are transposed through a per-wave LDS buffer to the lane-contiguous layout
+// of the reduction; the pad of one double per 8 keeps both directions free of
+// bank conflicts.
+static constexpr int SR_PAD = SR_TILE + SR_TILE / 8;
+
+__device__ inline int sr_col(int u, int lane) { return u * 72 + lane + (lane >> 3); }   // pad(64 u + lane)
+__device__ inline int sr_row(int u, int lane) { return lane * 9 + u; }                  // pad(8 lane + u)
+
+template <int NC, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
+                                                           const uint8_t *heads, const int64_t *tile_k0,
+                                                           const int64_t *fix_end, const int32_t *nzq,
+                                                           const double *ccinv6, double *m4, double4 *part)
 {
-    int lane = threadIdx.x & 63;
-    int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * SR_WAVES + (threadIdx.x >> 6);
+    __shared__ double tr[WAVES][NC][SR_PAD];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * WAVES + w;
     int64_t t0 = j * SR_TILE;
-    if (t0 >= nent) return;   // uniform over the wave
+    bool active = t0 < nent;   // uniform over the wave
+    if (active) {
+        CsrEnt e[SR_PER_LANE];
+#pragma unroll
+        for (int u = 0; u < SR_PER_LANE; ++u) {
+            int64_t idx = t0 + u * 64 + lane;
+            if (idx < nent) e[u] = ent[idx];
+            else { e[u].b = -1; e[u].w0 = e[u].w1 = e[u].w2 = 0.0f; }
+        }
+        double x[SR_PER_LANE];
+#pragma unroll
+        for (int u = 0; u < SR_PER_LANE; ++u) x[u] = e[u].b >= 0 ? pw[e[u].b] : 0.0;
+#pragma unroll
+        for (int u = 0; u < SR_PER_LANE; ++u) {
+            if (NC == 1) tr[w][0][sr_col(u, lane)] = x[u];
+            else {
+                tr[w][0][sr_col(u, lane)] = (double)e[u].w0 * x[u];
+                tr[w][NC > 1 ? 1 : 0][sr_col(u, lane)] = (double)e[u].w1 * x[u];
+                tr[w][NC > 2 ? 2 : 0][sr_col(u, lane)] = (double)e[u].w2 * x[u];
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
     int64_t e0 = t0 + lane * SR_PER_LANE;
     int64_t rem = nent - e0;
     int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
     unsigned hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
-    CsrEnt e[SR_PER_LANE];
-#pragma unroll
-    for (int u = 0; u < SR_PER_LANE; ++u)
-        if (u < nv) e[u] = ent[e0 + u];
-    double x[SR_PER_LANE];
-#pragma unroll
-    for (int u = 0; u < SR_PER_LANE; ++u) x[u] = u < nv ? pw[e[u].b] : 0.0;
     double prod[SR_PER_LANE][NC];
 #pragma unroll
-    for (int u = 0; u < SR_PER_LANE; ++u) {
-        if (NC == 1) prod[u][0] = x[u];
-        else {
-            bool ok = u < nv;
-            prod[u][0] = ok ? (double)e[u].w0 * x[u] : 0.0;
-            prod[u][NC > 1 ? 1 : 0] = ok ? (double)e[u].w1 * x[u] : 0.0;
-            prod[u][NC > 2 ? 2 : 0] = ok ? (double)e[u].w2 * x[u] : 0.0;
-        }
-    }
+    for (int u = 0; u < SR_PER_LANE; ++u)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) prod[u][c] = tr[w][c][sr_row(u, lane)];
     int64_t k0 = tile_k0[j];
     bool head0 = (heads[t0 / 8] & 1u) != 0;
     bool cont_last = fix_end[j] >= 0;
@@ -231,32 +252,41 @@ __global__ void __launch_bounds__(SR_TPB) k_sr_gather(int64_t nent, const PixW *
                                                       const int64_t *fix_end, const double *dwb, const double *nna,
                                                       const double *p, double *ap, double4 *part)
 {
-    int lane = threadIdx.x & 63;
-    int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * SR_WAVES + (threadIdx.x >> 6);
+    __shared__ double tr[SR_WAVES][SR_PAD];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * SR_WAVES + w;
     int64_t t0 = j * SR_TILE;
-    if (t0 >= nent) return;
+    bool active = t0 < nent;
+    if (active) {
+        PixW r[SR_PER_LANE];
+#pragma unroll
+        for (int u = 0; u < SR_PER_LANE; ++u) {
+            int64_t idx = t0 + u * 64 + lane;
+            if (idx < nent) r[u] = pixw[idx];
+            else r[u].pix = -1;
+        }
+        double4 m[SR_PER_LANE];
+#pragma unroll
+        for (int u = 0; u < SR_PER_LANE; ++u)
+            if (r[u].pix >= 0) m[u] = *(const double4 *)(m4 + (r[u].pix / ndeg) * 4);
+#pragma unroll
+        for (int u = 0; u < SR_PER_LANE; ++u) {
+            double v = 0.0;
+            if (r[u].pix >= 0)
+                v = NC == 1 ? m[u].x
+                            : (double)r[u].w0 * m[u].x + (double)r[u].w1 * m[u].y + (double)r[u].w2 * m[u].z;
+            tr[w][sr_col(u, lane)] = v;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
     int64_t e0 = t0 + lane * SR_PER_LANE;
     int64_t rem = nent - e0;
     int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
     unsigned hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
-    PixW r[SR_PER_LANE];
-#pragma unroll
-    for (int u = 0; u < SR_PER_LANE; ++u) {
-        if (u < nv) r[u] = pixw[e0 + u];
-        else r[u].pix = -1;
-    }
-    double4 m[SR_PER_LANE];
-#pragma unroll
-    for (int u = 0; u < SR_PER_LANE; ++u)
-        if (r[u].pix >= 0) m[u] = *(const double4 *)(m4 + (r[u].pix / ndeg) * 4);
     double prod[SR_PER_LANE][1];
 #pragma unroll
-    for (int u = 0; u < SR_PER_LANE; ++u) {
-        double v = 0.0;
-        if (r[u].pix >= 0)
-            v = NC == 1 ? m[u].x : (double)r[u].w0 * m[u].x + (double)r[u].w1 * m[u].y + (double)r[u].w2 * m[u].z;
-        prod[u][0] = v;
-    }
+    for (int u = 0; u < SR_PER_LANE; ++u) prod[u][0] = tr[w][sr_row(u, lane)];
     int64_t k0 = tile_k0[j];
     bool head0 = (heads[t0 / 8] & 1u) != 0;
     bool cont_last = fix_end[j] >= 0;
@@ -396,11 +426,12 @@ void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const CsrEnt *ent
 {
     int64_t nt = segred_ntiles(nent);
     if (nt == 0) return;
-    unsigned grid = (unsigned)((nt + SR_WAVES - 1) / SR_WAVES);
     if (nmap == 1)
-        k_sr_scatter<1><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, ccinv6, m4, part);
+        k_sr_scatter<1, 4><<<(unsigned)((nt + 3) / 4), 256, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq,
+                                                                    ccinv6, m4, part);
     else
-        k_sr_scatter<3><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, ccinv6, m4, part);
+        k_sr_scatter<3, 2><<<(unsigned)((nt + 1) / 2), 128, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq,
+                                                                    ccinv6, m4, part);
     k_sr_scatter_fix<<<sr_grid(nt, 1 << 30), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part, ccinv6, m4);
 }
 
