@@ -337,6 +337,7 @@ struct Context {
         bool ok = false;
     } sr_s, sr_g;
     DBuf<int32_t> d_nzq;
+    bool tiles_built = false;
     DBuf<double4> d_sr_part;
     // segments / prior
     std::vector<Segment> seg;
@@ -805,6 +806,7 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
     c.stats[ST_T_POINTING] = now() - t0;
     c.stats[ST_NACTIVE] = (double)total;
     c.pointed = true;
+    c.tiles_built = false;
 }
 
 // ------------------------------------------------------------------ solve
@@ -1295,10 +1297,13 @@ static void solve(Context &c, double *signal, int on_device)
                       p.incomplete_matrices, p.allow_decoupling, nullptr);
         c.d_m4x.alloc(c.npix_cross * 4);
         launch_ccmultiply(s, c.nmap, c.npix_cross, c.d_ccinv6x.p, rhs4x, c.d_m4x.p);
-        c.d_tilescratch.alloc(scatter_tiles_scratch_bytes(c.nactive));
-        c.d_tileinfo.alloc(scatter_tiles_info_bytes(c.nactive));
-        launch_tile_info(s, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_tileinfo.p);
-        build_segtiles(c);
+        if (!c.tiles_built) {   // tables of the projections: they depend on the pointing only
+            c.d_tilescratch.alloc(scatter_tiles_scratch_bytes(c.nactive));
+            c.d_tileinfo.alloc(scatter_tiles_info_bytes(c.nactive));
+            launch_tile_info(s, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_tileinfo.p);
+            build_segtiles(c);
+            c.tiles_built = true;
+        }
         c.d_yba.alloc(c.nbtot);
         c.d_nna.alloc(c.nbtot);
         int grp = group_for(c);
