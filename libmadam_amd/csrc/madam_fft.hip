@@ -270,10 +270,10 @@ void fft_plan_init(FftPlan &p, int64_t nof)
 {
     p.N = nof / 2;
     int logN = ilog2(p.N);
-    int l1 = (logN + 1) / 2;
-    if (l1 > 10) l1 = 10;
+    // columns (stride L2, CONV_NCOL per block) no longer than the rows (contiguous, two per block)
+    int l1 = logN / 2;
     int l2 = logN - l1;
-    if (l2 > 10) { l2 = 10; l1 = logN - 10; }
+    if (l2 > 11) { l2 = 11; l1 = logN - 11; }
     p.L1 = 1LL << l1;
     p.L2 = 1LL << l2;
     std::vector<double> tw(2 * p.N), twn(2 * (p.N / 2 + 1));
@@ -338,18 +338,245 @@ static void fft_inverse(hipStream_t s, const FftPlan &p, double2 *d, int64_t bat
                                                                  (const double2 *)p.tw_n, 1);
 }
 
+// ---------------------------------------------------------------- fused convolution
+// Three passes instead of eight (load, 2 x column, 2 x row, spectral, store):
+//   pass 1  forward column FFTs read x - mean straight from the baseline
+//           vector (zero padding never leaves LDS), twiddle, store to work;
+//   pass 2  one block per row pair (k1, L1 - k1): forward row FFTs, the
+//           real-spectrum split / filter multiply / merge (every pair
+//           {k, N - k} has one element in each row of the pair), inverse row
+//           FFTs, store;
+//   pass 3  inverse column FFTs write y = w / nof for the segment's
+//           baselines directly, with deterministic per-block partial sums of
+//           y for the mean, which a last pass subtracts.
+static constexpr int CONV_NCOL = 8;
+
+// grid = (L2 / CONV_NCOL, batch)
+__global__ void k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, const double *partial_in,
+                               double2 *work, int64_t N, int L1, int logL1, int L2, const double2 *tw)
+{
+    extern __shared__ double2 s[];
+    const int ncol = CONV_NCOL;
+    int64_t b = blockIdx.y;
+    int c0 = blockIdx.x * ncol;
+    Segment sg = seg[s0 + b];
+    double mean = seg_mean(partial_in, b, sg.noba);
+    const double *xs = x + sg.b0;
+    int64_t noba = sg.noba;
+    double2 *d = work + b * N;
+    int tot = L1 * ncol;
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int col = t % ncol, n1 = t / ncol;
+        int64_t j = 2 * ((int64_t)n1 * L2 + c0 + col);
+        double2 v;
+        v.x = j < noba ? xs[j] - mean : 0.0;
+        v.y = j + 1 < noba ? xs[j + 1] - mean : 0.0;
+        s[bitrev(n1, logL1) * ncol + col] = v;
+    }
+    __syncthreads();
+    lds_fft(s, L1, logL1, ncol, tw, N / L1, false);
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int col = t % ncol, k1 = t / ncol;
+        int64_t n2 = c0 + col;
+        double2 w = tw[(n2 * k1) % N];
+        d[(int64_t)k1 * L2 + n2] = cmul(s[k1 * ncol + col], w);
+    }
+}
+
+// spectral step on one pair {k, N - k}: X[k] at s[ik], X[N - k] at s[im], 0 < k <= N / 2
+__device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t N, const double2 *twnof,
+                                 const double *f)
+{
+    double2 a = s[ik], bc = s[im];
+    bc.y = -bc.y;
+    double2 E = make_double2(0.5 * (a.x + bc.x), 0.5 * (a.y + bc.y));
+    double2 O = make_double2(0.5 * (a.y - bc.y), -0.5 * (a.x - bc.x));
+    double2 W = twnof[k];
+    double2 WO = cmul(W, O);
+    double2 Xk = make_double2(E.x + WO.x, E.y + WO.y);
+    double2 Xm = make_double2(E.x - WO.x, -(E.y - WO.y));
+    double fk = f[k], fm = f[N - k];
+    double2 Yk = make_double2(Xk.x * fk, Xk.y * fk);
+    double2 Ym = make_double2(Xm.x * fm, Xm.y * fm);
+    double2 s1 = make_double2(Yk.x + Ym.x, Yk.y - Ym.y);
+    double2 d1 = make_double2(Yk.x - Ym.x, Yk.y + Ym.y);
+    double2 t1 = cmulc(d1, W);
+    s[ik] = make_double2(s1.x - t1.y, s1.y + t1.x);
+    if (ik != im) {
+        double2 s2 = make_double2(Ym.x + Yk.x, Ym.y - Yk.y);
+        double2 d2 = make_double2(Ym.x - Yk.x, Ym.y + Yk.y);
+        double2 t2 = cmul(d2, W);
+        s[im] = make_double2(s2.x + t2.y, s2.y - t2.x);
+    }
+}
+
+// grid = (L1 / 2 + 1, batch); LDS 2 * L2 double2, the pair's rows interleaved s[k2 * nr + r]
+__global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2, const double2 *tw,
+                            const double2 *twnof, const double *fc, int64_t fc_stride, const Segment *seg, int64_t s0)
+{
+    extern __shared__ double2 s[];
+    int64_t b = blockIdx.y;
+    int ra = blockIdx.x, rb = (L1 - ra) % L1;
+    int nr = ra == rb ? 1 : 2;
+    double2 *d = work + b * N;
+    const double *f = fc + (int64_t)seg[s0 + b].ipsd * fc_stride;
+    int tot = L2 * nr;
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int r = t / L2, n = t % L2;
+        int row = r == 0 ? ra : rb;
+        s[bitrev(n, logL2) * nr + r] = d[(int64_t)row * L2 + n];
+    }
+    __syncthreads();
+    lds_fft(s, L2, logL2, nr, tw, N / L2, false);
+    // X[ra + L1 k2] is at s[k2 * nr]; X[rb + L1 k2] at s[k2 * nr + 1]
+    if (ra == 0) {
+        for (int k2 = threadIdx.x; k2 <= L2 / 2; k2 += blockDim.x) {
+            if (k2 == 0) {
+                double2 a = s[0];
+                double x0 = a.x + a.y, xn = a.x - a.y;
+                double y0 = x0 * f[0], yn = xn * f[N];
+                s[0] = make_double2(y0 + yn, y0 - yn);
+            } else {
+                conv_pair(s, k2, L2 - k2, (int64_t)L1 * k2, N, twnof, f);
+            }
+        }
+    } else if (nr == 1) {   // ra == L1 / 2
+        for (int k2 = threadIdx.x; k2 < L2 / 2; k2 += blockDim.x)
+            conv_pair(s, k2, L2 - 1 - k2, (int64_t)ra + (int64_t)L1 * k2, N, twnof, f);
+    } else {
+        for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
+            int64_t kA = (int64_t)ra + (int64_t)L1 * k2;
+            int p = k2 * 2, q = (L2 - 1 - k2) * 2 + 1;
+            if (kA <= N / 2) conv_pair(s, p, q, kA, N, twnof, f);
+            else conv_pair(s, q, p, N - kA, N, twnof, f);
+        }
+    }
+    __syncthreads();
+    // back to bit-reversed order for the inverse transform
+    constexpr int PER = 16;   // tot <= 2 * 2048 = PER * FFT_TPB
+    double2 tmp[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        int t = threadIdx.x + u * FFT_TPB;
+        if (t < tot) tmp[u] = s[t];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        int t = threadIdx.x + u * FFT_TPB;
+        if (t < tot) {
+            int n = t / nr, r = t % nr;
+            s[bitrev(n, logL2) * nr + r] = tmp[u];
+        }
+    }
+    __syncthreads();
+    lds_fft(s, L2, logL2, nr, tw, N / L2, true);
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int r = t / L2, n = t % L2;
+        int row = r == 0 ? ra : rb;
+        d[(int64_t)row * L2 + n] = s[n * nr + r];
+    }
+}
+
+// grid = (L2 / CONV_NCOL, batch)
+__global__ void k_conv_col_inv(const double2 *work, const Segment *seg, int64_t s0, double *y, double *partial_out,
+                               int64_t N, int L1, int logL1, int L2, const double2 *tw)
+{
+    extern __shared__ double2 s[];
+    const int ncol = CONV_NCOL;
+    int64_t b = blockIdx.y;
+    int c0 = blockIdx.x * ncol;
+    Segment sg = seg[s0 + b];
+    int64_t noba = sg.noba;
+    const double2 *d = work + b * N;
+    double inv = 1 / (double)(2 * N);
+    int tot = L1 * ncol;
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int col = t % ncol, k1 = t / ncol;
+        int64_t n2 = c0 + col;
+        double2 w = tw[(n2 * k1) % N];
+        s[bitrev(k1, logL1) * ncol + col] = cmulc(d[(int64_t)k1 * L2 + n2], w);
+    }
+    __syncthreads();
+    lds_fft(s, L1, logL1, ncol, tw, N / L1, true);
+    double acc = 0;
+    double *ys = y + sg.b0;
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int col = t % ncol, n1 = t / ncol;
+        int64_t j = 2 * ((int64_t)n1 * L2 + c0 + col);
+        double2 v = s[n1 * ncol + col];
+        if (j < noba) {
+            double a = v.x * inv;
+            ys[j] = a;
+            acc += a;
+        }
+        if (j + 1 < noba) {
+            double a = v.y * inv;
+            ys[j + 1] = a;
+            acc += a;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ double sh[FFT_TPB / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int k = 0; k < FFT_TPB / 64; ++k) t += sh[k];
+        partial_out[b * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+// y[b0 + j] -= mean of the segment's y, from the nblk ordered partials
+__global__ void k_conv_sub_mean(double *y, const Segment *seg, int64_t s0, const double *partial_out, int nblk)
+{
+    int64_t b = blockIdx.y;
+    Segment sg = seg[s0 + b];
+    double t = 0;
+    for (int k = 0; k < nblk; ++k) t += partial_out[b * nblk + k];
+    double mean = t / sg.noba;
+    double *ys = y + sg.b0;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < sg.noba; j += (int64_t)gridDim.x * blockDim.x)
+        ys[j] -= mean;
+}
+
+// Convolution of the batch's segments: the three fused passes when the plan
+// allows them, otherwise the general eight-pass form (short FFTs).
+static bool conv_fusable(const FftPlan &p)
+{
+    return p.L2 >= CONV_NCOL && p.L1 >= 2 && p.L1 * CONV_NCOL <= 8192 && p.L2 <= 2048;
+}
+
 void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg, const double *x,
                            double *y, const double *fc, int64_t fc_stride, double *work, int64_t batch,
                            const Segment *hseg)
 {
     (void)hseg;
     int64_t nof = 2 * p.N;
-    int *ipsd = (int *)(work + batch * nof);                  // batch ints after the batch buffer
-    double *partial = work + batch * nof + (batch + 1) / 2 + 1; // batch * SEG_NB doubles
+    int *ipsd = (int *)(work + batch * nof);                        // batch ints after the batch buffer
+    double *partial = work + batch * nof + (batch + 1) / 2 + 1;    // batch * SEG_NB doubles
+    int logL1 = ilog2(p.L1), logL2 = ilog2(p.L2);
+    int ncb = (int)(p.L2 / CONV_NCOL);
+    double *partial_out = partial + batch * SEG_NB;                 // batch * ncb doubles
+    bool fused = conv_fusable(p);
     for (int64_t s0 = 0; s0 < nseg; s0 += batch) {
         int64_t nb = nseg - s0 < batch ? nseg - s0 : batch;
         dim3 gp(SEG_NB, (unsigned)nb);
         k_seg_partial<<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
+        if (fused) {
+            k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * p.L1 * CONV_NCOL, s>>>(
+                x, dseg, s0, partial, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, (const double2 *)p.tw_n);
+            k_conv_rows<<<dim3((unsigned)(p.L1 / 2 + 1), (unsigned)nb), FFT_TPB, sizeof(double2) * 2 * p.L2, s>>>(
+                (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, (const double2 *)p.tw_n,
+                (const double2 *)p.tw_nof, fc, fc_stride, dseg, s0);
+            k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * p.L1 * CONV_NCOL, s>>>(
+                (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
+                (const double2 *)p.tw_n);
+            unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
+            if (gl > 1024) gl = 1024;
+            k_conv_sub_mean<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
+            continue;
+        }
         unsigned gl = (unsigned)((nof + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
         if (gl > 1024) gl = 1024;
         k_seg_load<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(x, dseg, s0, nof, work, partial);
@@ -362,6 +589,12 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
         k_seg_partial<<<gp, FFT_TPB, 0, s>>>(work, nof, 0, dseg, s0, 1 / (double)nof, partial);
         k_seg_store<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(work, dseg, s0, nof, y, partial);
     }
+}
+
+size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch)
+{
+    return (size_t)batch * 2 * p.N + (size_t)(batch + 1) / 2 + 1 + (size_t)batch * SEG_NB +
+           (size_t)batch * (p.L2 / CONV_NCOL + 1) + 64;
 }
 
 __global__ void k_scale(double *x, double f, int64_t n)

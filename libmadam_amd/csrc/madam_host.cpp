@@ -1318,7 +1318,7 @@ static void solve(Context &c, double *signal, int on_device)
             build_filters(c);
             c.fft_batch = std::min<int64_t>(std::max<int64_t>((int64_t)c.segconv.size(), 1), 64);
             // batch buffer, ipsd ints and the per-segment partial sums
-            c.d_fftwork.alloc((size_t)c.fft_batch * c.nof + c.fft_batch + 2 + (size_t)c.fft_batch * 64);
+            c.d_fftwork.alloc(fft_convolve_work_doubles(c.fft, c.fft_batch));
         }
         build_preconditioner(c);
         MADAM_CHECK(hipStreamSynchronize(s));

@@ -178,6 +178,7 @@ void fft_plan_free(FftPlan &p);
 void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg_dev,
                            const double *x, double *y, const double *fc, int64_t fc_stride, double *work,
                            int64_t batch, const Segment *hseg);
+size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch);
 // inverse real FFT of fcov rows (invcov = c2r(fcov) / nof)
 void fft_c2r_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrows, double *out, double *work);
 // noise filter (build_filter) on the device

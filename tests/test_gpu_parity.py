@@ -224,3 +224,13 @@ def test_bitwise_reproducible():
     assert g1["niter"] == g2["niter"]
     for k in ("map", "binmap", "aa", "yba", "cc"):
         assert np.array_equal(np.asarray(g1[k]).view(np.uint64), np.asarray(g2[k]).view(np.uint64)), k
+
+
+def test_noise_prior_long_fft():
+    """One-sample baselines: 70000 baselines per period, nof = 2^18, so the
+    convolution runs with 256 x 512 four-step plans (the shape class of the
+    2^20 production filters)."""
+    case = ring(prior=True, ndet=2, nsamp=140000, nperiod=2, base=0.02, extra=dict(iter_max=40))
+    g, o, c1, c2 = both(case)
+    assert g["stats"]["nof"] == 2 ** 18
+    check_parity(g, o, c1, c2)
