@@ -230,7 +230,7 @@ def test_noise_prior_long_fft():
     """One-sample baselines: 70000 baselines per period, nof = 2^18, so the
     convolution runs with 256 x 512 four-step plans (the shape class of the
     2^20 production filters)."""
-    case = ring(prior=True, ndet=2, nsamp=140000, nperiod=2, base=0.02, extra=dict(iter_max=40))
+    case = ring(prior=True, ndet=2, nsamp=140000, nperiod=2, base=0.02, extra=dict(iter_max=20, use_fprecond=True))
     g, o, c1, c2 = both(case)
-    assert g["stats"]["nof"] == 2 ** 18
-    check_parity(g, o, c1, c2)
+    assert g["stats"]["nof"] == 2 ** 18 and g["stats"]["precond_kind"] == 2
+    check_parity(g, o, c1, c2, tol=1e-8)
