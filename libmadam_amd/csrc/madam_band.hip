@@ -133,21 +133,45 @@ __global__ void k_band_relayout(const ChunkInfo *ch, int64_t nch, int64_t NCP, i
 // is L(b0_c + i + m, b0_c + i) (m = 0 holds 1 / L(i, i)), so that the lanes of
 // a wave (consecutive chunks c) read consecutive addresses.
 
+// Rows are processed in groups of BAND_PF: the factor rows and right-hand
+// sides of a group are loaded first (they do not depend on the carry state),
+// so each thread keeps BAND_PF rows of loads in flight instead of one.
+// (fewer rows per group for the wide bands, to stay within the register file)
+template <int KD>
+struct BandPF {
+    static constexpr int value = KD <= 19 ? 8 : (KD <= 29 ? 4 : 2);
+};
+
 // forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
 // next KD rows (acc[0] for the first row of the chunk).  If store, writes y.
 template <int KD>
 __device__ inline void fwd_sweep(const double *Li, int64_t NCP, int64_t c, int64_t b0, int n, const double *r,
                                  double *acc, double *y, bool store, bool zero_rhs)
 {
-    const int K = KD + 1;
-    for (int i = 0; i < n; ++i) {
-        const double *row = Li + (int64_t)i * K * NCP + c;
-        double rhs = zero_rhs ? 0.0 : r[b0 + i];
-        double u = (rhs + acc[0]) * row[0];
+    constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
+    for (int i0 = 0; i0 < n; i0 += BAND_PF) {
+        double rw[BAND_PF][K], rh[BAND_PF];
 #pragma unroll
-        for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - row[m * NCP] * u;
-        acc[KD - 1] = -row[KD * NCP] * u;
-        if (store) y[b0 + i] = u;
+        for (int u = 0; u < BAND_PF; ++u) {
+            int i = i0 + u;
+            if (i < n) {
+                const double *row = Li + (int64_t)i * K * NCP + c;
+#pragma unroll
+                for (int m = 0; m < K; ++m) rw[u][m] = row[m * NCP];
+                rh[u] = zero_rhs ? 0.0 : r[b0 + i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BAND_PF; ++u) {
+            int i = i0 + u;
+            if (i < n) {
+                double v = (rh[u] + acc[0]) * rw[u][0];
+#pragma unroll
+                for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - rw[u][m] * v;
+                acc[KD - 1] = -rw[u][KD] * v;
+                if (store) y[b0 + i] = v;
+            }
+        }
     }
 }
 
@@ -156,17 +180,33 @@ template <int KD>
 __device__ inline void bwd_sweep(const double *Li, int64_t NCP, int64_t c, int64_t b0, int n, const double *y,
                                  double *xs, double *x, bool store, bool zero_rhs)
 {
-    const int K = KD + 1;
-    for (int i = n - 1; i >= 0; --i) {
-        const double *row = Li + (int64_t)i * K * NCP + c;
-        double s = zero_rhs ? 0.0 : y[b0 + i];
+    constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
+    for (int i0 = n - 1; i0 >= 0; i0 -= BAND_PF) {
+        double rw[BAND_PF][K], rh[BAND_PF];
 #pragma unroll
-        for (int m = 1; m <= KD; ++m) s -= row[m * NCP] * xs[m - 1];
-        double xi = s * row[0];
+        for (int u = 0; u < BAND_PF; ++u) {
+            int i = i0 - u;
+            if (i >= 0) {
+                const double *row = Li + (int64_t)i * K * NCP + c;
 #pragma unroll
-        for (int m = KD - 1; m > 0; --m) xs[m] = xs[m - 1];
-        xs[0] = xi;
-        if (store) x[b0 + i] = xi;
+                for (int m = 0; m < K; ++m) rw[u][m] = row[m * NCP];
+                rh[u] = zero_rhs ? 0.0 : y[b0 + i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BAND_PF; ++u) {
+            int i = i0 - u;
+            if (i >= 0) {
+                double s = rh[u];
+#pragma unroll
+                for (int m = 1; m <= KD; ++m) s -= rw[u][m] * xs[m - 1];
+                double xi = s * rw[u][0];
+#pragma unroll
+                for (int m = KD - 1; m > 0; --m) xs[m] = xs[m - 1];
+                xs[0] = xi;
+                if (store) x[b0 + i] = xi;
+            }
+        }
     }
 }
 
@@ -221,16 +261,32 @@ __global__ void k_band_chain(const int64_t *seg_first_chunk, int64_t nseg, const
     bool on = lane < KD;
     double st = 0.0;
     int64_t n = c1 - c0;
-    for (int64_t t = 0; t < n; ++t) {
-        int64_t c = dir == 0 ? c0 + t : c1 - 1 - t;
-        if (on) sin[c * KD + lane] = st;
-        double v = on ? s0[c * KD + lane] : 0.0;
-        double tr[KD];
+    // the chunks' s0 and T do not depend on the chained state: load BAND_PF steps at a time
+    constexpr int BAND_PF = BandPF<KD>::value;
+    for (int64_t t0 = 0; t0 < n; t0 += BAND_PF) {
+        double tv[BAND_PF], tr[BAND_PF][KD];
 #pragma unroll
-        for (int k = 0; k < KD; ++k) tr[k] = on ? T[(c * KD + lane) * KD + k] : 0.0;
+        for (int u = 0; u < BAND_PF; ++u) {
+            int64_t t = t0 + u;
+            if (t < n) {
+                int64_t c = dir == 0 ? c0 + t : c1 - 1 - t;
+                tv[u] = on ? s0[c * KD + lane] : 0.0;
 #pragma unroll
-        for (int k = 0; k < KD; ++k) v += tr[k] * __shfl(st, k, 64);
-        st = v;
+                for (int k = 0; k < KD; ++k) tr[u][k] = on ? T[(c * KD + lane) * KD + k] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < BAND_PF; ++u) {
+            int64_t t = t0 + u;
+            if (t < n) {
+                int64_t c = dir == 0 ? c0 + t : c1 - 1 - t;
+                if (on) sin[c * KD + lane] = st;
+                double v = tv[u];
+#pragma unroll
+                for (int k = 0; k < KD; ++k) v += tr[u][k] * __shfl(st, k, 64);
+                st = v;
+            }
+        }
     }
 }
 
