@@ -154,6 +154,9 @@ def main():
     ap.add_argument("--ndet", type=int, default=16)
     ap.add_argument("--log2nsamp", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, copy 1 GiB device-to-device three times (a plain copy kernel of known "
+                         "size, the reference point for the FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every GPU box here has one GPU, and the multi-GPU sharding is "
@@ -203,6 +206,16 @@ def main():
         "setup_s": {"pointing": r["t_pointing"], "bin": r["st_solve"]["t_bin"], "init": r["st_solve"]["t_init"],
                     "precond": r["st_solve"]["t_precond"]},
     }
+    if a.copy_calibration:
+        import torch
+        src = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("bench: copy calibration, 3 x %d bytes read and written" % (src.numel() * 8), file=sys.stderr,
+              flush=True)
+        del src, dst
     if not a.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(2, 2 ** 22, nside, fs, base)
