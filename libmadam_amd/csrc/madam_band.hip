@@ -42,7 +42,7 @@ struct ChunkInfo {
 // active (K x K) window, W[b] = A(j + a, j + b).  Per column: broadcast the
 // pivot, scale the column, rank-1 update, shift the window up by one row.
 template <int KD>
-__global__ void k_band_factor_w(const Segment *seg, int64_t nseg, const int *nband_seg, const double *invcov,
+__global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_t nseg, const int *nband_seg, const double *invcov,
                                 int64_t nof, const double *nna, double *Lb, int *info)
 {
     constexpr int K = KD + 1;
@@ -212,7 +212,7 @@ __device__ inline void bwd_sweep(const double *Li, int64_t NCP, int64_t c, int64
 
 // T matrices: response of the outgoing state to a unit incoming state
 template <int KD>
-__global__ void k_band_response(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, double *Tf,
+__global__ void __launch_bounds__(64) k_band_response(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, double *Tf,
                                 double *Tb)
 {
     int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -234,7 +234,7 @@ __global__ void k_band_response(const ChunkInfo *ch, int64_t nch, int64_t NCP, c
 }
 
 template <int KD>
-__global__ void k_band_fwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *r,
+__global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *r,
                             double *s0)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -251,7 +251,7 @@ __global__ void k_band_fwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const
 // chain carries inside each segment: one wave per segment, lane m < KD holds
 // state component m; s_next = s0[c] + T[c] s, sequential over the chunks
 template <int KD>
-__global__ void k_band_chain(const int64_t *seg_first_chunk, int64_t nseg, const double *s0, const double *T,
+__global__ void __launch_bounds__(64) k_band_chain(const int64_t *seg_first_chunk, int64_t nseg, const double *s0, const double *T,
                              double *sin, int dir)
 {
     int lane = threadIdx.x & 63;
@@ -293,7 +293,7 @@ __global__ void k_band_chain(const int64_t *seg_first_chunk, int64_t nseg, const
 // final forward sweep with true carries writes y into z, then the zero-carry
 // backward sweep of the same chunk gives its backward partial state
 template <int KD>
-__global__ void k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *r,
+__global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *r,
                                  const double *sinf, double *z, double *s0b)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -311,7 +311,7 @@ __global__ void k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, 
 }
 
 template <int KD>
-__global__ void k_band_bwd1(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *sinb,
+__global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *sinb,
                             double *z)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;

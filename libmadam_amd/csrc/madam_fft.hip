@@ -66,6 +66,28 @@ __device__ void lds_fft(double2 *s, int L, int logL, int ncol, const double2 *tw
     }
 }
 
+// Decimation-in-frequency form of lds_fft: natural-order input, output in
+// bit-reversed row order (s[bitrev(k) * ncol + col] = X[k]).
+__device__ void lds_fft_dif(double2 *s, int L, int logL, int ncol, const double2 *tw, int64_t twstride, bool inv)
+{
+    int nbf = (L / 2) * ncol;
+    for (int st = logL - 1; st >= 0; --st) {
+        int half = 1 << st;
+        for (int t = threadIdx.x; t < nbf; t += blockDim.x) {
+            int col = t % ncol;
+            int bf = t / ncol;
+            int grp = bf >> st, j = bf & (half - 1);
+            int i0 = grp * (2 * half) + j, i1 = i0 + half;
+            double2 w = tw[(int64_t)j * (L / (2 * half)) * twstride];
+            if (inv) w.y = -w.y;
+            double2 a = s[i0 * ncol + col], b = s[i1 * ncol + col];
+            s[i0 * ncol + col] = make_double2(a.x + b.x, a.y + b.y);
+            s[i1 * ncol + col] = cmul(make_double2(a.x - b.x, a.y - b.y), w);
+        }
+        __syncthreads();
+    }
+}
+
 // column pass; grid = (L2 / ncol, batch)
 __global__ void k_fft_col(double2 *data, int64_t N, int L1, int logL1, int L2, int ncol, const double2 *tw, int inv)
 {
@@ -452,29 +474,12 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
         }
     }
     __syncthreads();
-    // back to bit-reversed order for the inverse transform
-    constexpr int PER = 16;   // tot <= 2 * 2048 = PER * FFT_TPB
-    double2 tmp[PER];
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        int t = threadIdx.x + u * FFT_TPB;
-        if (t < tot) tmp[u] = s[t];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        int t = threadIdx.x + u * FFT_TPB;
-        if (t < tot) {
-            int n = t / nr, r = t % nr;
-            s[bitrev(n, logL2) * nr + r] = tmp[u];
-        }
-    }
-    __syncthreads();
-    lds_fft(s, L2, logL2, nr, tw, N / L2, true);
+    // inverse row FFTs from natural order; the DIF output is bit-reversed
+    lds_fft_dif(s, L2, logL2, nr, tw, N / L2, true);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t / L2, n = t % L2;
         int row = r == 0 ? ra : rb;
-        d[(int64_t)row * L2 + n] = s[n * nr + r];
+        d[(int64_t)row * L2 + n] = s[bitrev(n, logL2) * nr + r];
     }
 }
 
