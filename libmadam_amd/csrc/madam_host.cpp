@@ -363,8 +363,8 @@ struct Context {
     std::vector<double> rr_hist;
     int profile = 0;
     int pcg_steps_done = 0;
-    // projections: 0 segmented tiles (both sides), 1 lane group per baseline / wave per pixel,
-    // 2 LDS tiles, 3 segmented pixel side + LDS baseline side
+    // projections: 0 segmented pixel side + LDS-tile baseline side, 1 wave per pixel / lane
+    // group per baseline, 2 LDS tiles on both sides, 3 segmented tiles on both sides
     int kernel_variant = 0;
 
     ~Context()
@@ -1191,7 +1191,7 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         done++;
         c.pcg_steps_done++;
         if (prof) { t_it.start(s); t_sc.start(s); }
-        if (c.kernel_variant == 0 || c.kernel_variant == 3)
+        if (c.kernel_variant == 0 || c.kernel_variant == 3)   // segmented tiles (default)
             launch_scatter_seg(s, c.nmap, c.nactive, c.d_ent.p, c.d_pw.p, c.sr_s.heads.p, c.sr_s.k0.p,
                                c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.d_ccinv6x.p, c.d_m4x.p,
                                c.d_sr_part.p);
@@ -1206,7 +1206,7 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         else if (p.diagfilter != 0) launch_axpby_nna(s, c.d_ap.p, c.d_nna.p, c.d_p.p, p.diagfilter, nb);
         else launch_fill(s, c.d_ap.p, 0.0, nb);
         if (prof) { t_pr.stop(s); t_ga.start(s); }
-        if (c.kernel_variant == 0 && c.sr_g.ok)
+        if (c.kernel_variant == 3 && c.sr_g.ok)
             launch_gather_seg(s, c.nmap, (int64_t)c.ndet * c.nsamp, c.d_pixw.p, c.d_m4x.p, c.ndeg, c.sr_g.heads.p,
                               c.sr_g.k0.p, c.sr_g.fix_end.p, c.sr_g.fix_ord.p, c.d_dwb.p, c.d_nna.p, c.d_p.p,
                               c.d_ap.p, c.d_sr_part.p);

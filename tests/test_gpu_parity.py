@@ -117,8 +117,8 @@ def test_noise_prior_band_preconditioner():
 
 @pytest.mark.parametrize("variant", [1, 2, 3])
 def test_noise_prior_band_preconditioner_other_kernels(variant):
-    """the alternative projection kernels: 1 lane group per baseline / wave per
-    pixel, 2 LDS tiles, 3 segmented pixel side with the LDS baseline side"""
+    """the alternative projection kernels: 1 wave per pixel / lane group per
+    baseline, 2 LDS tiles on both sides, 3 segmented tiles on both sides"""
     import libmadam_amd
     case = ring(prior=True, nsamp=30000, nperiod=2)
     c1, c2 = mc.copy_case(case), mc.copy_case(case)
