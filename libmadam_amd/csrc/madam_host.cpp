@@ -337,6 +337,8 @@ struct Context {
         bool ok = false;
     } sr_s, sr_g;
     DBuf<int32_t> d_nzq;
+    DBuf<double> d_ccinv_c;   // inverse cross-pixel matrices in the pixel side's ordinal order
+    int64_t nzpix = 0;
     bool tiles_built = false;
     DBuf<double4> d_sr_part;
     // segments / prior
@@ -843,6 +845,7 @@ static void build_segtiles(Context &c)
         d2h(&last_flag, flag.p + npc - 1, 1, s);
         MADAM_CHECK(hipStreamSynchronize(s));
         int64_t nseg = last_pos + last_flag;
+        c.nzpix = nseg;
         segtiles_alloc(c.sr_s, c.nactive);
         segred_build(s, segstart.p, nseg, c.nactive, c.sr_s.heads.p, c.sr_s.k0.p, c.sr_s.fix_end.p,
                      c.sr_s.fix_ord.p);
@@ -1193,7 +1196,7 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         if (prof) { t_it.start(s); t_sc.start(s); }
         if (c.kernel_variant == 0 || c.kernel_variant == 3)   // segmented tiles (default)
             launch_scatter_seg(s, c.nmap, c.nactive, c.d_ent.p, c.d_pw.p, c.sr_s.heads.p, c.sr_s.k0.p,
-                               c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.d_ccinv6x.p, c.d_m4x.p,
+                               c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.d_ccinv_c.p, c.d_m4x.p,
                                c.d_sr_part.p);
         else if (c.kernel_variant == 2)
             launch_scatter_tiles(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_ent.p, c.d_pw.p,
@@ -1304,6 +1307,8 @@ static void solve(Context &c, double *signal, int on_device)
             build_segtiles(c);
             c.tiles_built = true;
         }
+        c.d_ccinv_c.alloc((size_t)std::max<int64_t>(c.nzpix, 1) * 6);
+        segred_compact_cc(s, c.d_nzq.p, c.nzpix, c.d_ccinv6x.p, c.d_ccinv_c.p);
         c.d_yba.alloc(c.nbtot);
         c.d_nna.alloc(c.nbtot);
         int grp = group_for(c);

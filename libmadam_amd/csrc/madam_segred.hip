@@ -31,6 +31,14 @@ static constexpr int SR_TILE = 64 * SR_PER_LANE;
 static constexpr int SR_TPB = 256;
 static constexpr int SR_WAVES = SR_TPB / 64;
 
+static int sr_grid(int64_t n, int cap = 65535 * 8)
+{
+    int64_t g = (n + SR_TPB - 1) / SR_TPB;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
 __device__ inline int64_t sr_xcd_block(int64_t bid, int64_t nblk)
 {
     // workgroups go round-robin over the 8 XCDs; give each XCD a contiguous range
@@ -168,65 +176,82 @@ static constexpr int SR_PAD = SR_TILE + SR_TILE / 8;
 __device__ inline int sr_col(int u, int lane) { return u * 72 + lane + (lane >> 3); }   // pad(64 u + lane)
 __device__ inline int sr_row(int u, int lane) { return lane * 9 + u; }                  // pad(8 lane + u)
 
+// The transpose goes one component at a time through a 4.6 KB per-wave
+// buffer (higher occupancy than one buffer per component), and the inverse
+// pixel matrices are read in compact ordinal order (cc_c[k] for the k-th
+// non-empty pixel), so finishing a pixel costs one round of independent loads.
 template <int NC, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
                                                            const uint8_t *heads, const int64_t *tile_k0,
                                                            const int64_t *fix_end, const int32_t *nzq,
-                                                           const double *ccinv6, double *m4, double4 *part)
+                                                           const double *cc_c, double *m4, double4 *part)
 {
-    __shared__ double tr[WAVES][NC][SR_PAD];
+    __shared__ double tr[WAVES][SR_PAD];
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * WAVES + w;
     int64_t t0 = j * SR_TILE;
-    bool active = t0 < nent;   // uniform over the wave
+    bool active = t0 < nent;   // uniform over the wave; every wave takes part in the barriers
+    CsrEnt e[SR_PER_LANE];
+    double x[SR_PER_LANE];
     if (active) {
-        CsrEnt e[SR_PER_LANE];
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
             int64_t idx = t0 + u * 64 + lane;
             if (idx < nent) e[u] = ent[idx];
             else { e[u].b = -1; e[u].w0 = e[u].w1 = e[u].w2 = 0.0f; }
         }
-        double x[SR_PER_LANE];
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) x[u] = e[u].b >= 0 ? pw[e[u].b] : 0.0;
+    }
+    double prod[SR_PER_LANE][NC];
 #pragma unroll
-        for (int u = 0; u < SR_PER_LANE; ++u) {
-            if (NC == 1) tr[w][0][sr_col(u, lane)] = x[u];
-            else {
-                tr[w][0][sr_col(u, lane)] = (double)e[u].w0 * x[u];
-                tr[w][NC > 1 ? 1 : 0][sr_col(u, lane)] = (double)e[u].w1 * x[u];
-                tr[w][NC > 2 ? 2 : 0][sr_col(u, lane)] = (double)e[u].w2 * x[u];
+    for (int c = 0; c < NC; ++c) {
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < SR_PER_LANE; ++u) {
+                float wc = NC == 1 ? 1.0f : (c == 0 ? e[u].w0 : (c == 1 ? e[u].w1 : e[u].w2));
+                tr[w][sr_col(u, lane)] = NC == 1 ? x[u] : (double)wc * x[u];
             }
         }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int u = 0; u < SR_PER_LANE; ++u) prod[u][c] = tr[w][sr_row(u, lane)];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (!active) return;
     int64_t e0 = t0 + lane * SR_PER_LANE;
     int64_t rem = nent - e0;
     int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
     unsigned hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
-    double prod[SR_PER_LANE][NC];
-#pragma unroll
-    for (int u = 0; u < SR_PER_LANE; ++u)
-#pragma unroll
-        for (int c = 0; c < NC; ++c) prod[u][c] = tr[w][c][sr_row(u, lane)];
     int64_t k0 = tile_k0[j];
     bool head0 = (heads[t0 / 8] & 1u) != 0;
     bool cont_last = fix_end[j] >= 0;
     const int nmap = NC == 1 ? 1 : 3;
     sr_reduce_tile<NC>(
         prod, hb, lane, k0, head0, cont_last,
-        [&](int64_t k, const double *s) {
-            int64_t q = nzq[k];
-            sr_ccmul_store(nmap, ccinv6 + q * 6, s, m4 + q * 4);
-        },
+        [&](int64_t k, const double *s) { sr_ccmul_store(nmap, cc_c + k * 6, s, m4 + (int64_t)nzq[k] * 4); },
         [&](int slot, const double *s) { sr_part_store(part, j, slot, s, NC); });
+}
+
+// compact copy of the inverse pixel matrices in non-empty-pixel order
+__global__ void k_sr_compact_cc(const int32_t *nzq, int64_t nseg, const double *ccinv6, double *cc_c)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nseg * 6; t += (int64_t)gridDim.x * blockDim.x)
+        cc_c[t] = ccinv6[(int64_t)nzq[t / 6] * 6 + t % 6];
+}
+
+void segred_compact_cc(hipStream_t s, const int32_t *nzq, int64_t nseg, const double *ccinv6, double *cc_c)
+{
+    if (nseg <= 0) return;
+    k_sr_compact_cc<<<sr_grid(nseg * 6), SR_TPB, 0, s>>>(nzq, nseg, ccinv6, cc_c);
+    MADAM_CHECK(hipGetLastError());
 }
 
 // one thread per tile: finish the segment that starts in tile j and continues
 __global__ void k_sr_scatter_fix(int nmap, int64_t ntiles, const int64_t *fix_end, const int64_t *fix_ord,
-                                 const int32_t *nzq, const double4 *part, const double *ccinv6, double *m4)
+                                 const int32_t *nzq, const double4 *part, const double *cc_c, double *m4)
 {
     int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (j >= ntiles) return;
@@ -239,9 +264,9 @@ __global__ void k_sr_scatter_fix(int nmap, int64_t ntiles, const int64_t *fix_en
         a.y += v.y;
         a.z += v.z;
     }
-    int64_t q = nzq[fix_ord[j]];
+    int64_t kk = fix_ord[j];
     double m[3] = {a.x, a.y, a.z};
-    sr_ccmul_store(nmap, ccinv6 + q * 6, m, m4 + q * 4);
+    sr_ccmul_store(nmap, cc_c + kk * 6, m, m4 + (int64_t)nzq[kk] * 4);
 }
 
 // ------------------------------------------------------------ baseline side
@@ -341,13 +366,6 @@ __global__ void k_sr_fill_i64(int64_t *x, int64_t v, int64_t n)
         x[i] = v;
 }
 
-static int sr_grid(int64_t n, int cap = 65535 * 8)
-{
-    int64_t g = (n + SR_TPB - 1) / SR_TPB;
-    if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return (int)g;
-}
 
 int64_t segred_ntiles(int64_t nent) { return (nent + SR_TILE - 1) / SR_TILE; }
 
@@ -422,17 +440,18 @@ void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbas
 
 void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const CsrEnt *ent, const double *pw,
                         const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
-                        const int32_t *nzq, const double *ccinv6, double *m4, double4 *part)
+                        const int32_t *nzq, const double *cc_c, double *m4, double4 *part)
 {
     int64_t nt = segred_ntiles(nent);
     if (nt == 0) return;
+    unsigned grid = (unsigned)((nt + SR_WAVES - 1) / SR_WAVES);
     if (nmap == 1)
-        k_sr_scatter<1, 4><<<(unsigned)((nt + 3) / 4), 256, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq,
-                                                                    ccinv6, m4, part);
+        k_sr_scatter<1, SR_WAVES><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, cc_c, m4,
+                                                          part);
     else
-        k_sr_scatter<3, 2><<<(unsigned)((nt + 1) / 2), 128, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq,
-                                                                    ccinv6, m4, part);
-    k_sr_scatter_fix<<<sr_grid(nt, 1 << 30), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part, ccinv6, m4);
+        k_sr_scatter<3, SR_WAVES><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, cc_c, m4,
+                                                          part);
+    k_sr_scatter_fix<<<sr_grid(nt, 1 << 30), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part, cc_c, m4);
 }
 
 void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,
