@@ -12,9 +12,12 @@
 // into chunks of CH rows: a chunk's sweep is an affine function of the KD-wide
 // carry state entering it, s_out = s0_out + T s_in, with T precomputed per
 // chunk.  A solve is then: sweep every chunk with zero carry (parallel), chain
-// the carries (sequential over chunks, KD x KD mat-vec each), sweep again with
-// the true carries (parallel).  Segments with a smaller band are zero-padded
-// to the largest KD.
+// the carries, sweep again with the true carries (parallel).  The chain is
+// itself split in three: groups of G chunks chain from zero in parallel, one
+// wave per segment chains the groups with the precomputed group products
+// T_last ... T_first, and the groups chain again from their true carries, so
+// the sequential depth is 2 G + nchunk / G instead of nchunk.  Segments with a
+// smaller band are zero-padded to the largest KD.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -26,7 +29,8 @@
 
 namespace madam {
 
-static constexpr int BAND_CH = 512;
+static constexpr int BAND_CH = 64;    // rows per chunk (one thread each in the sweeps)
+static constexpr int BAND_G = 64;     // chunks per group of the carry chain
 
 struct ChunkInfo {
     int64_t b0;    // first global baseline row of the chunk
@@ -250,43 +254,106 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
 
 // chain carries inside each segment: one wave per segment, lane m < KD holds
 // state component m; s_next = s0[c] + T[c] s, sequential over the chunks
+// Group tables: group g covers chunks gfirst[g] .. gfirst[g] + gn[g] - 1 of one segment;
+// the segment's groups are sfirst[s] .. sfirst[s + 1] - 1.
+// Wave per group: chain the group's chunks (increasing order for dir 0,
+// decreasing for dir 1) starting from zero (cg[g] = outgoing state) or from the
+// group's true incoming state Sg[g] (writing every chunk's incoming state sin).
 template <int KD>
-__global__ void __launch_bounds__(64) k_band_chain(const int64_t *seg_first_chunk, int64_t nseg, const double *s0, const double *T,
-                             double *sin, int dir)
+__global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, const int32_t *gn, int64_t ngrp,
+                                                         const double *s0, const double *T, int dir,
+                                                         const double *Sg, double *sin, double *cg)
 {
     int lane = threadIdx.x & 63;
-    int64_t s = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-    if (s >= nseg) return;
-    int64_t c0 = seg_first_chunk[s], c1 = seg_first_chunk[s + 1];
+    int64_t g = blockIdx.x;
+    if (g >= ngrp) return;
     bool on = lane < KD;
-    double st = 0.0;
-    int64_t n = c1 - c0;
-    // the chunks' s0 and T do not depend on the chained state: load BAND_PF steps at a time
-    constexpr int BAND_PF = BandPF<KD>::value;
-    for (int64_t t0 = 0; t0 < n; t0 += BAND_PF) {
-        double tv[BAND_PF], tr[BAND_PF][KD];
+    int64_t c0 = gfirst[g];
+    int n = gn[g];
+    double st = (Sg && on) ? Sg[g * KD + lane] : 0.0;
+    constexpr int PF = BandPF<KD>::value;
+    for (int t0 = 0; t0 < n; t0 += PF) {
+        double tv[PF], tr[PF][KD];
 #pragma unroll
-        for (int u = 0; u < BAND_PF; ++u) {
-            int64_t t = t0 + u;
+        for (int u = 0; u < PF; ++u) {
+            int t = t0 + u;
             if (t < n) {
-                int64_t c = dir == 0 ? c0 + t : c1 - 1 - t;
+                int64_t c = dir == 0 ? c0 + t : c0 + n - 1 - t;
                 tv[u] = on ? s0[c * KD + lane] : 0.0;
 #pragma unroll
-                for (int k = 0; k < KD; ++k) tr[u][k] = on ? T[(c * KD + lane) * KD + k] : 0.0;
+                for (int q = 0; q < KD; ++q) tr[u][q] = on ? T[(c * KD + lane) * KD + q] : 0.0;
             }
         }
 #pragma unroll
-        for (int u = 0; u < BAND_PF; ++u) {
-            int64_t t = t0 + u;
+        for (int u = 0; u < PF; ++u) {
+            int t = t0 + u;
             if (t < n) {
-                int64_t c = dir == 0 ? c0 + t : c1 - 1 - t;
-                if (on) sin[c * KD + lane] = st;
+                int64_t c = dir == 0 ? c0 + t : c0 + n - 1 - t;
+                if (sin && on) sin[c * KD + lane] = st;
                 double v = tv[u];
 #pragma unroll
-                for (int k = 0; k < KD; ++k) v += tr[u][k] * __shfl(st, k, 64);
+                for (int q = 0; q < KD; ++q) v += tr[u][q] * __shfl(st, q, 64);
                 st = v;
             }
         }
+    }
+    if (cg && on) cg[g * KD + lane] = st;
+}
+
+// wave per segment: S_{g+1} = cg[g] + Gm[g] S_g over the segment's groups in chain order
+template <int KD>
+__global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, int64_t nseg, const double *cg,
+                                                       const double *Gm, int dir, double *Sg)
+{
+    int lane = threadIdx.x & 63;
+    int64_t s = blockIdx.x;
+    if (s >= nseg) return;
+    bool on = lane < KD;
+    int64_t g0 = sfirst[s], g1 = sfirst[s + 1];
+    double st = 0.0;
+    for (int64_t t = 0; t < g1 - g0; ++t) {
+        int64_t g = dir == 0 ? g0 + t : g1 - 1 - t;
+        if (on) Sg[g * KD + lane] = st;
+        double v = on ? cg[g * KD + lane] : 0.0;
+#pragma unroll
+        for (int q = 0; q < KD; ++q) v += (on ? Gm[(g * KD + lane) * KD + q] : 0.0) * __shfl(st, q, 64);
+        st = v;
+    }
+}
+
+// wave per group: Gm = T_last ... T_first in chain order (lane m holds row m)
+template <int KD>
+__global__ void __launch_bounds__(64) k_band_group_mats(const int64_t *gfirst, const int32_t *gn, int64_t ngrp,
+                                                        const double *T, int dir, double *Gm)
+{
+    int lane = threadIdx.x & 63;
+    int64_t g = blockIdx.x;
+    if (g >= ngrp) return;
+    bool on = lane < KD;
+    double M[KD];
+#pragma unroll
+    for (int j = 0; j < KD; ++j) M[j] = (on && j == lane) ? 1.0 : 0.0;
+    int64_t c0 = gfirst[g];
+    int n = gn[g];
+    for (int t = 0; t < n; ++t) {
+        int64_t c = dir == 0 ? c0 + t : c0 + n - 1 - t;
+        double tr[KD];
+#pragma unroll
+        for (int q = 0; q < KD; ++q) tr[q] = on ? T[(c * KD + lane) * KD + q] : 0.0;
+        double Mn[KD];
+#pragma unroll
+        for (int j = 0; j < KD; ++j) {
+            double v = 0.0;
+#pragma unroll
+            for (int q = 0; q < KD; ++q) v += tr[q] * __shfl(M[j], q, 64);
+            Mn[j] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < KD; ++j) M[j] = Mn[j];
+    }
+    if (on) {
+#pragma unroll
+        for (int j = 0; j < KD; ++j) Gm[(g * KD + lane) * KD + j] = M[j];
     }
 }
 
@@ -332,6 +399,11 @@ struct BandImpl {
     int64_t *seg_first = nullptr;
     double *s0f = nullptr, *s0b = nullptr, *sinf = nullptr, *sinb = nullptr;
     int64_t nseg_used = 0;
+    // carry-chain groups
+    int64_t ngrp = 0;
+    int64_t *gfirst = nullptr, *sgfirst = nullptr;
+    int32_t *gn = nullptr;
+    double *Gf = nullptr, *Gb = nullptr, *cg = nullptr, *Sg = nullptr;
 };
 
 static int kd_class(int kd)
@@ -403,6 +475,35 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
     bp.nchunk = (int64_t)ch.size();
     BandImpl *im = new BandImpl();
     im->nseg_used = (int64_t)first.size() - 1;
+    // groups of BAND_G chunks inside each segment
+    std::vector<int64_t> gfirst, sgfirst;
+    std::vector<int32_t> gn;
+    for (int64_t si = 0; si < im->nseg_used; ++si) {
+        sgfirst.push_back((int64_t)gfirst.size());
+        for (int64_t c = first[si]; c < first[si + 1]; c += BAND_G) {
+            gfirst.push_back(c);
+            gn.push_back((int32_t)std::min<int64_t>(BAND_G, first[si + 1] - c));
+        }
+    }
+    sgfirst.push_back((int64_t)gfirst.size());
+    im->ngrp = (int64_t)gfirst.size();
+    {
+        int64_t ng = im->ngrp > 0 ? im->ngrp : 1;
+        MADAM_CHECK(hipMalloc(&im->gfirst, sizeof(int64_t) * ng));
+        MADAM_CHECK(hipMalloc(&im->gn, sizeof(int32_t) * ng));
+        MADAM_CHECK(hipMalloc(&im->sgfirst, sizeof(int64_t) * sgfirst.size()));
+        MADAM_CHECK(hipMalloc(&im->Gf, sizeof(double) * ng * KD * KD));
+        MADAM_CHECK(hipMalloc(&im->Gb, sizeof(double) * ng * KD * KD));
+        MADAM_CHECK(hipMalloc(&im->cg, sizeof(double) * ng * KD));
+        MADAM_CHECK(hipMalloc(&im->Sg, sizeof(double) * ng * KD));
+        if (im->ngrp > 0) {
+            MADAM_CHECK(hipMemcpyAsync(im->gfirst, gfirst.data(), sizeof(int64_t) * gfirst.size(),
+                                       hipMemcpyHostToDevice, s));
+            MADAM_CHECK(hipMemcpyAsync(im->gn, gn.data(), sizeof(int32_t) * gn.size(), hipMemcpyHostToDevice, s));
+        }
+        MADAM_CHECK(hipMemcpyAsync(im->sgfirst, sgfirst.data(), sizeof(int64_t) * sgfirst.size(),
+                                   hipMemcpyHostToDevice, s));
+    }
     int64_t nc = bp.nchunk > 0 ? bp.nchunk : 1;
     MADAM_CHECK(hipMalloc(&im->ch, sizeof(ChunkInfo) * nc));
     MADAM_CHECK(hipMalloc(&im->seg_first, sizeof(int64_t) * first.size()));
@@ -429,6 +530,13 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
         BAND_DISPATCH(KD, (k_band_response<KD><<<(unsigned)((nt + 63) / 64), 64, 0, s>>>(im->ch, bp.nchunk, im->NCP,
                                                                                          im->Li, bp.Tf, bp.Tb)));
         MADAM_CHECK(hipGetLastError());
+        if (im->ngrp > 0) {
+            BAND_DISPATCH(KD, ({
+                k_band_group_mats<KD><<<(unsigned)im->ngrp, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, bp.Tf, 0, im->Gf);
+                k_band_group_mats<KD><<<(unsigned)im->ngrp, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, bp.Tb, 1, im->Gb);
+            }));
+            MADAM_CHECK(hipGetLastError());
+        }
     }
     bp.chunk_b0 = (int64_t *)im;   // opaque implementation pointer
     MADAM_CHECK(hipStreamSynchronize(s));
@@ -442,11 +550,20 @@ void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
     int KD = bp.kd;
     unsigned g = (unsigned)((bp.nchunk + 63) / 64);
     unsigned gs = (unsigned)im->nseg_used;   // one 64-lane wave per segment
+    unsigned gg = (unsigned)im->ngrp;        // one wave per chunk group
     BAND_DISPATCH(KD, ({
         k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, r, im->s0f);
-        k_band_chain<KD><<<gs, 64, 0, s>>>(im->seg_first, im->nseg_used, im->s0f, bp.Tf, im->sinf, 0);
+        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, 0, nullptr, nullptr,
+                                                 im->cg);
+        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, 0, im->Sg);
+        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, 0, im->Sg, im->sinf,
+                                                 nullptr);
         k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, r, im->sinf, z, im->s0b);
-        k_band_chain<KD><<<gs, 64, 0, s>>>(im->seg_first, im->nseg_used, im->s0b, bp.Tb, im->sinb, 1);
+        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, 1, nullptr, nullptr,
+                                                 im->cg);
+        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, 1, im->Sg);
+        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, 1, im->Sg, im->sinb,
+                                                 nullptr);
         k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, im->sinb, z);
     }));
 }
@@ -462,6 +579,13 @@ void band_free(BandPrec &bp)
         (void)hipFree(im->s0b);
         (void)hipFree(im->sinf);
         (void)hipFree(im->sinb);
+        (void)hipFree(im->gfirst);
+        (void)hipFree(im->gn);
+        (void)hipFree(im->sgfirst);
+        (void)hipFree(im->Gf);
+        (void)hipFree(im->Gb);
+        (void)hipFree(im->cg);
+        (void)hipFree(im->Sg);
         delete im;
     }
     if (bp.L) (void)hipFree(bp.L);

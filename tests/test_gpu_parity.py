@@ -234,3 +234,12 @@ def test_noise_prior_long_fft():
     g, o, c1, c2 = both(case)
     assert g["stats"]["nof"] == 2 ** 18 and g["stats"]["precond_kind"] == 2
     check_parity(g, o, c1, c2, tol=1e-8)
+
+
+def test_band_preconditioner_chunk_groups():
+    """5000-baseline segments: 79 chunks of 64 rows in two chain groups per
+    segment, so the grouped carry chain (group products, top-level chain,
+    second local pass) is exercised."""
+    g, o, c1, c2 = both(ring(prior=True, ndet=2, nsamp=500000, nperiod=2))
+    assert g["stats"]["precond_kind"] == 3 and g["stats"]["nband"] >= 2
+    check_parity(g, o, c1, c2)
