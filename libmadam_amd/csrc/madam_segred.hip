@@ -193,7 +193,18 @@ __global__ void __launch_bounds__(64 * WAVES) k_sr_scatter(int64_t nent, const C
     bool active = t0 < nent;   // uniform over the wave; every wave takes part in the barriers
     CsrEnt e[SR_PER_LANE];
     double x[SR_PER_LANE];
+    int64_t k0 = 0;
+    bool head0 = false, cont_last = false;
+    unsigned hb = 0u;
+    int64_t e0 = t0 + lane * SR_PER_LANE;
     if (active) {
+        // tile metadata first: these loads are independent of the entry stream
+        int64_t rem = nent - e0;
+        int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
+        hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
+        k0 = tile_k0[j];
+        head0 = (heads[t0 / 8] & 1u) != 0;
+        cont_last = fix_end[j] >= 0;
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
             int64_t idx = t0 + u * 64 + lane;
@@ -221,13 +232,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_sr_scatter(int64_t nent, const C
         __syncthreads();
     }
     if (!active) return;
-    int64_t e0 = t0 + lane * SR_PER_LANE;
-    int64_t rem = nent - e0;
-    int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
-    unsigned hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
-    int64_t k0 = tile_k0[j];
-    bool head0 = (heads[t0 / 8] & 1u) != 0;
-    bool cont_last = fix_end[j] >= 0;
     const int nmap = NC == 1 ? 1 : 3;
     sr_reduce_tile<NC>(
         prod, hb, lane, k0, head0, cont_last,
