@@ -824,28 +824,14 @@ __global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t
     int64_t S0 = bs[0], S1 = bs[nb];
     for (int64_t t0 = S0; t0 < S1; t0 += FLAT_TILE) {
         int64_t t1 = t0 + FLAT_TILE < S1 ? t0 + FLAT_TILE : S1;
-        // rounds of FLAT_ROUND samples per thread: all stream loads of a round, then all map gathers
-        constexpr int FLAT_ROUND = 4;
-        for (int64_t gb = t0; gb < t1; gb += (int64_t)FLAT_ROUND * 256) {
-            PixW r[FLAT_ROUND];
-            double4 m[FLAT_ROUND];
-#pragma unroll
-            for (int u = 0; u < FLAT_ROUND; ++u) {
-                int64_t g = gb + u * 256 + threadIdx.x;
-                if (g < t1) r[u] = pixw[g];
-                else r[u].pix = -1;
+        for (int64_t g = t0 + threadIdx.x; g < t1; g += blockDim.x) {
+            PixW r = pixw[g];
+            double v = 0.0;
+            if (r.pix >= 0) {
+                double4 m = *(const double4 *)(m4 + (r.pix / ndeg) * 4);
+                v = nmap == 1 ? m.x : (double)r.w0 * m.x + (double)r.w1 * m.y + (double)r.w2 * m.z;
             }
-#pragma unroll
-            for (int u = 0; u < FLAT_ROUND; ++u)
-                if (r[u].pix >= 0) m[u] = *(const double4 *)(m4 + (r[u].pix / ndeg) * 4);
-#pragma unroll
-            for (int u = 0; u < FLAT_ROUND; ++u) {
-                int64_t g = gb + u * 256 + threadIdx.x;
-                double v = 0.0;
-                if (r[u].pix >= 0)
-                    v = nmap == 1 ? m[u].x : (double)r[u].w0 * m[u].x + (double)r[u].w1 * m[u].y + (double)r[u].w2 * m[u].z;
-                if (g < t1) vals[g - t0] = v;
-            }
+            vals[g - t0] = v;
         }
         __syncthreads();
         if (item < nb) {
