@@ -1196,7 +1196,7 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         if (prof) { t_it.start(s); t_sc.start(s); }
         if (c.kernel_variant == 0 || c.kernel_variant == 3)   // segmented tiles (default)
             launch_scatter_seg(s, c.nmap, c.nactive, c.d_ent.p, c.d_pw.p, c.sr_s.heads.p, c.sr_s.k0.p,
-                               c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.d_ccinv_c.p, c.d_m4x.p,
+                               c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.nzpix, c.d_ccinv_c.p, c.d_m4x.p,
                                c.d_sr_part.p);
         else if (c.kernel_variant == 2)
             launch_scatter_tiles(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_ent.p, c.d_pw.p,

@@ -109,7 +109,7 @@ void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbas
                               int64_t *segstart);
 void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const CsrEnt *ent, const double *pw,
                         const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
-                        const int32_t *nzq, const double *cc_c, double *m4, double4 *part);
+                        const int32_t *nzq, int64_t nzpix, const double *cc_c, double *m4, double4 *part);
 // cc_c[k] = ccinv6[nzq[k]]: the inverse pixel matrices in the pixel side's ordinal order
 void segred_compact_cc(hipStream_t s, const int32_t *nzq, int64_t nseg, const double *ccinv6, double *cc_c);
 void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,

@@ -818,8 +818,17 @@ __global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t
         bs[t] = t < nb ? base_gstart(bstart, nbase, nsamp, b)
                        : base_gstart(bstart, nbase, nsamp, b - 1) + (bstart[(b - 1) % nbase + 1] - bstart[(b - 1) % nbase]);
     }
-    __syncthreads();
     int item = threadIdx.x / FLAT_LANES, sub = threadIdx.x % FLAT_LANES;
+    // the per-baseline terms of the result, loaded before the sample stream
+    double pre_nna = 0.0, pre_p = 0.0, pre_dwb = 0.0, pre_ap = 0.0;
+    if (item < nb && sub == 0) {
+        int64_t b = b0 + item;
+        pre_nna = nna[b];
+        pre_p = p[b];
+        pre_dwb = dwb[b];
+        pre_ap = ap[b];
+    }
+    __syncthreads();
     double acc = 0.0;
     int64_t S0 = bs[0], S1 = bs[nb];
     for (int64_t t0 = S0; t0 < S1; t0 += FLAT_TILE) {
@@ -842,9 +851,8 @@ __global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t
     }
     acc = group_sum<FLAT_LANES>(acc);
     if (item < nb && sub == 0) {
-        int64_t b = b0 + item;
-        double v = nna[b] * p[b] - acc * dwb[b];
-        ap[b] = ap[b] + v;
+        double v = pre_nna * pre_p - acc * pre_dwb;
+        ap[b0 + item] = pre_ap + v;
     }
 }
 

@@ -181,12 +181,15 @@ __device__ inline int sr_row(int u, int lane) { return lane * 9 + u; }          
 // pixel matrices are read in compact ordinal order (cc_c[k] for the k-th
 // non-empty pixel), so finishing a pixel costs one round of independent loads.
 template <int NC, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(5, 8))) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
                                                            const uint8_t *heads, const int64_t *tile_k0,
                                                            const int64_t *fix_end, const int32_t *nzq,
-                                                           const double *cc_c, double *m4, double4 *part)
+                                                           int64_t nzpix, const double *cc_c, double *m4,
+                                                           double4 *part)
 {
     __shared__ double tr[WAVES][SR_PAD];
+    __shared__ double ccs[WAVES][64 * 6];   // inverse matrices of the tile's first 64 pixels
+    __shared__ int32_t qs[WAVES][64];       // and their pixel indices
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * WAVES + w;
     int64_t t0 = j * SR_TILE;
@@ -213,6 +216,12 @@ __global__ void __launch_bounds__(64 * WAVES) k_sr_scatter(int64_t nent, const C
         }
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) x[u] = e[u].b >= 0 ? pw[e[u].b] : 0.0;
+        // finishing data of the tile's pixels (ordinals k0 .. k0 + 63), with the gathers
+        if (k0 + lane < nzpix) {
+            qs[w][lane] = nzq[k0 + lane];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ccs[w][lane * 6 + q] = cc_c[(k0 + lane) * 6 + q];
+        }
     }
     double prod[SR_PER_LANE][NC];
 #pragma unroll
@@ -235,7 +244,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_sr_scatter(int64_t nent, const C
     const int nmap = NC == 1 ? 1 : 3;
     sr_reduce_tile<NC>(
         prod, hb, lane, k0, head0, cont_last,
-        [&](int64_t k, const double *s) { sr_ccmul_store(nmap, cc_c + k * 6, s, m4 + (int64_t)nzq[k] * 4); },
+        [&](int64_t k, const double *s) {
+            int64_t loc = k - k0;
+            if (loc < 64) sr_ccmul_store(nmap, &ccs[w][loc * 6], s, m4 + (int64_t)qs[w][loc] * 4);
+            else sr_ccmul_store(nmap, cc_c + k * 6, s, m4 + (int64_t)nzq[k] * 4);
+        },
         [&](int slot, const double *s) { sr_part_store(part, j, slot, s, NC); });
 }
 
@@ -444,17 +457,17 @@ void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbas
 
 void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const CsrEnt *ent, const double *pw,
                         const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
-                        const int32_t *nzq, const double *cc_c, double *m4, double4 *part)
+                        const int32_t *nzq, int64_t nzpix, const double *cc_c, double *m4, double4 *part)
 {
     int64_t nt = segred_ntiles(nent);
     if (nt == 0) return;
     unsigned grid = (unsigned)((nt + SR_WAVES - 1) / SR_WAVES);
     if (nmap == 1)
-        k_sr_scatter<1, SR_WAVES><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, cc_c, m4,
-                                                          part);
+        k_sr_scatter<1, SR_WAVES><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, nzpix, cc_c,
+                                                          m4, part);
     else
-        k_sr_scatter<3, SR_WAVES><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, cc_c, m4,
-                                                          part);
+        k_sr_scatter<3, SR_WAVES><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, nzpix, cc_c,
+                                                          m4, part);
     k_sr_scatter_fix<<<sr_grid(nt, 1 << 30), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part, cc_c, m4);
 }
 
