@@ -38,6 +38,12 @@ __device__ inline double2 cmulc(double2 a, double2 b)   // a * conj(b)
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
+// W^m from a two-level table: a[m % 1024] * b[m / 1024]
+__device__ inline double2 tw2(const double2 *a, const double2 *b, int64_t m)
+{
+    return cmul(a[m & 1023], b[m >> 10]);
+}
+
 __device__ inline unsigned bitrev(unsigned v, int bits)
 {
     return __brev(v) >> (32 - bits);
@@ -45,8 +51,8 @@ __device__ inline unsigned bitrev(unsigned v, int bits)
 
 // In-place radix-2 FFT of `ncol` interleaved columns of length L held in LDS
 // as s[row * ncol + col]; input already in bit-reversed row order.
-// tw = exp(-2 pi i j / N) table, stride N / L gives W_L.
-__device__ void lds_fft(double2 *s, int L, int logL, int ncol, const double2 *tw, int64_t twstride, bool inv)
+// twL[j] = W_L^j for j < L / 2.
+__device__ void lds_fft(double2 *s, int L, int logL, int ncol, const double2 *twL, bool inv)
 {
     int nbf = (L / 2) * ncol;
     for (int st = 0; st < logL; ++st) {
@@ -56,7 +62,7 @@ __device__ void lds_fft(double2 *s, int L, int logL, int ncol, const double2 *tw
             int bf = t / ncol;
             int grp = bf >> st, j = bf & (half - 1);
             int i0 = grp * (2 * half) + j, i1 = i0 + half;
-            double2 w = tw[(int64_t)j * (L / (2 * half)) * twstride];
+            double2 w = twL[j * (L / (2 * half))];
             if (inv) w.y = -w.y;
             double2 a = s[i0 * ncol + col], b = cmul(s[i1 * ncol + col], w);
             s[i0 * ncol + col] = make_double2(a.x + b.x, a.y + b.y);
@@ -68,7 +74,7 @@ __device__ void lds_fft(double2 *s, int L, int logL, int ncol, const double2 *tw
 
 // Decimation-in-frequency form of lds_fft: natural-order input, output in
 // bit-reversed row order (s[bitrev(k) * ncol + col] = X[k]).
-__device__ void lds_fft_dif(double2 *s, int L, int logL, int ncol, const double2 *tw, int64_t twstride, bool inv)
+__device__ void lds_fft_dif(double2 *s, int L, int logL, int ncol, const double2 *twL, bool inv)
 {
     int nbf = (L / 2) * ncol;
     for (int st = logL - 1; st >= 0; --st) {
@@ -78,7 +84,7 @@ __device__ void lds_fft_dif(double2 *s, int L, int logL, int ncol, const double2
             int bf = t / ncol;
             int grp = bf >> st, j = bf & (half - 1);
             int i0 = grp * (2 * half) + j, i1 = i0 + half;
-            double2 w = tw[(int64_t)j * (L / (2 * half)) * twstride];
+            double2 w = twL[j * (L / (2 * half))];
             if (inv) w.y = -w.y;
             double2 a = s[i0 * ncol + col], b = s[i1 * ncol + col];
             s[i0 * ncol + col] = make_double2(a.x + b.x, a.y + b.y);
@@ -89,13 +95,12 @@ __device__ void lds_fft_dif(double2 *s, int L, int logL, int ncol, const double2
 }
 
 // column pass; grid = (L2 / ncol, batch)
-__global__ void k_fft_col(double2 *data, int64_t N, int L1, int logL1, int L2, int ncol, const double2 *tw, int inv)
+__global__ void k_fft_col(double2 *data, int64_t N, int L1, int logL1, int L2, int ncol, FftPlan P, int inv)
 {
     extern __shared__ double2 s[];
     int64_t b = blockIdx.y;
     int c0 = blockIdx.x * ncol;
     double2 *d = data + b * N;
-    int64_t twstride = N / L1;
     int tot = L1 * ncol;
     if (!inv) {
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
@@ -103,22 +108,22 @@ __global__ void k_fft_col(double2 *data, int64_t N, int L1, int logL1, int L2, i
             s[bitrev(n1, logL1) * ncol + col] = d[(int64_t)n1 * L2 + c0 + col];
         }
         __syncthreads();
-        lds_fft(s, L1, logL1, ncol, tw, twstride, false);
+        lds_fft(s, L1, logL1, ncol, P.sub1, false);
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
             int col = t % ncol, k1 = t / ncol;
             int64_t n2 = c0 + col;
-            double2 w = tw[(n2 * k1) % N];
+            double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
             d[(int64_t)k1 * L2 + n2] = cmul(s[k1 * ncol + col], w);
         }
     } else {
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
             int col = t % ncol, k1 = t / ncol;
             int64_t n2 = c0 + col;
-            double2 w = tw[(n2 * k1) % N];
+            double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
             s[bitrev(k1, logL1) * ncol + col] = cmulc(d[(int64_t)k1 * L2 + n2], w);
         }
         __syncthreads();
-        lds_fft(s, L1, logL1, ncol, tw, twstride, true);
+        lds_fft(s, L1, logL1, ncol, P.sub1, true);
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
             int col = t % ncol, n1 = t / ncol;
             d[(int64_t)n1 * L2 + c0 + col] = s[n1 * ncol + col];
@@ -127,13 +132,12 @@ __global__ void k_fft_col(double2 *data, int64_t N, int L1, int logL1, int L2, i
 }
 
 // row pass; grid = (L1 / nrow, batch); rows are contiguous runs of L2
-__global__ void k_fft_row(double2 *data, int64_t N, int L1, int L2, int logL2, int nrow, const double2 *tw, int inv)
+__global__ void k_fft_row(double2 *data, int64_t N, int L1, int L2, int logL2, int nrow, FftPlan P, int inv)
 {
     extern __shared__ double2 s[];
     int64_t b = blockIdx.y;
     int r0 = blockIdx.x * nrow;
     double2 *d = data + b * N;
-    int64_t twstride = N / L2;
     int tot = L2 * nrow;
     // s[n * nrow + row]
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
@@ -141,7 +145,7 @@ __global__ void k_fft_row(double2 *data, int64_t N, int L1, int L2, int logL2, i
         s[bitrev(n, logL2) * nrow + row] = d[(int64_t)(r0 + row) * L2 + n];
     }
     __syncthreads();
-    lds_fft(s, L2, logL2, nrow, tw, twstride, inv != 0);
+    lds_fft(s, L2, logL2, nrow, P.sub2, inv != 0);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int row = t / L2, n = t % L2;
         d[(int64_t)(r0 + row) * L2 + n] = s[n * nrow + row];
@@ -155,7 +159,7 @@ __device__ inline int64_t kidx(int64_t k, int L1, int L2)
 }
 
 // real-spectrum split, filter multiply, merge; grid = (ceil((N/2+1)/TPB), batch)
-__global__ void k_spectral(double2 *data, int64_t N, int L1, int L2, const double2 *twnof, const double *fc,
+__global__ void k_spectral(double2 *data, int64_t N, int L1, int L2, FftPlan P, const double *fc,
                            int64_t fc_stride, const int *ipsd_of_batch)
 {
     int64_t b = blockIdx.y;
@@ -174,7 +178,7 @@ __global__ void k_spectral(double2 *data, int64_t N, int L1, int L2, const doubl
         bc.y = -bc.y;   // conj(Z_{N-k})
         double2 E = make_double2(0.5 * (a.x + bc.x), 0.5 * (a.y + bc.y));
         double2 O = make_double2(0.5 * (a.y - bc.y), -0.5 * (a.x - bc.x));
-        double2 W = twnof[k];
+        double2 W = tw2(P.fa, P.fb, k);
         double2 WO = cmul(W, O);
         double2 Xk = make_double2(E.x + WO.x, E.y + WO.y);
         double2 Xm = make_double2(E.x - WO.x, -(E.y - WO.y));   // X_{N-k} = conj(E - W O)
@@ -199,14 +203,14 @@ __global__ void k_spectral(double2 *data, int64_t N, int L1, int L2, const doubl
 }
 
 // plain c2r of one filter row (no segment): imaginary parts ignored like FFTW
-__global__ void k_c2r_prep(double2 *data, int64_t N, int L1, int L2, const double2 *twnof, const double *fc,
+__global__ void k_c2r_prep(double2 *data, int64_t N, int L1, int L2, FftPlan P, const double *fc,
                            int64_t fc_stride)
 {
     int64_t b = blockIdx.y;
     double2 *d = data + b * N;
     const double *Y = fc + b * fc_stride;   // real spectrum values
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < N; k += (int64_t)gridDim.x * blockDim.x) {
-        double2 W = twnof[k <= N / 2 ? k : N - k];
+        double2 W = tw2(P.fa, P.fb, k <= N / 2 ? k : N - k);
         if (k > N / 2) { W.x = -W.x; }   // exp(-2 pi i k / nof) for k > N/2 from the mirror: -conj(W_{N-k})
         // with purely real Y: z_k = (Y_k + Y_{N-k}) + i W^-k (Y_k - Y_{N-k}), Y_N at index N
         double yk = Y[k], ym = Y[N - k];
@@ -298,28 +302,40 @@ void fft_plan_init(FftPlan &p, int64_t nof)
     if (l2 > 11) { l2 = 11; l1 = logN - 11; }
     p.L1 = 1LL << l1;
     p.L2 = 1LL << l2;
-    std::vector<double> tw(2 * p.N), twn(2 * (p.N / 2 + 1));
-    for (int64_t j = 0; j < p.N; ++j) {
-        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)p.N;
-        tw[2 * j] = (double)cosl(a);
-        tw[2 * j + 1] = (double)sinl(a);
-    }
-    for (int64_t k = 0; k <= p.N / 2; ++k) {
-        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)nof;
-        twn[2 * k] = (double)cosl(a);
-        twn[2 * k + 1] = (double)sinl(a);
-    }
-    MADAM_CHECK(hipMalloc(&p.tw_n, sizeof(double) * 2 * p.N));
-    MADAM_CHECK(hipMalloc(&p.tw_nof, sizeof(double) * 2 * (p.N / 2 + 1)));
-    MADAM_CHECK(hipMemcpy(p.tw_n, tw.data(), sizeof(double) * 2 * p.N, hipMemcpyHostToDevice));
-    MADAM_CHECK(hipMemcpy(p.tw_nof, twn.data(), sizeof(double) * 2 * (p.N / 2 + 1), hipMemcpyHostToDevice));
+    // exp(-2 pi i num / den), computed in long double and rounded once
+    auto W = [](long double num, long double den, double *out) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * num / den;
+        out[0] = (double)cosl(a);
+        out[1] = (double)sinl(a);
+    };
+    int64_t n1 = std::max<int64_t>(p.L1 / 2, 1), n2 = std::max<int64_t>(p.L2 / 2, 1);
+    int64_t nna = 1024, nnb = p.N / 1024 + 1;
+    int64_t nfa = 1024, nfb = (p.N / 2) / 1024 + 1;
+    int64_t tot = n1 + n2 + nna + nnb + nfa + nfb;
+    std::vector<double> h(2 * tot);
+    double *q = h.data();
+    for (int64_t j = 0; j < n1; ++j, q += 2) W(j, p.L1, q);
+    for (int64_t j = 0; j < n2; ++j, q += 2) W(j, p.L2, q);
+    for (int64_t j = 0; j < nna; ++j, q += 2) W(j, p.N, q);
+    for (int64_t j = 0; j < nnb; ++j, q += 2) W(1024.0L * j, p.N, q);
+    for (int64_t j = 0; j < nfa; ++j, q += 2) W(j, nof, q);
+    for (int64_t j = 0; j < nfb; ++j, q += 2) W(1024.0L * j, nof, q);
+    MADAM_CHECK(hipMalloc(&p.tw_mem, sizeof(double) * h.size()));
+    MADAM_CHECK(hipMemcpy(p.tw_mem, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    const double2 *b = (const double2 *)p.tw_mem;
+    p.sub1 = b;
+    p.sub2 = p.sub1 + n1;
+    p.na = p.sub2 + n2;
+    p.nb = p.na + nna;
+    p.fa = p.nb + nnb;
+    p.fb = p.fa + nfa;
 }
 
 void fft_plan_free(FftPlan &p)
 {
-    if (p.tw_n) (void)hipFree(p.tw_n);
-    if (p.tw_nof) (void)hipFree(p.tw_nof);
-    p.tw_n = p.tw_nof = nullptr;
+    if (p.tw_mem) (void)hipFree(p.tw_mem);
+    p.tw_mem = nullptr;
+    p.sub1 = p.sub2 = p.na = p.nb = p.fa = p.fb = nullptr;
 }
 
 static void fft_forward(hipStream_t s, const FftPlan &p, double2 *d, int64_t batch)
@@ -330,14 +346,14 @@ static void fft_forward(hipStream_t s, const FftPlan &p, double2 *d, int64_t bat
     if (ncol < 1) ncol = 1;
     dim3 gc((unsigned)(p.L2 / ncol), (unsigned)batch);
     k_fft_col<<<gc, FFT_TPB, sizeof(double2) * p.L1 * ncol, s>>>(d, p.N, (int)p.L1, logL1, (int)p.L2, ncol,
-                                                                 (const double2 *)p.tw_n, 0);
+                                                                 p, 0);
     if (p.L2 > 1) {
         int nrow = (int)(4096 / p.L2);
         if (nrow > p.L1) nrow = (int)p.L1;
         if (nrow < 1) nrow = 1;
         dim3 gr((unsigned)(p.L1 / nrow), (unsigned)batch);
         k_fft_row<<<gr, FFT_TPB, sizeof(double2) * p.L2 * nrow, s>>>(d, p.N, (int)p.L1, (int)p.L2, logL2, nrow,
-                                                                     (const double2 *)p.tw_n, 0);
+                                                                     p, 0);
     }
 }
 
@@ -350,14 +366,14 @@ static void fft_inverse(hipStream_t s, const FftPlan &p, double2 *d, int64_t bat
         if (nrow < 1) nrow = 1;
         dim3 gr((unsigned)(p.L1 / nrow), (unsigned)batch);
         k_fft_row<<<gr, FFT_TPB, sizeof(double2) * p.L2 * nrow, s>>>(d, p.N, (int)p.L1, (int)p.L2, logL2, nrow,
-                                                                     (const double2 *)p.tw_n, 1);
+                                                                     p, 1);
     }
     int ncol = (int)(p.L2 < 4 ? p.L2 : 4);
     if (p.L1 * ncol > 4096) ncol = (int)(4096 / p.L1);
     if (ncol < 1) ncol = 1;
     dim3 gc((unsigned)(p.L2 / ncol), (unsigned)batch);
     k_fft_col<<<gc, FFT_TPB, sizeof(double2) * p.L1 * ncol, s>>>(d, p.N, (int)p.L1, logL1, (int)p.L2, ncol,
-                                                                 (const double2 *)p.tw_n, 1);
+                                                                 p, 1);
 }
 
 // ---------------------------------------------------------------- fused convolution
@@ -375,7 +391,7 @@ static constexpr int CONV_NCOL = 8;
 
 // grid = (L2 / CONV_NCOL, batch)
 __global__ void k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, const double *partial_in,
-                               double2 *work, int64_t N, int L1, int logL1, int L2, const double2 *tw)
+                               double2 *work, int64_t N, int L1, int logL1, int L2, FftPlan P)
 {
     extern __shared__ double2 s[];
     const int ncol = CONV_NCOL;
@@ -396,24 +412,24 @@ __global__ void k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, 
         s[bitrev(n1, logL1) * ncol + col] = v;
     }
     __syncthreads();
-    lds_fft(s, L1, logL1, ncol, tw, N / L1, false);
+    lds_fft(s, L1, logL1, ncol, P.sub1, false);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
-        double2 w = tw[(n2 * k1) % N];
+        double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
         d[(int64_t)k1 * L2 + n2] = cmul(s[k1 * ncol + col], w);
     }
 }
 
 // spectral step on one pair {k, N - k}: X[k] at s[ik], X[N - k] at s[im], 0 < k <= N / 2
-__device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t N, const double2 *twnof,
+__device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t N, const FftPlan &P,
                                  const double *f)
 {
     double2 a = s[ik], bc = s[im];
     bc.y = -bc.y;
     double2 E = make_double2(0.5 * (a.x + bc.x), 0.5 * (a.y + bc.y));
     double2 O = make_double2(0.5 * (a.y - bc.y), -0.5 * (a.x - bc.x));
-    double2 W = twnof[k];
+    double2 W = tw2(P.fa, P.fb, k);
     double2 WO = cmul(W, O);
     double2 Xk = make_double2(E.x + WO.x, E.y + WO.y);
     double2 Xm = make_double2(E.x - WO.x, -(E.y - WO.y));
@@ -433,8 +449,8 @@ __device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t 
 }
 
 // grid = (L1 / 2 + 1, batch); LDS 2 * L2 double2, the pair's rows interleaved s[k2 * nr + r]
-__global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2, const double2 *tw,
-                            const double2 *twnof, const double *fc, int64_t fc_stride, const Segment *seg, int64_t s0)
+__global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2, FftPlan P, const double *fc,
+                            int64_t fc_stride, const Segment *seg, int64_t s0)
 {
     extern __shared__ double2 s[];
     int64_t b = blockIdx.y;
@@ -449,7 +465,7 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
         s[bitrev(n, logL2) * nr + r] = d[(int64_t)row * L2 + n];
     }
     __syncthreads();
-    lds_fft(s, L2, logL2, nr, tw, N / L2, false);
+    lds_fft(s, L2, logL2, nr, P.sub2, false);
     // X[ra + L1 k2] is at s[k2 * nr]; X[rb + L1 k2] at s[k2 * nr + 1]
     if (ra == 0) {
         for (int k2 = threadIdx.x; k2 <= L2 / 2; k2 += blockDim.x) {
@@ -459,23 +475,23 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
                 double y0 = x0 * f[0], yn = xn * f[N];
                 s[0] = make_double2(y0 + yn, y0 - yn);
             } else {
-                conv_pair(s, k2, L2 - k2, (int64_t)L1 * k2, N, twnof, f);
+                conv_pair(s, k2, L2 - k2, (int64_t)L1 * k2, N, P, f);
             }
         }
     } else if (nr == 1) {   // ra == L1 / 2
         for (int k2 = threadIdx.x; k2 < L2 / 2; k2 += blockDim.x)
-            conv_pair(s, k2, L2 - 1 - k2, (int64_t)ra + (int64_t)L1 * k2, N, twnof, f);
+            conv_pair(s, k2, L2 - 1 - k2, (int64_t)ra + (int64_t)L1 * k2, N, P, f);
     } else {
         for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
             int64_t kA = (int64_t)ra + (int64_t)L1 * k2;
             int p = k2 * 2, q = (L2 - 1 - k2) * 2 + 1;
-            if (kA <= N / 2) conv_pair(s, p, q, kA, N, twnof, f);
-            else conv_pair(s, q, p, N - kA, N, twnof, f);
+            if (kA <= N / 2) conv_pair(s, p, q, kA, N, P, f);
+            else conv_pair(s, q, p, N - kA, N, P, f);
         }
     }
     __syncthreads();
     // inverse row FFTs from natural order; the DIF output is bit-reversed
-    lds_fft_dif(s, L2, logL2, nr, tw, N / L2, true);
+    lds_fft_dif(s, L2, logL2, nr, P.sub2, true);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t / L2, n = t % L2;
         int row = r == 0 ? ra : rb;
@@ -485,7 +501,7 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
 
 // grid = (L2 / CONV_NCOL, batch)
 __global__ void k_conv_col_inv(const double2 *work, const Segment *seg, int64_t s0, double *y, double *partial_out,
-                               int64_t N, int L1, int logL1, int L2, const double2 *tw)
+                               int64_t N, int L1, int logL1, int L2, FftPlan P)
 {
     extern __shared__ double2 s[];
     const int ncol = CONV_NCOL;
@@ -499,11 +515,11 @@ __global__ void k_conv_col_inv(const double2 *work, const Segment *seg, int64_t 
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
-        double2 w = tw[(n2 * k1) % N];
+        double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
         s[bitrev(k1, logL1) * ncol + col] = cmulc(d[(int64_t)k1 * L2 + n2], w);
     }
     __syncthreads();
-    lds_fft(s, L1, logL1, ncol, tw, N / L1, true);
+    lds_fft(s, L1, logL1, ncol, P.sub1, true);
     double acc = 0;
     double *ys = y + sg.b0;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
@@ -570,13 +586,12 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
         k_seg_partial<<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
         if (fused) {
             k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * p.L1 * CONV_NCOL, s>>>(
-                x, dseg, s0, partial, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, (const double2 *)p.tw_n);
+                x, dseg, s0, partial, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
             k_conv_rows<<<dim3((unsigned)(p.L1 / 2 + 1), (unsigned)nb), FFT_TPB, sizeof(double2) * 2 * p.L2, s>>>(
-                (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, (const double2 *)p.tw_n,
-                (const double2 *)p.tw_nof, fc, fc_stride, dseg, s0);
+                (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc, fc_stride, dseg, s0);
             k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * p.L1 * CONV_NCOL, s>>>(
                 (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
-                (const double2 *)p.tw_n);
+                p);
             unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
             if (gl > 1024) gl = 1024;
             k_conv_sub_mean<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
@@ -588,7 +603,7 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
         k_seg_ipsd<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(dseg, s0, nb, ipsd);
         fft_forward(s, p, (double2 *)work, nb);
         dim3 gs((unsigned)((p.N / 2 + 1 + FFT_TPB - 1) / FFT_TPB), (unsigned)nb);
-        k_spectral<<<gs, FFT_TPB, 0, s>>>((double2 *)work, p.N, (int)p.L1, (int)p.L2, (const double2 *)p.tw_nof, fc,
+        k_spectral<<<gs, FFT_TPB, 0, s>>>((double2 *)work, p.N, (int)p.L1, (int)p.L2, p, fc,
                                            fc_stride, ipsd);
         fft_inverse(s, p, (double2 *)work, nb);
         k_seg_partial<<<gp, FFT_TPB, 0, s>>>(work, nof, 0, dseg, s0, 1 / (double)nof, partial);
@@ -612,7 +627,7 @@ void fft_c2r_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nro
     (void)work;
     // out[row][nof] = c2r(fc[row][N+1]) / nof
     dim3 g((unsigned)((p.N + FFT_TPB - 1) / FFT_TPB), (unsigned)nrows);
-    k_c2r_prep<<<g, FFT_TPB, 0, s>>>((double2 *)out, p.N, (int)p.L1, (int)p.L2, (const double2 *)p.tw_nof, fc,
+    k_c2r_prep<<<g, FFT_TPB, 0, s>>>((double2 *)out, p.N, (int)p.L1, (int)p.L2, p, fc,
                                       p.N + 1);
     fft_inverse(s, p, (double2 *)out, nrows);
     int64_t n = nrows * 2 * p.N;

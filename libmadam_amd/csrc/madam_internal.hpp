@@ -167,11 +167,15 @@ void launch_cg_p(hipStream_t s, const Segment *seg, int64_t nseg, const int *act
 
 // ------------------------------------------------------------------ fft.hip
 
+// Twiddle factors, all from one allocation and small enough to stay cached:
+//   sub1[j] = W_L1^j (j < L1 / 2), sub2[j] = W_L2^j (j < L2 / 2) for the LDS sub-FFTs;
+//   W_N^m = na[m % 1024] * nb[m / 1024] and W_nof^k = fa[k % 1024] * fb[k / 1024]
+//   (two-level tables; W_x = exp(-2 pi i / x)).
 struct FftPlan {
     int64_t N;       // complex length (= nof / 2)
     int64_t L1, L2;  // N = L1 * L2, column pass length L1, row pass length L2
-    double *tw_n;    // [N] complex exp(-2 pi i j / N)
-    double *tw_nof;  // [N/2 + 1] complex exp(-2 pi i k / nof)
+    double *tw_mem;
+    const double2 *sub1, *sub2, *na, *nb, *fa, *fb;
 };
 
 void fft_plan_init(FftPlan &p, int64_t nof);
