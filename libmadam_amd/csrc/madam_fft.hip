@@ -376,6 +376,91 @@ static void fft_inverse(hipStream_t s, const FftPlan &p, double2 *d, int64_t bat
                                                                  p, 1);
 }
 
+// ---------------------------------------------------------------- fused-stage LDS FFTs
+// Three radix-2 stages per LDS round trip: a work item holds the 8 elements
+// base + m * h (m < 8) that those stages combine among themselves, applies the
+// three butterfly levels in registers and writes them back.  The LDS layout is
+// padded by one element in eight (lp) so that the strided accesses of the
+// early stages do not collide on the same banks.
+__device__ inline int lp(int idx) { return idx + (idx >> 3); }
+
+// DIT, bit-reversed input order, natural output order; twL[j] = W_L^j (j < L / 2)
+__device__ void lds_fft8(double2 *s, int L, int logL, int ncol, const double2 *twL, bool inv)
+{
+    for (int st = 0; st < logL; st += 3) {
+        int T = logL - st < 3 ? logL - st : 3;
+        int h0 = 1 << st, E = 1 << T;
+        int nitems = (L >> T) * ncol;
+        for (int t = threadIdx.x; t < nitems; t += blockDim.x) {
+            int col = t % ncol, it = t / ncol;
+            int j = it & (h0 - 1), grp = it >> st;
+            int base = grp * (E * h0) + j;
+            double2 a[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+                if (m < E) a[m] = s[lp((base + m * h0) * ncol + col)];
+#pragma unroll
+            for (int tt = 0; tt < 3; ++tt) {
+                if (tt >= T) break;
+                int h = h0 << tt;
+#pragma unroll
+                for (int m0 = 0; m0 < 8; ++m0) {
+                    if (m0 >= E || (m0 & (1 << tt))) continue;
+                    int r = m0 & ((1 << tt) - 1);
+                    double2 w = twL[(j + r * h0) * (L / (2 * h))];
+                    if (inv) w.y = -w.y;
+                    double2 b = cmul(a[m0 + (1 << tt)], w), x = a[m0];
+                    a[m0] = make_double2(x.x + b.x, x.y + b.y);
+                    a[m0 + (1 << tt)] = make_double2(x.x - b.x, x.y - b.y);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+                if (m < E) s[lp((base + m * h0) * ncol + col)] = a[m];
+        }
+        __syncthreads();
+    }
+}
+
+// DIF, natural input order, bit-reversed output order
+__device__ void lds_fft8_dif(double2 *s, int L, int logL, int ncol, const double2 *twL, bool inv)
+{
+    for (int hi = logL - 1; hi >= 0; hi -= 3) {
+        int T = hi + 1 < 3 ? hi + 1 : 3;
+        int lo = hi - T + 1;
+        int h0 = 1 << lo, E = 1 << T;
+        int nitems = (L >> T) * ncol;
+        for (int t = threadIdx.x; t < nitems; t += blockDim.x) {
+            int col = t % ncol, it = t / ncol;
+            int j = it & (h0 - 1), grp = it >> lo;
+            int base = grp * (E * h0) + j;
+            double2 a[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+                if (m < E) a[m] = s[lp((base + m * h0) * ncol + col)];
+#pragma unroll
+            for (int tt = 2; tt >= 0; --tt) {
+                if (tt >= T) continue;
+                int h = h0 << tt;
+#pragma unroll
+                for (int m0 = 0; m0 < 8; ++m0) {
+                    if (m0 >= E || (m0 & (1 << tt))) continue;
+                    int r = m0 & ((1 << tt) - 1);
+                    double2 w = twL[(j + r * h0) * (L / (2 * h))];
+                    if (inv) w.y = -w.y;
+                    double2 x = a[m0], y = a[m0 + (1 << tt)];
+                    a[m0] = make_double2(x.x + y.x, x.y + y.y);
+                    a[m0 + (1 << tt)] = cmul(make_double2(x.x - y.x, x.y - y.y), w);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+                if (m < E) s[lp((base + m * h0) * ncol + col)] = a[m];
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- fused convolution
 // Three passes instead of eight (load, 2 x column, 2 x row, spectral, store):
 //   pass 1  forward column FFTs read x - mean straight from the baseline
@@ -388,6 +473,9 @@ static void fft_inverse(hipStream_t s, const FftPlan &p, double2 *d, int64_t bat
 //           baselines directly, with deterministic per-block partial sums of
 //           y for the mean, which a last pass subtracts.
 static constexpr int CONV_NCOL = 8;
+
+// LDS elements (padded) for n double2
+__host__ __device__ inline int conv_lds_elems(int n) { return n + (n >> 3) + 1; }
 
 // grid = (L2 / CONV_NCOL, batch)
 __global__ void k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, const double *partial_in,
@@ -405,23 +493,24 @@ __global__ void k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, 
     int tot = L1 * ncol;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, n1 = t / ncol;
-        int64_t j = 2 * ((int64_t)n1 * L2 + c0 + col);
+        int64_t jj = 2 * ((int64_t)n1 * L2 + c0 + col);
         double2 v;
-        v.x = j < noba ? xs[j] - mean : 0.0;
-        v.y = j + 1 < noba ? xs[j + 1] - mean : 0.0;
-        s[bitrev(n1, logL1) * ncol + col] = v;
+        v.x = jj < noba ? xs[jj] - mean : 0.0;
+        v.y = jj + 1 < noba ? xs[jj + 1] - mean : 0.0;
+        s[lp(bitrev(n1, logL1) * ncol + col)] = v;
     }
     __syncthreads();
-    lds_fft(s, L1, logL1, ncol, P.sub1, false);
+    lds_fft8(s, L1, logL1, ncol, P.sub1, false);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
         double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
-        d[(int64_t)k1 * L2 + n2] = cmul(s[k1 * ncol + col], w);
+        d[(int64_t)k1 * L2 + n2] = cmul(s[lp(k1 * ncol + col)], w);
     }
 }
 
-// spectral step on one pair {k, N - k}: X[k] at s[ik], X[N - k] at s[im], 0 < k <= N / 2
+// spectral step on one pair {k, N - k}: X[k] at s[ik], X[N - k] at s[im] (padded
+// LDS indices), 0 < k <= N / 2
 __device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t N, const FftPlan &P,
                                  const double *f)
 {
@@ -448,7 +537,7 @@ __device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t 
     }
 }
 
-// grid = (L1 / 2 + 1, batch); LDS 2 * L2 double2, the pair's rows interleaved s[k2 * nr + r]
+// grid = (L1 / 2 + 1, batch); the pair's rows interleaved in LDS at k2 * nr + r (padded)
 __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2, FftPlan P, const double *fc,
                             int64_t fc_stride, const Segment *seg, int64_t s0)
 {
@@ -462,40 +551,40 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t / L2, n = t % L2;
         int row = r == 0 ? ra : rb;
-        s[bitrev(n, logL2) * nr + r] = d[(int64_t)row * L2 + n];
+        s[lp(bitrev(n, logL2) * nr + r)] = d[(int64_t)row * L2 + n];
     }
     __syncthreads();
-    lds_fft(s, L2, logL2, nr, P.sub2, false);
-    // X[ra + L1 k2] is at s[k2 * nr]; X[rb + L1 k2] at s[k2 * nr + 1]
+    lds_fft8(s, L2, logL2, nr, P.sub2, false);
+    // X[ra + L1 k2] is at k2 * nr; X[rb + L1 k2] at k2 * nr + 1
     if (ra == 0) {
         for (int k2 = threadIdx.x; k2 <= L2 / 2; k2 += blockDim.x) {
             if (k2 == 0) {
-                double2 a = s[0];
+                double2 a = s[lp(0)];
                 double x0 = a.x + a.y, xn = a.x - a.y;
                 double y0 = x0 * f[0], yn = xn * f[N];
-                s[0] = make_double2(y0 + yn, y0 - yn);
+                s[lp(0)] = make_double2(y0 + yn, y0 - yn);
             } else {
-                conv_pair(s, k2, L2 - k2, (int64_t)L1 * k2, N, P, f);
+                conv_pair(s, lp(k2), lp(L2 - k2), (int64_t)L1 * k2, N, P, f);
             }
         }
     } else if (nr == 1) {   // ra == L1 / 2
         for (int k2 = threadIdx.x; k2 < L2 / 2; k2 += blockDim.x)
-            conv_pair(s, k2, L2 - 1 - k2, (int64_t)ra + (int64_t)L1 * k2, N, P, f);
+            conv_pair(s, lp(k2), lp(L2 - 1 - k2), (int64_t)ra + (int64_t)L1 * k2, N, P, f);
     } else {
         for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
             int64_t kA = (int64_t)ra + (int64_t)L1 * k2;
-            int p = k2 * 2, q = (L2 - 1 - k2) * 2 + 1;
-            if (kA <= N / 2) conv_pair(s, p, q, kA, N, P, f);
-            else conv_pair(s, q, p, N - kA, N, P, f);
+            int pp = lp(k2 * 2), qq = lp((L2 - 1 - k2) * 2 + 1);
+            if (kA <= N / 2) conv_pair(s, pp, qq, kA, N, P, f);
+            else conv_pair(s, qq, pp, N - kA, N, P, f);
         }
     }
     __syncthreads();
     // inverse row FFTs from natural order; the DIF output is bit-reversed
-    lds_fft_dif(s, L2, logL2, nr, P.sub2, true);
+    lds_fft8_dif(s, L2, logL2, nr, P.sub2, true);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t / L2, n = t % L2;
         int row = r == 0 ? ra : rb;
-        d[(int64_t)row * L2 + n] = s[bitrev(n, logL2) * nr + r];
+        d[(int64_t)row * L2 + n] = s[lp(bitrev(n, logL2) * nr + r)];
     }
 }
 
@@ -516,24 +605,24 @@ __global__ void k_conv_col_inv(const double2 *work, const Segment *seg, int64_t 
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
         double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
-        s[bitrev(k1, logL1) * ncol + col] = cmulc(d[(int64_t)k1 * L2 + n2], w);
+        s[lp(bitrev(k1, logL1) * ncol + col)] = cmulc(d[(int64_t)k1 * L2 + n2], w);
     }
     __syncthreads();
-    lds_fft(s, L1, logL1, ncol, P.sub1, true);
+    lds_fft8(s, L1, logL1, ncol, P.sub1, true);
     double acc = 0;
     double *ys = y + sg.b0;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, n1 = t / ncol;
-        int64_t j = 2 * ((int64_t)n1 * L2 + c0 + col);
-        double2 v = s[n1 * ncol + col];
-        if (j < noba) {
+        int64_t jj = 2 * ((int64_t)n1 * L2 + c0 + col);
+        double2 v = s[lp(n1 * ncol + col)];
+        if (jj < noba) {
             double a = v.x * inv;
-            ys[j] = a;
+            ys[jj] = a;
             acc += a;
         }
-        if (j + 1 < noba) {
+        if (jj + 1 < noba) {
             double a = v.y * inv;
-            ys[j + 1] = a;
+            ys[jj + 1] = a;
             acc += a;
         }
     }
@@ -543,7 +632,7 @@ __global__ void k_conv_col_inv(const double2 *work, const Segment *seg, int64_t 
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0;
-        for (int k = 0; k < FFT_TPB / 64; ++k) t += sh[k];
+        for (int q = 0; q < FFT_TPB / 64; ++q) t += sh[q];
         partial_out[b * gridDim.x + blockIdx.x] = t;
     }
 }
@@ -585,11 +674,11 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
         dim3 gp(SEG_NB, (unsigned)nb);
         k_seg_partial<<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
         if (fused) {
-            k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * p.L1 * CONV_NCOL, s>>>(
+            k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 x, dseg, s0, partial, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
-            k_conv_rows<<<dim3((unsigned)(p.L1 / 2 + 1), (unsigned)nb), FFT_TPB, sizeof(double2) * 2 * p.L2, s>>>(
+            k_conv_rows<<<dim3((unsigned)(p.L1 / 2 + 1), (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(2 * p.L2)), s>>>(
                 (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc, fc_stride, dseg, s0);
-            k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * p.L1 * CONV_NCOL, s>>>(
+            k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
                 p);
             unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
