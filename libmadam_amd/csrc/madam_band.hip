@@ -143,7 +143,7 @@ __global__ void k_band_relayout(const ChunkInfo *ch, int64_t nch, int64_t NCP, i
 // (fewer rows per group for the wide bands, to stay within the register file)
 template <int KD>
 struct BandPF {
-    static constexpr int value = KD <= 19 ? 8 : (KD <= 29 ? 4 : 2);
+    static constexpr int value = KD <= 9 ? 4 : (KD <= 19 ? 8 : (KD <= 29 ? 4 : 2));
 };
 
 // forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
@@ -311,13 +311,32 @@ __global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, in
     bool on = lane < KD;
     int64_t g0 = sfirst[s], g1 = sfirst[s + 1];
     double st = 0.0;
-    for (int64_t t = 0; t < g1 - g0; ++t) {
-        int64_t g = dir == 0 ? g0 + t : g1 - 1 - t;
-        if (on) Sg[g * KD + lane] = st;
-        double v = on ? cg[g * KD + lane] : 0.0;
+    int64_t n = g1 - g0;
+    constexpr int PF = BandPF<KD>::value;
+    for (int64_t t0 = 0; t0 < n; t0 += PF) {
+        double tv[PF], tr[PF][KD];
 #pragma unroll
-        for (int q = 0; q < KD; ++q) v += (on ? Gm[(g * KD + lane) * KD + q] : 0.0) * __shfl(st, q, 64);
-        st = v;
+        for (int u = 0; u < PF; ++u) {
+            int64_t t = t0 + u;
+            if (t < n) {
+                int64_t g = dir == 0 ? g0 + t : g1 - 1 - t;
+                tv[u] = on ? cg[g * KD + lane] : 0.0;
+#pragma unroll
+                for (int q = 0; q < KD; ++q) tr[u][q] = on ? Gm[(g * KD + lane) * KD + q] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            int64_t t = t0 + u;
+            if (t < n) {
+                int64_t g = dir == 0 ? g0 + t : g1 - 1 - t;
+                if (on) Sg[g * KD + lane] = st;
+                double v = tv[u];
+#pragma unroll
+                for (int q = 0; q < KD; ++q) v += tr[u][q] * __shfl(st, q, 64);
+                st = v;
+            }
+        }
     }
 }
 
