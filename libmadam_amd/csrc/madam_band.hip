@@ -31,6 +31,7 @@ namespace madam {
 
 static constexpr int BAND_CH = 64;    // rows per chunk (one thread each in the sweeps)
 static constexpr int BAND_G = 64;     // chunks per group of the carry chain
+static_assert(BAND_CH > 0, "");
 
 struct ChunkInfo {
     int64_t b0;    // first global baseline row of the chunk
@@ -117,13 +118,24 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
 // ---------------------------------------------------------------- partitioned solves
 
 
+// The factor is re-laid out per chunk for the sweeps, blocked by wave:
+// Li[(((c / 64) * CH + i) * K + m) * 64 + c % 64] is L(b0_c + i + m, b0_c + i)
+// (m = 0 holds 1 / L(i, i)).  The lanes of a wave (consecutive chunks c) read
+// consecutive addresses, and a wave's whole sweep is one contiguous block.
+__device__ inline const double *li_chunk(const double *Li, int64_t c, int K)
+{
+    return Li + (c >> 6) * (int64_t)BAND_CH * K * 64 + (c & 63);
+}
+
 __global__ void k_band_relayout(const ChunkInfo *ch, int64_t nch, int64_t NCP, int K, int CH, const double *Lb,
                                 double *Li)
 {
     int64_t tot = NCP * (int64_t)CH * K;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c = t % NCP, im = t / NCP;
-        int64_t i = im / K, m = im % K;
+        int64_t lane = t & 63, rest = t >> 6;
+        int64_t m = rest % K, rest2 = rest / K;
+        int64_t i = rest2 % CH, w = rest2 / CH;
+        int64_t c = w * 64 + lane;
         double v = 0.0;
         if (c < nch) {
             ChunkInfo ci = ch[c];
@@ -132,10 +144,6 @@ __global__ void k_band_relayout(const ChunkInfo *ch, int64_t nch, int64_t NCP, i
         Li[t] = v;
     }
 }
-
-// The factor is re-laid out per chunk for the sweeps: Li[(i * K + m) * NCP + c]
-// is L(b0_c + i + m, b0_c + i) (m = 0 holds 1 / L(i, i)), so that the lanes of
-// a wave (consecutive chunks c) read consecutive addresses.
 
 // Rows are processed in groups of BAND_PF: the factor rows and right-hand
 // sides of a group are loaded first (they do not depend on the carry state),
@@ -159,9 +167,9 @@ __device__ inline void fwd_sweep(const double *Li, int64_t NCP, int64_t c, int64
         for (int u = 0; u < BAND_PF; ++u) {
             int i = i0 + u;
             if (i < n) {
-                const double *row = Li + (int64_t)i * K * NCP + c;
+                const double *row = li_chunk(Li, c, K) + (int64_t)i * K * 64;
 #pragma unroll
-                for (int m = 0; m < K; ++m) rw[u][m] = row[m * NCP];
+                for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
                 rh[u] = zero_rhs ? 0.0 : r[b0 + i];
             }
         }
@@ -191,9 +199,9 @@ __device__ inline void bwd_sweep(const double *Li, int64_t NCP, int64_t c, int64
         for (int u = 0; u < BAND_PF; ++u) {
             int i = i0 - u;
             if (i >= 0) {
-                const double *row = Li + (int64_t)i * K * NCP + c;
+                const double *row = li_chunk(Li, c, K) + (int64_t)i * K * 64;
 #pragma unroll
-                for (int m = 0; m < K; ++m) rw[u][m] = row[m * NCP];
+                for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
                 rh[u] = zero_rhs ? 0.0 : y[b0 + i];
             }
         }
