@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "madam_fits.hpp"
 #include "madam_internal.hpp"
 
 #define F32(x) ((double)(float)(x))
@@ -62,6 +63,22 @@ struct Params {
     int rm_monopole = 0, write_tod = 0, ndetset = 0, nsurvey = 0, mcmode = 0;
     bool has_inmask = false, has_spectrum = false, has_covmat = false;
     int nshort = 0;
+    // products (src/commonparam.f90:200-305, src/inputparam.f90:310-360)
+    std::string path_output, file_root = "madam", unit_tod = "unknown";
+    int write_map = 1, write_binmap = 0, write_hits = 0, write_mask = 0, write_matrix = 0, write_wcov = 0;
+    int write_cut = 0, bin_subsets = 0, force_pol = 0, skip_existing = 0;
+    struct DetSet {
+        std::string name;
+        std::vector<std::string> dets;
+        bool nopol = false;
+    };
+    struct Survey {
+        std::string name;
+        std::vector<double> starts, stops;
+    };
+    std::vector<DetSet> detsets;     // detsets(1..ndetset)
+    std::vector<Survey> surveys;     // surveys(1..nsurvey)
+    std::string detset0, survey0;    // names of the full detector set / full data span
 };
 
 static std::string strip(const std::string &s)
@@ -143,10 +160,15 @@ static void read_line(Params &p, std::string line)
     else if (key == "nside_submap") ok = parse_int(value, p.nside_submap);
     else if (key == "good_baseline_fraction") ok = parse_real(value, p.good_baseline_fraction);
     else if (key == "concatenate_messages" || key == "allreduce" || key == "reassign_submaps" ||
-             key == "flag_by_horn" || key == "write_cut" || key == "checknan" || key == "sync_output" ||
-             key == "skip_existing" || key == "force_pol" || key == "unaligned_fft" || key == "write_map" ||
-             key == "write_matrix" || key == "write_wcov" || key == "write_base" || key == "write_leakmatrix" ||
-             key == "binary_output" || key == "bin_subsets") ok = parse_bool(value, ig);
+             key == "flag_by_horn" || key == "checknan" || key == "sync_output" || key == "unaligned_fft" ||
+             key == "write_base" || key == "write_leakmatrix" || key == "binary_output") ok = parse_bool(value, ig);
+    else if (key == "write_cut") ok = parse_bool(value, p.write_cut);
+    else if (key == "skip_existing") ok = parse_bool(value, p.skip_existing);
+    else if (key == "force_pol") ok = parse_bool(value, p.force_pol);
+    else if (key == "write_map") ok = parse_bool(value, p.write_map);
+    else if (key == "write_matrix") ok = parse_bool(value, p.write_matrix);
+    else if (key == "write_wcov") ok = parse_bool(value, p.write_wcov);
+    else if (key == "bin_subsets") ok = parse_bool(value, p.bin_subsets);
     else if (key == "pixmode_map") ok = parse_int(value, p.pixmode_map);
     else if (key == "pixmode_cross") ok = parse_int(value, p.pixmode_cross);
     else if (key == "pixlim_map") ok = parse_real(value, p.pixlim_map);
@@ -177,24 +199,73 @@ static void read_line(Params &p, std::string line)
     else if (key == "use_fprecond") ok = parse_bool(value, p.use_fprecond);
     else if (key == "use_cgprecond") ok = parse_bool(value, p.use_cgprecond);
     else if (key == "rm_monopole") ok = parse_bool(value, p.rm_monopole);
-    else if (key == "path_output" || key == "file_root" || key == "unit_tod" || key == "file_gap_out" ||
-             key == "file_mc" || key == "file_gap") {
-    } else if (key == "write_binmap" || key == "write_hits" || key == "write_mask") ok = parse_bool(value, ig);
+    else if (key == "path_output") p.path_output = value;
+    else if (key == "file_root") p.file_root = value;
+    else if (key == "unit_tod") p.unit_tod = value;
+    else if (key == "file_gap_out" || key == "file_mc" || key == "file_gap") {
+    } else if (key == "write_binmap") ok = parse_bool(value, p.write_binmap);
+    else if (key == "write_hits") ok = parse_bool(value, p.write_hits);
+    else if (key == "write_mask") ok = parse_bool(value, p.write_mask);
     else if (key == "write_tod") ok = parse_bool(value, p.write_tod);
     else if (key == "file_inmask") p.has_inmask = !value.empty();
     else if (key == "file_spectrum") p.has_spectrum = !value.empty();
     else if (key == "file_covmat") p.has_covmat = !value.empty();
     else if (key == "detset" || key == "detset_nopol") {
+        // parse_detset (src/inputparam.f90:375-434): an empty or "all" member names the full set
         size_t c = value.find(':');
         if (c == std::string::npos) fail("Failed to parse: %s for valid detector set name", value.c_str());
-        if (!is_all(strip(value.substr(c + 1)))) p.ndetset++;
+        Params::DetSet ds;
+        ds.name = strip(value.substr(0, c));
+        if (ds.name.empty()) fail("Empty detset name");
+        ds.nopol = key == "detset_nopol";
+        std::string rest = strip(value.substr(c + 1));
+        bool full = false;
+        for (;;) {
+            size_t comma = rest.find(',');
+            std::string d = strip(comma == std::string::npos ? rest : rest.substr(0, comma));
+            if (d.empty() || is_all(d)) {
+                full = true;
+                break;
+            }
+            ds.dets.push_back(d);
+            if (comma == std::string::npos) break;
+            rest = strip(rest.substr(comma + 1));
+        }
+        if (full) p.detset0 = ds.name;
+        else p.detsets.push_back(ds);
+        p.ndetset = (int)p.detsets.size();
     } else if (key == "survey") {
+        // parse_survey (src/inputparam.f90:438-487): "name : start - stop, start - stop" or "name : all"
         size_t c = value.find(':');
         if (c == std::string::npos) fail("Failed to parse: %s for valid survey name", value.c_str());
+        Params::Survey sv;
+        sv.name = strip(value.substr(0, c));
+        if (sv.name.empty()) fail("Empty survey name");
         std::string rest = strip(value.substr(c + 1));
-        if (rest.find('-') == std::string::npos) {
-            if (!is_all(rest)) fail("Failed to parse: %s for valid survey span marker (-)", rest.c_str());
-        } else p.nsurvey++;
+        bool full = false;
+        for (;;) {
+            size_t dash = rest.find('-');
+            if (dash == std::string::npos) {
+                if (rest.empty() || is_all(rest)) {
+                    full = true;
+                    break;
+                }
+                fail("Failed to parse: %s for valid survey span marker (-)", rest.c_str());
+            }
+            double a = 0, b = 0;
+            if (!parse_real(strip(rest.substr(0, dash)), a)) fail("Failed to parse: %s for valid survey start", rest.c_str());
+            rest = strip(rest.substr(dash + 1));
+            size_t comma = rest.find(',');
+            if (!parse_real(strip(comma == std::string::npos ? rest : rest.substr(0, comma)), b))
+                fail("Failed to parse: %s for valid survey stop", rest.c_str());
+            sv.starts.push_back(a);
+            sv.stops.push_back(b);
+            if (comma == std::string::npos) break;
+            rest = strip(rest.substr(comma + 1));
+        }
+        if (full) p.survey0 = sv.name;
+        else p.surveys.push_back(sv);
+        p.nsurvey = (int)p.surveys.size();
     } else if (key == "mcmode") ok = parse_bool(value, p.mcmode);
     else fail("Unknown parameter: %s", key.c_str());
     if (!ok) fail("Failed to parse %s from %s", key.c_str(), value.c_str());
@@ -288,6 +359,7 @@ struct Context {
     int64_t nsamp = 0, npix_map = 0, npix_cross = 0, ndeg = 1;
     int pass = 0;
     std::vector<Detector> det;
+    std::vector<std::string> det_names;
     std::vector<double> psdfreqs;
     // intervals
     int nival = 0;
@@ -428,7 +500,18 @@ static void configure(Context &c, const char *parstring, long ndet, const char *
                       const long *npsd, long npsdtot, const double *psdstarts, long npsdbin, const double *psdfreqs,
                       long npsdval, const double *psdvals, int pass)
 {
-    (void)detstring;
+    c.det_names.clear();
+    {
+        std::string ds = detstring ? detstring : "";
+        size_t a = 0;
+        while (a <= ds.size() && (long)c.det_names.size() < ndet) {
+            size_t b = ds.find(',', a);
+            if (b == std::string::npos) b = ds.size();
+            c.det_names.push_back(strip(ds.substr(a, b - a)));
+            a = b + 1;
+        }
+        while ((long)c.det_names.size() < ndet) c.det_names.push_back("det" + std::to_string(c.det_names.size()));
+    }
     (void)npsdval;
     ensure_stream(c);
     Params p = read_parameters(parstring);
@@ -522,6 +605,7 @@ static void configure(Context &c, const char *parstring, long ndet, const char *
     if (pass < 0 || pass > p.nsubchunk) fail("subchunk pass %d out of range", pass);
     if (p.good_baseline_fraction < 0 || p.good_baseline_fraction > 1) fail("ERROR: good_baseline_fraction must be in 0..1");
     if (p.nside_cross < 0) p.nside_cross = p.nside_map;
+    p.nside_submap = std::min(p.nside_submap, std::min(p.nside_map, p.nside_cross));   // parameter_control.f90:253
     for (int ns : {p.nside_map, p.nside_submap, p.nside_cross})
         if (ns < 1 || (ns & (ns - 1))) fail("nside must be a power of two");
     if (p.nside_map < p.nside_cross) fail("nside_cross > nside_map not supported by Madam/TOAST");
@@ -1425,6 +1509,390 @@ static void get_maps(Context &c, double *map, double *binmap, int32_t *hits, dou
     if (mask) std::copy(mk.begin(), mk.end(), mask);
 }
 
+// ------------------------------------------------------------------ FITS products
+// src/output.f90:114-1175 (write_map, write_binmap, write_mask, write_hits,
+// write_matrix), file naming src/parameter_control.f90:47-72, src/output.f90:86-106
+// (addpath), 1329-1366 (check_outfile), src/smadam.F90:1285-1314 (add_subchunk_id).
+
+static const float HEALPIX_UNDEF = -1.6375e30f;
+
+struct OutNames {
+    std::string map, binmap, hit, mask, matrix, wcov;
+};
+
+static bool file_exists(const std::string &f)
+{
+    FILE *x = fopen(f.c_str(), "rb");
+    if (x) fclose(x);
+    return x != nullptr;
+}
+
+static std::string check_outfile(const std::string &fn)
+{
+    if (fn.empty() || fn[0] == '!' || !file_exists(fn)) return fn;
+    size_t n = fn.rfind('.');
+    std::string base = n == std::string::npos ? fn : fn.substr(0, n);
+    std::string ending = n == std::string::npos ? "" : fn.substr(n);
+    for (int k = 1; k < 1000; ++k) {
+        char s[16];
+        snprintf(s, sizeof s, "_%03d", k);
+        std::string f = base + s + ending;
+        if (!file_exists(f)) return f;
+    }
+    return "";
+}
+
+static std::string add_subchunk_id(const std::string &fn, int isub, int nsub)
+{
+    if (fn.empty() || isub == 0) return fn;
+    char s[32];
+    if (nsub < 10) snprintf(s, sizeof s, "_sub%1dof%1d", isub, nsub);
+    else snprintf(s, sizeof s, "_sub%02dof%02d", isub, nsub);
+    size_t i = fn.rfind('.');
+    if (i == std::string::npos) return fn + s;
+    return fn.substr(0, i) + s + fn.substr(i);
+}
+
+static std::string add_path(std::string path, const std::string &fn)
+{
+    if (path.empty() || fn.empty()) return fn;
+    if (path.back() != '/') path += '/';
+    if (fn[0] == '!') return "!" + path + fn.substr(1);
+    return path + fn;
+}
+
+static OutNames output_names(const Params &p, const std::string &path, int isub)
+{
+    std::string sub = p.file_root;
+    if (!p.detset0.empty()) sub += "_" + p.detset0;
+    if (!p.survey0.empty()) sub += "_" + p.survey0;
+    OutNames n;
+    auto mk = [&](bool on, const char *suffix) {
+        return on ? add_path(path, add_subchunk_id(sub + suffix, isub, p.nsubchunk)) : std::string();
+    };
+    n.map = mk(p.write_map, "_map.fits");
+    n.binmap = mk(p.write_binmap, "_bmap.fits");
+    n.hit = mk(p.write_hits, "_hmap.fits");
+    n.matrix = mk(p.write_matrix, "_wcov_inv.fits");
+    n.wcov = mk(p.write_wcov, "_wcov.fits");
+    n.mask = mk(p.write_mask, "_mask.fits");
+    return n;
+}
+
+struct HostProducts {
+    std::vector<double> map, binmap, cc6, ci6, crit;
+    std::vector<int32_t> hits, mask;
+};
+
+static void fetch_products(Context &c, HostProducts &h)
+{
+    int64_t np = c.npix_map;
+    h.map.resize(np * c.nmap);
+    h.binmap.resize(np * c.nmap);
+    h.hits.resize(np);
+    h.mask.resize(np);
+    h.crit.resize(np);
+    get_maps(c, h.map.data(), h.binmap.data(), h.hits.data(), nullptr, h.mask.data(), h.crit.data());
+    h.cc6.resize(np * 6);
+    h.ci6.resize(np * 6);
+    d2h(h.cc6.data(), c.d_cc6.p, np * 6, c.stream);
+    d2h(h.ci6.data(), c.d_ccinv6.p, np * 6, c.stream);
+    MADAM_CHECK(hipStreamSynchronize(c.stream));
+}
+
+// header keyword groups (src/output.f90:1376-1590)
+static void hdr_code(std::vector<FitsKey> &k)
+{
+    k.push_back(FitsKey::note("----------------------------------"));
+    k.push_back(FitsKey::note("         CODE             "));
+    k.push_back(FitsKey::note("----------------------------------"));
+    k.push_back(FitsKey::s("CREATOR", "Madam", "Software creating FITS file"));
+    k.push_back(FitsKey::s("VERSION", "3.7", "Software version"));
+    k.push_back(FitsKey::note("Serial version"));
+    k.push_back(FitsKey::s("PARFILE", "", "Parameter file"));
+}
+
+static void hdr_simulation(const Context &c, int isub, std::vector<FitsKey> &k)
+{
+    const Params &p = c.p;
+    double to_d = 1.0 / p.fsample / 24.0 / 3600.0;
+    k.push_back(FitsKey::note("-------------------------------"));
+    k.push_back(FitsKey::note("          Simulation           "));
+    k.push_back(FitsKey::note("-------------------------------"));
+    k.push_back(FitsKey::r("FSAMPLE", F32(p.fsample), "Sampling frequency (Hz)"));
+    k.push_back(FitsKey::r("TMISSION", F32(c.nsamp * to_d), "Mission duration/days"));
+    k.push_back(FitsKey::r("TSTART", 0.0, "Mission start/days"));
+    k.push_back(FitsKey::i("IMISSION", c.nsamp, "Mission duration/samples"));
+    k.push_back(FitsKey::i("ISTART", 0, "Mission start/samples"));
+    k.push_back(FitsKey::i("NSUBCHNK", p.nsubchunk, "Number of subchunks"));
+    k.push_back(FitsKey::i("ISUBCHNK", isub, "Index of subchunk"));
+    k.push_back(FitsKey::note("---------------------------------"));
+    k.push_back(FitsKey::note("         Detector info           "));
+    k.push_back(FitsKey::note("---------------------------------"));
+    k.push_back(FitsKey::i("NODET", c.ndet, "Number of detectors"));
+    k.push_back(FitsKey::i("MDETW", p.mode_detweight, "Detector weighting mode"));
+    for (int i = 0; i < c.ndet; ++i)
+        k.push_back(FitsKey::s("DETNAM" + std::to_string(i + 1), c.det_names[i], "Detector name"));
+    k.push_back(FitsKey::b("TEMPONLY", p.temperature_only != 0, "Temperature map only"));
+    k.push_back(FitsKey::note("----------------------------"));
+    k.push_back(FitsKey::note("          Pointing          "));
+    k.push_back(FitsKey::note("----------------------------"));
+}
+
+static void hdr_destriping(const Context &c, std::vector<FitsKey> &k)
+{
+    const Params &p = c.p;
+    k.push_back(FitsKey::note("---------------------------"));
+    k.push_back(FitsKey::note("     Data distribution     "));
+    k.push_back(FitsKey::note("---------------------------"));
+    k.push_back(FitsKey::note("Standard mode"));
+    k.push_back(FitsKey::i("NSIDESUB", p.nside_submap, "Submap resolution"));
+    k.push_back(FitsKey::note("-------------------------------------------"));
+    k.push_back(FitsKey::note("        Destriping scheme                  "));
+    k.push_back(FitsKey::note("-------------------------------------------"));
+    k.push_back(FitsKey::b("KFIRST", p.kfirst != 0, "First destriping phase on/off"));
+    if (p.kfirst) {
+        k.push_back(FitsKey::i("NSHORT", p.nshort, "Baseline length "));
+        k.push_back(FitsKey::b("KFILTER", p.kfilter != 0, "Noise filter on/off"));
+        k.push_back(FitsKey::s("BASIS", "LEGENDRE", "Destriping function basis"));
+        k.push_back(FitsKey::i("BORDER", p.basis_order, "Destriping function order"));
+    }
+    k.push_back(FitsKey::i("NSIDECR", p.nside_cross, "Crossing point resolution"));
+    k.push_back(FitsKey::i("PIXMCR", p.pixmode_cross, "Pixel removal criterion"));
+    k.push_back(FitsKey::r("PIXLCR", F32(p.pixlim_cross), "Pixel removal limit"));
+    k.push_back(FitsKey::r("CGLIMIT", F32(p.cglimit), "Conjugate gradient convergence limit"));
+    k.push_back(FitsKey::i("ITERMAX", p.iter_max, "Maximum number of iterations"));
+    if (p.kfilter) {
+        k.push_back(FitsKey::note("---------------------------------------"));
+        k.push_back(FitsKey::note("           Noise filter                "));
+        k.push_back(FitsKey::note("---------------------------------------"));
+        k.push_back(FitsKey::r("FILTTIM", (double)c.noba_pp_max * p.nshort / p.fsample, "Noise filter length/seconds"));
+        for (int i = 0; i < c.ndet; ++i) {
+            k.push_back(FitsKey::note(c.det_names[i]));
+            k.push_back(FitsKey::r("SIGMA" + std::to_string(i + 1), F32(c.det[i].sigmas[0]), "White noise std"));
+            k.push_back(FitsKey::r("PLATEAU" + std::to_string(i + 1), F32(c.det[i].plateaus[0]),
+                                   "White noise plateau"));
+        }
+    }
+}
+
+static void hdr_healpix(const Context &c, std::vector<FitsKey> &k)
+{
+    const Params &p = c.p;
+    k.push_back(FitsKey::note("----------------------------------"));
+    k.push_back(FitsKey::note("   Healpix Map Specific Keywords  "));
+    k.push_back(FitsKey::note("----------------------------------"));
+    k.push_back(FitsKey::s("PIXTYPE", "HEALPIX", "HEALPIX Pixelisation"));
+    k.push_back(FitsKey::s("ORDERING", "NESTED", "Pixel ordering scheme"));
+    k.push_back(FitsKey::i("NSIDE", p.nside_map, "Resolution parameter for HEALPIX"));
+    k.push_back(FitsKey::i("FIRSTPIX", 0, "First pixel # (0 based)"));
+    k.push_back(FitsKey::i("LASTPIX", 12LL * p.nside_map * p.nside_map - 1, "Last pixel # (0 based)"));
+    k.push_back(FitsKey::b("POLAR", c.nmap != 1, "Polarisation included (True/False)"));
+    k.push_back(FitsKey::s("POLCCONV", "COSMO", "Coord. convention for polarisation (COSMO/IAU)"));
+    if (p.write_cut) {
+        k.push_back(FitsKey::note("Cut sky data"));
+        k.push_back(FitsKey::s("OBJECT", "PARTIAL"));
+        k.push_back(FitsKey::s("INDXSCHM", "EXPLICIT", " Indexing : IMPLICIT or EXPLICIT"));
+        k.push_back(FitsKey::i("GRAIN", 1, " Grain of pixel indexing"));
+    } else {
+        k.push_back(FitsKey::note("Full sky data"));
+        k.push_back(FitsKey::s("OBJECT", "FULLSKY"));
+        k.push_back(FitsKey::s("INDXSCHM", "IMPLICIT", " Indexing : IMPLICIT or EXPLICIT"));
+        k.push_back(FitsKey::i("GRAIN", 0, " Grain of pixel indexing"));
+    }
+    k.push_back(FitsKey::note("GRAIN=0 : no indexing of pixel data                         (IMPLICIT)"));
+    k.push_back(FitsKey::note("GRAIN=1 : 1 pixel index -> 1 pixel data                     (EXPLICIT)"));
+    k.push_back(FitsKey::note("GRAIN>1 : 1 pixel index -> data of GRAIN consecutive pixels (EXPLICIT)"));
+}
+
+static void hdr_sky(const Context &c, int64_t skycover, std::vector<FitsKey> &k)
+{
+    const Params &p = c.p;
+    k.push_back(FitsKey::note("----------------------------------"));
+    k.push_back(FitsKey::note("         Sky coverage             "));
+    k.push_back(FitsKey::note("----------------------------------"));
+    k.push_back(FitsKey::i("PIXCOV", skycover, "Recovered pixels"));
+    k.push_back(FitsKey::r("SKYCOV", F32(skycover / (12.0 * p.nside_map * p.nside_map)), "Sky coverage"));
+    k.push_back(FitsKey::i("PIXMMAP", p.pixmode_map, "Pixel removal criterion"));
+    k.push_back(FitsKey::r("PIXLMAP", F32(p.pixlim_map), "Pixel removal limit"));
+}
+
+// One product file.  sel: pixels written in cut-sky form (write_cut), vals:
+// per-column data over all pixels (float or double by type, int32 for 'J').
+struct ProductCol {
+    std::string name, unit;
+    char type;
+    std::vector<float> f;
+    std::vector<double> d;
+    std::vector<int32_t> j;
+};
+
+static void write_product(const Context &c, const std::string &fname, const std::vector<ProductCol> &cols,
+                          const std::vector<char> &sel, int64_t repeat_full, const std::vector<FitsKey> &keys)
+{
+    std::string fn = check_outfile(fname);
+    if (fn.empty()) return;
+    bool overwrite = fn[0] == '!';
+    if (overwrite) fn = fn.substr(1);
+    int64_t np = c.npix_map;
+    std::vector<FitsColumn> fc;
+    std::vector<int64_t> idx;
+    std::vector<ProductCol> packed;
+    int64_t nrows;
+    if (c.p.write_cut) {
+        for (int64_t q = 0; q < np; ++q)
+            if (sel[q]) idx.push_back(q);
+        packed = cols;
+        for (auto &pc : packed) {
+            int64_t n = 0;
+            for (int64_t q : idx) {
+                if (pc.type == 'E') pc.f[n] = pc.f[q];
+                else if (pc.type == 'D') pc.d[n] = pc.d[q];
+                else pc.j[n] = pc.j[q];
+                ++n;
+            }
+        }
+        FitsColumn ic;
+        ic.name = "Pixel index";
+        ic.type = 'K';
+        ic.repcount = 1;
+        ic.data = idx.data();
+        fc.push_back(ic);
+        nrows = (int64_t)idx.size();
+    } else {
+        packed = cols;
+        nrows = np / repeat_full;
+    }
+    for (auto &pc : packed) {
+        FitsColumn x;
+        x.name = pc.name;
+        x.unit = pc.unit;
+        x.type = pc.type;
+        x.repcount = c.p.write_cut ? 1 : (int)repeat_full;
+        x.data = pc.type == 'E' ? (const void *)pc.f.data() : pc.type == 'D' ? (const void *)pc.d.data()
+                                                                              : (const void *)pc.j.data();
+        fc.push_back(x);
+    }
+    std::string err = fits_write_bintable(fn, fc, nrows, keys, c.p.write_cut ? "CUT SKY MAP" : "FULL SKY MAP");
+    if (!err.empty()) fail("FITS output failed: %s", err.c_str());
+    (void)overwrite;
+}
+
+static void write_products(Context &c, const OutNames &n, int isub)
+{
+    if (n.map.empty() && n.binmap.empty() && n.hit.empty() && n.mask.empty() && n.matrix.empty() && n.wcov.empty())
+        return;
+    const Params &p = c.p;
+    HostProducts h;
+    fetch_products(c, h);
+    int64_t np = c.npix_map;
+    int nm = c.nmap;
+    int64_t nosubpix = (int64_t)(p.nside_map / p.nside_submap) * (p.nside_map / p.nside_submap);
+    std::vector<char> good(np), hit(np), ccnz(np);
+    int64_t skycover = 0;
+    for (int64_t q = 0; q < np; ++q) {
+        good[q] = h.mask[q] > 0;
+        hit[q] = h.hits[q] > 0;
+        ccnz[q] = h.cc6[q * 6] != 0;
+        skycover += good[q];
+    }
+    static const char *names3[3] = {"TEMPERATURE", "Q-POLARISATION", "U-POLARISATION"};
+    auto map_cols = [&](const std::vector<double> &m) {
+        std::vector<ProductCol> cols(nm);
+        for (int a = 0; a < nm; ++a) {
+            cols[a].name = names3[a];
+            cols[a].unit = p.unit_tod;
+            cols[a].type = 'E';
+            cols[a].f.resize(np);
+            for (int64_t q = 0; q < np; ++q) cols[a].f[q] = good[q] ? (float)m[q * nm + a] : HEALPIX_UNDEF;
+        }
+        return cols;
+    };
+    auto matrix_cols = [&](const std::vector<double> &m6) {
+        // cc(i, j), i <= j, from the 6-element symmetric storage (11, 12, 13, 22, 23, 33)
+        std::vector<ProductCol> cols;
+        static const int sl[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        for (int a = 0; a < nm; ++a)
+            for (int b = a; b < nm; ++b) {
+                ProductCol pc;
+                pc.name = "cc(" + std::to_string(a + 1) + "," + std::to_string(b + 1) + ")";
+                pc.type = 'D';
+                pc.d.resize(np);
+                for (int64_t q = 0; q < np; ++q) pc.d[q] = m6[q * 6 + sl[a][b]];
+                cols.push_back(pc);
+            }
+        return cols;
+    };
+    auto base_keys = [&](bool destriping) {
+        std::vector<FitsKey> k;
+        hdr_code(k);
+        hdr_simulation(c, isub, k);
+        if (destriping) hdr_destriping(c, k);
+        return k;
+    };
+    if (!n.matrix.empty()) {
+        auto k = base_keys(false);
+        k.push_back(FitsKey::s("objtype", "madam.matrix", "Object type"));
+        hdr_healpix(c, k);
+        write_product(c, n.matrix, matrix_cols(h.cc6), ccnz, nosubpix, k);
+        printf(" Pixel matrix written in %s\n", n.matrix.c_str());
+    }
+    if (!n.wcov.empty()) {
+        auto k = base_keys(false);
+        k.push_back(FitsKey::s("objtype", "madam.matrix", "Object type"));
+        hdr_healpix(c, k);
+        write_product(c, n.wcov, matrix_cols(h.ci6), good, nosubpix, k);
+        printf(" Pixel matrix written in %s\n", n.wcov.c_str());
+    }
+    if (!n.map.empty()) {
+        auto k = base_keys(true);
+        k.push_back(FitsKey::s("objtype", "madam.map", "Object type"));
+        hdr_healpix(c, k);
+        hdr_sky(c, skycover, k);
+        k.push_back(FitsKey::b("MONOPOLE", p.rm_monopole != 0, "Monopole removed (T map)"));
+        write_product(c, n.map, map_cols(h.map), good, nosubpix, k);
+        printf(" Map written in %s\n", n.map.c_str());
+    }
+    if (!n.binmap.empty()) {
+        auto k = base_keys(false);
+        k.push_back(FitsKey::s("objtype", "madam.binmap", "Object type"));
+        hdr_healpix(c, k);
+        hdr_sky(c, skycover, k);
+        k.push_back(FitsKey::b("MONOPOLE", p.rm_monopole != 0, "Monopole removed (T map)"));
+        write_product(c, n.binmap, map_cols(h.binmap), good, nosubpix, k);
+        printf(" Binned map written in %s\n", n.binmap.c_str());
+    }
+    if (!n.mask.empty()) {
+        auto k = base_keys(true);
+        k.push_back(FitsKey::s("objtype", "madam.mask", "Object type"));
+        hdr_healpix(c, k);
+        hdr_sky(c, skycover, k);
+        std::vector<ProductCol> cols(2);
+        cols[0].name = "Mask";
+        cols[0].type = 'J';
+        cols[0].j = h.mask;
+        cols[1].name = "criterion";
+        cols[1].type = 'E';
+        cols[1].f.resize(np);
+        for (int64_t q = 0; q < np; ++q) cols[1].f[q] = (float)h.crit[q];
+        if (p.write_cut) cols.erase(cols.begin());   // the mask is redundant in cut-sky form
+        write_product(c, n.mask, cols, good, nosubpix, k);
+        printf(" Mask written in %s\n", n.mask.c_str());
+    }
+    if (!n.hit.empty()) {
+        auto k = base_keys(false);
+        k.push_back(FitsKey::s("objtype", "madam.hits", "Object type"));
+        hdr_healpix(c, k);
+        std::vector<ProductCol> cols(1);
+        cols[0].name = "Total hits";
+        cols[0].type = 'J';
+        cols[0].j = h.hits;
+        int64_t rep = std::min<int64_t>(std::max<int64_t>(12 * 1024, nosubpix), np);
+        write_product(c, n.hit, cols, hit, rep, k);
+        printf(" Hit count written in %s\n", n.hit.c_str());
+    }
+}
+
 static Context *g_cache = nullptr;   // destripe(mcmode=T) keeps its context here
 
 }  // namespace madam
@@ -1649,9 +2117,27 @@ MADAM_API void destripe(int comm, char *parstring, long ndet, char *detstring, d
         MADAM_CHECK(hipGetDevice(&c->device));
         configure(*c, parstring, ndet, detstring, detweights, nsamp, nnz, (const double *)timestamps, nperiod, periods,
                   npsd, npsdtot, psdstarts, npsdbin, psdfreqs, npsdval, psdvals, 0);
-        set_pointing(*c, (int32_t *)pix, (float *)pixweights, 0);
-        solve(*c, (double *)signal, 0);
-        write_back_pixels(*c, (int32_t *)pix);
+        // the subchunk loop (src/smadam.F90:193-201): passes isubchunk + 1 .. nsubchunk, then the full data (0)
+        const Params p0 = c->p;
+        for (int counter = p0.isubchunk; counter <= p0.nsubchunk; ++counter) {
+            int isub = (counter + 1) % (p0.nsubchunk + 1);
+            OutNames names = output_names(p0, p0.path_output, isub);
+            if (p0.skip_existing && !names.map.empty() && file_exists(names.map)) continue;
+            if (isub == 0) {
+                set_pointing(*c, (int32_t *)pix, (float *)pixweights, 0);
+                solve(*c, (double *)signal, 0);
+                write_products(*c, names, 0);
+            } else {
+                Context sub;
+                sub.device = c->device;
+                configure(sub, parstring, ndet, detstring, detweights, nsamp, nnz, (const double *)timestamps,
+                          nperiod, periods, npsd, npsdtot, psdstarts, npsdbin, psdfreqs, npsdval, psdvals, isub);
+                set_pointing(sub, (int32_t *)pix, (float *)pixweights, 0);
+                solve(sub, (double *)signal, 0);
+                write_products(sub, names, isub);
+            }
+        }
+        if (c->solved) write_back_pixels(*c, (int32_t *)pix);
     });
     if (rc) {
         delete c;
@@ -1666,7 +2152,6 @@ MADAM_API void destripe_with_cache(int comm, long ndet, long nsamp, long nnz, vo
                                    void *pixweights, void *signal, char *outpath)
 {
     (void)comm;
-    (void)outpath;   // FITS output is not produced by this library
     (void)timestamps;
     g_last_status = 0;
     int rc = guarded([&] {
@@ -1676,6 +2161,14 @@ MADAM_API void destripe_with_cache(int comm, long ndet, long nsamp, long nnz, vo
         set_pointing(c, (int32_t *)pix, (float *)pixweights, 0);
         solve(c, (double *)signal, 0);
         write_back_pixels(c, (int32_t *)pix);
+        // src/smadam.F90:643-656, 797-803: map and binned map into outpath when given
+        std::string path = (outpath && outpath[0]) ? strip(outpath) : c.p.path_output;
+        OutNames names = output_names(c.p, path, 0);
+        names.hit.clear();
+        names.mask.clear();
+        names.matrix.clear();
+        names.wcov.clear();
+        write_products(c, names, 0);
     });
     if (rc) abort_like_reference("destripe_with_cache");
 }

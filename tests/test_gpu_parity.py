@@ -150,11 +150,11 @@ def test_subchunk_pass():
         check_parity(g, o, c1, c2)
 
 
-def test_wrapper_drop_in_mutates_like_reference():
+def test_wrapper_drop_in_mutates_like_reference(tmp_path):
     import os
     from libmadam_amd import libmadam_wrapper as madam
     os.environ["MADAM_RAISE"] = "1"
-    case = ring(prior=True, nsamp=20000, extra=dict(write_tod=True))
+    case = ring(prior=True, nsamp=20000, extra=dict(write_tod=True, path_output=str(tmp_path)))
     case["pixels"][100:200] = -1
     c1 = mc.copy_case(case)
     c2 = mc.copy_case(case)
@@ -188,14 +188,14 @@ def test_inner_cg_preconditioner_forced():
     check_parity(g, o, c1, c2)
 
 
-def test_mc_mode_cache_cycle():
+def test_mc_mode_cache_cycle(tmp_path):
     """destripe(mcmode=T) -> destripe_with_cache(new signal) -> clear_caches, as in
     python/tests/test_libmadam_mc.py:132-191; the cached solve equals a fresh
     solve of the new signal (clean right-hand side)."""
     import os
     from libmadam_amd import libmadam_wrapper as madam
     os.environ["MADAM_RAISE"] = "1"
-    case = ring(prior=True, nsamp=20000, extra=dict(mcmode=True, write_tod=True))
+    case = ring(prior=True, nsamp=20000, extra=dict(mcmode=True, write_tod=True, path_output=str(tmp_path)))
     c1 = mc.copy_case(case)
     madam.destripe(None, c1["pars"], c1["dets"], c1["weights"], c1["timestamps"], c1["pixels"], c1["pixweights"],
                    c1["signal"], c1["periods"], c1["npsd"], c1["psdstarts"], c1["psdfreqs"], c1["psdvals"])
@@ -203,12 +203,16 @@ def test_mc_mode_cache_cycle():
     sig2 = case["signal"] + rng.standard_normal(case["signal"].size) * 1e-4
     c2 = mc.copy_case(case)
     c2["signal"] = sig2.copy()
-    madam.destripe_with_cache(None, c2["timestamps"], c2["pixels"], c2["pixweights"], c2["signal"], "unused")
+    (tmp_path / "mc").mkdir()
+    madam.destripe_with_cache(None, c2["timestamps"], c2["pixels"], c2["pixweights"], c2["signal"],
+                              str(tmp_path / "mc"))
+    assert (tmp_path / "mc" / "madam_map.fits").exists()
     madam.clear_caches()
     madam.clear_caches()   # a second call only warns
     c3 = mc.copy_case(case)
     c3["signal"] = sig2.copy()
     c3["pars"] = dict(case["pars"], mcmode=False)
+    c3["pars"].pop("path_output")
     oracle.destripe(*mc.call_args(c3))
     assert rel_rms(c2["signal"], c3["signal"]) < RTOL_RMS
     assert np.array_equal(c2["pixels"], c3["pixels"])

@@ -1,0 +1,108 @@
+"""FITS products of destripe() (src/output.f90), read back with a small
+numpy FITS reader standing in for healpy.
+
+The first test replays python/tests/test_libmadam.py:41-180 through the
+drop-in wrapper: two destripe calls (Nside 8, then 16) into one output
+directory, nsubchunk = 2, and the reference's own checks on the files it
+reads back: hits equal to the bincount, the binned map equal to the per-pixel
+mean, and the two subchunk hit maps summing to the full one.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests import fits_io
+from tests import madam_cases as mc
+
+pytestmark = pytest.mark.gpu
+
+
+def _destripe(madam, pars, c):
+    madam.destripe(None, pars, c["dets"], c["weights"], c["timestamps"], c["pixels"], c["pixweights"], c["signal"],
+                   c["periods"], c["npsd"], c["psdstarts"], c["psdfreqs"], c["psdvals"])
+
+
+def test_reference_test_products(tmp_path):
+    os.environ["MADAM_RAISE"] = "1"
+    from libmadam_amd import libmadam_wrapper as madam
+    case, gold = mc.reference_test_case()
+    out = tmp_path / "pymaps"
+    out.mkdir()
+    pars = dict(case["pars"], path_output=str(out) + "/")
+    c = mc.copy_case(case)
+    nside = 8
+    for _ in range(2):   # the reference calls destripe twice with the same arrays
+        _destripe(madam, pars, c)
+        nside *= 2
+        pars["nside_map"] = nside
+    hmap = fits_io.read_map(str(out / "madam_pytest_hmap.fits"))
+    bmap = fits_io.read_map(str(out / "madam_pytest_bmap.fits"))
+    np.testing.assert_allclose(hmap, gold["hmap"])
+    np.testing.assert_allclose(bmap, gold["bmap32"].astype(np.float64))
+    h1 = fits_io.read_map(str(out / "madam_pytest_hmap_sub1of2.fits"))
+    h2 = fits_io.read_map(str(out / "madam_pytest_hmap_sub2of2.fits"))
+    np.testing.assert_allclose(hmap, h1 + h2)
+    # the second call (Nside 16) found the files and numbered its own (check_outfile)
+    hdr, _, _ = fits_io.read_bintable(str(out / "madam_pytest_hmap_001.fits"))
+    assert hdr["NSIDE"] == 16
+    for name in ("map", "bmap", "hmap", "wcov", "wcov_inv"):
+        hdr, comments, cols = fits_io.read_bintable(str(out / ("madam_pytest_%s.fits" % name)))
+        assert hdr["PIXTYPE"] == "HEALPIX" and hdr["ORDERING"] == "NESTED" and hdr["NSIDE"] == 8
+        assert hdr["EXTNAME"] == "FULL SKY MAP" and hdr["INDXSCHM"] == "IMPLICIT"
+        assert hdr["CREATOR"] == "Madam" and hdr["NODET"] == 4 and hdr["DETNAM2"] == "LFI27S"
+        n = next(iter(cols.values())).size
+        assert n == 12 * 8 ** 2
+    hdr, _, cols = fits_io.read_bintable(str(out / "madam_pytest_map.fits"))
+    assert list(cols) == ["TEMPERATURE"] and hdr["TFORM1"] == "16E" and hdr["NAXIS2"] == 48
+    assert hdr["objtype"] == "madam.map" and hdr["KFILTER"] is True and hdr["NSUBCHNK"] == 2
+    hdr, _, cols = fits_io.read_bintable(str(out / "madam_pytest_wcov.fits"))
+    assert list(cols) == ["cc(1,1)"] and hdr["TFORM1"] == "16D"
+
+
+def test_products_match_in_memory_results(tmp_path):
+    """IQU products of one call against the same solve's in-memory outputs,
+    full-sky and cut-sky (write_cut) forms, with the mask file."""
+    os.environ["MADAM_RAISE"] = "1"
+    import libmadam_amd
+    from libmadam_amd import libmadam_wrapper as madam
+    from tests.test_gpu_parity import ring
+    case = ring(prior=True, nside=8, nsamp=20000, extra=dict(write_binmap=True, write_hits=True, write_mask=True,
+                                                              write_wcov=True, write_matrix=True,
+                                                              nside_submap=2, unit_tod="K_CMB"))
+    case["pixels"][300:400] = -1
+    ref = libmadam_amd.run(*mc.call_args(mc.copy_case(case)))
+    npix = 12 * 8 ** 2
+    good = ref["mask"] > 0
+    for cut in (False, True):
+        d = tmp_path / ("cut%d" % cut)
+        d.mkdir()
+        c = mc.copy_case(case)
+        pars = dict(c["pars"], path_output=str(d), file_root="run", write_cut=cut)
+        _destripe(madam, pars, c)
+        hdr, _, cols = fits_io.read_bintable(str(d / "run_map.fits"))
+        assert list(cols)[-3:] == ["TEMPERATURE", "Q-POLARISATION", "U-POLARISATION"]
+        assert hdr["TUNIT%d" % (2 if cut else 1)] == "K_CMB"
+        assert hdr["PIXCOV"] == int(good.sum())
+        hdr_h, _, hcols = fits_io.read_bintable(str(d / "run_hmap.fits"))
+        hdr_w, _, wcols = fits_io.read_bintable(str(d / "run_wcov.fits"))
+        hdr_m, _, mcols = fits_io.read_bintable(str(d / "run_mask.fits"))
+        if not cut:
+            for a, name in enumerate(["TEMPERATURE", "Q-POLARISATION", "U-POLARISATION"]):
+                exp = np.where(good, ref["map"][:, a].astype(np.float32), np.float32(-1.6375e30))
+                assert np.array_equal(cols[name], exp), name
+            assert np.array_equal(hcols["Total hits"], ref["hits"])
+            assert np.array_equal(mcols["Mask"], ref["mask"])
+            np.testing.assert_array_equal(wcols["cc(1,2)"], ref["cc"][:, 0, 1])
+            assert hdr["INDXSCHM"] == "IMPLICIT" and hdr["EXTNAME"] == "FULL SKY MAP"
+        else:
+            idx = cols["Pixel index"]
+            assert np.array_equal(idx, np.nonzero(good)[0])
+            assert np.array_equal(cols["U-POLARISATION"], ref["map"][good, 2].astype(np.float32))
+            assert np.array_equal(hcols["Pixel index"], np.nonzero(ref["hits"] > 0)[0])
+            assert np.array_equal(hcols["Total hits"], ref["hits"][ref["hits"] > 0])
+            np.testing.assert_array_equal(wcols["cc(3,3)"], ref["cc"][good, 2, 2])
+            assert list(mcols) == ["Pixel index", "criterion"]
+            assert hdr["INDXSCHM"] == "EXPLICIT" and hdr["EXTNAME"] == "CUT SKY MAP"
+        _, _, icols = fits_io.read_bintable(str(d / "run_wcov_inv.fits"))
+        assert "cc(2,3)" in icols
