@@ -360,6 +360,9 @@ struct Context {
     int pass = 0;
     std::vector<Detector> det;
     std::vector<std::string> det_names;
+    std::vector<double> timestamps;   // host copy, for the survey spans of the subset products
+    bool subsets_enabled = false;     // destripe(): bin the detset / survey subsets inside the solve
+    std::string subsets_path;
     std::vector<double> psdfreqs;
     // intervals
     int nival = 0;
@@ -500,12 +503,13 @@ static void configure(Context &c, const char *parstring, long ndet, const char *
                       const long *npsd, long npsdtot, const double *psdstarts, long npsdbin, const double *psdfreqs,
                       long npsdval, const double *psdvals, int pass)
 {
+    c.timestamps.assign(timestamps, timestamps + (nsamp > 0 ? nsamp : 0));
     c.det_names.clear();
     {
         std::string ds = detstring ? detstring : "";
         size_t a = 0;
         while (a <= ds.size() && (long)c.det_names.size() < ndet) {
-            size_t b = ds.find(',', a);
+            size_t b = ds.find(';', a);   // names separated by ';' (dets2detstring)
             if (b == std::string::npos) b = ds.size();
             c.det_names.push_back(strip(ds.substr(a, b - a)));
             a = b + 1;
@@ -1341,6 +1345,8 @@ static int pcg_steps(Context &c, int nmax, bool converge)
     return done;
 }
 
+static void run_subsets(Context &c);
+
 static void solve(Context &c, double *signal, int on_device)
 {
     if (!c.pointed) fail("madam_solve called before the pointing was set");
@@ -1438,6 +1444,10 @@ static void solve(Context &c, double *signal, int on_device)
     launch_ccmultiply(s, c.nmap, c.npix_map, c.d_ccinv6.p, c.d_rhs4.p, c.d_binmap4.p);
     if (p.kfirst) launch_ccmultiply(s, c.nmap, c.npix_map, c.d_ccinv6.p, c.d_rhsd4.p, c.d_map4.p);
     else MADAM_CHECK(hipMemsetAsync(c.d_map4.p, 0, sizeof(double) * c.npix_map * 4, s));
+    MADAM_CHECK(hipStreamSynchronize(s));
+    c.solved = true;
+    // subset maps (src/smadam.F90:430, 954-1257) see the TOD before the final clean
+    if (c.subsets_enabled) run_subsets(c);
     // clean_tod (src/smadam_routines.f90:1545-1585), final full-data pass only
     bool clean = c.pass == 0 && p.kfirst && (p.write_tod || p.ndetset > 0 || p.nsurvey > 0);
     if (clean) {
@@ -1579,14 +1589,20 @@ static OutNames output_names(const Params &p, const std::string &path, int isub)
     return n;
 }
 
+// host copy of one set of products (the solve's, or one subset's)
 struct HostProducts {
-    std::vector<double> map, binmap, cc6, ci6, crit;
+    int nmap = 3;
+    bool temponly = false, kfirst = true;
+    std::vector<double> map, binmap, cc6, ci6, crit;   // maps [npix][nmap]
     std::vector<int32_t> hits, mask;
 };
 
 static void fetch_products(Context &c, HostProducts &h)
 {
     int64_t np = c.npix_map;
+    h.nmap = c.nmap;
+    h.temponly = c.p.temperature_only != 0;
+    h.kfirst = c.p.kfirst != 0;
     h.map.resize(np * c.nmap);
     h.binmap.resize(np * c.nmap);
     h.hits.resize(np);
@@ -1612,7 +1628,7 @@ static void hdr_code(std::vector<FitsKey> &k)
     k.push_back(FitsKey::s("PARFILE", "", "Parameter file"));
 }
 
-static void hdr_simulation(const Context &c, int isub, std::vector<FitsKey> &k)
+static void hdr_simulation(const Context &c, int isub, const HostProducts &h, std::vector<FitsKey> &k)
 {
     const Params &p = c.p;
     double to_d = 1.0 / p.fsample / 24.0 / 3600.0;
@@ -1633,13 +1649,13 @@ static void hdr_simulation(const Context &c, int isub, std::vector<FitsKey> &k)
     k.push_back(FitsKey::i("MDETW", p.mode_detweight, "Detector weighting mode"));
     for (int i = 0; i < c.ndet; ++i)
         k.push_back(FitsKey::s("DETNAM" + std::to_string(i + 1), c.det_names[i], "Detector name"));
-    k.push_back(FitsKey::b("TEMPONLY", p.temperature_only != 0, "Temperature map only"));
+    k.push_back(FitsKey::b("TEMPONLY", h.temponly, "Temperature map only"));
     k.push_back(FitsKey::note("----------------------------"));
     k.push_back(FitsKey::note("          Pointing          "));
     k.push_back(FitsKey::note("----------------------------"));
 }
 
-static void hdr_destriping(const Context &c, std::vector<FitsKey> &k)
+static void hdr_destriping(const Context &c, const HostProducts &h, std::vector<FitsKey> &k)
 {
     const Params &p = c.p;
     k.push_back(FitsKey::note("---------------------------"));
@@ -1650,8 +1666,8 @@ static void hdr_destriping(const Context &c, std::vector<FitsKey> &k)
     k.push_back(FitsKey::note("-------------------------------------------"));
     k.push_back(FitsKey::note("        Destriping scheme                  "));
     k.push_back(FitsKey::note("-------------------------------------------"));
-    k.push_back(FitsKey::b("KFIRST", p.kfirst != 0, "First destriping phase on/off"));
-    if (p.kfirst) {
+    k.push_back(FitsKey::b("KFIRST", h.kfirst, "First destriping phase on/off"));
+    if (h.kfirst) {
         k.push_back(FitsKey::i("NSHORT", p.nshort, "Baseline length "));
         k.push_back(FitsKey::b("KFILTER", p.kfilter != 0, "Noise filter on/off"));
         k.push_back(FitsKey::s("BASIS", "LEGENDRE", "Destriping function basis"));
@@ -1676,7 +1692,7 @@ static void hdr_destriping(const Context &c, std::vector<FitsKey> &k)
     }
 }
 
-static void hdr_healpix(const Context &c, std::vector<FitsKey> &k)
+static void hdr_healpix(const Context &c, const HostProducts &h, std::vector<FitsKey> &k)
 {
     const Params &p = c.p;
     k.push_back(FitsKey::note("----------------------------------"));
@@ -1687,7 +1703,7 @@ static void hdr_healpix(const Context &c, std::vector<FitsKey> &k)
     k.push_back(FitsKey::i("NSIDE", p.nside_map, "Resolution parameter for HEALPIX"));
     k.push_back(FitsKey::i("FIRSTPIX", 0, "First pixel # (0 based)"));
     k.push_back(FitsKey::i("LASTPIX", 12LL * p.nside_map * p.nside_map - 1, "Last pixel # (0 based)"));
-    k.push_back(FitsKey::b("POLAR", c.nmap != 1, "Polarisation included (True/False)"));
+    k.push_back(FitsKey::b("POLAR", h.nmap != 1, "Polarisation included (True/False)"));
     k.push_back(FitsKey::s("POLCCONV", "COSMO", "Coord. convention for polarisation (COSMO/IAU)"));
     if (p.write_cut) {
         k.push_back(FitsKey::note("Cut sky data"));
@@ -1727,11 +1743,11 @@ struct ProductCol {
     std::vector<int32_t> j;
 };
 
-static void write_product(const Context &c, const std::string &fname, const std::vector<ProductCol> &cols,
-                          const std::vector<char> &sel, int64_t repeat_full, const std::vector<FitsKey> &keys)
+static std::string write_product(const Context &c, const std::string &fname, const std::vector<ProductCol> &cols,
+                                 const std::vector<char> &sel, int64_t repeat_full, const std::vector<FitsKey> &keys)
 {
     std::string fn = check_outfile(fname);
-    if (fn.empty()) return;
+    if (fn.empty()) return fn;
     bool overwrite = fn[0] == '!';
     if (overwrite) fn = fn.substr(1);
     int64_t np = c.npix_map;
@@ -1776,17 +1792,14 @@ static void write_product(const Context &c, const std::string &fname, const std:
     std::string err = fits_write_bintable(fn, fc, nrows, keys, c.p.write_cut ? "CUT SKY MAP" : "FULL SKY MAP");
     if (!err.empty()) fail("FITS output failed: %s", err.c_str());
     (void)overwrite;
+    return fn;
 }
 
-static void write_products(Context &c, const OutNames &n, int isub)
+static void write_product_set(Context &c, const OutNames &n, int isub, const HostProducts &h)
 {
-    if (n.map.empty() && n.binmap.empty() && n.hit.empty() && n.mask.empty() && n.matrix.empty() && n.wcov.empty())
-        return;
     const Params &p = c.p;
-    HostProducts h;
-    fetch_products(c, h);
     int64_t np = c.npix_map;
-    int nm = c.nmap;
+    int nm = h.nmap;
     int64_t nosubpix = (int64_t)(p.nside_map / p.nside_submap) * (p.nside_map / p.nside_submap);
     std::vector<char> good(np), hit(np), ccnz(np);
     int64_t skycover = 0;
@@ -1826,46 +1839,46 @@ static void write_products(Context &c, const OutNames &n, int isub)
     auto base_keys = [&](bool destriping) {
         std::vector<FitsKey> k;
         hdr_code(k);
-        hdr_simulation(c, isub, k);
-        if (destriping) hdr_destriping(c, k);
+        hdr_simulation(c, isub, h, k);
+        if (destriping) hdr_destriping(c, h, k);
         return k;
     };
     if (!n.matrix.empty()) {
         auto k = base_keys(false);
         k.push_back(FitsKey::s("objtype", "madam.matrix", "Object type"));
-        hdr_healpix(c, k);
-        write_product(c, n.matrix, matrix_cols(h.cc6), ccnz, nosubpix, k);
-        printf(" Pixel matrix written in %s\n", n.matrix.c_str());
+        hdr_healpix(c, h, k);
+        std::string fw = write_product(c, n.matrix, matrix_cols(h.cc6), ccnz, nosubpix, k);
+        printf(" Pixel matrix written in %s\n", fw.c_str());
     }
     if (!n.wcov.empty()) {
         auto k = base_keys(false);
         k.push_back(FitsKey::s("objtype", "madam.matrix", "Object type"));
-        hdr_healpix(c, k);
-        write_product(c, n.wcov, matrix_cols(h.ci6), good, nosubpix, k);
-        printf(" Pixel matrix written in %s\n", n.wcov.c_str());
+        hdr_healpix(c, h, k);
+        std::string fw = write_product(c, n.wcov, matrix_cols(h.ci6), good, nosubpix, k);
+        printf(" Pixel matrix written in %s\n", fw.c_str());
     }
     if (!n.map.empty()) {
         auto k = base_keys(true);
         k.push_back(FitsKey::s("objtype", "madam.map", "Object type"));
-        hdr_healpix(c, k);
+        hdr_healpix(c, h, k);
         hdr_sky(c, skycover, k);
         k.push_back(FitsKey::b("MONOPOLE", p.rm_monopole != 0, "Monopole removed (T map)"));
-        write_product(c, n.map, map_cols(h.map), good, nosubpix, k);
-        printf(" Map written in %s\n", n.map.c_str());
+        std::string fw = write_product(c, n.map, map_cols(h.map), good, nosubpix, k);
+        printf(" Map written in %s\n", fw.c_str());
     }
     if (!n.binmap.empty()) {
         auto k = base_keys(false);
         k.push_back(FitsKey::s("objtype", "madam.binmap", "Object type"));
-        hdr_healpix(c, k);
+        hdr_healpix(c, h, k);
         hdr_sky(c, skycover, k);
         k.push_back(FitsKey::b("MONOPOLE", p.rm_monopole != 0, "Monopole removed (T map)"));
-        write_product(c, n.binmap, map_cols(h.binmap), good, nosubpix, k);
-        printf(" Binned map written in %s\n", n.binmap.c_str());
+        std::string fw = write_product(c, n.binmap, map_cols(h.binmap), good, nosubpix, k);
+        printf(" Binned map written in %s\n", fw.c_str());
     }
     if (!n.mask.empty()) {
         auto k = base_keys(true);
         k.push_back(FitsKey::s("objtype", "madam.mask", "Object type"));
-        hdr_healpix(c, k);
+        hdr_healpix(c, h, k);
         hdr_sky(c, skycover, k);
         std::vector<ProductCol> cols(2);
         cols[0].name = "Mask";
@@ -1876,20 +1889,168 @@ static void write_products(Context &c, const OutNames &n, int isub)
         cols[1].f.resize(np);
         for (int64_t q = 0; q < np; ++q) cols[1].f[q] = (float)h.crit[q];
         if (p.write_cut) cols.erase(cols.begin());   // the mask is redundant in cut-sky form
-        write_product(c, n.mask, cols, good, nosubpix, k);
-        printf(" Mask written in %s\n", n.mask.c_str());
+        std::string fw = write_product(c, n.mask, cols, good, nosubpix, k);
+        printf(" Mask written in %s\n", fw.c_str());
     }
     if (!n.hit.empty()) {
         auto k = base_keys(false);
         k.push_back(FitsKey::s("objtype", "madam.hits", "Object type"));
-        hdr_healpix(c, k);
+        hdr_healpix(c, h, k);
         std::vector<ProductCol> cols(1);
         cols[0].name = "Total hits";
         cols[0].type = 'J';
         cols[0].j = h.hits;
         int64_t rep = std::min<int64_t>(std::max<int64_t>(12 * 1024, nosubpix), np);
-        write_product(c, n.hit, cols, hit, rep, k);
-        printf(" Hit count written in %s\n", n.hit.c_str());
+        std::string fw = write_product(c, n.hit, cols, hit, rep, k);
+        printf(" Hit count written in %s\n", fw.c_str());
+    }
+}
+
+static void write_products(Context &c, const OutNames &n, int isub)
+{
+    if (n.map.empty() && n.binmap.empty() && n.hit.empty() && n.mask.empty() && n.matrix.empty() && n.wcov.empty())
+        return;
+    HostProducts h;
+    fetch_products(c, h);
+    write_product_set(c, n, isub, h);
+}
+
+// run_subsets (src/smadam.F90:954-1257): binned (and, with write_map, destriped)
+// maps of every detector set x survey combination, with their hits, pixel
+// matrices and masks on the first pass.
+static void run_subsets(Context &c)
+{
+    const Params &p = c.p;
+    if (!p.bin_subsets || (p.detsets.empty() && p.surveys.empty())) return;
+    hipStream_t s = c.stream;
+    int64_t n = (int64_t)c.ndet * c.nsamp, np = c.npix_map;
+    bool do_map = p.write_map != 0, do_binmap = p.write_binmap != 0;
+    int npass = (do_binmap && do_map) ? 2 : 1;
+    if (do_map) do_binmap = true;
+    DBuf<double> cc6, rhs4, ci6, crit, b4, tclean;
+    DBuf<int32_t> hits, mask;
+    DBuf<uint8_t> smask;
+    cc6.alloc(np * 6);
+    rhs4.alloc(np * 4);
+    ci6.alloc(np * 6);
+    crit.alloc(np);
+    b4.alloc(np * 4);
+    hits.alloc(np);
+    mask.alloc(np);
+    smask.alloc(n);
+    std::vector<uint8_t> hm(n);
+    std::string root = add_path(c.subsets_path, p.file_root);
+    for (int pass = 1; pass <= npass; ++pass) {
+        const double *tod = c.d_tod;
+        if (pass == npass && do_map && p.kfirst) {
+            // binned maps of the destriped TOD: tod - F a on a copy
+            tclean.alloc(n);
+            MADAM_CHECK(hipMemcpyAsync(tclean.p, c.d_tod, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            launch_clean_tod(s, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_aa.p, tclean.p);
+            tod = tclean.p;
+        }
+        for (int isv = 0; isv <= (int)p.surveys.size(); ++isv) {
+            std::vector<char> sflag(c.nsamp, 1);
+            std::string svname = p.survey0;
+            if (isv > 0) {
+                const auto &sv = p.surveys[isv - 1];
+                svname = sv.name;
+                int64_t nh = 0;
+                for (int64_t i = 0; i < c.nsamp; ++i) {
+                    char f = 0;
+                    for (size_t j = 0; j < sv.starts.size(); ++j)
+                        if (sv.starts[j] <= c.timestamps[i] && c.timestamps[i] <= sv.stops[j]) f = 1;
+                    sflag[i] = f;
+                    nh += f;
+                }
+                if (nh == 0) {
+                    if (p.info > 0) printf(" %s : no overlap\n", svname.c_str());
+                    continue;
+                }
+            }
+            for (int ids = 0; ids <= (int)p.detsets.size(); ++ids) {
+                if (ids == 0 && isv == 0) continue;   // the full data set is the main product
+                std::vector<char> dflag(c.ndet, 1);
+                std::string dsname = p.detset0;
+                bool temponly = p.temperature_only != 0;
+                if (ids > 0) {
+                    const auto &ds = p.detsets[ids - 1];
+                    dsname = ds.name;
+                    int nd = 0;
+                    for (int d = 0; d < c.ndet; ++d) {
+                        char f = 0;
+                        for (const auto &nm : ds.dets)
+                            if (nm == c.det_names[d]) f = 1;
+                        dflag[d] = f;
+                        nd += f;
+                    }
+                    temponly = ds.nopol && !p.force_pol;
+                    if (nd == 0) {
+                        if (p.info > 0) printf(" %s : no overlap\n", dsname.c_str());
+                        continue;
+                    }
+                }
+                int nm = temponly ? 1 : c.nmap;
+                std::string sub = root;
+                if (!dsname.empty()) sub += "_" + dsname;
+                if (!svname.empty()) sub += "_" + svname;
+                auto nmf = [&](bool on, const char *suffix) {
+                    std::string f = on ? add_subchunk_id(sub + suffix, c.pass, p.nsubchunk) : std::string();
+                    if (!f.empty() && file_exists(f[0] == '!' ? f.substr(1) : f)) f.clear();   // existing: skipped
+                    return f;
+                };
+                OutNames on;
+                on.binmap = (pass == npass && do_map) ? nmf(true, "_map.fits") : nmf(do_binmap, "_bmap.fits");
+                if (pass == 1) {
+                    on.hit = nmf(p.write_hits, "_hmap.fits");
+                    on.matrix = nmf(p.write_matrix, "_wcov_inv.fits");
+                    on.wcov = nmf(p.write_wcov, "_wcov.fits");
+                    on.mask = nmf(p.write_mask, "_mask.fits");
+                }
+                if (on.binmap.empty() && on.hit.empty() && on.matrix.empty() && on.wcov.empty() && on.mask.empty())
+                    continue;
+                for (int d = 0; d < c.ndet; ++d)
+                    for (int64_t i = 0; i < c.nsamp; ++i) hm[(size_t)d * c.nsamp + i] = dflag[d] && sflag[i];
+                h2d(smask.p, hm.data(), n, s);
+                launch_bin_subset(s, nm, np, c.d_off.p, c.d_ent.p, c.d_src.p, smask.p, tod, c.d_dwb.p, c.d_sdwb.p,
+                                  cc6.p, rhs4.p, hits.p);
+                MADAM_CHECK(hipMemsetAsync(crit.p, 0, sizeof(double) * np, s));
+                launch_invert(s, nm, np, cc6.p, ci6.p, crit.p, mask.p, p.pixmode_map, p.pixlim_map,
+                              p.incomplete_matrices, p.allow_decoupling, nullptr);
+                launch_ccmultiply(s, nm, np, ci6.p, rhs4.p, b4.p);
+                MADAM_CHECK(hipGetLastError());
+                HostProducts h;
+                h.nmap = nm;
+                h.temponly = temponly;
+                h.kfirst = false;
+                std::vector<double> m4(np * 4);
+                h.cc6.resize(np * 6);
+                h.ci6.resize(np * 6);
+                h.crit.resize(np);
+                h.hits.resize(np);
+                h.mask.resize(np);
+                d2h(m4.data(), b4.p, np * 4, s);
+                d2h(h.cc6.data(), cc6.p, np * 6, s);
+                d2h(h.ci6.data(), ci6.p, np * 6, s);
+                d2h(h.crit.data(), crit.p, np, s);
+                d2h(h.hits.data(), hits.p, np, s);
+                d2h(h.mask.data(), mask.p, np, s);
+                MADAM_CHECK(hipStreamSynchronize(s));
+                if (p.rm_monopole) {   // makemaps (src/map_routines.f90:341-353)
+                    int64_t cover = 0;
+                    double mono = 0;
+                    for (int64_t q = 0; q < np; ++q)
+                        if (h.mask[q] > 0) { cover++; mono += m4[q * 4]; }
+                    if (cover > 0)
+                        for (int64_t q = 0; q < np; ++q)
+                            if (h.mask[q] > 0) m4[q * 4] -= mono / cover;
+                }
+                h.binmap.resize(np * nm);
+                for (int64_t q = 0; q < np; ++q)
+                    for (int a = 0; a < nm; ++a) h.binmap[q * nm + a] = m4[q * 4 + a];
+                write_product_set(c, on, c.pass, h);
+            }
+        }
     }
 }
 
@@ -2124,6 +2285,8 @@ MADAM_API void destripe(int comm, char *parstring, long ndet, char *detstring, d
             OutNames names = output_names(p0, p0.path_output, isub);
             if (p0.skip_existing && !names.map.empty() && file_exists(names.map)) continue;
             if (isub == 0) {
+                c->subsets_enabled = true;
+                c->subsets_path = p0.path_output;
                 set_pointing(*c, (int32_t *)pix, (float *)pixweights, 0);
                 solve(*c, (double *)signal, 0);
                 write_products(*c, names, 0);
@@ -2132,6 +2295,8 @@ MADAM_API void destripe(int comm, char *parstring, long ndet, char *detstring, d
                 sub.device = c->device;
                 configure(sub, parstring, ndet, detstring, detweights, nsamp, nnz, (const double *)timestamps,
                           nperiod, periods, npsd, npsdtot, psdstarts, npsdbin, psdfreqs, npsdval, psdvals, isub);
+                sub.subsets_enabled = true;
+                sub.subsets_path = p0.path_output;
                 set_pointing(sub, (int32_t *)pix, (float *)pixweights, 0);
                 solve(sub, (double *)signal, 0);
                 write_products(sub, names, isub);
@@ -2158,11 +2323,13 @@ MADAM_API void destripe_with_cache(int comm, long ndet, long nsamp, long nnz, vo
         if (!g_cache) fail("destripe_with_cache called without cached data (run destripe with mcmode=T first)");
         Context &c = *g_cache;
         if (ndet != c.ndet || nsamp != c.nsamp || nnz != c.nnz) fail("destripe_with_cache: data shape differs from the cached run");
+        // src/smadam.F90:643-656, 797-803: map and binned map into outpath when given
+        std::string path = (outpath && outpath[0]) ? strip(outpath) : c.p.path_output;
+        c.subsets_enabled = true;
+        c.subsets_path = path;
         set_pointing(c, (int32_t *)pix, (float *)pixweights, 0);
         solve(c, (double *)signal, 0);
         write_back_pixels(c, (int32_t *)pix);
-        // src/smadam.F90:643-656, 797-803: map and binned map into outpath when given
-        std::string path = (outpath && outpath[0]) ? strip(outpath) : c.p.path_output;
         OutNames names = output_names(c.p, path, 0);
         names.hit.clear();
         names.mask.clear();

@@ -73,6 +73,9 @@ void launch_bin(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, c
                 const double *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4);
 void launch_degrade(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const double *cc6, const double *rhs4,
                     double *cc6x, double *rhs4x);
+void launch_bin_subset(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent,
+                       const uint32_t *src, const uint8_t *smask, const double *tod, const double *dwb,
+                       const double *sdwb, double *cc6, double *rhs4, int32_t *hits);
 void launch_invert(hipStream_t s, int nmap, int64_t npix, const double *cc6, double *ccinv6, double *crit,
                    int32_t *mask, int pixmode, double pixlim, int incomplete, int allow_decoupling,
                    unsigned long long *ngood);

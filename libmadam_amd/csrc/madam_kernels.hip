@@ -354,6 +354,63 @@ void launch_bin(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, c
                                                                cc6, rhs4);
 }
 
+// pixel_matrix + count_hits + bin_tod over the samples of one subset (detector
+// set x survey spans, src/smadam.F90:954-1257): smask[g] != 0 selects sample g
+__global__ void k_bin_subset(int nmap, int64_t npix, const int64_t *off, const CsrEnt *ent, const uint32_t *src,
+                             const uint8_t *smask, const double *tod, const double *dwb, const double *sdwb,
+                             double *cc6, double *rhs4, int32_t *hits)
+{
+    int lane = threadIdx.x & 63;
+    int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t q = wave; q < npix; q += nwaves) {
+        int64_t j0 = off[q], j1 = off[q + 1];
+        double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0, r0 = 0, r1 = 0, r2 = 0;
+        int nh = 0;
+        for (int64_t j = j0 + lane; j < j1; j += 64) {
+            uint32_t g = src[j];
+            if (!smask[g]) continue;
+            CsrEnt e = ent[j];
+            double d = tod[g];
+            double dw = dwb[e.b];
+            ++nh;
+            if (nmap == 3) {
+                double sq = sdwb[e.b];
+                double a0 = (double)e.w0 * sq, a1 = (double)e.w1 * sq, a2 = (double)e.w2 * sq;
+                c0 += a0 * a0; c1 += a0 * a1; c2 += a0 * a2;
+                c3 += a1 * a1; c4 += a1 * a2; c5 += a2 * a2;
+                r0 += (double)e.w0 * d * dw;
+                r1 += (double)e.w1 * d * dw;
+                r2 += (double)e.w2 * d * dw;
+            } else {
+                c0 += dw;
+                r0 += d * dw;
+            }
+        }
+        c0 = wave_sum(c0); r0 = wave_sum(r0);
+        nh = wave_sum(nh);
+        if (nmap == 3) {
+            c1 = wave_sum(c1); c2 = wave_sum(c2); c3 = wave_sum(c3); c4 = wave_sum(c4); c5 = wave_sum(c5);
+            r1 = wave_sum(r1); r2 = wave_sum(r2);
+        }
+        if (lane == 0) {
+            double *C = cc6 + q * 6;
+            C[0] = c0; C[1] = c1; C[2] = c2; C[3] = c3; C[4] = c4; C[5] = c5;
+            double *R = rhs4 + q * 4;
+            R[0] = r0; R[1] = r1; R[2] = r2; R[3] = 0.0;
+            hits[q] = nh;
+        }
+    }
+}
+
+void launch_bin_subset(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent,
+                       const uint32_t *src, const uint8_t *smask, const double *tod, const double *dwb,
+                       const double *sdwb, double *cc6, double *rhs4, int32_t *hits)
+{
+    k_bin_subset<<<grid_for(npix_map, TPB / 64, 1 << 20), TPB, 0, s>>>(nmap, npix_map, off, ent, src, smask, tod,
+                                                                      dwb, sdwb, cc6, rhs4, hits);
+}
+
 __global__ void k_degrade(int nmap, int64_t npc, int64_t ndeg, const double *cc6, const double *rhs4, double *cc6x,
                           double *rhs4x)
 {

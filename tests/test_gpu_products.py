@@ -58,6 +58,30 @@ def test_reference_test_products(tmp_path):
     assert hdr["objtype"] == "madam.map" and hdr["KFILTER"] is True and hdr["NSUBCHNK"] == 2
     hdr, _, cols = fits_io.read_bintable(str(out / "madam_pytest_wcov.fits"))
     assert list(cols) == ["cc(1,1)"] and hdr["TFORM1"] == "16D"
+    # the survey subset 'hm1 : 0 - 500' (bin_subsets = T, src/smadam.F90:954-1257):
+    # binned map and hits of the samples with 0 <= t <= 500, and the binned
+    # map of the destriped TOD (baselines from the oracle's solve)
+    import oracle
+    sel = case["timestamps"] <= 500
+    pix = case["pixels"].reshape(4, -1)[:, sel].ravel()
+    sig = case["signal"].reshape(4, -1)
+    h_exp = np.bincount(pix, minlength=768)
+    s_exp = np.bincount(pix, weights=sig[:, sel].ravel(), minlength=768)
+    seen = h_exp > 0
+    assert np.array_equal(fits_io.read_map(str(out / "madam_pytest_hm1_hmap.fits")), h_exp)
+    bm = fits_io.read_map(str(out / "madam_pytest_hm1_bmap.fits"))
+    np.testing.assert_allclose(bm[seen], (s_exp[seen] / h_exp[seen]).astype(np.float32), rtol=1e-6)
+    o = oracle.destripe(*mc.call_args(mc.copy_case(case)))
+    clean = sig.copy()
+    for d in range(4):
+        for kb in range(o["aa"].shape[1]):
+            a, n = o["base_start"][kb], o["base_len"][kb]
+            clean[d, a:a + n] -= o["aa"][d, kb]
+    c_exp = np.bincount(pix, weights=clean[:, sel].ravel(), minlength=768)
+    dm = fits_io.read_map(str(out / "madam_pytest_hm1_map.fits"))
+    np.testing.assert_allclose(dm[seen], c_exp[seen] / h_exp[seen], rtol=1e-6, atol=1e-6)
+    for name in ("hm1_hmap_sub1of2", "hm1_bmap_sub2of2", "hm1_map_sub1of2", "hm1_wcov"):
+        assert (out / ("madam_pytest_%s.fits" % name)).exists(), name
 
 
 def test_products_match_in_memory_results(tmp_path):
