@@ -11,7 +11,9 @@
  * the reference, flagged samples are left as 12*nside_max**2 in the caller's
  * pix array, and the baselines are subtracted from signal when write_tod or a
  * detset / survey is given.  Not re-entrant: one call at a time per process.
- * FITS files are not written; use madam_get_maps for the products.
+ * destripe() writes the reference's FITS products (map, binned map, hits,
+ * mask, pixel matrices; subchunk and subset variants) under path_output;
+ * madam_get_maps returns the same products in memory.
  */
 #ifndef LIBMADAM_AMD_MADAM_H
 #define LIBMADAM_AMD_MADAM_H
