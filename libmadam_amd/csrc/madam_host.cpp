@@ -66,7 +66,7 @@ struct Params {
     // products (src/commonparam.f90:200-305, src/inputparam.f90:310-360)
     std::string path_output, file_root = "madam", unit_tod = "unknown";
     int write_map = 1, write_binmap = 0, write_hits = 0, write_mask = 0, write_matrix = 0, write_wcov = 0;
-    int write_cut = 0, bin_subsets = 0, force_pol = 0, skip_existing = 0;
+    int write_cut = 0, bin_subsets = 0, force_pol = 0, skip_existing = 0, write_leakmatrix = 0;
     struct DetSet {
         std::string name;
         std::vector<std::string> dets;
@@ -161,7 +161,8 @@ static void read_line(Params &p, std::string line)
     else if (key == "good_baseline_fraction") ok = parse_real(value, p.good_baseline_fraction);
     else if (key == "concatenate_messages" || key == "allreduce" || key == "reassign_submaps" ||
              key == "flag_by_horn" || key == "checknan" || key == "sync_output" || key == "unaligned_fft" ||
-             key == "write_base" || key == "write_leakmatrix" || key == "binary_output") ok = parse_bool(value, ig);
+             key == "write_base" || key == "binary_output") ok = parse_bool(value, ig);
+    else if (key == "write_leakmatrix") ok = parse_bool(value, p.write_leakmatrix);
     else if (key == "write_cut") ok = parse_bool(value, p.write_cut);
     else if (key == "skip_existing") ok = parse_bool(value, p.skip_existing);
     else if (key == "force_pol") ok = parse_bool(value, p.force_pol);
@@ -1528,6 +1529,7 @@ static const float HEALPIX_UNDEF = -1.6375e30f;
 
 struct OutNames {
     std::string map, binmap, hit, mask, matrix, wcov;
+    std::string leak;   // file_leakmatrix: "<leak>_<detector>.fits" per detector
 };
 
 static bool file_exists(const std::string &f)
@@ -1586,6 +1588,7 @@ static OutNames output_names(const Params &p, const std::string &path, int isub)
     n.matrix = mk(p.write_matrix, "_wcov_inv.fits");
     n.wcov = mk(p.write_wcov, "_wcov.fits");
     n.mask = mk(p.write_mask, "_mask.fits");
+    n.leak = mk(p.write_leakmatrix, "_leakmatrix");
     return n;
 }
 
@@ -1906,13 +1909,73 @@ static void write_product_set(Context &c, const OutNames &n, int isub, const Hos
     }
 }
 
+// leakmatrix + write_leakmatrix (src/smadam_routines.f90:38-105, src/output.f90:1181-1320):
+// per detector, gamma = C^-1 C_det with C_det the pixel matrix of that detector's
+// samples (restricted to sflag's time spans), C^-1 the products' inverse matrices
+static void write_leakmatrices(Context &c, const std::string &base, const std::vector<char> *sflag,
+                               const std::vector<char> &dallowed, int isub, const HostProducts &h)
+{
+    if (base.empty()) return;
+    const Params &p = c.p;
+    hipStream_t s = c.stream;
+    int64_t n = (int64_t)c.ndet * c.nsamp, np = c.npix_map;
+    int nm = h.nmap;
+    DBuf<double> cc6, rhs4;
+    DBuf<int32_t> hits;
+    DBuf<uint8_t> smask;
+    cc6.alloc(np * 6);
+    rhs4.alloc(np * 4);
+    hits.alloc(np);
+    smask.alloc(n);
+    std::vector<uint8_t> hm(n);
+    std::vector<double> cd(np * 6);
+    std::vector<char> good(np);
+    for (int64_t q = 0; q < np; ++q) good[q] = h.mask[q] > 0;
+    static const int sl[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    for (int d = 0; d < c.ndet; ++d) {
+        if (!dallowed[d]) continue;
+        for (int dd = 0; dd < c.ndet; ++dd)
+            for (int64_t i = 0; i < c.nsamp; ++i)
+                hm[(size_t)dd * c.nsamp + i] = dd == d && (!sflag || (*sflag)[i]);
+        h2d(smask.p, hm.data(), n, s);
+        launch_bin_subset(s, nm, np, c.d_off.p, c.d_ent.p, c.d_src.p, smask.p, c.d_tod, c.d_dwb.p, c.d_sdwb.p,
+                          cc6.p, rhs4.p, hits.p);
+        MADAM_CHECK(hipGetLastError());
+        d2h(cd.data(), cc6.p, np * 6, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        std::vector<ProductCol> cols(nm * nm);
+        for (int a = 0; a < nm; ++a)
+            for (int b = 0; b < nm; ++b) {
+                ProductCol &pc = cols[a * nm + b];
+                pc.name = "gamma(" + std::to_string(a + 1) + "," + std::to_string(b + 1) + ")";
+                pc.type = 'D';
+                pc.d.resize(np);
+                for (int64_t q = 0; q < np; ++q) {
+                    double v = 0;
+                    for (int m = 0; m < nm; ++m) v += h.ci6[q * 6 + sl[a][m]] * cd[q * 6 + sl[m][b]];
+                    pc.d[q] = v;
+                }
+            }
+        std::vector<FitsKey> k;
+        hdr_code(k);
+        hdr_simulation(c, isub, h, k);
+        k.push_back(FitsKey::s("objtype", "madam.leakmatrix", "Object type"));
+        hdr_healpix(c, h, k);
+        int64_t nosubpix = (int64_t)(p.nside_map / p.nside_submap) * (p.nside_map / p.nside_submap);
+        std::string fw = write_product(c, base + "_" + c.det_names[d] + ".fits", cols, good, nosubpix, k);
+        printf(" Leakage matrix written in %s\n", fw.c_str());
+    }
+}
+
 static void write_products(Context &c, const OutNames &n, int isub)
 {
-    if (n.map.empty() && n.binmap.empty() && n.hit.empty() && n.mask.empty() && n.matrix.empty() && n.wcov.empty())
+    if (n.map.empty() && n.binmap.empty() && n.hit.empty() && n.mask.empty() && n.matrix.empty() && n.wcov.empty() &&
+        n.leak.empty())
         return;
     HostProducts h;
     fetch_products(c, h);
     write_product_set(c, n, isub, h);
+    write_leakmatrices(c, n.leak, nullptr, std::vector<char>(c.ndet, 1), isub, h);
 }
 
 // run_subsets (src/smadam.F90:954-1257): binned (and, with write_map, destriped)
@@ -2001,13 +2064,16 @@ static void run_subsets(Context &c)
                 };
                 OutNames on;
                 on.binmap = (pass == npass && do_map) ? nmf(true, "_map.fits") : nmf(do_binmap, "_bmap.fits");
+                std::string leak;
                 if (pass == 1) {
+                    leak = p.write_leakmatrix ? add_subchunk_id(sub + "_leakmatrix.fits", c.pass, p.nsubchunk) : "";
                     on.hit = nmf(p.write_hits, "_hmap.fits");
                     on.matrix = nmf(p.write_matrix, "_wcov_inv.fits");
                     on.wcov = nmf(p.write_wcov, "_wcov.fits");
                     on.mask = nmf(p.write_mask, "_mask.fits");
                 }
-                if (on.binmap.empty() && on.hit.empty() && on.matrix.empty() && on.wcov.empty() && on.mask.empty())
+                if (on.binmap.empty() && on.hit.empty() && on.matrix.empty() && on.wcov.empty() && on.mask.empty() &&
+                    leak.empty())
                     continue;
                 for (int d = 0; d < c.ndet; ++d)
                     for (int64_t i = 0; i < c.nsamp; ++i) hm[(size_t)d * c.nsamp + i] = dflag[d] && sflag[i];
@@ -2049,6 +2115,7 @@ static void run_subsets(Context &c)
                 for (int64_t q = 0; q < np; ++q)
                     for (int a = 0; a < nm; ++a) h.binmap[q * nm + a] = m4[q * 4 + a];
                 write_product_set(c, on, c.pass, h);
+                write_leakmatrices(c, leak, isv > 0 ? &sflag : nullptr, dflag, c.pass, h);
             }
         }
     }

@@ -93,7 +93,8 @@ def test_products_match_in_memory_results(tmp_path):
     from tests.test_gpu_parity import ring
     case = ring(prior=True, nside=8, nsamp=20000, extra=dict(write_binmap=True, write_hits=True, write_mask=True,
                                                               write_wcov=True, write_matrix=True,
-                                                              nside_submap=2, unit_tod="K_CMB"))
+                                                              write_leakmatrix=True, nside_submap=2,
+                                                              unit_tod="K_CMB"))
     case["pixels"][300:400] = -1
     ref = libmadam_amd.run(*mc.call_args(mc.copy_case(case)))
     npix = 12 * 8 ** 2
@@ -130,3 +131,21 @@ def test_products_match_in_memory_results(tmp_path):
             assert hdr["INDXSCHM"] == "EXPLICIT" and hdr["EXTNAME"] == "CUT SKY MAP"
         _, _, icols = fits_io.read_bintable(str(d / "run_wcov_inv.fits"))
         assert "cc(2,3)" in icols
+        # leakage matrices gamma_d = C^-1 C_d: the detectors' C_d sum to C, so
+        # sum_d gamma_d is the identity on every pixel the mask keeps
+        leak = sorted(d.glob("run_leakmatrix_*.fits"))
+        assert len(leak) == len(case["dets"])
+        tot = None
+        for f in leak:
+            hdr_l, _, lcols = fits_io.read_bintable(str(f))
+            assert hdr_l["objtype"] == "madam.leakmatrix" and "gamma(3,1)" in lcols
+            g = np.stack([lcols["gamma(%d,%d)" % (i, j)] for i in (1, 2, 3) for j in (1, 2, 3)], -1)
+            tot = g if tot is None else tot + g
+        ng = tot.shape[0]
+        if not cut:
+            ok = good
+            tot_ok = tot[ok]
+        else:
+            tot_ok = tot
+        np.testing.assert_allclose(tot_ok, np.tile(np.eye(3).ravel(), (ng if cut else int(good.sum()), 1)),
+                                   atol=1e-9)
