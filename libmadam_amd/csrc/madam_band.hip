@@ -46,11 +46,35 @@ struct ChunkInfo {
 // Register-window variant: one wave per segment, lane a holds row j + a of the
 // active (K x K) window, W[b] = A(j + a, j + b).  Per column: broadcast the
 // pivot, scale the column, rank-1 update, shift the window up by one row.
+// broadcast of lane `src` (wave-uniform) through the scalar unit
+__device__ inline double readlane_d(double v, int src)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_readlane(lo, src);
+    hi = __builtin_amdgcn_readlane(hi, src);
+    return __hiloint2double(hi, lo);
+}
+
+// lane l takes lane l + 1's value (row_shl:1 inside 16-lane rows; lane 15 of a row gets 0)
+__device__ inline double shift_down_row(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x101, 0xf, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x101, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Register-window variant: one wave per segment, lane a holds row j + a of the
+// active (K x K) window, W[b] = A(j + a, j + b).  Per column: broadcast the
+// pivot, scale the column, rank-1 update, shift the window up by one row.  The
+// diagonal terms nna of the rows entering the window are loaded 64 rows ahead.
 template <int KD>
-__global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_t nseg, const int *nband_seg, const double *invcov,
-                                int64_t nof, const double *nna, double *Lb, int *info)
+__global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_t nseg, const int *nband_seg,
+                                                      const double *invcov, int64_t nof, const double *nna,
+                                                      double *Lb, int *info)
 {
     constexpr int K = KD + 1;
+    constexpr bool DPP_SHIFT = K <= 15;   // the window fits one 16-lane DPP row
     int lane = threadIdx.x & 63;
     int64_t s = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     if (s >= nseg) return;
@@ -76,36 +100,48 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
         }
         W[bb] = v;
     }
+    // nna of rows base .. base + 63 in lane order, and the next 64
+    int64_t base = K;
+    double nbuf = base + lane < n ? nn[base + lane] : 0.0;
+    double nnext = base + 64 + lane < n ? nn[base + 64 + lane] : 0.0;
     for (int64_t j = 0; j < n; ++j) {
-        double ajj = __shfl(W[0], 0, 64);
+        double ajj = readlane_d(W[0], 0);
         if (!(ajj > 0)) {
             if (lane == 0) info[s] = (int)(j + 1);
             return;
         }
         double d = sqrt(ajj);
-        double l = (lane >= 1 && lane < K && j + lane < n) ? W[0] / d : 0.0;
-        if (lane == 0) Lb[(sg.b0 + j) * K] = 1.0 / d;
+        double dinv = 1.0 / d;
+        double l = (lane >= 1 && lane < K && j + lane < n) ? W[0] * dinv : 0.0;
+        if (lane == 0) Lb[(sg.b0 + j) * K] = dinv;
         else if (lane < K) Lb[(sg.b0 + j) * K + lane] = l;
         // rank-1 update of rows 1..KD: W[bb] -= l_a l_bb for 1 <= bb <= a
 #pragma unroll
         for (int bb = 1; bb < K; ++bb) {
-            double lb = __shfl(l, bb, 64);
+            double lb = readlane_d(l, bb);
             if (bb <= lane) W[bb] -= l * lb;
         }
+        // the row entering the window: i = j + K
+        int64_t i = j + K;
+        if (i >= base + 64) {   // uniform over the wave
+            base += 64;
+            nbuf = nnext;
+            nnext = base + 64 + lane < n ? nn[base + 64 + lane] : 0.0;
+        }
+        double nni = __shfl(nbuf, (int)(i - base), 64);
         // shift: new row a = old row a + 1, columns shifted left by one
         double Wn[K];
 #pragma unroll
-        for (int bb = 0; bb < K - 1; ++bb) Wn[bb] = __shfl(W[bb + 1], lane + 1, 64);
+        for (int bb = 0; bb < K - 1; ++bb) Wn[bb] = DPP_SHIFT ? shift_down_row(W[bb + 1]) : __shfl(W[bb + 1], lane + 1, 64);
         Wn[K - 1] = 0.0;
-        // new last row: i = j + K, entries A(i, j + 1 + bb) = ic[K - 1 - bb] (+ nna on the diagonal)
+        // new last row: entries A(i, j + 1 + bb) = ic[K - 1 - bb] (+ nna on the diagonal)
         if (lane == K - 1) {
-            int64_t i = j + K;
 #pragma unroll
             for (int bb = 0; bb < K; ++bb) {
                 double v = 0.0;
                 if (i < n) {
                     v = icv[K - 1 - bb];
-                    if (bb == K - 1) v += nn[i];
+                    if (bb == K - 1) v += nni;
                 }
                 Wn[bb] = v;
             }
