@@ -55,6 +55,18 @@ __device__ inline double readlane_d(double v, int src)
     return __hiloint2double(hi, lo);
 }
 
+// 1 / sqrt(a) for a > 0: the hardware estimate refined by two Newton steps
+// y <- y (3 - a y^2) / 2 (relative error of the estimate ~2^-23, so the result is
+// within about an ulp); shorter than a correctly rounded sqrt followed by a divide
+__device__ inline double rsqrt_nr(double a)
+{
+    double y = __builtin_amdgcn_rsq(a);
+    double h = 0.5 * a;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
 // lane l takes lane l + 1's value (row_shl:1 inside 16-lane rows; lane 15 of a row gets 0)
 __device__ inline double shift_down_row(double v)
 {
@@ -110,8 +122,7 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
             if (lane == 0) info[s] = (int)(j + 1);
             return;
         }
-        double d = sqrt(ajj);
-        double dinv = 1.0 / d;
+        double dinv = rsqrt_nr(ajj);
         double l = (lane >= 1 && lane < K && j + lane < n) ? W[0] * dinv : 0.0;
         if (lane == 0) Lb[(sg.b0 + j) * K] = dinv;
         else if (lane < K) Lb[(sg.b0 + j) * K + lane] = l;
@@ -128,7 +139,7 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
             nbuf = nnext;
             nnext = base + 64 + lane < n ? nn[base + 64 + lane] : 0.0;
         }
-        double nni = __shfl(nbuf, (int)(i - base), 64);
+        double nni = readlane_d(nbuf, (int)(i - base));
         // shift: new row a = old row a + 1, columns shifted left by one
         double Wn[K];
 #pragma unroll
