@@ -112,53 +112,55 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
         }
         W[bb] = v;
     }
-    // nna of rows base .. base + 63 in lane order, and the next 64
-    int64_t base = K;
-    double nbuf = base + lane < n ? nn[base + lane] : 0.0;
-    double nnext = base + 64 + lane < n ? nn[base + 64 + lane] : 0.0;
-    for (int64_t j = 0; j < n; ++j) {
-        double ajj = readlane_d(W[0], 0);
-        if (!(ajj > 0)) {
-            if (lane == 0) info[s] = (int)(j + 1);
-            return;
-        }
-        double dinv = rsqrt_nr(ajj);
-        double l = (lane >= 1 && lane < K && j + lane < n) ? W[0] * dinv : 0.0;
-        if (lane == 0) Lb[(sg.b0 + j) * K] = dinv;
-        else if (lane < K) Lb[(sg.b0 + j) * K + lane] = l;
-        // rank-1 update of rows 1..KD: W[bb] -= l_a l_bb for 1 <= bb <= a
-#pragma unroll
-        for (int bb = 1; bb < K; ++bb) {
-            double lb = readlane_d(l, bb);
-            if (bb <= lane) W[bb] -= l * lb;
-        }
-        // the row entering the window: i = j + K
-        int64_t i = j + K;
-        if (i >= base + 64) {   // uniform over the wave
-            base += 64;
-            nbuf = nnext;
-            nnext = base + 64 + lane < n ? nn[base + 64 + lane] : 0.0;
-        }
-        double nni = readlane_d(nbuf, (int)(i - base));
-        // shift: new row a = old row a + 1, columns shifted left by one
-        double Wn[K];
-#pragma unroll
-        for (int bb = 0; bb < K - 1; ++bb) Wn[bb] = DPP_SHIFT ? shift_down_row(W[bb + 1]) : __shfl(W[bb + 1], lane + 1, 64);
-        Wn[K - 1] = 0.0;
-        // new last row: entries A(i, j + 1 + bb) = ic[K - 1 - bb] (+ nna on the diagonal)
-        if (lane == K - 1) {
-#pragma unroll
-            for (int bb = 0; bb < K; ++bb) {
-                double v = 0.0;
-                if (i < n) {
-                    v = icv[K - 1 - bb];
-                    if (bb == K - 1) v += nni;
-                }
-                Wn[bb] = v;
+    // Columns go in blocks of 64: block j0 needs the nna of rows j0 + K .. j0 + K + 63
+    // (lane t holds row j0 + K + t); the next block's values are loaded one block
+    // ahead, so the only memory wait is once per block.
+    double nnext = K + lane < n ? nn[K + lane] : 0.0;
+    for (int64_t j0 = 0; j0 < n; j0 += 64) {
+        double nbuf = nnext;
+        nnext = j0 + 64 + K + lane < n ? nn[j0 + 64 + K + lane] : 0.0;
+        int tn = n - j0 < 64 ? (int)(n - j0) : 64;
+        for (int t = 0; t < tn; ++t) {
+            int64_t j = j0 + t;
+            double ajj = readlane_d(W[0], 0);
+            if (!(ajj > 0)) {
+                if (lane == 0) info[s] = (int)(j + 1);
+                return;
             }
-        }
+            double dinv = rsqrt_nr(ajj);
+            double l = (lane >= 1 && lane < K && j + lane < n) ? W[0] * dinv : 0.0;
+            if (lane == 0) Lb[(sg.b0 + j) * K] = dinv;
+            else if (lane < K) Lb[(sg.b0 + j) * K + lane] = l;
+            // rank-1 update of rows 1..KD: W[bb] -= l_a l_bb for 1 <= bb <= a
 #pragma unroll
-        for (int bb = 0; bb < K; ++bb) W[bb] = Wn[bb];
+            for (int bb = 1; bb < K; ++bb) {
+                double lb = readlane_d(l, bb);
+                if (bb <= lane) W[bb] -= l * lb;
+            }
+            // the row entering the window: i = j + K
+            int64_t i = j + K;
+            double nni = readlane_d(nbuf, t);
+            // shift: new row a = old row a + 1, columns shifted left by one
+            double Wn[K];
+#pragma unroll
+            for (int bb = 0; bb < K - 1; ++bb)
+                Wn[bb] = DPP_SHIFT ? shift_down_row(W[bb + 1]) : __shfl(W[bb + 1], lane + 1, 64);
+            Wn[K - 1] = 0.0;
+            // new last row: entries A(i, j + 1 + bb) = ic[K - 1 - bb] (+ nna on the diagonal)
+            if (lane == K - 1) {
+#pragma unroll
+                for (int bb = 0; bb < K; ++bb) {
+                    double v = 0.0;
+                    if (i < n) {
+                        v = icv[K - 1 - bb];
+                        if (bb == K - 1) v += nni;
+                    }
+                    Wn[bb] = v;
+                }
+            }
+#pragma unroll
+            for (int bb = 0; bb < K; ++bb) W[bb] = Wn[bb];
+        }
     }
 }
 
