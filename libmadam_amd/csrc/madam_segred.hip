@@ -173,6 +173,17 @@ __device__ inline void sr_ccmul_store(int nmap, const double *cc, const double *
 // bank conflicts.
 static constexpr int SR_PAD = SR_TILE + SR_TILE / 8;
 
+// Orders this wave's LDS accesses: the LDS serves one wave's requests in
+// issue order, so a wave that only exchanges data through its own LDS slice
+// needs no workgroup barrier, only a fence that keeps the compiler from
+// moving accesses across this point.
+__device__ inline void sr_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline int sr_col(int u, int lane) { return u * 72 + lane + (lane >> 3); }   // pad(64 u + lane)
 __device__ inline int sr_row(int u, int lane) { return lane * 9 + u; }                  // pad(8 lane + u)
 
@@ -193,14 +204,14 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * WAVES + w;
     int64_t t0 = j * SR_TILE;
-    bool active = t0 < nent;   // uniform over the wave; every wave takes part in the barriers
+    if (t0 >= nent) return;   // uniform over the wave; waves synchronise only with themselves
     CsrEnt e[SR_PER_LANE];
     double x[SR_PER_LANE];
     int64_t k0 = 0;
     bool head0 = false, cont_last = false;
     unsigned hb = 0u;
     int64_t e0 = t0 + lane * SR_PER_LANE;
-    if (active) {
+    {
         // tile metadata first: these loads are independent of the entry stream
         int64_t rem = nent - e0;
         int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
@@ -226,21 +237,17 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     double prod[SR_PER_LANE][NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        if (active) {
 #pragma unroll
-            for (int u = 0; u < SR_PER_LANE; ++u) {
-                float wc = NC == 1 ? 1.0f : (c == 0 ? e[u].w0 : (c == 1 ? e[u].w1 : e[u].w2));
-                tr[w][sr_col(u, lane)] = NC == 1 ? x[u] : (double)wc * x[u];
-            }
+        for (int u = 0; u < SR_PER_LANE; ++u) {
+            float wc = NC == 1 ? 1.0f : (c == 0 ? e[u].w0 : (c == 1 ? e[u].w1 : e[u].w2));
+            tr[w][sr_col(u, lane)] = NC == 1 ? x[u] : (double)wc * x[u];
         }
-        __syncthreads();
-        if (active) {
+        sr_wave_sync();
 #pragma unroll
-            for (int u = 0; u < SR_PER_LANE; ++u) prod[u][c] = tr[w][sr_row(u, lane)];
-        }
-        __syncthreads();
+        for (int u = 0; u < SR_PER_LANE; ++u) prod[u][c] = tr[w][sr_row(u, lane)];
+        sr_wave_sync();
     }
-    if (!active) return;
+    sr_wave_sync();   // the prefetched finishing data in LDS
     const int nmap = NC == 1 ? 1 : 3;
     sr_reduce_tile<NC>(
         prod, hb, lane, k0, head0, cont_last,
