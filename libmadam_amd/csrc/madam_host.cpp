@@ -392,6 +392,11 @@ struct Context {
     DBuf<int32_t> d_hits;
     DBuf<int64_t> d_off, d_scan_tmp;
     DBuf<CsrEnt> d_ent;
+    // locality order of the baselines (the CSR entries carry slot numbers):
+    // bord[slot] = baseline, bslot[baseline] = slot, and the baseline vectors
+    // the entries index, in slot order
+    DBuf<int32_t> d_bord, d_bslot;
+    DBuf<double> d_dwb_s, d_sdwb_s, d_aa_s;
     DBuf<uint32_t> d_src;
     int64_t nactive = 0;
     bool pix_host_dirty = false;
@@ -891,7 +896,11 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
     c.nactive = total;
     c.d_ent.alloc(total > 0 ? total : 1);
     c.d_src.alloc(total > 0 ? total : 1);
-    csr_build_sorted(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.npix_map, total, c.d_ent.p, c.d_src.p);
+    c.d_bord.alloc(std::max<int64_t>(c.nbtot, 1));
+    c.d_bslot.alloc(std::max<int64_t>(c.nbtot, 1));
+    baseline_slots(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.d_bord.p, c.d_bslot.p);
+    csr_build_sorted(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.npix_map, total, c.d_bslot.p,
+                     c.d_ent.p, c.d_src.p);
     MADAM_CHECK(hipGetLastError());
     MADAM_CHECK(hipStreamSynchronize(s));
     c.stats[ST_T_POINTING] = now() - t0;
@@ -1250,7 +1259,7 @@ static void pcg_init(Context &c)
     launch_copy(s, c.d_r.p, c.d_yba.p, nb);
     apply_precond(c, c.d_r.p, c.d_z.p);
     launch_copy(s, c.d_p.p, c.d_z.p, nb);
-    launch_mul(s, c.d_p.p, c.d_dwb.p, c.d_pw.p, nb);
+    launch_pw_slots(s, c.d_p.p, c.d_bord.p, c.d_dwb_s.p, c.d_pw.p, nb);
     launch_dots(s, 1, c.d_r.p, c.d_z.p, c.d_r.p, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
     launch_finalize(s, c.d_part.p, nblk, 3, c.d_scal.p + 2);
     double hs[8];
@@ -1328,7 +1337,10 @@ static int pcg_steps(Context &c, int nmax, bool converge)
             if (rr / rrinit < p.cglimit && c.pcg_steps_done > p.iter_min) last = true;
             if (rz == 0) last = true;
         }
-        if (!last) launch_pcg_update_p(s, c.d_scal.p, c.d_p.p, c.d_z.p, c.d_pw.p, c.d_dwb.p, nb);
+        if (!last) {
+            launch_pcg_update_p(s, c.d_scal.p, c.d_p.p, c.d_z.p, nb);
+            launch_pw_slots(s, c.d_p.p, c.d_bord.p, c.d_dwb_s.p, c.d_pw.p, nb);
+        }
         if (prof) {
             t_it.stop(s);
             c.stats[ST_T_SCATTER] += t_sc.ms() * 1e-3;
@@ -1368,7 +1380,11 @@ static void solve(Context &c, double *signal, int on_device)
     tm.start(s);
     c.d_cc6.alloc(c.npix_map * 6);
     c.d_rhs4.alloc(c.npix_map * 4);
-    launch_bin(s, c.nmap, c.npix_map, c.d_off.p, c.d_ent.p, c.d_src.p, c.d_tod, c.d_dwb.p, c.d_sdwb.p, c.d_cc6.p,
+    c.d_dwb_s.alloc(std::max<int64_t>(c.nbtot, 1));
+    c.d_sdwb_s.alloc(std::max<int64_t>(c.nbtot, 1));
+    launch_gather_order(s, c.d_dwb.p, c.d_bord.p, c.d_dwb_s.p, c.nbtot);
+    launch_gather_order(s, c.d_sdwb.p, c.d_bord.p, c.d_sdwb_s.p, c.nbtot);
+    launch_bin(s, c.nmap, c.npix_map, c.d_off.p, c.d_ent.p, c.d_src.p, c.d_tod, c.d_dwb_s.p, c.d_sdwb_s.p, c.d_cc6.p,
                c.d_rhs4.p);
     double *cc6x = c.d_cc6.p, *rhs4x = c.d_rhs4.p;
     if (c.ndeg > 1) {
@@ -1432,8 +1448,11 @@ static void solve(Context &c, double *signal, int on_device)
     tm.start(s);
     c.d_rhsd4.alloc(c.npix_map * 4);
     MADAM_CHECK(hipMemcpyAsync(c.d_rhsd4.p, c.d_rhs4.p, sizeof(double) * c.npix_map * 4, hipMemcpyDeviceToDevice, s));
-    if (p.kfirst)
-        launch_subtract_baselines(s, c.nmap, c.npix_map, c.d_off.p, c.d_ent.p, c.d_aa.p, c.d_dwb.p, c.d_rhsd4.p);
+    if (p.kfirst) {
+        c.d_aa_s.alloc(std::max<int64_t>(c.nbtot, 1));
+        launch_gather_order(s, c.d_aa.p, c.d_bord.p, c.d_aa_s.p, c.nbtot);
+        launch_subtract_baselines(s, c.nmap, c.npix_map, c.d_off.p, c.d_ent.p, c.d_aa_s.p, c.d_dwb_s.p, c.d_rhsd4.p);
+    }
     c.d_ccinv6.alloc(c.npix_map * 6);
     c.d_mask.alloc(c.npix_map);
     c.d_crit.alloc(c.npix_map);
@@ -1938,7 +1957,7 @@ static void write_leakmatrices(Context &c, const std::string &base, const std::v
             for (int64_t i = 0; i < c.nsamp; ++i)
                 hm[(size_t)dd * c.nsamp + i] = dd == d && (!sflag || (*sflag)[i]);
         h2d(smask.p, hm.data(), n, s);
-        launch_bin_subset(s, nm, np, c.d_off.p, c.d_ent.p, c.d_src.p, smask.p, c.d_tod, c.d_dwb.p, c.d_sdwb.p,
+        launch_bin_subset(s, nm, np, c.d_off.p, c.d_ent.p, c.d_src.p, smask.p, c.d_tod, c.d_dwb_s.p, c.d_sdwb_s.p,
                           cc6.p, rhs4.p, hits.p);
         MADAM_CHECK(hipGetLastError());
         d2h(cd.data(), cc6.p, np * 6, s);
@@ -2078,7 +2097,7 @@ static void run_subsets(Context &c)
                 for (int d = 0; d < c.ndet; ++d)
                     for (int64_t i = 0; i < c.nsamp; ++i) hm[(size_t)d * c.nsamp + i] = dflag[d] && sflag[i];
                 h2d(smask.p, hm.data(), n, s);
-                launch_bin_subset(s, nm, np, c.d_off.p, c.d_ent.p, c.d_src.p, smask.p, tod, c.d_dwb.p, c.d_sdwb.p,
+                launch_bin_subset(s, nm, np, c.d_off.p, c.d_ent.p, c.d_src.p, smask.p, tod, c.d_dwb_s.p, c.d_sdwb_s.p,
                                   cc6.p, rhs4.p, hits.p);
                 MADAM_CHECK(hipMemsetAsync(crit.p, 0, sizeof(double) * np, s));
                 launch_invert(s, nm, np, cc6.p, ci6.p, crit.p, mask.p, p.pixmode_map, p.pixlim_map,

@@ -119,8 +119,16 @@ void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, 
                        const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
                        const double *dwb, const double *nna, const double *p, double *ap, double4 *part);
 // stable regrouping of the samples by map pixel (off from the hit histogram)
+// bslot (may be null): slot number of each baseline, stored in the entries
 void csr_build_sorted(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
-                      int64_t npix_map, int64_t total, CsrEnt *ent, uint32_t *src);
+                      int64_t npix_map, int64_t total, const int32_t *bslot, CsrEnt *ent, uint32_t *src);
+// locality order of the baselines: bord[slot] = baseline, bslot[baseline] = slot
+void baseline_slots(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
+                    int32_t *bord, int32_t *bslot);
+// out[i] = in[ord[i]]
+void launch_gather_order(hipStream_t s, const double *in, const int32_t *ord, double *out, int64_t n);
+// pw[i] = p[ord[i]] * dwb_s[i]: the pixel-side input in slot order
+void launch_pw_slots(hipStream_t s, const double *p, const int32_t *ord, const double *dwb_s, double *pw, int64_t n);
 void launch_subtract_baselines(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent,
                                const double *aa, const double *dwb, double *rhs4);
 void launch_clean_tod(hipStream_t s, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
@@ -143,9 +151,7 @@ void launch_finalize(hipStream_t s, const double *partials, int nblocks, int nva
 // PCG vector steps, scalars read from device memory
 void launch_pcg_update_r(hipStream_t s, const double *scal, double *aa, double *r, double *ro, const double *p,
                          const double *ap, int64_t n);
-void launch_pcg_update_p(hipStream_t s, const double *scal, double *p, const double *z, double *pw,
-                         const double *dwb, int64_t n);
-void launch_mul(hipStream_t s, const double *a, const double *b, double *out, int64_t n);
+void launch_pcg_update_p(hipStream_t s, const double *scal, double *p, const double *z, int64_t n);
 void launch_axpby_nna(hipStream_t s, double *ap, const double *nna, const double *p, double factor, int64_t n);
 
 void launch_minmax(hipStream_t s, const int32_t *pix, int64_t n, int *mn, int *mx);

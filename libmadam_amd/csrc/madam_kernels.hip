@@ -1319,16 +1319,6 @@ void launch_prec_diag_build(hipStream_t s, const double *nna, const double *wb_i
     k_prec_diag<<<grid_for(n, TPB), TPB, 0, s>>>(nna, wb_inv, haspsd, pd, n);
 }
 
-__global__ void k_mul(const double *a, const double *b, double *o, int64_t n)
-{
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        o[i] = a[i] * b[i];
-}
-void launch_mul(hipStream_t s, const double *a, const double *b, double *out, int64_t n)
-{
-    k_mul<<<grid_for(n, TPB), TPB, 0, s>>>(a, b, out, n);
-}
-
 __global__ void k_nna_term(double *ap, const double *nna, const double *p, double f, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1431,20 +1421,38 @@ void launch_pcg_update_r(hipStream_t s, const double *scal, double *aa, double *
     k_update_r<<<grid_for(n, TPB), TPB, 0, s>>>(scal, aa, r, ro, p, ap, n);
 }
 
-__global__ void k_update_p(const double *scal, double *p, const double *z, double *pw, const double *dwb, int64_t n)
+__global__ void k_update_p(const double *scal, double *p, const double *z, int64_t n)
 {
     double beta = scal[5];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double pi = z[i] + beta * p[i];
-        p[i] = pi;
-        pw[i] = pi * dwb[i];
-    }
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
 }
 
-void launch_pcg_update_p(hipStream_t s, const double *scal, double *p, const double *z, double *pw, const double *dwb,
-                         int64_t n)
+void launch_pcg_update_p(hipStream_t s, const double *scal, double *p, const double *z, int64_t n)
 {
-    k_update_p<<<grid_for(n, TPB), TPB, 0, s>>>(scal, p, z, pw, dwb, n);
+    k_update_p<<<grid_for(n, TPB), TPB, 0, s>>>(scal, p, z, n);
+}
+
+__global__ void k_gather_order(const double *in, const int32_t *ord, double *out, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[ord[i]];
+}
+
+void launch_gather_order(hipStream_t s, const double *in, const int32_t *ord, double *out, int64_t n)
+{
+    if (n > 0) k_gather_order<<<grid_for(n, TPB), TPB, 0, s>>>(in, ord, out, n);
+}
+
+__global__ void k_pw_slots(const double *p, const int32_t *ord, const double *dwb_s, double *pw, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        pw[i] = p[ord[i]] * dwb_s[i];
+}
+
+void launch_pw_slots(hipStream_t s, const double *p, const int32_t *ord, const double *dwb_s, double *pw, int64_t n)
+{
+    if (n > 0) k_pw_slots<<<grid_for(n, TPB), TPB, 0, s>>>(p, ord, dwb_s, pw, n);
 }
 
 }  // namespace madam

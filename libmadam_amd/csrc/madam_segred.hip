@@ -1,4 +1,4 @@
-// Segmented reductions of a sorted entry stream, one wave per 512-entry tile:
+// Segmented reductions of a sorted entry stream, one wave per 256-entry tile:
 // the two per-iteration pointing projections of the PCG.
 //
 //   pixel side    m_q = ccinv_q . sum_{j in pixel q} w_j p[b_j]
@@ -10,7 +10,7 @@
 // Entries are consumed in stream order; a segment (a cross pixel's CSR range,
 // or a baseline's sample range) starts where the head bitmask has a bit.  The
 // wave loads its tile coalesced and transposes the per-entry products through
-// LDS so that lane l holds entries t0 + 8 l .. t0 + 8 l + 7: it sums its runs
+// LDS so that lane l holds entries t0 + 4 l .. t0 + 4 l + 3: it sums its runs
 // sequentially, finishes segments that begin and end inside it, and one
 // segmented inclusive scan over the 64 lanes joins runs that cross lanes.  A
 // segment that crosses a tile edge leaves a partial sum per tile (slot 1 in
@@ -26,7 +26,7 @@
 
 namespace madam {
 
-static constexpr int SR_PER_LANE = 8;
+static constexpr int SR_PER_LANE = 4;
 static constexpr int SR_TILE = 64 * SR_PER_LANE;
 static constexpr int SR_TPB = 256;
 static constexpr int SR_WAVES = SR_TPB / 64;
@@ -73,9 +73,10 @@ __device__ inline void sr_seg_scan(double (&v)[NC], bool head, int lane)
 // end), hb the head bits of its 8 entries, k0 the ordinal of the segment that
 // holds the tile's first entry, cont_last whether the segment that starts last
 // in the tile continues into the next tile.  fin(ordinal, sum) finishes a
-// segment; part(slot, sum) stores a partial sum of this tile.
+// segment; part(slot, sum) stores a partial sum of this tile.  Returns the
+// number of segment heads in the tile.
 template <int NC, typename Fin, typename Part>
-__device__ inline void sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsigned hb, int lane, int64_t k0,
+__device__ inline int sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsigned hb, int lane, int64_t k0,
                                       bool head0, bool cont_last, Fin fin, Part part)
 {
     int nh = __popc(hb);
@@ -137,6 +138,7 @@ __device__ inline void sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], uns
         else if (cont_last) part(1, v);
         else fin(kb + Htot, v);
     }
+    return Htot;
 }
 
 __device__ inline void sr_part_store(double4 *part, int64_t j, int slot, const double *v, int nc)
@@ -169,9 +171,9 @@ __device__ inline void sr_ccmul_store(int nmap, const double *cc, const double *
 
 // Entries are loaded coalesced (lane l takes t0 + 64 u + l) and their products
 // are transposed through a per-wave LDS buffer to the lane-contiguous layout
-// of the reduction; the pad of one double per 8 keeps both directions free of
-// bank conflicts.
-static constexpr int SR_PAD = SR_TILE + SR_TILE / 8;
+// of the reduction; the pad of one double per 16 keeps both directions free of
+// bank conflicts (16 lanes of a 64-bit access cover the 32 banks once).
+static constexpr int SR_PAD = SR_TILE + SR_TILE / 16;
 
 // Orders this wave's LDS accesses: the LDS serves one wave's requests in
 // issue order, so a wave that only exchanges data through its own LDS slice
@@ -184,23 +186,37 @@ __device__ inline void sr_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ inline int sr_col(int u, int lane) { return u * 72 + lane + (lane >> 3); }   // pad(64 u + lane)
-__device__ inline int sr_row(int u, int lane) { return lane * 9 + u; }                  // pad(8 lane + u)
+__device__ inline int sr_pad(int i) { return i + (i >> 4); }
+__device__ inline int sr_col(int u, int lane) { return sr_pad(u * 64 + lane); }
+__device__ inline int sr_row(int u, int lane) { return sr_pad(lane * SR_PER_LANE + u); }
 
-// The transpose goes one component at a time through a 4.6 KB per-wave
-// buffer (higher occupancy than one buffer per component), and the inverse
-// pixel matrices are read in compact ordinal order (cc_c[k] for the k-th
-// non-empty pixel), so finishing a pixel costs one round of independent loads.
+// head bits of the lane's SR_PER_LANE entries starting at e0 (a multiple of SR_PER_LANE)
+// (the head bytes are padded to whole tiles, so the load is always in bounds)
+__device__ inline unsigned sr_head_bits(const uint8_t *heads, int64_t e0, int64_t nent)
+{
+    int64_t rem = nent - e0;
+    int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
+    unsigned byte = heads[e0 >> 3];
+    return (byte >> (e0 & 7)) & ((1u << nv) - 1u);
+}
+
+// The transpose goes one component at a time through a 2.2 KB per-wave
+// buffer, the only LDS the kernel uses; with four entries per lane the kernel
+// fits in 72 registers, seven waves per SIMD.
+// The pixels finished in a tile have consecutive ordinals: the lane that
+// completes a sum deposits it in the transpose buffer at slot k - k0, and one
+// uniform pass then has lane l finish ordinal k0 + l, with the inverse matrix
+// (compact ordinal order) and the pixel index that lane l loaded before the
+// reduction.  Only a tile that finishes more than 64 pixels reads the
+// finishing data of the rest where they complete.
 template <int NC, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(5, 8))) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(7, 8))) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
                                                            const uint8_t *heads, const int64_t *tile_k0,
                                                            const int64_t *fix_end, const int32_t *nzq,
                                                            int64_t nzpix, const double *cc_c, double *m4,
                                                            double4 *part)
 {
     __shared__ double tr[WAVES][SR_PAD];
-    __shared__ double ccs[WAVES][64 * 6];   // inverse matrices of the tile's first 64 pixels
-    __shared__ int32_t qs[WAVES][64];       // and their pixel indices
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * WAVES + w;
     int64_t t0 = j * SR_TILE;
@@ -213,25 +229,22 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     int64_t e0 = t0 + lane * SR_PER_LANE;
     {
         // tile metadata first: these loads are independent of the entry stream
-        int64_t rem = nent - e0;
-        int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
-        hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
+        hb = sr_head_bits(heads, e0, nent);
         k0 = tile_k0[j];
         head0 = (heads[t0 / 8] & 1u) != 0;
         cont_last = fix_end[j] >= 0;
+        // unconditional loads from clamped indices, so that they all issue
+        // before the first wait; entries past the end are neutralised after
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
             int64_t idx = t0 + u * 64 + lane;
-            if (idx < nent) e[u] = ent[idx];
-            else { e[u].b = -1; e[u].w0 = e[u].w1 = e[u].w2 = 0.0f; }
+            e[u] = ent[idx < nent ? idx : nent - 1];
         }
 #pragma unroll
-        for (int u = 0; u < SR_PER_LANE; ++u) x[u] = e[u].b >= 0 ? pw[e[u].b] : 0.0;
-        // finishing data of the tile's pixels (ordinals k0 .. k0 + 63), with the gathers
-        if (k0 + lane < nzpix) {
-            qs[w][lane] = nzq[k0 + lane];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) ccs[w][lane * 6 + q] = cc_c[(k0 + lane) * 6 + q];
+        for (int u = 0; u < SR_PER_LANE; ++u) {
+            bool ok = t0 + u * 64 + lane < nent && e[u].b >= 0;
+            double v = pw[ok ? e[u].b : 0];
+            x[u] = ok ? v : 0.0;
         }
     }
     double prod[SR_PER_LANE][NC];
@@ -247,16 +260,55 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
         for (int u = 0; u < SR_PER_LANE; ++u) prod[u][c] = tr[w][sr_row(u, lane)];
         sr_wave_sync();
     }
-    sr_wave_sync();   // the prefetched finishing data in LDS
+    // heads in the tile, from one ballot per entry slot (scalar work)
+    int htot = 0;
+#pragma unroll
+    for (int u = 0; u < SR_PER_LANE; ++u) htot += __popcll(__ballot((hb >> u) & 1u));
+    // finished ordinals are kb + 1 .. kb + htot - cont_last with kb = k0 - head0,
+    // i.e. slots lo .. hi relative to k0
+    int lo = head0 ? 0 : 1;
+    int hi = htot - (cont_last ? 1 : 0) - (head0 ? 1 : 0);
     const int nmap = NC == 1 ? 1 : 3;
-    sr_reduce_tile<NC>(
-        prod, hb, lane, k0, head0, cont_last,
-        [&](int64_t k, const double *s) {
-            int64_t loc = k - k0;
-            if (loc < 64) sr_ccmul_store(nmap, &ccs[w][loc * 6], s, m4 + (int64_t)qs[w][loc] * 4);
-            else sr_ccmul_store(nmap, cc_c + k * 6, s, m4 + (int64_t)nzq[k] * 4);
-        },
-        [&](int slot, const double *s) { sr_part_store(part, j, slot, s, NC); });
+    auto part_fn = [&](int slot, const double *s) { sr_part_store(part, j, slot, s, NC); };
+    if (hi < 64) {
+        // the common case: every finished pixel has a slot in the transpose buffer
+        // only the lanes that finish a pixel load its inverse matrix (three
+        // 16-byte loads of 48 contiguous bytes); issued here, the loads
+        // overlap the reduction
+        bool fin_lane = lane >= lo && lane <= hi;
+        double2 cc[3];
+        int32_t q = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) cc[i] = make_double2(0.0, 0.0);
+        if (fin_lane) {
+            q = nzq[k0 + lane];
+            const double2 *src = (const double2 *)(cc_c + (k0 + lane) * 6);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) cc[i] = src[i];
+        }
+        sr_reduce_tile<NC>(
+            prod, hb, lane, k0, head0, cont_last,
+            [&](int64_t k, const double *s) {
+                int loc = (int)(k - k0);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) tr[w][loc * NC + c] = s[c];
+            },
+            part_fn);
+        sr_wave_sync();
+        if (fin_lane) {
+            double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < NC; ++c) s[c] = tr[w][lane * NC + c];
+            const double ccd[6] = {cc[0].x, cc[0].y, cc[1].x, cc[1].y, cc[2].x, cc[2].y};
+            sr_ccmul_store(nmap, ccd, s, m4 + (int64_t)q * 4);
+        }
+    } else {
+        // a tile of many short segments: finish each where it completes
+        sr_reduce_tile<NC>(
+            prod, hb, lane, k0, head0, cont_last,
+            [&](int64_t k, const double *s) { sr_ccmul_store(nmap, cc_c + k * 6, s, m4 + (int64_t)nzq[k] * 4); },
+            part_fn);
+    }
 }
 
 // compact copy of the inverse pixel matrices in non-empty-pixel order
@@ -330,9 +382,7 @@ __global__ void __launch_bounds__(SR_TPB) k_sr_gather(int64_t nent, const PixW *
     __syncthreads();
     if (!active) return;
     int64_t e0 = t0 + lane * SR_PER_LANE;
-    int64_t rem = nent - e0;
-    int nv = rem <= 0 ? 0 : (rem >= SR_PER_LANE ? SR_PER_LANE : (int)rem);
-    unsigned hb = nv > 0 ? (unsigned)heads[e0 / 8] & ((1u << nv) - 1u) : 0u;
+    unsigned hb = sr_head_bits(heads, e0, nent);
     double prod[SR_PER_LANE][1];
 #pragma unroll
     for (int u = 0; u < SR_PER_LANE; ++u) prod[u][0] = tr[w][sr_row(u, lane)];
