@@ -171,9 +171,36 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
 // Li[(((c / 64) * CH + i) * K + m) * 64 + c % 64] is L(b0_c + i + m, b0_c + i)
 // (m = 0 holds 1 / L(i, i)).  The lanes of a wave (consecutive chunks c) read
 // consecutive addresses, and a wave's whole sweep is one contiguous block.
-__device__ inline const double *li_chunk(const double *Li, int64_t c, int K)
+//
+// Where the band matrix is stationary (equal baseline lengths, no flags) the
+// factor's rows reach a bitwise fixed point or a short cycle (period 1, 2 or
+// 4, all dividing the chunk length) within a few hundred rows, so most chunks
+// repeat the previous one exactly.  bmap[w] names a block whose 64 chunks are
+// bitwise identical to those of block w (the first block of a run of equal
+// blocks), and cmap[c] the first chunk of c's run of equal chunks: the sweeps
+// and the carry chains read through them, so a run streams its factor and its
+// transfer matrices from cache once instead of from memory per chunk.  The
+// values read are the same bits, so the results do not change.
+__device__ inline const double *li_chunk(const double *Li, const int32_t *bmap, int64_t c, int K)
 {
-    return Li + (c >> 6) * (int64_t)BAND_CH * K * 64 + (c & 63);
+    int64_t w = bmap ? (int64_t)bmap[c >> 6] : (c >> 6);
+    return Li + w * (int64_t)BAND_CH * K * 64 + (c & 63);
+}
+
+// ceq[c] = 1 if chunk c holds the same row count and bitwise the same factor
+// rows as chunk c - 1
+__global__ void k_band_chunk_eq(const ChunkInfo *ch, int64_t nch, int K, const double *Li, uint8_t *ceq)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nch; c += (int64_t)gridDim.x * blockDim.x) {
+        bool eq = c > 0 && ch[c].n == ch[c - 1].n;
+        if (eq) {
+            const double *a = li_chunk(Li, nullptr, c, K), *b = li_chunk(Li, nullptr, c - 1, K);
+            int n = ch[c].n;
+            for (int64_t t = 0; t < (int64_t)n * K && eq; ++t)
+                eq = __double_as_longlong(a[t * 64]) == __double_as_longlong(b[t * 64]);
+        }
+        ceq[c] = eq ? 1 : 0;
+    }
 }
 
 __global__ void k_band_relayout(const ChunkInfo *ch, int64_t nch, int64_t NCP, int K, int CH, const double *Lb,
@@ -206,8 +233,8 @@ struct BandPF {
 // forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
 // next KD rows (acc[0] for the first row of the chunk).  If store, writes y.
 template <int KD>
-__device__ inline void fwd_sweep(const double *Li, int64_t NCP, int64_t c, int64_t b0, int n, const double *r,
-                                 double *acc, double *y, bool store, bool zero_rhs)
+__device__ inline void fwd_sweep(const double *Li, const int32_t *bmap, int64_t c, int64_t b0, int n,
+                                 const double *r, double *acc, double *y, bool store, bool zero_rhs)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
     for (int i0 = 0; i0 < n; i0 += BAND_PF) {
@@ -216,7 +243,7 @@ __device__ inline void fwd_sweep(const double *Li, int64_t NCP, int64_t c, int64
         for (int u = 0; u < BAND_PF; ++u) {
             int i = i0 + u;
             if (i < n) {
-                const double *row = li_chunk(Li, c, K) + (int64_t)i * K * 64;
+                const double *row = li_chunk(Li, bmap, c, K) + (int64_t)i * K * 64;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
                 rh[u] = zero_rhs ? 0.0 : r[b0 + i];
@@ -238,8 +265,8 @@ __device__ inline void fwd_sweep(const double *Li, int64_t NCP, int64_t c, int64
 
 // backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
 template <int KD>
-__device__ inline void bwd_sweep(const double *Li, int64_t NCP, int64_t c, int64_t b0, int n, const double *y,
-                                 double *xs, double *x, bool store, bool zero_rhs)
+__device__ inline void bwd_sweep(const double *Li, const int32_t *bmap, int64_t c, int64_t b0, int n,
+                                 const double *y, double *xs, double *x, bool store, bool zero_rhs)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
     for (int i0 = n - 1; i0 >= 0; i0 -= BAND_PF) {
@@ -248,7 +275,7 @@ __device__ inline void bwd_sweep(const double *Li, int64_t NCP, int64_t c, int64
         for (int u = 0; u < BAND_PF; ++u) {
             int i = i0 - u;
             if (i >= 0) {
-                const double *row = li_chunk(Li, c, K) + (int64_t)i * K * 64;
+                const double *row = li_chunk(Li, bmap, c, K) + (int64_t)i * K * 64;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
                 rh[u] = zero_rhs ? 0.0 : y[b0 + i];
@@ -284,19 +311,19 @@ __global__ void __launch_bounds__(64) k_band_response(const ChunkInfo *ch, int64
     double st[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    fwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, nullptr, st, nullptr, false, true);
+    fwd_sweep<KD>(Lb, nullptr, c, ci.b0, ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tf[(c * KD + m) * KD + j] = st[m];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    bwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, nullptr, st, nullptr, false, true);
+    bwd_sweep<KD>(Lb, nullptr, c, ci.b0, ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tb[(c * KD + m) * KD + j] = st[m];
 }
 
 template <int KD>
-__global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *r,
-                            double *s0)
+__global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
+                                                  const double *Lb, const double *r, double *s0)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
@@ -304,7 +331,7 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    fwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, r, acc, nullptr, false, false);
+    fwd_sweep<KD>(Lb, bmap, c, ci.b0, ci.n, r, acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
 }
@@ -318,8 +345,8 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
 // group's true incoming state Sg[g] (writing every chunk's incoming state sin).
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, const int32_t *gn, int64_t ngrp,
-                                                         const double *s0, const double *T, int dir,
-                                                         const double *Sg, double *sin, double *cg)
+                                                         const double *s0, const double *T, const int32_t *cmap,
+                                                         int dir, const double *Sg, double *sin, double *cg)
 {
     int lane = threadIdx.x & 63;
     int64_t g = blockIdx.x;
@@ -336,9 +363,10 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
             int t = t0 + u;
             if (t < n) {
                 int64_t c = dir == 0 ? c0 + t : c0 + n - 1 - t;
+                int64_t ct = cmap ? (int64_t)cmap[c] : c;
                 tv[u] = on ? s0[c * KD + lane] : 0.0;
 #pragma unroll
-                for (int q = 0; q < KD; ++q) tr[u][q] = on ? T[(c * KD + lane) * KD + q] : 0.0;
+                for (int q = 0; q < KD; ++q) tr[u][q] = on ? T[(ct * KD + lane) * KD + q] : 0.0;
             }
         }
 #pragma unroll
@@ -436,8 +464,9 @@ __global__ void __launch_bounds__(64) k_band_group_mats(const int64_t *gfirst, c
 // final forward sweep with true carries writes y into z, then the zero-carry
 // backward sweep of the same chunk gives its backward partial state
 template <int KD>
-__global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *r,
-                                 const double *sinf, double *z, double *s0b)
+__global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
+                                                       const double *Lb, const double *r, const double *sinf,
+                                                       double *z, double *s0b)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
@@ -445,17 +474,17 @@ __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int6
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
-    fwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, r, acc, z, true, false);
+    fwd_sweep<KD>(Lb, bmap, c, ci.b0, ci.n, r, acc, z, true, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    bwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, z, acc, nullptr, false, false);
+    bwd_sweep<KD>(Lb, bmap, c, ci.b0, ci.n, z, acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
 }
 
 template <int KD>
-__global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t nch, int64_t NCP, const double *Lb, const double *sinb,
-                            double *z)
+__global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
+                                                  const double *Lb, const double *sinb, double *z)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
@@ -463,7 +492,7 @@ __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t n
     double xs[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
-    bwd_sweep<KD>(Lb, NCP, c, ci.b0, ci.n, z, xs, z, true, false);
+    bwd_sweep<KD>(Lb, bmap, c, ci.b0, ci.n, z, xs, z, true, false);
 }
 
 // ---------------------------------------------------------------- host side
@@ -480,6 +509,9 @@ struct BandImpl {
     int64_t *gfirst = nullptr, *sgfirst = nullptr;
     int32_t *gn = nullptr;
     double *Gf = nullptr, *Gb = nullptr, *cg = nullptr, *Sg = nullptr;
+    // runs of bitwise-equal chunks and blocks of the factor (see li_chunk)
+    int32_t *cmap = nullptr, *bmap = nullptr;
+    int64_t nblk_unique = 0, nchunk_unique = 0;
 };
 
 static int kd_class(int kd)
@@ -613,6 +645,32 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
             }));
             MADAM_CHECK(hipGetLastError());
         }
+        // runs of equal chunks and blocks
+        uint8_t *dceq = nullptr;
+        MADAM_CHECK(hipMalloc(&dceq, bp.nchunk));
+        k_band_chunk_eq<<<(unsigned)std::min<int64_t>((bp.nchunk + 255) / 256, 65535), 256, 0, s>>>(
+            im->ch, bp.nchunk, K, im->Li, dceq);
+        MADAM_CHECK(hipGetLastError());
+        std::vector<uint8_t> ceq(bp.nchunk);
+        MADAM_CHECK(hipMemcpyAsync(ceq.data(), dceq, bp.nchunk, hipMemcpyDeviceToHost, s));
+        MADAM_CHECK(hipStreamSynchronize(s));
+        (void)hipFree(dceq);
+        std::vector<int32_t> cmap(bp.nchunk), bmap(im->NCP / 64);
+        for (int64_t c = 0; c < bp.nchunk; ++c) {
+            cmap[c] = ceq[c] ? cmap[c - 1] : (int32_t)c;
+            im->nchunk_unique += ceq[c] ? 0 : 1;
+        }
+        for (int64_t w = 0; w < (int64_t)bmap.size(); ++w) {
+            bool same = w > 0;
+            for (int64_t c = w * 64; c < std::min<int64_t>(w * 64 + 64, bp.nchunk) && same; ++c)
+                same = cmap[c] == cmap[c - 64];
+            bmap[w] = same ? bmap[w - 1] : (int32_t)w;
+            im->nblk_unique += same ? 0 : 1;
+        }
+        MADAM_CHECK(hipMalloc(&im->cmap, sizeof(int32_t) * cmap.size()));
+        MADAM_CHECK(hipMalloc(&im->bmap, sizeof(int32_t) * bmap.size()));
+        MADAM_CHECK(hipMemcpyAsync(im->cmap, cmap.data(), sizeof(int32_t) * cmap.size(), hipMemcpyHostToDevice, s));
+        MADAM_CHECK(hipMemcpyAsync(im->bmap, bmap.data(), sizeof(int32_t) * bmap.size(), hipMemcpyHostToDevice, s));
     }
     bp.chunk_b0 = (int64_t *)im;   // opaque implementation pointer
     MADAM_CHECK(hipStreamSynchronize(s));
@@ -628,19 +686,19 @@ void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
     unsigned gs = (unsigned)im->nseg_used;   // one 64-lane wave per segment
     unsigned gg = (unsigned)im->ngrp;        // one wave per chunk group
     BAND_DISPATCH(KD, ({
-        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, r, im->s0f);
-        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, 0, nullptr, nullptr,
-                                                 im->cg);
+        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, r, im->s0f);
+        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, nullptr,
+                                                 nullptr, im->cg);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, 0, im->Sg);
-        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, 0, im->Sg, im->sinf,
-                                                 nullptr);
-        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, r, im->sinf, z, im->s0b);
-        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, 1, nullptr, nullptr,
-                                                 im->cg);
+        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, im->Sg,
+                                                 im->sinf, nullptr);
+        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, r, im->sinf, z, im->s0b);
+        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, nullptr,
+                                                 nullptr, im->cg);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, 1, im->Sg);
-        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, 1, im->Sg, im->sinb,
-                                                 nullptr);
-        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->NCP, im->Li, im->sinb, z);
+        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, im->Sg,
+                                                 im->sinb, nullptr);
+        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->sinb, z);
     }));
 }
 
@@ -649,6 +707,8 @@ void band_free(BandPrec &bp)
     BandImpl *im = (BandImpl *)bp.chunk_b0;
     if (im) {
         if (im->Li) (void)hipFree(im->Li);
+        if (im->cmap) (void)hipFree(im->cmap);
+        if (im->bmap) (void)hipFree(im->bmap);
         (void)hipFree(im->ch);
         (void)hipFree(im->seg_first);
         (void)hipFree(im->s0f);

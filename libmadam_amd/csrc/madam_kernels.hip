@@ -851,35 +851,40 @@ static constexpr int FLAT_TILE = 2048;
 static constexpr int FLAT_ITEMS = 64;    // baselines / pixels per block
 static constexpr int FLAT_LANES = 4;     // lanes per item in the reduction phase
 
-// global sample index of the start of global baseline b
-__device__ inline int64_t base_gstart(const int64_t *bstart, int64_t nbase, int64_t nsamp, int64_t b)
-{
-    int64_t det = b / nbase, k = b - det * nbase;
-    return det * nsamp + bstart[k];
-}
+// Baseline side: a block owns FG_K consecutive baselines of FG_D consecutive
+// detectors, i.e. one time window of every one of those detectors (the
+// baseline boundaries are common to all detectors), and the samples are
+// loaded time-major across the detectors: a wave's lanes hold 64 / FG_D
+// consecutive samples of each.  Co-pointed detectors (the polarisation pair
+// of one horn, detectors that share a scan path) then read each map pixel in
+// the same load instruction, one cache-line access for all of them; the loads
+// of the pointing stream stay in whole 256-byte runs.
+static constexpr int FG_D = 4;                    // detectors per block
+static constexpr int FG_K = FLAT_ITEMS / FG_D;    // baselines per detector per block
+static constexpr int FG_T = FLAT_TILE / FG_D;     // samples per detector per LDS tile
 
 __global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t nsamp, int64_t nbase,
                                                      const int64_t *bstart, const PixW *pixw, const double *m4,
                                                      int64_t ndeg, const double *dwb, const double *nna,
                                                      const double *p, double *ap)
 {
-    __shared__ double vals[FLAT_TILE];
-    __shared__ int64_t bs[FLAT_ITEMS + 1];   // global sample starts of the block's baselines, bs[n] = end
-    int64_t nbt = (int64_t)ndet * nbase;
+    __shared__ double vals[FLAT_TILE];    // vals[d * FG_T + t]
+    __shared__ int64_t bs[FG_K + 1];      // sample starts of the block's baselines, bs[nk] = end
+    int64_t ngrp = (ndet + FG_D - 1) / FG_D;
     int64_t lb = xcd_block(blockIdx.x, gridDim.x);
-    int64_t b0 = lb * FLAT_ITEMS, b1 = b0 + FLAT_ITEMS < nbt ? b0 + FLAT_ITEMS : nbt;
-    int nb = (int)(b1 - b0);
-    if (nb <= 0) return;
-    for (int t = threadIdx.x; t <= nb; t += blockDim.x) {
-        int64_t b = b0 + t;
-        bs[t] = t < nb ? base_gstart(bstart, nbase, nsamp, b)
-                       : base_gstart(bstart, nbase, nsamp, b - 1) + (bstart[(b - 1) % nbase + 1] - bstart[(b - 1) % nbase]);
-    }
+    int64_t kb = lb / ngrp, g = lb - kb * ngrp;   // time-major: neighbouring blocks share the sky
+    int64_t k0 = kb * FG_K;
+    if (k0 >= nbase) return;
+    int nk = (int)(nbase - k0 < FG_K ? nbase - k0 : FG_K);
+    int d0 = (int)g * FG_D, nd = ndet - d0 < FG_D ? ndet - d0 : FG_D;
+    for (int t = threadIdx.x; t <= nk; t += blockDim.x) bs[t] = bstart[k0 + t];
     int item = threadIdx.x / FLAT_LANES, sub = threadIdx.x % FLAT_LANES;
+    int id = item / FG_K, ik = item % FG_K;   // item = (detector, baseline)
+    bool own = id < nd && ik < nk;
+    int64_t b = (int64_t)(d0 + id) * nbase + k0 + ik;
     // the per-baseline terms of the result, loaded before the sample stream
     double pre_nna = 0.0, pre_p = 0.0, pre_dwb = 0.0, pre_ap = 0.0;
-    if (item < nb && sub == 0) {
-        int64_t b = b0 + item;
+    if (own && sub == 0) {
         pre_nna = nna[b];
         pre_p = p[b];
         pre_dwb = dwb[b];
@@ -887,30 +892,31 @@ __global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t
     }
     __syncthreads();
     double acc = 0.0;
-    int64_t S0 = bs[0], S1 = bs[nb];
-    for (int64_t t0 = S0; t0 < S1; t0 += FLAT_TILE) {
-        int64_t t1 = t0 + FLAT_TILE < S1 ? t0 + FLAT_TILE : S1;
-        for (int64_t g = t0 + threadIdx.x; g < t1; g += blockDim.x) {
-            PixW r = pixw[g];
+    int64_t S0 = bs[0], S1 = bs[nk];
+    for (int64_t t0 = S0; t0 < S1; t0 += FG_T) {
+        int len = (int)(S1 - t0 < FG_T ? S1 - t0 : FG_T);
+        for (int j = threadIdx.x; j < len * FG_D; j += blockDim.x) {
+            int tl = j / FG_D, d = j % FG_D;
             double v = 0.0;
-            if (r.pix >= 0) {
-                double4 m = *(const double4 *)(m4 + (r.pix / ndeg) * 4);
-                v = nmap == 1 ? m.x : (double)r.w0 * m.x + (double)r.w1 * m.y + (double)r.w2 * m.z;
+            if (d < nd) {
+                PixW r = pixw[(int64_t)(d0 + d) * nsamp + t0 + tl];
+                if (r.pix >= 0) {
+                    double4 m = *(const double4 *)(m4 + (r.pix / ndeg) * 4);
+                    v = nmap == 1 ? m.x : (double)r.w0 * m.x + (double)r.w1 * m.y + (double)r.w2 * m.z;
+                }
             }
-            vals[g - t0] = v;
+            vals[d * FG_T + tl] = v;
         }
         __syncthreads();
-        if (item < nb) {
-            int64_t a = bs[item] > t0 ? bs[item] : t0, e = bs[item + 1] < t1 ? bs[item + 1] : t1;
-            for (int64_t g = a + sub; g < e; g += FLAT_LANES) acc += vals[g - t0];
+        if (own) {
+            int64_t t1 = t0 + len;
+            int64_t a = bs[ik] > t0 ? bs[ik] : t0, e = bs[ik + 1] < t1 ? bs[ik + 1] : t1;
+            for (int64_t t = a + sub; t < e; t += FLAT_LANES) acc += vals[id * FG_T + (t - t0)];
         }
         __syncthreads();
     }
     acc = group_sum<FLAT_LANES>(acc);
-    if (item < nb && sub == 0) {
-        double v = pre_nna * pre_p - acc * pre_dwb;
-        ap[b0 + item] = pre_ap + v;
-    }
+    if (own && sub == 0) ap[b] = pre_ap + (pre_nna * pre_p - acc * pre_dwb);
 }
 
 __global__ void __launch_bounds__(256) k_scatter_flat(int nmap, int64_t npc, int64_t ndeg, const int64_t *off,
@@ -1109,8 +1115,10 @@ void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_
                         const PixW *pixw, const double *m4, int64_t ndeg, const double *dwb, const double *nna,
                         const double *p, double *ap)
 {
-    int64_t grid = ((int64_t)ndet * nbase + FLAT_ITEMS - 1) / FLAT_ITEMS;
-    k_gather_flat<<<(unsigned)grid, 256, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p, ap);
+    int64_t grid = ((ndet + FG_D - 1) / FG_D) * ((nbase + FG_K - 1) / FG_K);
+    if (grid > 0)
+        k_gather_flat<<<(unsigned)grid, 256, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p,
+                                                     ap);
 }
 
 void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
