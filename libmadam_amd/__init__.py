@@ -24,7 +24,8 @@ PSD_TYPE = np.float64
 # order of madam_get_stats
 STAT_NAMES = ["niter", "rrinit", "rzinit", "t_pointing", "t_bin", "t_init", "t_precond", "t_iter", "t_final",
               "t_scatter", "t_gather", "t_prior", "t_apply_prec", "n_scatter", "n_gather", "precond_kind",
-              "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total", "t_step", "ncg"]
+              "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total", "t_step", "ncg",
+              "band_chunks", "band_chunks_unique", "band_blocks", "band_blocks_unique"]
 
 
 class MadamError(RuntimeError):

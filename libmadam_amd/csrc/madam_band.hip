@@ -232,9 +232,11 @@ struct BandPF {
 
 // forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
 // next KD rows (acc[0] for the first row of the chunk).  If store, writes y.
+// lic is the chunk's factor column, li_chunk(...) resolved once by the caller
+// (a map load inside the row loop would wait for every load in flight).
 template <int KD>
-__device__ inline void fwd_sweep(const double *Li, const int32_t *bmap, int64_t c, int64_t b0, int n,
-                                 const double *r, double *acc, double *y, bool store, bool zero_rhs)
+__device__ inline void fwd_sweep(const double *lic, int64_t b0, int n, const double *r, double *acc, double *y,
+                                 bool store, bool zero_rhs)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
     for (int i0 = 0; i0 < n; i0 += BAND_PF) {
@@ -243,7 +245,7 @@ __device__ inline void fwd_sweep(const double *Li, const int32_t *bmap, int64_t 
         for (int u = 0; u < BAND_PF; ++u) {
             int i = i0 + u;
             if (i < n) {
-                const double *row = li_chunk(Li, bmap, c, K) + (int64_t)i * K * 64;
+                const double *row = lic + (int64_t)i * K * 64;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
                 rh[u] = zero_rhs ? 0.0 : r[b0 + i];
@@ -265,8 +267,8 @@ __device__ inline void fwd_sweep(const double *Li, const int32_t *bmap, int64_t 
 
 // backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
 template <int KD>
-__device__ inline void bwd_sweep(const double *Li, const int32_t *bmap, int64_t c, int64_t b0, int n,
-                                 const double *y, double *xs, double *x, bool store, bool zero_rhs)
+__device__ inline void bwd_sweep(const double *lic, int64_t b0, int n, const double *y, double *xs, double *x,
+                                 bool store, bool zero_rhs)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
     for (int i0 = n - 1; i0 >= 0; i0 -= BAND_PF) {
@@ -275,7 +277,7 @@ __device__ inline void bwd_sweep(const double *Li, const int32_t *bmap, int64_t 
         for (int u = 0; u < BAND_PF; ++u) {
             int i = i0 - u;
             if (i >= 0) {
-                const double *row = li_chunk(Li, bmap, c, K) + (int64_t)i * K * 64;
+                const double *row = lic + (int64_t)i * K * 64;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
                 rh[u] = zero_rhs ? 0.0 : y[b0 + i];
@@ -311,12 +313,12 @@ __global__ void __launch_bounds__(64) k_band_response(const ChunkInfo *ch, int64
     double st[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    fwd_sweep<KD>(Lb, nullptr, c, ci.b0, ci.n, nullptr, st, nullptr, false, true);
+    fwd_sweep<KD>(li_chunk(Lb, nullptr, c, KD + 1), ci.b0, ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tf[(c * KD + m) * KD + j] = st[m];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    bwd_sweep<KD>(Lb, nullptr, c, ci.b0, ci.n, nullptr, st, nullptr, false, true);
+    bwd_sweep<KD>(li_chunk(Lb, nullptr, c, KD + 1), ci.b0, ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tb[(c * KD + m) * KD + j] = st[m];
 }
@@ -331,7 +333,7 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    fwd_sweep<KD>(Lb, bmap, c, ci.b0, ci.n, r, acc, nullptr, false, false);
+    fwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.b0, ci.n, r, acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
 }
@@ -474,10 +476,11 @@ __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int6
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
-    fwd_sweep<KD>(Lb, bmap, c, ci.b0, ci.n, r, acc, z, true, false);
+    const double *lic = li_chunk(Lb, bmap, c, KD + 1);
+    fwd_sweep<KD>(lic, ci.b0, ci.n, r, acc, z, true, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    bwd_sweep<KD>(Lb, bmap, c, ci.b0, ci.n, z, acc, nullptr, false, false);
+    bwd_sweep<KD>(lic, ci.b0, ci.n, z, acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
 }
@@ -492,7 +495,7 @@ __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t n
     double xs[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
-    bwd_sweep<KD>(Lb, bmap, c, ci.b0, ci.n, z, xs, z, true, false);
+    bwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.b0, ci.n, z, xs, z, true, false);
 }
 
 // ---------------------------------------------------------------- host side
@@ -511,7 +514,6 @@ struct BandImpl {
     double *Gf = nullptr, *Gb = nullptr, *cg = nullptr, *Sg = nullptr;
     // runs of bitwise-equal chunks and blocks of the factor (see li_chunk)
     int32_t *cmap = nullptr, *bmap = nullptr;
-    int64_t nblk_unique = 0, nchunk_unique = 0;
 };
 
 static int kd_class(int kd)
@@ -658,28 +660,29 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
         std::vector<int32_t> cmap(bp.nchunk), bmap(im->NCP / 64);
         for (int64_t c = 0; c < bp.nchunk; ++c) {
             cmap[c] = ceq[c] ? cmap[c - 1] : (int32_t)c;
-            im->nchunk_unique += ceq[c] ? 0 : 1;
+            bp.nchunk_unique += ceq[c] ? 0 : 1;
         }
         for (int64_t w = 0; w < (int64_t)bmap.size(); ++w) {
             bool same = w > 0;
             for (int64_t c = w * 64; c < std::min<int64_t>(w * 64 + 64, bp.nchunk) && same; ++c)
                 same = cmap[c] == cmap[c - 64];
             bmap[w] = same ? bmap[w - 1] : (int32_t)w;
-            im->nblk_unique += same ? 0 : 1;
+            bp.nblock_unique += same ? 0 : 1;
         }
+        bp.nblock = (int64_t)bmap.size();
         MADAM_CHECK(hipMalloc(&im->cmap, sizeof(int32_t) * cmap.size()));
         MADAM_CHECK(hipMalloc(&im->bmap, sizeof(int32_t) * bmap.size()));
         MADAM_CHECK(hipMemcpyAsync(im->cmap, cmap.data(), sizeof(int32_t) * cmap.size(), hipMemcpyHostToDevice, s));
         MADAM_CHECK(hipMemcpyAsync(im->bmap, bmap.data(), sizeof(int32_t) * bmap.size(), hipMemcpyHostToDevice, s));
     }
-    bp.chunk_b0 = (int64_t *)im;   // opaque implementation pointer
+    bp.impl = im;
     MADAM_CHECK(hipStreamSynchronize(s));
     return 0;
 }
 
 void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
 {
-    BandImpl *im = (BandImpl *)bp.chunk_b0;
+    BandImpl *im = (BandImpl *)bp.impl;
     if (!im || bp.nchunk == 0) return;
     int KD = bp.kd;
     unsigned g = (unsigned)((bp.nchunk + 63) / 64);
@@ -704,7 +707,7 @@ void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
 
 void band_free(BandPrec &bp)
 {
-    BandImpl *im = (BandImpl *)bp.chunk_b0;
+    BandImpl *im = (BandImpl *)bp.impl;
     if (im) {
         if (im->Li) (void)hipFree(im->Li);
         if (im->cmap) (void)hipFree(im->cmap);

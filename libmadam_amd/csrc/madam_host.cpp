@@ -349,7 +349,8 @@ struct DBuf {
 
 enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_PRECOND, ST_T_ITER, ST_T_FINAL,
        ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
-       ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_NCG, ST_COUNT };
+       ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_NCG,
+       ST_BAND_CHUNKS, ST_BAND_CHUNKS_UNIQUE, ST_BAND_BLOCKS, ST_BAND_BLOCKS_UNIQUE, ST_COUNT };
 
 struct Context {
     int device = 0;
@@ -1229,6 +1230,10 @@ static void build_preconditioner(Context &c)
     int nbmax = 0;
     for (int v : nband) nbmax = std::max(nbmax, v);
     c.stats[ST_NBAND] = nbmax;
+    c.stats[ST_BAND_CHUNKS] = (double)c.band.nchunk;
+    c.stats[ST_BAND_CHUNKS_UNIQUE] = (double)c.band.nchunk_unique;
+    c.stats[ST_BAND_BLOCKS] = (double)c.band.nblock;
+    c.stats[ST_BAND_BLOCKS_UNIQUE] = (double)c.band.nblock_unique;
     c.segcg.clear();
     c.ic0cg.clear();
     for (size_t is = 0; is < c.seg.size(); ++is)

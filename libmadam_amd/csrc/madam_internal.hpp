@@ -207,15 +207,12 @@ struct BandPrec {
     int kd = 0;             // max bandwidth - 1
     int chunk = 0;          // rows per partition
     int64_t nchunk = 0;     // total partitions over all segments
-    double *L = nullptr;    // interleaved factor [row_in_chunk][kd+1][chunk_id]
+    double *L = nullptr;    // factor rows while building (freed once re-laid out)
     double *Tf = nullptr;   // forward carry response [chunk][kd][kd]
     double *Tb = nullptr;   // backward carry response
-    int64_t *chunk_b0 = nullptr;  // first global baseline of each partition
-    int32_t *chunk_n = nullptr;   // rows in each partition
-    int32_t *chunk_first = nullptr;  // 1 if first partition of its segment
-    int32_t *chunk_nband = nullptr;  // bandwidth of the segment
-    double *carry = nullptr;      // [nchunk][kd] carries
-    double *u = nullptr;          // scratch per baseline row
+    void *impl = nullptr;   // chunk tables, interleaved factor, carry-chain groups
+    // partitions (and 64-partition blocks) that are not bitwise repeats of the previous one
+    int64_t nchunk_unique = 0, nblock = 0, nblock_unique = 0;
 };
 
 int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *dseg, int64_t nseg,
