@@ -180,11 +180,77 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
 // blocks), and cmap[c] the first chunk of c's run of equal chunks: the sweeps
 // and the carry chains read through them, so a run streams its factor and its
 // transfer matrices from cache once instead of from memory per chunk.  The
-// values read are the same bits, so the results do not change.
+// values read are the same bits, so the results do not change.  A negative
+// map entry -1 - w marks a block whose 64 chunks are all equal to each other:
+// every lane then reads lane 0's column of block w, one cache line per load.
 __device__ inline const double *li_chunk(const double *Li, const int32_t *bmap, int64_t c, int K)
 {
-    int64_t w = bmap ? (int64_t)bmap[c >> 6] : (c >> 6);
-    return Li + w * (int64_t)BAND_CH * K * 64 + (c & 63);
+    int64_t w = c >> 6, lane = c & 63;
+    if (bmap) {
+        int32_t m = bmap[w];
+        w = m >= 0 ? m : -1 - m;
+        if (m < 0) lane = 0;
+    }
+    return Li + w * (int64_t)BAND_CH * K * 64 + lane;
+}
+
+// Vectors of the sweeps in the factor's interleaved layout: row i of chunk c
+// at V[((c / 64) * CH + i) * 64 + c % 64].
+__device__ inline const double *vi_chunk(const double *V, int64_t c)
+{
+    return V + (c >> 6) * (int64_t)BAND_CH * 64 + (c & 63);
+}
+__device__ inline double *vi_chunk(double *V, int64_t c)
+{
+    return V + (c >> 6) * (int64_t)BAND_CH * 64 + (c & 63);
+}
+
+// To and from the interleaved layout, one 64-chunk block per workgroup through
+// a padded LDS tile: both sides access whole 512-byte rows.
+static constexpr int BAND_TP = BAND_CH + 1;
+__global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_t nch, const double *v, double *vi)
+{
+    __shared__ double tile[64 * BAND_TP];
+    __shared__ int64_t b0s[64];
+    __shared__ int ns[64];
+    int64_t w = blockIdx.x;
+    if (threadIdx.x < 64) {
+        int64_t c = w * 64 + threadIdx.x;
+        b0s[threadIdx.x] = c < nch ? ch[c].b0 : 0;
+        ns[threadIdx.x] = c < nch ? ch[c].n : 0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
+        int j = t / BAND_CH, i = t % BAND_CH;   // chunk j of the block, row i
+        tile[j * BAND_TP + i] = i < ns[j] ? v[b0s[j] + i] : 0.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
+        int i = t / 64, j = t % 64;
+        vi[(w * BAND_CH + i) * 64 + j] = tile[j * BAND_TP + i];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int64_t nch, const double *vi, double *v)
+{
+    __shared__ double tile[64 * BAND_TP];
+    __shared__ int64_t b0s[64];
+    __shared__ int ns[64];
+    int64_t w = blockIdx.x;
+    if (threadIdx.x < 64) {
+        int64_t c = w * 64 + threadIdx.x;
+        b0s[threadIdx.x] = c < nch ? ch[c].b0 : 0;
+        ns[threadIdx.x] = c < nch ? ch[c].n : 0;
+    }
+    for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
+        int i = t / 64, j = t % 64;
+        tile[j * BAND_TP + i] = vi[(w * BAND_CH + i) * 64 + j];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
+        int j = t / BAND_CH, i = t % BAND_CH;
+        if (i < ns[j]) v[b0s[j] + i] = tile[j * BAND_TP + i];
+    }
 }
 
 // ceq[c] = 1 if chunk c holds the same row count and bitwise the same factor
@@ -233,10 +299,12 @@ struct BandPF {
 // forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
 // next KD rows (acc[0] for the first row of the chunk).  If store, writes y.
 // lic is the chunk's factor column, li_chunk(...) resolved once by the caller
-// (a map load inside the row loop would wait for every load in flight).
+// (a map load inside the row loop would wait for every load in flight); r and
+// y are the chunk's columns of vectors in the same interleaved layout (row i
+// at [i * 64]), so the lanes of a wave read and write consecutive addresses.
 template <int KD>
-__device__ inline void fwd_sweep(const double *lic, int64_t b0, int n, const double *r, double *acc, double *y,
-                                 bool store, bool zero_rhs)
+__device__ inline void fwd_sweep(const double *lic, int n, const double *r, double *acc, double *y, bool store,
+                                 bool zero_rhs)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
     for (int i0 = 0; i0 < n; i0 += BAND_PF) {
@@ -248,7 +316,7 @@ __device__ inline void fwd_sweep(const double *lic, int64_t b0, int n, const dou
                 const double *row = lic + (int64_t)i * K * 64;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
-                rh[u] = zero_rhs ? 0.0 : r[b0 + i];
+                rh[u] = zero_rhs ? 0.0 : r[(int64_t)i * 64];
             }
         }
 #pragma unroll
@@ -259,7 +327,7 @@ __device__ inline void fwd_sweep(const double *lic, int64_t b0, int n, const dou
 #pragma unroll
                 for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - rw[u][m] * v;
                 acc[KD - 1] = -rw[u][KD] * v;
-                if (store) y[b0 + i] = v;
+                if (store) y[(int64_t)i * 64] = v;
             }
         }
     }
@@ -267,8 +335,8 @@ __device__ inline void fwd_sweep(const double *lic, int64_t b0, int n, const dou
 
 // backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
 template <int KD>
-__device__ inline void bwd_sweep(const double *lic, int64_t b0, int n, const double *y, double *xs, double *x,
-                                 bool store, bool zero_rhs)
+__device__ inline void bwd_sweep(const double *lic, int n, const double *y, double *xs, double *x, bool store,
+                                 bool zero_rhs)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
     for (int i0 = n - 1; i0 >= 0; i0 -= BAND_PF) {
@@ -280,7 +348,7 @@ __device__ inline void bwd_sweep(const double *lic, int64_t b0, int n, const dou
                 const double *row = lic + (int64_t)i * K * 64;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
-                rh[u] = zero_rhs ? 0.0 : y[b0 + i];
+                rh[u] = zero_rhs ? 0.0 : y[(int64_t)i * 64];
             }
         }
 #pragma unroll
@@ -294,7 +362,7 @@ __device__ inline void bwd_sweep(const double *lic, int64_t b0, int n, const dou
 #pragma unroll
                 for (int m = KD - 1; m > 0; --m) xs[m] = xs[m - 1];
                 xs[0] = xi;
-                if (store) x[b0 + i] = xi;
+                if (store) x[(int64_t)i * 64] = xi;
             }
         }
     }
@@ -313,12 +381,12 @@ __global__ void __launch_bounds__(64) k_band_response(const ChunkInfo *ch, int64
     double st[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    fwd_sweep<KD>(li_chunk(Lb, nullptr, c, KD + 1), ci.b0, ci.n, nullptr, st, nullptr, false, true);
+    fwd_sweep<KD>(li_chunk(Lb, nullptr, c, KD + 1), ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tf[(c * KD + m) * KD + j] = st[m];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    bwd_sweep<KD>(li_chunk(Lb, nullptr, c, KD + 1), ci.b0, ci.n, nullptr, st, nullptr, false, true);
+    bwd_sweep<KD>(li_chunk(Lb, nullptr, c, KD + 1), ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tb[(c * KD + m) * KD + j] = st[m];
 }
@@ -333,7 +401,7 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    fwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.b0, ci.n, r, acc, nullptr, false, false);
+    fwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(r, c), acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
 }
@@ -477,10 +545,10 @@ __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int6
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
     const double *lic = li_chunk(Lb, bmap, c, KD + 1);
-    fwd_sweep<KD>(lic, ci.b0, ci.n, r, acc, z, true, false);
+    fwd_sweep<KD>(lic, ci.n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    bwd_sweep<KD>(lic, ci.b0, ci.n, z, acc, nullptr, false, false);
+    bwd_sweep<KD>(lic, ci.n, vi_chunk(z, c), acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
 }
@@ -495,7 +563,7 @@ __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t n
     double xs[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
-    bwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.b0, ci.n, z, xs, z, true, false);
+    bwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(z, c), xs, vi_chunk(z, c), true, false);
 }
 
 // ---------------------------------------------------------------- host side
@@ -514,6 +582,8 @@ struct BandImpl {
     double *Gf = nullptr, *Gb = nullptr, *cg = nullptr, *Sg = nullptr;
     // runs of bitwise-equal chunks and blocks of the factor (see li_chunk)
     int32_t *cmap = nullptr, *bmap = nullptr;
+    // right-hand side and solution in the interleaved layout (see vi_chunk)
+    double *ri = nullptr, *zi = nullptr;
 };
 
 static int kd_class(int kd)
@@ -629,6 +699,8 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
     if (bp.nchunk > 0) {
         im->NCP = (bp.nchunk + 63) / 64 * 64;
         MADAM_CHECK(hipMalloc(&im->Li, sizeof(double) * im->NCP * BAND_CH * K));
+        MADAM_CHECK(hipMalloc(&im->ri, sizeof(double) * im->NCP * BAND_CH));
+        MADAM_CHECK(hipMalloc(&im->zi, sizeof(double) * im->NCP * BAND_CH));
         int64_t tot = im->NCP * BAND_CH * K;
         unsigned g = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535 * 16);
         k_band_relayout<<<g, 256, 0, s>>>(im->ch, bp.nchunk, im->NCP, K, BAND_CH, bp.L, im->Li);
@@ -669,6 +741,13 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
             bmap[w] = same ? bmap[w - 1] : (int32_t)w;
             bp.nblock_unique += same ? 0 : 1;
         }
+        // blocks whose chunks are all one run read a single column (see li_chunk)
+        for (int64_t w = 0; w < (int64_t)bmap.size(); ++w) {
+            int64_t c0 = w * 64, c1 = std::min<int64_t>(w * 64 + 64, bp.nchunk);
+            bool uni = true;
+            for (int64_t c = c0 + 1; c < c1 && uni; ++c) uni = cmap[c] == cmap[c0];
+            if (uni) bmap[w] = -1 - bmap[w];
+        }
         bp.nblock = (int64_t)bmap.size();
         MADAM_CHECK(hipMalloc(&im->cmap, sizeof(int32_t) * cmap.size()));
         MADAM_CHECK(hipMalloc(&im->bmap, sizeof(int32_t) * bmap.size()));
@@ -689,19 +768,22 @@ void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
     unsigned gs = (unsigned)im->nseg_used;   // one 64-lane wave per segment
     unsigned gg = (unsigned)im->ngrp;        // one wave per chunk group
     BAND_DISPATCH(KD, ({
-        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, r, im->s0f);
+        k_band_to_int<<<g, 256, 0, s>>>(im->ch, bp.nchunk, r, im->ri);
+        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->ri, im->s0f);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, nullptr,
                                                  nullptr, im->cg);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, 0, im->Sg);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, im->Sg,
                                                  im->sinf, nullptr);
-        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, r, im->sinf, z, im->s0b);
+        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->ri, im->sinf, im->zi,
+                                              im->s0b);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, nullptr,
                                                  nullptr, im->cg);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, 1, im->Sg);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, im->Sg,
                                                  im->sinb, nullptr);
-        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->sinb, z);
+        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->sinb, im->zi);
+        k_band_from_int<<<g, 256, 0, s>>>(im->ch, bp.nchunk, im->zi, z);
     }));
 }
 
@@ -711,6 +793,8 @@ void band_free(BandPrec &bp)
     if (im) {
         if (im->Li) (void)hipFree(im->Li);
         if (im->cmap) (void)hipFree(im->cmap);
+        if (im->ri) (void)hipFree(im->ri);
+        if (im->zi) (void)hipFree(im->zi);
         if (im->bmap) (void)hipFree(im->bmap);
         (void)hipFree(im->ch);
         (void)hipFree(im->seg_first);
