@@ -413,11 +413,26 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
 // Wave per group: chain the group's chunks (increasing order for dir 0,
 // decreasing for dir 1) starting from zero (cg[g] = outgoing state) or from the
 // group's true incoming state Sg[g] (writing every chunk's incoming state sin).
+// Both chain levels run one wave per chain with lane m < KD holding state
+// component m.  The chain's additive terms are staged in LDS before the loop,
+// the state is broadcast with readlane, and the transfer matrix stays in
+// registers while consecutive steps share it (runs of equal chunks or groups,
+// through cmap / gmap); it is reloaded only where the run changes.  The sums
+// are taken in the same order on every path.
+template <int KD>
+__device__ inline double chain_step(double v, const double (&tr)[KD], double st)
+{
+#pragma unroll
+    for (int q = 0; q < KD; ++q) v += tr[q] * readlane_d(st, q);
+    return v;
+}
+
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, const int32_t *gn, int64_t ngrp,
                                                          const double *s0, const double *T, const int32_t *cmap,
                                                          int dir, const double *Sg, double *sin, double *cg)
 {
+    __shared__ double s0l[BAND_G * KD];
     int lane = threadIdx.x & 63;
     int64_t g = blockIdx.x;
     if (g >= ngrp) return;
@@ -425,72 +440,64 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
     int64_t c0 = gfirst[g];
     int n = gn[g];
     double st = (Sg && on) ? Sg[g * KD + lane] : 0.0;
-    constexpr int PF = BandPF<KD>::value;
-    for (int t0 = 0; t0 < n; t0 += PF) {
-        double tv[PF], tr[PF][KD];
+    for (int t = lane; t < n * KD; t += 64) s0l[t] = s0[c0 * KD + t];
+    __syncthreads();
+    int64_t cur = -1;
+    double tr[KD];
 #pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            int t = t0 + u;
-            if (t < n) {
-                int64_t c = dir == 0 ? c0 + t : c0 + n - 1 - t;
-                int64_t ct = cmap ? (int64_t)cmap[c] : c;
-                tv[u] = on ? s0[c * KD + lane] : 0.0;
+    for (int q = 0; q < KD; ++q) tr[q] = 0.0;
+    for (int t = 0; t < n; ++t) {
+        int lc = dir == 0 ? t : n - 1 - t;
+        int64_t c = c0 + lc;
+        int64_t ct = cmap ? (int64_t)cmap[c] : c;
+        if (ct != cur) {   // uniform over the wave
+            cur = ct;
 #pragma unroll
-                for (int q = 0; q < KD; ++q) tr[u][q] = on ? T[(ct * KD + lane) * KD + q] : 0.0;
-            }
+            for (int q = 0; q < KD; ++q) tr[q] = on ? T[(ct * KD + lane) * KD + q] : 0.0;
         }
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            int t = t0 + u;
-            if (t < n) {
-                int64_t c = dir == 0 ? c0 + t : c0 + n - 1 - t;
-                if (sin && on) sin[c * KD + lane] = st;
-                double v = tv[u];
-#pragma unroll
-                for (int q = 0; q < KD; ++q) v += tr[u][q] * __shfl(st, q, 64);
-                st = v;
-            }
-        }
+        if (sin && on) sin[c * KD + lane] = st;
+        st = chain_step<KD>(on ? s0l[lc * KD + lane] : 0.0, tr, st);
     }
     if (cg && on) cg[g * KD + lane] = st;
 }
 
-// wave per segment: S_{g+1} = cg[g] + Gm[g] S_g over the segment's groups in chain order
+// wave per segment: S_{g+1} = cg[g] + Gm[g] S_g over the segment's groups in
+// chain order, staging cg in windows of 64 groups
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, int64_t nseg, const double *cg,
-                                                       const double *Gm, int dir, double *Sg)
+                                                       const double *Gm, const int32_t *gmap, int dir, double *Sg)
 {
+    constexpr int W = 64;
+    __shared__ double cgl[W * KD];
     int lane = threadIdx.x & 63;
     int64_t s = blockIdx.x;
     if (s >= nseg) return;
     bool on = lane < KD;
     int64_t g0 = sfirst[s], g1 = sfirst[s + 1];
-    double st = 0.0;
     int64_t n = g1 - g0;
-    constexpr int PF = BandPF<KD>::value;
-    for (int64_t t0 = 0; t0 < n; t0 += PF) {
-        double tv[PF], tr[PF][KD];
+    double st = 0.0;
+    int64_t cur = -1;
+    double tr[KD];
 #pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            int64_t t = t0 + u;
-            if (t < n) {
-                int64_t g = dir == 0 ? g0 + t : g1 - 1 - t;
-                tv[u] = on ? cg[g * KD + lane] : 0.0;
+    for (int q = 0; q < KD; ++q) tr[q] = 0.0;
+    for (int64_t t0 = 0; t0 < n; t0 += W) {
+        int64_t m = n - t0 < W ? n - t0 : W;
+        // window of chain positions t0 .. t0 + m - 1: groups lg0 .. lg0 + m - 1
+        int64_t lg0 = dir == 0 ? t0 : n - t0 - m;
+        __syncthreads();
+        for (int64_t t = lane; t < m * KD; t += 64) cgl[t] = cg[(g0 + lg0) * KD + t];
+        __syncthreads();
+        for (int64_t t = 0; t < m; ++t) {
+            int64_t lw = dir == 0 ? t : m - 1 - t;   // position in the window
+            int64_t g = g0 + lg0 + lw;
+            int64_t gt = gmap ? (int64_t)gmap[g] : g;
+            if (gt != cur) {
+                cur = gt;
 #pragma unroll
-                for (int q = 0; q < KD; ++q) tr[u][q] = on ? Gm[(g * KD + lane) * KD + q] : 0.0;
+                for (int q = 0; q < KD; ++q) tr[q] = on ? Gm[(gt * KD + lane) * KD + q] : 0.0;
             }
-        }
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            int64_t t = t0 + u;
-            if (t < n) {
-                int64_t g = dir == 0 ? g0 + t : g1 - 1 - t;
-                if (on) Sg[g * KD + lane] = st;
-                double v = tv[u];
-#pragma unroll
-                for (int q = 0; q < KD; ++q) v += tr[u][q] * __shfl(st, q, 64);
-                st = v;
-            }
+            if (on) Sg[g * KD + lane] = st;
+            st = chain_step<KD>(on ? cgl[lw * KD + lane] : 0.0, tr, st);
         }
     }
 }
@@ -584,6 +591,8 @@ struct BandImpl {
     int32_t *cmap = nullptr, *bmap = nullptr;
     // right-hand side and solution in the interleaved layout (see vi_chunk)
     double *ri = nullptr, *zi = nullptr;
+    // runs of groups with equal chunk sequences (equal group matrices)
+    int32_t *gmap = nullptr;
 };
 
 static int kd_class(int kd)
@@ -749,6 +758,16 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
             if (uni) bmap[w] = -1 - bmap[w];
         }
         bp.nblock = (int64_t)bmap.size();
+        // groups whose chunks repeat the previous group's one for one have the
+        // same group matrices (computed from the same bits in the same order)
+        std::vector<int32_t> gmap(std::max<int64_t>(im->ngrp, 1));
+        for (int64_t g = 0; g < im->ngrp; ++g) {
+            bool same = g > 0 && gn[g] == gn[g - 1];
+            for (int t = 0; t < gn[g] && same; ++t) same = cmap[gfirst[g] + t] == cmap[gfirst[g - 1] + t];
+            gmap[g] = same ? gmap[g - 1] : (int32_t)g;
+        }
+        MADAM_CHECK(hipMalloc(&im->gmap, sizeof(int32_t) * gmap.size()));
+        MADAM_CHECK(hipMemcpyAsync(im->gmap, gmap.data(), sizeof(int32_t) * gmap.size(), hipMemcpyHostToDevice, s));
         MADAM_CHECK(hipMalloc(&im->cmap, sizeof(int32_t) * cmap.size()));
         MADAM_CHECK(hipMalloc(&im->bmap, sizeof(int32_t) * bmap.size()));
         MADAM_CHECK(hipMemcpyAsync(im->cmap, cmap.data(), sizeof(int32_t) * cmap.size(), hipMemcpyHostToDevice, s));
@@ -772,14 +791,14 @@ void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
         k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->ri, im->s0f);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, nullptr,
                                                  nullptr, im->cg);
-        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, 0, im->Sg);
+        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, im->gmap, 0, im->Sg);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, im->Sg,
                                                  im->sinf, nullptr);
         k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->ri, im->sinf, im->zi,
                                               im->s0b);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, nullptr,
                                                  nullptr, im->cg);
-        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, 1, im->Sg);
+        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, im->gmap, 1, im->Sg);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, im->Sg,
                                                  im->sinb, nullptr);
         k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->sinb, im->zi);
@@ -794,6 +813,7 @@ void band_free(BandPrec &bp)
         if (im->Li) (void)hipFree(im->Li);
         if (im->cmap) (void)hipFree(im->cmap);
         if (im->ri) (void)hipFree(im->ri);
+        if (im->gmap) (void)hipFree(im->gmap);
         if (im->zi) (void)hipFree(im->zi);
         if (im->bmap) (void)hipFree(im->bmap);
         (void)hipFree(im->ch);
