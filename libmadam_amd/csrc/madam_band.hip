@@ -30,7 +30,8 @@
 namespace madam {
 
 static constexpr int BAND_CH = 64;    // rows per chunk (one thread each in the sweeps)
-static constexpr int BAND_G = 64;     // chunks per group of the carry chain
+static constexpr int BAND_G = 64;     // chunks per group of the carry chain (at most 64: one per lane)
+static_assert(BAND_G <= 64, "");
 static_assert(BAND_CH > 0, "");
 
 struct ChunkInfo {
@@ -253,6 +254,17 @@ __global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int6
     }
 }
 
+// compact row-major copy [i][m] of lane 0's column of block reps[u] (a block
+// whose chunks are all equal), for the scalar-load sweeps
+__global__ void k_band_compact_uniform(const double *Li, int K, const int32_t *reps, int64_t nu, double *Lu)
+{
+    int64_t tot = nu * BAND_CH * K;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t u = t / (BAND_CH * K), im = t % (BAND_CH * K);
+        Lu[t] = Li[((int64_t)reps[u] * BAND_CH * K + im) * 64];
+    }
+}
+
 // ceq[c] = 1 if chunk c holds the same row count and bitwise the same factor
 // rows as chunk c - 1
 __global__ void k_band_chunk_eq(const ChunkInfo *ch, int64_t nch, int K, const double *Li, uint8_t *ceq)
@@ -299,10 +311,12 @@ struct BandPF {
 // forward sweep of one chunk: L y = r, carry state acc[KD] = pending sums for the
 // next KD rows (acc[0] for the first row of the chunk).  If store, writes y.
 // lic is the chunk's factor column, li_chunk(...) resolved once by the caller
-// (a map load inside the row loop would wait for every load in flight); r and
+// (a map load inside the row loop would wait for every load in flight), with
+// element m of row i at lic[(i * K + m) * MS]: MS = 64 for the interleaved
+// factor, MS = 1 for a block's compact copy read with scalar loads; r and
 // y are the chunk's columns of vectors in the same interleaved layout (row i
 // at [i * 64]), so the lanes of a wave read and write consecutive addresses.
-template <int KD>
+template <int KD, int MS = 64>
 __device__ inline void fwd_sweep(const double *lic, int n, const double *r, double *acc, double *y, bool store,
                                  bool zero_rhs)
 {
@@ -313,9 +327,9 @@ __device__ inline void fwd_sweep(const double *lic, int n, const double *r, doub
         for (int u = 0; u < BAND_PF; ++u) {
             int i = i0 + u;
             if (i < n) {
-                const double *row = lic + (int64_t)i * K * 64;
+                const double *row = lic + (int64_t)i * K * MS;
 #pragma unroll
-                for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
+                for (int m = 0; m < K; ++m) rw[u][m] = row[m * MS];
                 rh[u] = zero_rhs ? 0.0 : r[(int64_t)i * 64];
             }
         }
@@ -334,7 +348,7 @@ __device__ inline void fwd_sweep(const double *lic, int n, const double *r, doub
 }
 
 // backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
-template <int KD>
+template <int KD, int MS = 64>
 __device__ inline void bwd_sweep(const double *lic, int n, const double *y, double *xs, double *x, bool store,
                                  bool zero_rhs)
 {
@@ -345,9 +359,9 @@ __device__ inline void bwd_sweep(const double *lic, int n, const double *y, doub
         for (int u = 0; u < BAND_PF; ++u) {
             int i = i0 - u;
             if (i >= 0) {
-                const double *row = lic + (int64_t)i * K * 64;
+                const double *row = lic + (int64_t)i * K * MS;
 #pragma unroll
-                for (int m = 0; m < K; ++m) rw[u][m] = row[m * 64];
+                for (int m = 0; m < K; ++m) rw[u][m] = row[m * MS];
                 rh[u] = zero_rhs ? 0.0 : y[(int64_t)i * 64];
             }
         }
@@ -393,7 +407,8 @@ __global__ void __launch_bounds__(64) k_band_response(const ChunkInfo *ch, int64
 
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
-                                                  const double *Lb, const double *r, double *s0)
+                                                  const double *Lb, const int32_t *uidx,
+                                                  const double *__restrict__ Lu, const double *r, double *s0)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
@@ -401,7 +416,12 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
-    fwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(r, c), acc, nullptr, false, false);
+    int32_t ui = uidx[blockIdx.x];
+    if (ui >= 0)
+        fwd_sweep<KD, 1>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ci.n), vi_chunk(r, c),
+                         acc, nullptr, false, false);
+    else
+        fwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(r, c), acc, nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
 }
@@ -441,6 +461,8 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
     int n = gn[g];
     double st = (Sg && on) ? Sg[g * KD + lane] : 0.0;
     for (int t = lane; t < n * KD; t += 64) s0l[t] = s0[c0 * KD + t];
+    // lane t holds the transfer-matrix index of the group's chunk t (n <= 64)
+    int myct = lane < n ? (cmap ? cmap[c0 + lane] : (int32_t)(c0 + lane)) : 0;
     __syncthreads();
     int64_t cur = -1;
     double tr[KD];
@@ -449,7 +471,7 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
     for (int t = 0; t < n; ++t) {
         int lc = dir == 0 ? t : n - 1 - t;
         int64_t c = c0 + lc;
-        int64_t ct = cmap ? (int64_t)cmap[c] : c;
+        int64_t ct = __builtin_amdgcn_readlane(myct, lc);
         if (ct != cur) {   // uniform over the wave
             cur = ct;
 #pragma unroll
@@ -486,11 +508,13 @@ __global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, in
         int64_t lg0 = dir == 0 ? t0 : n - t0 - m;
         __syncthreads();
         for (int64_t t = lane; t < m * KD; t += 64) cgl[t] = cg[(g0 + lg0) * KD + t];
+        // lane t holds the group-matrix index of the window's group t
+        int mygt = lane < m ? (gmap ? gmap[g0 + lg0 + lane] : (int32_t)(g0 + lg0 + lane)) : 0;
         __syncthreads();
         for (int64_t t = 0; t < m; ++t) {
-            int64_t lw = dir == 0 ? t : m - 1 - t;   // position in the window
+            int lw = (int)(dir == 0 ? t : m - 1 - t);   // position in the window
             int64_t g = g0 + lg0 + lw;
-            int64_t gt = gmap ? (int64_t)gmap[g] : g;
+            int64_t gt = __builtin_amdgcn_readlane(mygt, lw);
             if (gt != cur) {
                 cur = gt;
 #pragma unroll
@@ -542,8 +566,9 @@ __global__ void __launch_bounds__(64) k_band_group_mats(const int64_t *gfirst, c
 // backward sweep of the same chunk gives its backward partial state
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
-                                                       const double *Lb, const double *r, const double *sinf,
-                                                       double *z, double *s0b)
+                                                       const double *Lb, const int32_t *uidx,
+                                                       const double *__restrict__ Lu, const double *r,
+                                                       const double *sinf, double *z, double *s0b)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
@@ -551,18 +576,29 @@ __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int6
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
-    const double *lic = li_chunk(Lb, bmap, c, KD + 1);
-    fwd_sweep<KD>(lic, ci.n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
+    int32_t ui = uidx[blockIdx.x];
+    if (ui >= 0) {
+        const double *lu = Lu + (int64_t)ui * BAND_CH * (KD + 1);
+        int n = __builtin_amdgcn_readfirstlane(ci.n);
+        fwd_sweep<KD, 1>(lu, n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
 #pragma unroll
-    for (int m = 0; m < KD; ++m) acc[m] = 0;
-    bwd_sweep<KD>(lic, ci.n, vi_chunk(z, c), acc, nullptr, false, false);
+        for (int m = 0; m < KD; ++m) acc[m] = 0;
+        bwd_sweep<KD, 1>(lu, n, vi_chunk(z, c), acc, nullptr, false, false);
+    } else {
+        const double *lic = li_chunk(Lb, bmap, c, KD + 1);
+        fwd_sweep<KD>(lic, ci.n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
+#pragma unroll
+        for (int m = 0; m < KD; ++m) acc[m] = 0;
+        bwd_sweep<KD>(lic, ci.n, vi_chunk(z, c), acc, nullptr, false, false);
+    }
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
 }
 
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
-                                                  const double *Lb, const double *sinb, double *z)
+                                                  const double *Lb, const int32_t *uidx,
+                                                  const double *__restrict__ Lu, const double *sinb, double *z)
 {
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
@@ -570,7 +606,12 @@ __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t n
     double xs[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
-    bwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(z, c), xs, vi_chunk(z, c), true, false);
+    int32_t ui = uidx[blockIdx.x];
+    if (ui >= 0)
+        bwd_sweep<KD, 1>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ci.n), vi_chunk(z, c),
+                         xs, vi_chunk(z, c), true, false);
+    else
+        bwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(z, c), xs, vi_chunk(z, c), true, false);
 }
 
 // ---------------------------------------------------------------- host side
@@ -593,6 +634,9 @@ struct BandImpl {
     double *ri = nullptr, *zi = nullptr;
     // runs of groups with equal chunk sequences (equal group matrices)
     int32_t *gmap = nullptr;
+    // uniform blocks: uidx[w] = slot of the block's compact factor copy in Lu, or -1
+    int32_t *uidx = nullptr;
+    double *Lu = nullptr;
 };
 
 static int kd_class(int kd)
@@ -757,6 +801,28 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
             for (int64_t c = c0 + 1; c < c1 && uni; ++c) uni = cmap[c] == cmap[c0];
             if (uni) bmap[w] = -1 - bmap[w];
         }
+        // one compact copy per distinct uniform block
+        std::vector<int32_t> uidx(bmap.size(), -1), reps;
+        for (int64_t w = 0; w < (int64_t)bmap.size(); ++w) {
+            if (bmap[w] >= 0) continue;
+            int32_t wr = -1 - bmap[w];
+            if (reps.empty() || reps.back() != wr) reps.push_back(wr);   // runs are contiguous
+            uidx[w] = (int32_t)reps.size() - 1;
+        }
+        MADAM_CHECK(hipMalloc(&im->uidx, sizeof(int32_t) * uidx.size()));
+        MADAM_CHECK(hipMemcpyAsync(im->uidx, uidx.data(), sizeof(int32_t) * uidx.size(), hipMemcpyHostToDevice, s));
+        MADAM_CHECK(hipMalloc(&im->Lu, sizeof(double) * std::max<size_t>(reps.size(), 1) * BAND_CH * K));
+        if (!reps.empty()) {
+            int32_t *dreps = nullptr;
+            MADAM_CHECK(hipMalloc(&dreps, sizeof(int32_t) * reps.size()));
+            MADAM_CHECK(hipMemcpyAsync(dreps, reps.data(), sizeof(int32_t) * reps.size(), hipMemcpyHostToDevice, s));
+            int64_t tot = (int64_t)reps.size() * BAND_CH * K;
+            k_band_compact_uniform<<<(unsigned)std::min<int64_t>((tot + 255) / 256, 65535), 256, 0, s>>>(
+                im->Li, K, dreps, (int64_t)reps.size(), im->Lu);
+            MADAM_CHECK(hipGetLastError());
+            MADAM_CHECK(hipStreamSynchronize(s));
+            (void)hipFree(dreps);
+        }
         bp.nblock = (int64_t)bmap.size();
         // groups whose chunks repeat the previous group's one for one have the
         // same group matrices (computed from the same bits in the same order)
@@ -788,20 +854,20 @@ void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
     unsigned gg = (unsigned)im->ngrp;        // one wave per chunk group
     BAND_DISPATCH(KD, ({
         k_band_to_int<<<g, 256, 0, s>>>(im->ch, bp.nchunk, r, im->ri);
-        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->ri, im->s0f);
+        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->ri, im->s0f);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, nullptr,
                                                  nullptr, im->cg);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, im->gmap, 0, im->Sg);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, im->Sg,
                                                  im->sinf, nullptr);
-        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->ri, im->sinf, im->zi,
-                                              im->s0b);
+        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->ri,
+                                              im->sinf, im->zi, im->s0b);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, nullptr,
                                                  nullptr, im->cg);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, im->gmap, 1, im->Sg);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, im->Sg,
                                                  im->sinb, nullptr);
-        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->sinb, im->zi);
+        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->sinb, im->zi);
         k_band_from_int<<<g, 256, 0, s>>>(im->ch, bp.nchunk, im->zi, z);
     }));
 }
@@ -814,6 +880,8 @@ void band_free(BandPrec &bp)
         if (im->cmap) (void)hipFree(im->cmap);
         if (im->ri) (void)hipFree(im->ri);
         if (im->gmap) (void)hipFree(im->gmap);
+        if (im->uidx) (void)hipFree(im->uidx);
+        if (im->Lu) (void)hipFree(im->Lu);
         if (im->zi) (void)hipFree(im->zi);
         if (im->bmap) (void)hipFree(im->bmap);
         (void)hipFree(im->ch);
