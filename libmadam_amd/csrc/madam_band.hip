@@ -706,6 +706,8 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
     }
     first.push_back((int64_t)ch.size());
     bp.nchunk = (int64_t)ch.size();
+    bp.nrows = 0;
+    for (const ChunkInfo &ci : ch) bp.nrows += ci.n;
     BandImpl *im = new BandImpl();
     im->nseg_used = (int64_t)first.size() - 1;
     // groups of BAND_G chunks inside each segment

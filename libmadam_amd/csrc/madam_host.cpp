@@ -428,6 +428,9 @@ struct Context {
     DBuf<Segment> d_seg;
     std::vector<Segment> segconv;      // segments the prior convolves (PSD found)
     DBuf<Segment> d_segconv;
+    // the convolved segments (the band and CG preconditioner segments) cover
+    // every baseline, so the identity copy outside them is skipped
+    bool prior_covers_all = false, prec_covers_all = false;
     DBuf<unsigned long long> d_cnt;
     DBuf<double> d_fcov, d_fprec, d_invcov, d_fftwork, d_tab;
     FftPlan fft{};
@@ -1007,7 +1010,7 @@ static void apply_prior(Context &c, const double *x, double *y)
 {
     // cinvmul: y = x outside segments, convolution inside (src/noise_routines.f90:745-913);
     // intervals without a PSD are left as y = x (convolve_interval returns early)
-    launch_copy(c.stream, y, x, c.nbtot);
+    if (!c.prior_covers_all) launch_copy(c.stream, y, x, c.nbtot);
     if (c.segconv.empty()) return;
     fft_convolve_segments(c.stream, c.fft, c.d_segconv.p, (int64_t)c.segconv.size(), x, y, c.d_fcov.p, c.nof / 2 + 1,
                           c.d_fftwork.p, c.fft_batch, c.segconv.data());
@@ -1095,7 +1098,7 @@ static void apply_precond(Context &c, const double *r, double *z)
                                   c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, c.segconv.data());
         break;
     default:
-        launch_copy(s, z, r, c.nbtot);
+        if (!c.prec_covers_all) launch_copy(s, z, r, c.nbtot);
         band_apply(s, c.band, r, z);
         cg_apply(c, r, z);
         break;
@@ -1134,8 +1137,14 @@ static void build_segments(Context &c)
     }
     if (!c.seg.empty()) h2d(c.d_seg.p, c.seg.data(), c.seg.size(), s);
     c.segconv.clear();
+    c.prior_covers_all = false;
     for (auto &g : c.seg)
         if (g.ipsd >= 0) c.segconv.push_back(g);
+    {
+        int64_t rows = 0;
+        for (const Segment &g : c.segconv) rows += g.noba;
+        c.prior_covers_all = rows == c.nbtot;   // segments are disjoint
+    }
     if (!c.segconv.empty()) {
         c.d_segconv.alloc(c.segconv.size());
         h2d(c.d_segconv.p, c.segconv.data(), c.segconv.size(), s);
@@ -1152,6 +1161,7 @@ static void build_preconditioner(Context &c)
     hipStream_t s = c.stream;
     const Params &p = c.p;
     if (c.band_ready) { band_free(c.band); c.band_ready = false; }
+    c.prec_covers_all = false;
     if (p.precond_width_max < 1) { c.precond_kind = 0; return; }
     if (!p.kfilter || p.precond_width_max == 1) {
         c.precond_kind = 1;
@@ -1249,6 +1259,11 @@ static void build_preconditioner(Context &c)
         MADAM_CHECK(hipStreamSynchronize(s));
     }
     c.stats[ST_NCG] = (double)c.segcg.size();
+    {
+        int64_t rows = c.band.nrows;
+        for (const Segment &g : c.segcg) rows += g.noba;
+        c.prec_covers_all = rows == c.nbtot;
+    }
 }
 
 // r = yba, z = M r, p = z (src/smadam_routines.f90:732-748)

@@ -213,6 +213,7 @@ struct BandPrec {
     void *impl = nullptr;   // chunk tables, interleaved factor, carry-chain groups
     // partitions (and 64-partition blocks) that are not bitwise repeats of the previous one
     int64_t nchunk_unique = 0, nblock = 0, nblock_unique = 0;
+    int64_t nrows = 0;      // baseline rows the factored segments cover
 };
 
 int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *dseg, int64_t nseg,
