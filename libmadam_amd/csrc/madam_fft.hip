@@ -511,8 +511,9 @@ __global__ void k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, 
 
 // spectral step on one pair {k, N - k}: X[k] at s[ik], X[N - k] at s[im] (padded
 // LDS indices), 0 < k <= N / 2
-__device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t N, const FftPlan &P,
-                                 const double *f)
+// fk = f[k], fm = f[N - k]
+__device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t N, const FftPlan &P, double fk,
+                                 double fm)
 {
     double2 a = s[ik], bc = s[im];
     bc.y = -bc.y;
@@ -522,7 +523,6 @@ __device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t 
     double2 WO = cmul(W, O);
     double2 Xk = make_double2(E.x + WO.x, E.y + WO.y);
     double2 Xm = make_double2(E.x - WO.x, -(E.y - WO.y));
-    double fk = f[k], fm = f[N - k];
     double2 Yk = make_double2(Xk.x * fk, Xk.y * fk);
     double2 Ym = make_double2(Xm.x * fm, Xm.y * fm);
     double2 s1 = make_double2(Yk.x + Ym.x, Yk.y - Ym.y);
@@ -537,7 +537,9 @@ __device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t 
     }
 }
 
-// grid = (L1 / 2 + 1, batch); the pair's rows interleaved in LDS at k2 * nr + r (padded)
+// grid = (L1 / 2 + 1, batch); the pair's rows interleaved in LDS at k2 * nr + r (padded).
+// fc holds the filter spectra in row order (fft_spectrum_rows): f[r + L1 k2] at
+// [r * L2 + k2], f[N] at [N], so a row pair reads two contiguous rows.
 __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2, FftPlan P, const double *fc,
                             int64_t fc_stride, const Segment *seg, int64_t s0)
 {
@@ -561,21 +563,23 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
             if (k2 == 0) {
                 double2 a = s[lp(0)];
                 double x0 = a.x + a.y, xn = a.x - a.y;
-                double y0 = x0 * f[0], yn = xn * f[N];
+                double y0 = x0 * f[0], yn = xn * f[N];   // f[0] at [0]
                 s[lp(0)] = make_double2(y0 + yn, y0 - yn);
             } else {
-                conv_pair(s, lp(k2), lp(L2 - k2), (int64_t)L1 * k2, N, P, f);
+                conv_pair(s, lp(k2), lp(L2 - k2), (int64_t)L1 * k2, N, P, f[k2], f[L2 - k2]);
             }
         }
     } else if (nr == 1) {   // ra == L1 / 2
         for (int k2 = threadIdx.x; k2 < L2 / 2; k2 += blockDim.x)
-            conv_pair(s, lp(k2), lp(L2 - 1 - k2), (int64_t)ra + (int64_t)L1 * k2, N, P, f);
+            conv_pair(s, lp(k2), lp(L2 - 1 - k2), (int64_t)ra + (int64_t)L1 * k2, N, P, f[(int64_t)ra * L2 + k2],
+                      f[(int64_t)ra * L2 + L2 - 1 - k2]);
     } else {
         for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
             int64_t kA = (int64_t)ra + (int64_t)L1 * k2;
             int pp = lp(k2 * 2), qq = lp((L2 - 1 - k2) * 2 + 1);
-            if (kA <= N / 2) conv_pair(s, pp, qq, kA, N, P, f);
-            else conv_pair(s, qq, pp, N - kA, N, P, f);
+            double fA = f[(int64_t)ra * L2 + k2], fB = f[(int64_t)rb * L2 + L2 - 1 - k2];   // f[kA], f[N - kA]
+            if (kA <= N / 2) conv_pair(s, pp, qq, kA, N, P, fA, fB);
+            else conv_pair(s, qq, pp, N - kA, N, P, fB, fA);
         }
     }
     __syncthreads();
@@ -650,6 +654,27 @@ __global__ void k_conv_sub_mean(double *y, const Segment *seg, int64_t s0, const
         ys[j] -= mean;
 }
 
+// fT[r * L2 + k2] = f[r + L1 * k2], fT[N] = f[N], per spectrum row
+__global__ void k_spectrum_rows(const double *fc, int64_t nrow, int64_t stride, int64_t N, int L1, int L2,
+                                double *fT)
+{
+    int64_t tot = nrow * (N + 1);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t row = t / (N + 1), i = t % (N + 1);
+        int64_t src = i == N ? N : (i % L2) * L1 + i / L2;
+        fT[row * stride + i] = fc[row * stride + src];
+    }
+}
+
+void fft_spectrum_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrow, int64_t stride, double *fT)
+{
+    int64_t tot = nrow * (p.N + 1);
+    if (tot <= 0) return;
+    k_spectrum_rows<<<(unsigned)std::min<int64_t>((tot + FFT_TPB - 1) / FFT_TPB, 65535 * 8), FFT_TPB, 0, s>>>(
+        fc, nrow, stride, p.N, (int)p.L1, (int)p.L2, fT);
+    MADAM_CHECK(hipGetLastError());
+}
+
 // Convolution of the batch's segments: the three fused passes when the plan
 // allows them, otherwise the general eight-pass form (short FFTs).
 static bool conv_fusable(const FftPlan &p)
@@ -658,8 +683,8 @@ static bool conv_fusable(const FftPlan &p)
 }
 
 void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg, const double *x,
-                           double *y, const double *fc, int64_t fc_stride, double *work, int64_t batch,
-                           const Segment *hseg)
+                           double *y, const double *fc, const double *fc_rows, int64_t fc_stride, double *work,
+                           int64_t batch, const Segment *hseg)
 {
     (void)hseg;
     int64_t nof = 2 * p.N;
@@ -677,7 +702,7 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
             k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 x, dseg, s0, partial, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
             k_conv_rows<<<dim3((unsigned)(p.L1 / 2 + 1), (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(2 * p.L2)), s>>>(
-                (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc, fc_stride, dseg, s0);
+                (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
             k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
                 p);

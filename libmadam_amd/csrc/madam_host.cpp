@@ -433,6 +433,7 @@ struct Context {
     bool prior_covers_all = false, prec_covers_all = false;
     DBuf<unsigned long long> d_cnt;
     DBuf<double> d_fcov, d_fprec, d_invcov, d_fftwork, d_tab;
+    DBuf<double> d_fcov_rows, d_fprec_rows;   // the spectra in the fused row pass's order
     FftPlan fft{};
     bool fft_ready = false;
     int64_t fft_batch = 0;
@@ -822,6 +823,8 @@ static void build_filters(Context &c)
         fft_plan_init(c.fft, c.nof);
         c.fft_ready = true;
     }
+    c.d_fcov_rows.alloc((size_t)nh * c.npsdtot);
+    fft_spectrum_rows(s, c.fft, c.d_fcov.p, c.npsdtot, nh, c.d_fcov_rows.p);
 }
 
 // ------------------------------------------------------------------ pointing
@@ -1012,8 +1015,8 @@ static void apply_prior(Context &c, const double *x, double *y)
     // intervals without a PSD are left as y = x (convolve_interval returns early)
     if (!c.prior_covers_all) launch_copy(c.stream, y, x, c.nbtot);
     if (c.segconv.empty()) return;
-    fft_convolve_segments(c.stream, c.fft, c.d_segconv.p, (int64_t)c.segconv.size(), x, y, c.d_fcov.p, c.nof / 2 + 1,
-                          c.d_fftwork.p, c.fft_batch, c.segconv.data());
+    fft_convolve_segments(c.stream, c.fft, c.d_segconv.p, (int64_t)c.segconv.size(), x, y, c.d_fcov.p,
+                          c.d_fcov_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, c.segconv.data());
 }
 
 // apply_CG_preconditioner (src/noise_routines.f90:1305-1409) on every segment
@@ -1054,7 +1057,7 @@ static void cg_apply(Context &c, const double *r, double *z)
         h2d(c.d_segact.p, live.data(), live.size(), s);
         // ap = convolve(p) + nna p
         fft_convolve_segments(s, c.fft, c.d_segact.p, (int64_t)live.size(), c.d_cg_p.p, c.d_cg_ap.p, c.d_fcov.p,
-                              c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, live.data());
+                              c.d_fcov_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, live.data());
         launch_cg_nna(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_nna.p, c.d_cg_p.p, c.d_cg_ap.p);
         launch_cg_dot(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_p.p, c.d_cg_ap.p, c.d_cg_part.p, an_d);
         d2h(an.data(), an_d, ns, s);
@@ -1095,7 +1098,7 @@ static void apply_precond(Context &c, const double *r, double *z)
         launch_copy(s, z, r, c.nbtot);
         if (!c.segconv.empty())
             fft_convolve_segments(s, c.fft, c.d_segconv.p, (int64_t)c.segconv.size(), r, z, c.d_fprec.p,
-                                  c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, c.segconv.data());
+                                  c.d_fprec_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, c.segconv.data());
         break;
     default:
         if (!c.prec_covers_all) launch_copy(s, z, r, c.nbtot);
@@ -1176,6 +1179,8 @@ static void build_preconditioner(Context &c)
         wp.alloc(c.npsdtot);
         h2d(wp.p, c.psd_w.data(), c.npsdtot, s);
         launch_fprecond(s, c.d_fcov.p, nh, wp.p, p.nshort, c.d_fprec.p, c.npsdtot);
+        c.d_fprec_rows.alloc((size_t)nh * c.npsdtot);
+        fft_spectrum_rows(s, c.fft, c.d_fprec.p, c.npsdtot, nh, c.d_fprec_rows.p);
         MADAM_CHECK(hipStreamSynchronize(s));
     }
     if (p.use_fprecond) { c.precond_kind = 2; return; }

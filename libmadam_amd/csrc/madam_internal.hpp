@@ -190,9 +190,12 @@ struct FftPlan {
 void fft_plan_init(FftPlan &p, int64_t nof);
 void fft_plan_free(FftPlan &p);
 // convolve every segment: y = x outside segments is the caller's job (copy first)
+// fc: filter spectra, nrow x fc_stride; fc_rows: the same in row order (fft_spectrum_rows)
 void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg_dev,
-                           const double *x, double *y, const double *fc, int64_t fc_stride, double *work,
-                           int64_t batch, const Segment *hseg);
+                           const double *x, double *y, const double *fc, const double *fc_rows, int64_t fc_stride,
+                           double *work, int64_t batch, const Segment *hseg);
+// fT = fc reordered for the fused row pass: f[r + L1 k2] at [r * L2 + k2], f[N] at [N]
+void fft_spectrum_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrow, int64_t stride, double *fT);
 size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch);
 // inverse real FFT of fcov rows (invcov = c2r(fcov) / nof)
 void fft_c2r_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrows, double *out, double *work);
