@@ -476,9 +476,12 @@ static constexpr int CONV_NCOL = 8;
 
 // LDS elements (padded) for n double2
 __host__ __device__ inline int conv_lds_elems(int n) { return n + (n >> 3) + 1; }
+// The column passes hold L1 * CONV_NCOL points (74 KB of LDS at L1 = 512), so
+// only two blocks fit on a CU: they run 512 threads, one 8-point work item each.
+static constexpr int CONV_COL_TPB = 512;
 
 // grid = (L2 / CONV_NCOL, batch)
-__global__ void k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, const double *partial_in,
+__global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, const double *partial_in,
                                double2 *work, int64_t N, int L1, int logL1, int L2, FftPlan P)
 {
     extern __shared__ double2 s[];
@@ -593,7 +596,7 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
 }
 
 // grid = (L2 / CONV_NCOL, batch)
-__global__ void k_conv_col_inv(const double2 *work, const Segment *seg, int64_t s0, double *y, double *partial_out,
+__global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *work, const Segment *seg, int64_t s0, double *y, double *partial_out,
                                int64_t N, int L1, int logL1, int L2, FftPlan P)
 {
     extern __shared__ double2 s[];
@@ -631,12 +634,12 @@ __global__ void k_conv_col_inv(const double2 *work, const Segment *seg, int64_t 
         }
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    __shared__ double sh[FFT_TPB / 64];
+    __shared__ double sh[CONV_COL_TPB / 64];
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0;
-        for (int q = 0; q < FFT_TPB / 64; ++q) t += sh[q];
+        for (int q = 0; q < CONV_COL_TPB / 64; ++q) t += sh[q];
         partial_out[b * gridDim.x + blockIdx.x] = t;
     }
 }
@@ -699,11 +702,11 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
         dim3 gp(SEG_NB, (unsigned)nb);
         k_seg_partial<<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
         if (fused) {
-            k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
+            k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 x, dseg, s0, partial, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
             k_conv_rows<<<dim3((unsigned)(p.L1 / 2 + 1), (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(2 * p.L2)), s>>>(
                 (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
-            k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
+            k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
                 p);
             unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));

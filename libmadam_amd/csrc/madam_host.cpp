@@ -639,6 +639,8 @@ static void configure(Context &c, const char *parstring, long ndet, const char *
     c.npix_map = 12LL * p.nside_map * p.nside_map;
     c.npix_cross = 12LL * p.nside_cross * p.nside_cross;
     c.ndeg = c.npix_map / c.npix_cross;
+    if (c.ndeg < 1 || (c.ndeg & (c.ndeg - 1)) != 0)
+        fail("nside_cross must not exceed nside_map, and both must be powers of two");
     p.nshort = (int)(p.dnshort + .5);
     if (!p.kfirst) {
         p.dnshort = 1e6;

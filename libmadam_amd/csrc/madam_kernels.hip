@@ -862,13 +862,15 @@ static constexpr int FLAT_LANES = 4;     // lanes per item in the reduction phas
 static constexpr int FG_D = 4;                    // detectors per block
 static constexpr int FG_K = FLAT_ITEMS / FG_D;    // baselines per detector per block
 static constexpr int FG_T = FLAT_TILE / FG_D;     // samples per detector per LDS tile
+static constexpr int FG_U = 4;                    // samples in flight per thread
 
-__global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t nsamp, int64_t nbase,
+template <int NMAP>
+__global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, int64_t nbase,
                                                      const int64_t *bstart, const PixW *pixw, const double *m4,
                                                      int64_t ndeg, const double *dwb, const double *nna,
                                                      const double *p, double *ap)
 {
-    __shared__ double vals[FLAT_TILE];    // vals[d * FG_T + t]
+    __shared__ double vals[FLAT_TILE + 1];    // vals[d * FG_T + t]
     __shared__ int64_t bs[FG_K + 1];      // sample starts of the block's baselines, bs[nk] = end
     int64_t ngrp = (ndet + FG_D - 1) / FG_D;
     int64_t lb = xcd_block(blockIdx.x, gridDim.x);
@@ -892,20 +894,41 @@ __global__ void __launch_bounds__(256) k_gather_flat(int nmap, int ndet, int64_t
     }
     __syncthreads();
     double acc = 0.0;
+    const int dsh = __ffsll((unsigned long long)ndeg) - 1;   // ndeg is a power of 4
     int64_t S0 = bs[0], S1 = bs[nk];
     for (int64_t t0 = S0; t0 < S1; t0 += FG_T) {
         int len = (int)(S1 - t0 < FG_T ? S1 - t0 : FG_T);
-        for (int j = threadIdx.x; j < len * FG_D; j += blockDim.x) {
-            int tl = j / FG_D, d = j % FG_D;
-            double v = 0.0;
-            if (d < nd) {
-                PixW r = pixw[(int64_t)(d0 + d) * nsamp + t0 + tl];
-                if (r.pix >= 0) {
-                    double4 m = *(const double4 *)(m4 + (r.pix / ndeg) * 4);
-                    v = nmap == 1 ? m.x : (double)r.w0 * m.x + (double)r.w1 * m.y + (double)r.w2 * m.z;
-                }
+        int nj = len * FG_D;
+        // FG_U samples per thread per round: their pointing loads, then their
+        // map gathers, all issued before the first wait, from clamped indices
+        // (out-of-range samples read a valid element and are dropped)
+        for (int j0 = threadIdx.x; j0 < nj; j0 += FG_U * blockDim.x) {
+            PixW r[FG_U];
+#pragma unroll
+            for (int u = 0; u < FG_U; ++u) {
+                int j = j0 + u * blockDim.x;
+                j = j < nj ? j : nj - 1;
+                int tl = j / FG_D, d = j % FG_D;
+                d = d < nd ? d : 0;
+                r[u] = pixw[(int64_t)(d0 + d) * nsamp + t0 + tl];
             }
-            vals[d * FG_T + tl] = v;
+            double4 m[FG_U];
+#pragma unroll
+            for (int u = 0; u < FG_U; ++u) {
+                int q = r[u].pix >= 0 ? (r[u].pix >> dsh) : 0;
+                m[u] = *(const double4 *)(m4 + (int64_t)q * 4);
+            }
+            // keeps the loads above from being sunk into the per-sample code below
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int u = 0; u < FG_U; ++u) {
+                int j = j0 + u * blockDim.x;
+                int tl = j / FG_D, d = j % FG_D;
+                double v = NMAP == 1 ? m[u].x
+                                     : (double)r[u].w0 * m[u].x + (double)r[u].w1 * m[u].y + (double)r[u].w2 * m[u].z;
+                v = (d < nd && r[u].pix >= 0) ? v : 0.0;
+                vals[j < nj ? d * FG_T + tl : FLAT_TILE] = v;   // past the end: the spare slot
+            }
         }
         __syncthreads();
         if (own) {
@@ -1117,8 +1140,14 @@ void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_
 {
     int64_t grid = ((ndet + FG_D - 1) / FG_D) * ((nbase + FG_K - 1) / FG_K);
     if (grid > 0)
-        k_gather_flat<<<(unsigned)grid, 256, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p,
-                                                     ap);
+    {
+        if (nmap == 1)
+            k_gather_flat<1><<<(unsigned)grid, 256, 0, s>>>(ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p,
+                                                            ap);
+        else
+            k_gather_flat<3><<<(unsigned)grid, 256, 0, s>>>(ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p,
+                                                            ap);
+    }
 }
 
 void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
