@@ -644,17 +644,26 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *wo
     }
 }
 
-// y[b0 + j] -= mean of the segment's y, from the nblk ordered partials
-__global__ void k_conv_sub_mean(double *y, const Segment *seg, int64_t s0, const double *partial_out, int nblk)
+// mean[b] of the batch's segment b from its nblk ordered partials, then
+// y[b0 + j] -= mean[b] over the segment
+__global__ void k_conv_seg_mean(const Segment *seg, int64_t s0, int64_t nb, const double *partial_out, int nblk,
+                                double *mean)
+{
+    int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    double t = 0;
+    for (int k = 0; k < nblk; ++k) t += partial_out[b * nblk + k];
+    mean[b] = t / seg[s0 + b].noba;
+}
+
+__global__ void k_conv_sub_mean(double *y, const Segment *seg, int64_t s0, const double *mean)
 {
     int64_t b = blockIdx.y;
     Segment sg = seg[s0 + b];
-    double t = 0;
-    for (int k = 0; k < nblk; ++k) t += partial_out[b * nblk + k];
-    double mean = t / sg.noba;
+    double mb = mean[b];
     double *ys = y + sg.b0;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < sg.noba; j += (int64_t)gridDim.x * blockDim.x)
-        ys[j] -= mean;
+        ys[j] -= mb;
 }
 
 // fT[r * L2 + k2] = f[r + L1 * k2], fT[N] = f[N], per spectrum row
@@ -709,9 +718,11 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
             k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
                 p);
+            double *means = partial_out + batch * ncb;   // batch doubles
+            k_conv_seg_mean<<<(unsigned)((nb + 63) / 64), 64, 0, s>>>(dseg, s0, nb, partial_out, ncb, means);
             unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
             if (gl > 1024) gl = 1024;
-            k_conv_sub_mean<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
+            k_conv_sub_mean<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, means);
             continue;
         }
         unsigned gl = (unsigned)((nof + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
