@@ -384,15 +384,18 @@ static void fft_inverse(hipStream_t s, const FftPlan &p, double2 *d, int64_t bat
 // early stages do not collide on the same banks.
 __device__ inline int lp(int idx) { return idx + (idx >> 3); }
 
-// DIT, bit-reversed input order, natural output order; twL[j] = W_L^j (j < L / 2)
-__device__ void lds_fft8(double2 *s, int L, int logL, int ncol, const double2 *twL, bool inv)
+// DIT, bit-reversed input order, natural output order; twL[j] = W_L^j (j < L / 2).
+// 2^logncol interleaved transforms; every length is a power of two, so the
+// index arithmetic is shifts and masks.
+__device__ void lds_fft8(double2 *s, int L, int logL, int logncol, const double2 *twL, bool inv)
 {
+    const int ncol = 1 << logncol;
     for (int st = 0; st < logL; st += 3) {
         int T = logL - st < 3 ? logL - st : 3;
         int h0 = 1 << st, E = 1 << T;
-        int nitems = (L >> T) * ncol;
+        int nitems = (L >> T) << logncol;
         for (int t = threadIdx.x; t < nitems; t += blockDim.x) {
-            int col = t % ncol, it = t / ncol;
+            int col = t & (ncol - 1), it = t >> logncol;
             int j = it & (h0 - 1), grp = it >> st;
             int base = grp * (E * h0) + j;
             double2 a[8];
@@ -402,12 +405,11 @@ __device__ void lds_fft8(double2 *s, int L, int logL, int ncol, const double2 *t
 #pragma unroll
             for (int tt = 0; tt < 3; ++tt) {
                 if (tt >= T) break;
-                int h = h0 << tt;
 #pragma unroll
                 for (int m0 = 0; m0 < 8; ++m0) {
                     if (m0 >= E || (m0 & (1 << tt))) continue;
                     int r = m0 & ((1 << tt) - 1);
-                    double2 w = twL[(j + r * h0) * (L / (2 * h))];
+                    double2 w = twL[(j + r * h0) << (logL - st - tt - 1)];   // * L / (2 h)
                     if (inv) w.y = -w.y;
                     double2 b = cmul(a[m0 + (1 << tt)], w), x = a[m0];
                     a[m0] = make_double2(x.x + b.x, x.y + b.y);
@@ -423,15 +425,16 @@ __device__ void lds_fft8(double2 *s, int L, int logL, int ncol, const double2 *t
 }
 
 // DIF, natural input order, bit-reversed output order
-__device__ void lds_fft8_dif(double2 *s, int L, int logL, int ncol, const double2 *twL, bool inv)
+__device__ void lds_fft8_dif(double2 *s, int L, int logL, int logncol, const double2 *twL, bool inv)
 {
+    const int ncol = 1 << logncol;
     for (int hi = logL - 1; hi >= 0; hi -= 3) {
         int T = hi + 1 < 3 ? hi + 1 : 3;
         int lo = hi - T + 1;
         int h0 = 1 << lo, E = 1 << T;
-        int nitems = (L >> T) * ncol;
+        int nitems = (L >> T) << logncol;
         for (int t = threadIdx.x; t < nitems; t += blockDim.x) {
-            int col = t % ncol, it = t / ncol;
+            int col = t & (ncol - 1), it = t >> logncol;
             int j = it & (h0 - 1), grp = it >> lo;
             int base = grp * (E * h0) + j;
             double2 a[8];
@@ -441,12 +444,11 @@ __device__ void lds_fft8_dif(double2 *s, int L, int logL, int ncol, const double
 #pragma unroll
             for (int tt = 2; tt >= 0; --tt) {
                 if (tt >= T) continue;
-                int h = h0 << tt;
 #pragma unroll
                 for (int m0 = 0; m0 < 8; ++m0) {
                     if (m0 >= E || (m0 & (1 << tt))) continue;
                     int r = m0 & ((1 << tt) - 1);
-                    double2 w = twL[(j + r * h0) * (L / (2 * h))];
+                    double2 w = twL[(j + r * h0) << (logL - lo - tt - 1)];   // * L / (2 h)
                     if (inv) w.y = -w.y;
                     double2 x = a[m0], y = a[m0 + (1 << tt)];
                     a[m0] = make_double2(x.x + y.x, x.y + y.y);
@@ -473,6 +475,7 @@ __device__ void lds_fft8_dif(double2 *s, int L, int logL, int ncol, const double
 //           baselines directly, with deterministic per-block partial sums of
 //           y for the mean, which a last pass subtracts.
 static constexpr int CONV_NCOL = 8;
+static_assert(CONV_NCOL == 8, "the column passes call the LDS FFTs with log2(CONV_NCOL) = 3");
 
 // LDS elements (padded) for n double2
 __host__ __device__ inline int conv_lds_elems(int n) { return n + (n >> 3) + 1; }
@@ -503,7 +506,7 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const double *x, 
         s[lp(bitrev(n1, logL1) * ncol + col)] = v;
     }
     __syncthreads();
-    lds_fft8(s, L1, logL1, ncol, P.sub1, false);
+    lds_fft8(s, L1, logL1, 3, P.sub1, false);   // CONV_NCOL = 8 columns
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
@@ -554,12 +557,12 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
     const double *f = fc + (int64_t)seg[s0 + b].ipsd * fc_stride;
     int tot = L2 * nr;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-        int r = t / L2, n = t % L2;
+        int r = t >> logL2, n = t & (L2 - 1);
         int row = r == 0 ? ra : rb;
         s[lp(bitrev(n, logL2) * nr + r)] = d[(int64_t)row * L2 + n];
     }
     __syncthreads();
-    lds_fft8(s, L2, logL2, nr, P.sub2, false);
+    lds_fft8(s, L2, logL2, nr - 1, P.sub2, false);   // nr = 1 or 2 rows
     // X[ra + L1 k2] is at k2 * nr; X[rb + L1 k2] at k2 * nr + 1
     if (ra == 0) {
         for (int k2 = threadIdx.x; k2 <= L2 / 2; k2 += blockDim.x) {
@@ -587,9 +590,9 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
     }
     __syncthreads();
     // inverse row FFTs from natural order; the DIF output is bit-reversed
-    lds_fft8_dif(s, L2, logL2, nr, P.sub2, true);
+    lds_fft8_dif(s, L2, logL2, nr - 1, P.sub2, true);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-        int r = t / L2, n = t % L2;
+        int r = t >> logL2, n = t & (L2 - 1);
         int row = r == 0 ? ra : rb;
         d[(int64_t)row * L2 + n] = s[lp(bitrev(n, logL2) * nr + r)];
     }
@@ -615,7 +618,7 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *wo
         s[lp(bitrev(k1, logL1) * ncol + col)] = cmulc(d[(int64_t)k1 * L2 + n2], w);
     }
     __syncthreads();
-    lds_fft8(s, L1, logL1, ncol, P.sub1, true);
+    lds_fft8(s, L1, logL1, 3, P.sub1, true);
     double acc = 0;
     double *ys = y + sg.b0;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
