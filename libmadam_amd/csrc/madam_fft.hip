@@ -112,14 +112,14 @@ __global__ void k_fft_col(double2 *data, int64_t N, int L1, int logL1, int L2, i
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
             int col = t % ncol, k1 = t / ncol;
             int64_t n2 = c0 + col;
-            double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
+            double2 w = tw2(P.na, P.nb, (n2 * k1) & (N - 1))   /* N is a power of two */;
             d[(int64_t)k1 * L2 + n2] = cmul(s[k1 * ncol + col], w);
         }
     } else {
         for (int t = threadIdx.x; t < tot; t += blockDim.x) {
             int col = t % ncol, k1 = t / ncol;
             int64_t n2 = c0 + col;
-            double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
+            double2 w = tw2(P.na, P.nb, (n2 * k1) & (N - 1))   /* N is a power of two */;
             s[bitrev(k1, logL1) * ncol + col] = cmulc(d[(int64_t)k1 * L2 + n2], w);
         }
         __syncthreads();
@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const double *x, 
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
-        double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
+        double2 w = tw2(P.na, P.nb, (n2 * k1) & (N - 1))   /* N is a power of two */;
         d[(int64_t)k1 * L2 + n2] = cmul(s[lp(k1 * ncol + col)], w);
     }
 }
@@ -614,7 +614,7 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *wo
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
-        double2 w = tw2(P.na, P.nb, (n2 * k1) % N);
+        double2 w = tw2(P.na, P.nb, (n2 * k1) & (N - 1))   /* N is a power of two */;
         s[lp(bitrev(k1, logL1) * ncol + col)] = cmulc(d[(int64_t)k1 * L2 + n2], w);
     }
     __syncthreads();
