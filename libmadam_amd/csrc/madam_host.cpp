@@ -350,7 +350,7 @@ struct DBuf {
 enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_PRECOND, ST_T_ITER, ST_T_FINAL,
        ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
        ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_NCG,
-       ST_BAND_CHUNKS, ST_BAND_CHUNKS_UNIQUE, ST_BAND_BLOCKS, ST_BAND_BLOCKS_UNIQUE, ST_COUNT };
+       ST_BAND_CHUNKS, ST_BAND_CHUNKS_UNIQUE, ST_BAND_BLOCKS, ST_BAND_BLOCKS_UNIQUE, ST_REC_BYTES, ST_COUNT };
 
 struct Context {
     int device = 0;
@@ -397,6 +397,10 @@ struct Context {
     // bord[slot] = baseline, bslot[baseline] = slot, and the baseline vectors
     // the entries index, in slot order
     DBuf<int32_t> d_bord, d_bslot;
+    // the projections' pointing records in the compact layout rec_layout
+    // (REC12 / REC4), or unused (REC16: d_pixw and d_ent themselves)
+    int rec_layout = 0;
+    DBuf<char> d_pixc, d_entc;
     DBuf<double> d_dwb_s, d_sdwb_s, d_aa_s;
     DBuf<uint32_t> d_src;
     int64_t nactive = 0;
@@ -910,6 +914,35 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
     baseline_slots(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.d_bord.p, c.d_bslot.p);
     csr_build_sorted(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.npix_map, total, c.d_bslot.p,
                      c.d_ent.p, c.d_src.p);
+    // compact records for the projections: temperature-only projections use no
+    // weight (src/smadam_routines.f90:1038-1070), polarised ones drop w0 when
+    // it is 1 for every active sample
+    {
+        int rl = REC16;
+        if (c.nmap == 1) rl = REC4;
+        else {
+            DBuf<unsigned long long> cnt;
+            cnt.alloc(1);
+            MADAM_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
+            launch_count_nonunit_w0(s, c.d_pixw.p, n, cnt.p);
+            unsigned long long h = 0;
+            d2h(&h, cnt.p, 1, s);
+            MADAM_CHECK(hipStreamSynchronize(s));
+            if (h == 0) rl = REC12;
+        }
+        c.rec_layout = rl;
+        c.stats[ST_REC_BYTES] = rec_bytes(rl);
+        if (rl != REC16) {
+            c.d_pixc.alloc((size_t)n * rec_bytes(rl));
+            c.d_entc.alloc((size_t)std::max<int64_t>(total, 1) * rec_bytes(rl));
+            launch_compact_records(s, c.d_pixw.p, n, rl, c.d_pixc.p);
+            launch_compact_records(s, (const PixW *)c.d_ent.p, total, rl, c.d_entc.p);
+            MADAM_CHECK(hipGetLastError());
+        } else {
+            c.d_pixc.release();
+            c.d_entc.release();
+        }
+    }
     MADAM_CHECK(hipGetLastError());
     MADAM_CHECK(hipStreamSynchronize(s));
     c.stats[ST_T_POINTING] = now() - t0;
@@ -1320,7 +1353,8 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         c.pcg_steps_done++;
         if (prof) { t_it.start(s); t_sc.start(s); }
         if (c.kernel_variant == 0 || c.kernel_variant == 3)   // segmented tiles (default)
-            launch_scatter_seg(s, c.nmap, c.nactive, c.d_ent.p, c.d_pw.p, c.sr_s.heads.p, c.sr_s.k0.p,
+            launch_scatter_seg(s, c.nmap, c.nactive, c.rec_layout == REC16 ? (const void *)c.d_ent.p : c.d_entc.p,
+                               c.rec_layout, c.d_pw.p, c.sr_s.heads.p, c.sr_s.k0.p,
                                c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.nzpix, c.d_ccinv_c.p, c.d_m4x.p,
                                c.d_sr_part.p);
         else if (c.kernel_variant == 2)
@@ -1339,8 +1373,9 @@ static int pcg_steps(Context &c, int nmax, bool converge)
                               c.sr_g.k0.p, c.sr_g.fix_end.p, c.sr_g.fix_ord.p, c.d_dwb.p, c.d_nna.p, c.d_p.p,
                               c.d_ap.p, c.d_sr_part.p);
         else if (c.kernel_variant != 1)
-            launch_gather_flat(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
-                               c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p);
+            launch_gather_flat(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p,
+                               c.rec_layout == REC16 ? (const void *)c.d_pixw.p : c.d_pixc.p, c.rec_layout,
+                               c.d_m4x.p, c.ndeg, c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p);
         else
             launch_gather_baselines(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_m4x.p, c.ndeg,
                                     c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p, grp);

@@ -25,6 +25,43 @@ struct PixW {
     float w0, w1, w2;
 };
 
+// Pointing records of the two per-iteration projections.  REC16 is the full
+// record {index, w0, w1, w2}; when every active sample has an intensity weight
+// of exactly 1 (the (1, cos 2psi, sin 2psi) convention), REC12 {index, w1, w2}
+// and, for temperature-only maps, REC4 {index} carry the same information in
+// fewer bytes (1.0f * x == x, so the results are the same bits).
+enum RecLayout { REC16 = 0, REC12 = 1, REC4 = 2 };
+struct Rec12 {
+    int32_t i;
+    float w1, w2;
+};
+inline int rec_bytes(int rl) { return rl == REC16 ? 16 : (rl == REC12 ? 12 : 4); }
+// number of active records (index >= 0) whose w0 is not exactly 1; *count must start at 0
+void launch_count_nonunit_w0(hipStream_t s, const PixW *rec, int64_t n, unsigned long long *count);
+// compact copy of 16-byte records (PixW or CsrEnt) in layout rl
+void launch_compact_records(hipStream_t s, const PixW *rec, int64_t n, int rl, void *out);
+
+// record i of a record array in layout RL, as a full record (w0 = 1 for the compact layouts)
+template <int RL>
+__device__ inline PixW load_rec(const void *base, int64_t i)
+{
+    PixW r;
+    if constexpr (RL == REC16) {
+        r = ((const PixW *)base)[i];
+    } else if constexpr (RL == REC12) {
+        Rec12 c = ((const Rec12 *)base)[i];
+        r.pix = c.i;
+        r.w0 = 1.0f;
+        r.w1 = c.w1;
+        r.w2 = c.w2;
+    } else {
+        r.pix = ((const int32_t *)base)[i];
+        r.w0 = 1.0f;
+        r.w1 = r.w2 = 0.0f;
+    }
+    return r;
+}
+
 struct CsrEnt {
     int32_t b;   // global baseline index
     float w0, w1, w2;
@@ -88,9 +125,10 @@ void launch_scatter_pixels(hipStream_t s, int nmap, int64_t npix_cross, int64_t 
 void launch_gather_baselines(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase,
                              const int64_t *bstart, const PixW *pixw, const double *m4, int64_t ndeg,
                              const double *dwb, const double *nna, const double *p, double *ap, int group);
+// rec: the pointing records in layout rl (PixW for REC16)
 void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
-                        const PixW *pixw, const double *m4, int64_t ndeg, const double *dwb, const double *nna,
-                        const double *p, double *ap);
+                        const void *rec, int rl, const double *m4, int64_t ndeg, const double *dwb,
+                        const double *nna, const double *p, double *ap);
 void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4);
 // entry-balanced pixel-side projection: tinfo from launch_tile_info (once per
@@ -110,7 +148,8 @@ void segred_pixel_segments(hipStream_t s, const int64_t *off, int64_t npc, int64
                            int64_t *scan_tmp, int64_t nchunks, int32_t *nzq, int64_t *segstart);
 void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbase, int ndet, int64_t nsamp,
                               int64_t *segstart);
-void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const CsrEnt *ent, const double *pw,
+// ent: the CSR entries in layout rl (CsrEnt for REC16)
+void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const void *ent, int rl, const double *pw,
                         const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
                         const int32_t *nzq, int64_t nzpix, const double *cc_c, double *m4, double4 *part);
 // cc_c[k] = ccinv6[nzq[k]]: the inverse pixel matrices in the pixel side's ordinal order

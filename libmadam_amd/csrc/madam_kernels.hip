@@ -840,6 +840,45 @@ void launch_scatter_pixels(hipStream_t s, int nmap, int64_t npix_cross, int64_t 
     k_scatter_pixels<<<grid, TPB, 0, s>>>(nmap, npix_cross, ndeg, off, ent, pw, ccinv6, m4);
 }
 
+// ---------------------------------------------------------------- pointing records
+
+__global__ void k_count_nonunit_w0(const PixW *rec, int64_t n, unsigned long long *count)
+{
+    unsigned long long c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        PixW r = rec[i];
+        c += (r.pix >= 0 && r.w0 != 1.0f) ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+void launch_count_nonunit_w0(hipStream_t s, const PixW *rec, int64_t n, unsigned long long *count)
+{
+    if (n > 0) k_count_nonunit_w0<<<grid_for(n, TPB), TPB, 0, s>>>(rec, n, count);
+}
+
+__global__ void k_compact_records(const PixW *rec, int64_t n, int rl, void *out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        PixW r = rec[i];
+        if (rl == REC12) {
+            Rec12 c;
+            c.i = r.pix;
+            c.w1 = r.w1;
+            c.w2 = r.w2;
+            ((Rec12 *)out)[i] = c;
+        } else {
+            ((int32_t *)out)[i] = r.pix;
+        }
+    }
+}
+
+void launch_compact_records(hipStream_t s, const PixW *rec, int64_t n, int rl, void *out)
+{
+    if (n > 0) k_compact_records<<<grid_for(n, TPB), TPB, 0, s>>>(rec, n, rl, out);
+}
+
 // ---------------------------------------------------------------- flat tiles
 // Flat variants of the two per-iteration projections: every thread of a block
 // issues coalesced loads for a tile of the stream (maximum memory-level
@@ -864,9 +903,9 @@ static constexpr int FG_K = FLAT_ITEMS / FG_D;    // baselines per detector per 
 static constexpr int FG_T = FLAT_TILE / FG_D;     // samples per detector per LDS tile
 static constexpr int FG_U = 4;                    // samples in flight per thread
 
-template <int NMAP>
+template <int NMAP, int RL>
 __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, int64_t nbase,
-                                                     const int64_t *bstart, const PixW *pixw, const double *m4,
+                                                     const int64_t *bstart, const void *pixw, const double *m4,
                                                      int64_t ndeg, const double *dwb, const double *nna,
                                                      const double *p, double *ap)
 {
@@ -910,7 +949,7 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
                 j = j < nj ? j : nj - 1;
                 int tl = j / FG_D, d = j % FG_D;
                 d = d < nd ? d : 0;
-                r[u] = pixw[(int64_t)(d0 + d) * nsamp + t0 + tl];
+                r[u] = load_rec<RL>(pixw, (int64_t)(d0 + d) * nsamp + t0 + tl);
             }
             double4 m[FG_U];
 #pragma unroll
@@ -1135,18 +1174,22 @@ size_t scatter_tiles_scratch_bytes(int64_t nent)
 }
 
 void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
-                        const PixW *pixw, const double *m4, int64_t ndeg, const double *dwb, const double *nna,
-                        const double *p, double *ap)
+                        const void *pixw, int rl, const double *m4, int64_t ndeg, const double *dwb,
+                        const double *nna, const double *p, double *ap)
 {
     int64_t grid = ((ndet + FG_D - 1) / FG_D) * ((nbase + FG_K - 1) / FG_K);
     if (grid > 0)
     {
-        if (nmap == 1)
-            k_gather_flat<1><<<(unsigned)grid, 256, 0, s>>>(ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p,
-                                                            ap);
-        else
-            k_gather_flat<3><<<(unsigned)grid, 256, 0, s>>>(ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p,
-                                                            ap);
+#define MADAM_GF(NM, RLV)                                                                                        \
+    k_gather_flat<NM, RLV><<<(unsigned)grid, 256, 0, s>>>(ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p, ap)
+        if (nmap == 1) {
+            if (rl == REC4) MADAM_GF(1, REC4);
+            else MADAM_GF(1, REC16);
+        } else {
+            if (rl == REC12) MADAM_GF(3, REC12);
+            else MADAM_GF(3, REC16);
+        }
+#undef MADAM_GF
     }
 }
 

@@ -209,8 +209,8 @@ __device__ inline unsigned sr_head_bits(const uint8_t *heads, int64_t e0, int64_
 // (compact ordinal order) and the pixel index that lane l loaded before the
 // reduction.  Only a tile that finishes more than 64 pixels reads the
 // finishing data of the rest where they complete.
-template <int NC, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(7, 8))) k_sr_scatter(int64_t nent, const CsrEnt *ent, const double *pw,
+template <int NC, int WAVES, int RL>
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(7, 8))) k_sr_scatter(int64_t nent, const void *ent, const double *pw,
                                                            const uint8_t *heads, const int64_t *tile_k0,
                                                            const int64_t *fix_end, const int32_t *nzq,
                                                            int64_t nzpix, const double *cc_c, double *m4,
@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * WAVES + w;
     int64_t t0 = j * SR_TILE;
     if (t0 >= nent) return;   // uniform over the wave; waves synchronise only with themselves
-    CsrEnt e[SR_PER_LANE];
+    PixW e[SR_PER_LANE];   // the entries as full records: .pix is the baseline slot
     double x[SR_PER_LANE];
     int64_t k0 = 0;
     bool head0 = false, cont_last = false;
@@ -238,12 +238,12 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
             int64_t idx = t0 + u * 64 + lane;
-            e[u] = ent[idx < nent ? idx : nent - 1];
+            e[u] = load_rec<RL>(ent, idx < nent ? idx : nent - 1);
         }
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
-            bool ok = t0 + u * 64 + lane < nent && e[u].b >= 0;
-            double v = pw[ok ? e[u].b : 0];
+            bool ok = t0 + u * 64 + lane < nent && e[u].pix >= 0;
+            double v = pw[ok ? e[u].pix : 0];
             x[u] = ok ? v : 0.0;
         }
     }
@@ -512,19 +512,24 @@ void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbas
 
 // ------------------------------------------------------------ launchers
 
-void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const CsrEnt *ent, const double *pw,
+void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const void *ent, int rl, const double *pw,
                         const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
                         const int32_t *nzq, int64_t nzpix, const double *cc_c, double *m4, double4 *part)
 {
     int64_t nt = segred_ntiles(nent);
     if (nt == 0) return;
     unsigned grid = (unsigned)((nt + SR_WAVES - 1) / SR_WAVES);
-    if (nmap == 1)
-        k_sr_scatter<1, SR_WAVES><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, nzpix, cc_c,
-                                                          m4, part);
-    else
-        k_sr_scatter<3, SR_WAVES><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, nzpix, cc_c,
-                                                          m4, part);
+#define MADAM_SC(NCV, RLV)                                                                                       \
+    k_sr_scatter<NCV, SR_WAVES, RLV><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, nzpix,      \
+                                                              cc_c, m4, part)
+    if (nmap == 1) {
+        if (rl == REC4) MADAM_SC(1, REC4);
+        else MADAM_SC(1, REC16);
+    } else {
+        if (rl == REC12) MADAM_SC(3, REC12);
+        else MADAM_SC(3, REC16);
+    }
+#undef MADAM_SC
     k_sr_scatter_fix<<<sr_grid(nt, 1 << 30), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part, cc_c, m4);
 }
 

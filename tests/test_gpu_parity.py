@@ -115,6 +115,26 @@ def test_noise_prior_band_preconditioner():
     check_parity(g, o, c1, c2)
 
 
+def test_pointing_record_layouts():
+    """The projections read 12-byte records {index, w1, w2} when every
+    intensity weight is exactly 1, 4-byte records for temperature-only maps,
+    and the full 16-byte records otherwise (here a per-sample gain on all three
+    weights); each layout against the oracle."""
+    g, o, c1, c2 = both(ring(prior=True, nsamp=30000, nperiod=2))
+    assert g["stats"]["rec_bytes"] == 12
+    check_parity(g, o, c1, c2)
+    g, o, c1, c2 = both(ring(prior=True, nsamp=30000, nperiod=2, nnz=1))
+    assert g["stats"]["rec_bytes"] == 4
+    check_parity(g, o, c1, c2)
+    case = ring(prior=True, nsamp=30000, nperiod=2)
+    w = case["pixweights"].reshape(-1, 3)
+    gain = 1.0 + 0.05 * np.random.default_rng(5).standard_normal((w.shape[0], 1))
+    w[:] = (w * gain).astype(np.float32)
+    g, o, c1, c2 = both(case)
+    assert g["stats"]["rec_bytes"] == 16
+    check_parity(g, o, c1, c2)
+
+
 @pytest.mark.parametrize("variant", [1, 2, 3])
 def test_noise_prior_band_preconditioner_other_kernels(variant):
     """the alternative projection kernels: 1 wave per pixel / lane group per
