@@ -941,21 +941,29 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
         // FG_U samples per thread per round: their pointing loads, then their
         // map gathers, all issued before the first wait, from clamped indices
         // (out-of-range samples read a valid element and are dropped)
-        for (int j0 = threadIdx.x; j0 < nj; j0 += FG_U * blockDim.x) {
+        // a slot whose whole wave lies past the end issues no loads (the
+        // address path costs per instruction, whatever the exec mask)
+        for (int j0 = threadIdx.x; j0 - (threadIdx.x & 63) < nj; j0 += FG_U * blockDim.x) {
+            // (a skipped slot's registers stay undefined: its products go to the spare slot)
+            const int jb = __builtin_amdgcn_readfirstlane(j0 - (int)(threadIdx.x & 63));
             PixW r[FG_U];
-#pragma unroll
-            for (int u = 0; u < FG_U; ++u) {
-                int j = j0 + u * blockDim.x;
-                j = j < nj ? j : nj - 1;
-                int tl = j / FG_D, d = j % FG_D;
-                d = d < nd ? d : 0;
-                r[u] = load_rec<RL>(pixw, (int64_t)(d0 + d) * nsamp + t0 + tl);
-            }
             double4 m[FG_U];
 #pragma unroll
             for (int u = 0; u < FG_U; ++u) {
-                int q = r[u].pix >= 0 ? (r[u].pix >> dsh) : 0;
-                m[u] = *(const double4 *)(m4 + (int64_t)q * 4);
+                if (jb + u * (int)blockDim.x < nj) {   // uniform over the wave
+                    int j = j0 + u * blockDim.x;
+                    j = j < nj ? j : nj - 1;
+                    int tl = j / FG_D, d = j % FG_D;
+                    d = d < nd ? d : 0;
+                    r[u] = load_rec<RL>(pixw, (int64_t)(d0 + d) * nsamp + t0 + tl);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < FG_U; ++u) {
+                if (jb + u * (int)blockDim.x < nj) {
+                    int q = r[u].pix >= 0 ? (r[u].pix >> dsh) : 0;
+                    m[u] = *(const double4 *)(m4 + (int64_t)q * 4);
+                }
             }
             // keeps the loads above from being sunk into the per-sample code below
             asm volatile("" ::: "memory");
