@@ -191,6 +191,14 @@ __device__ inline int sr_col(int u, int lane) { return sr_pad(u * 64 + lane); }
 __device__ inline int sr_row(int u, int lane) { return sr_pad(lane * SR_PER_LANE + u); }
 
 // head bits of the lane's SR_PER_LANE entries starting at e0 (a multiple of SR_PER_LANE)
+// Per-tile metadata in one int64: the ordinal of the segment holding the
+// tile's first entry (bits 0-60), whether that entry is a segment head (bit
+// 61) and whether the segment that starts last in the tile continues into the
+// next tile (bit 62).
+__host__ __device__ inline int64_t sr_meta_k0(int64_t m) { return m & ((1ll << 61) - 1); }
+__host__ __device__ inline bool sr_meta_head0(int64_t m) { return (m >> 61) & 1; }
+__host__ __device__ inline bool sr_meta_cont(int64_t m) { return (m >> 62) & 1; }
+
 // (the head bytes are padded to whole tiles, so the load is always in bounds)
 __device__ inline unsigned sr_head_bits(const uint8_t *heads, int64_t e0, int64_t nent)
 {
@@ -230,9 +238,10 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     {
         // tile metadata first: these loads are independent of the entry stream
         hb = sr_head_bits(heads, e0, nent);
-        k0 = tile_k0[j];
-        head0 = (heads[t0 / 8] & 1u) != 0;
-        cont_last = fix_end[j] >= 0;
+        int64_t meta = tile_k0[j];   // one load: ordinal and the two flags (sr_meta_*)
+        k0 = sr_meta_k0(meta);
+        head0 = sr_meta_head0(meta);
+        cont_last = sr_meta_cont(meta);
         // unconditional loads from clamped indices, so that they all issue
         // before the first wait; entries past the end are neutralised after
 #pragma unroll
@@ -386,9 +395,10 @@ __global__ void __launch_bounds__(SR_TPB) k_sr_gather(int64_t nent, const PixW *
     double prod[SR_PER_LANE][1];
 #pragma unroll
     for (int u = 0; u < SR_PER_LANE; ++u) prod[u][0] = tr[w][sr_row(u, lane)];
-    int64_t k0 = tile_k0[j];
-    bool head0 = (heads[t0 / 8] & 1u) != 0;
-    bool cont_last = fix_end[j] >= 0;
+    int64_t meta = tile_k0[j];
+    int64_t k0 = sr_meta_k0(meta);
+    bool head0 = sr_meta_head0(meta);
+    bool cont_last = sr_meta_cont(meta);
     sr_reduce_tile<1>(
         prod, hb, lane, k0, head0, cont_last,
         [&](int64_t b, const double *s) { ap[b] = ap[b] + (nna[b] * p[b] - s[0] * dwb[b]); },
@@ -434,6 +444,16 @@ __global__ void k_sr_tiles(const int64_t *segstart, int64_t nseg, int64_t *tile_
     }
 }
 
+__global__ void k_sr_pack_meta(int64_t nt, const uint8_t *heads, const int64_t *fix_end, int64_t *tile_k0)
+{
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nt; j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t m = tile_k0[j];
+        if (heads[j * (SR_TILE / 8)] & 1u) m |= 1ll << 61;
+        if (fix_end[j] >= 0) m |= 1ll << 62;
+        tile_k0[j] = m;
+    }
+}
+
 __global__ void k_sr_fill_i64(int64_t *x, int64_t v, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -462,6 +482,7 @@ void segred_build(hipStream_t s, const int64_t *segstart, int64_t nseg, int64_t 
         k_sr_heads<<<sr_grid(nseg), SR_TPB, 0, s>>>(segstart, nseg, (uint32_t *)heads);
         k_sr_tiles<<<sr_grid(nseg), SR_TPB, 0, s>>>(segstart, nseg, tile_k0, fix_end, fix_ord);
     }
+    k_sr_pack_meta<<<sr_grid(nt), SR_TPB, 0, s>>>(nt, heads, fix_end, tile_k0);
     MADAM_CHECK(hipGetLastError());
 }
 
