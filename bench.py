@@ -173,8 +173,9 @@ def main():
     npix = 12 * nside * nside
     nbtot = ndet * ((nsamp // nperiod - 1) // int(round(base * fs)) + 1) * nperiod
     nact = st["nactive"]
-    bytes_gather = N * 16 + npix * 24 + nbtot * 5 * 8
-    bytes_scatter = nact * 16 + nbtot * 8 + npix * (8 + 48 + 32)
+    rec = st.get("rec_bytes") or 16   # bytes per pointing record the projections stream (16, 12 or 4)
+    bytes_gather = N * rec + npix * 24 + nbtot * 5 * 8
+    bytes_scatter = nact * rec + nbtot * 8 + npix * (8 + 48 + 32)
     t_g = st["t_gather"] / max(st["n_gather"], 1)
     t_s = st["t_scatter"] / max(st["n_scatter"], 1)
     kern = {"gather_baselines": dict(avg_ms=t_g * 1e3, alg_bytes=bytes_gather, gbs=bytes_gather / t_g / 1e9),
