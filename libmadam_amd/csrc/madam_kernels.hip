@@ -937,44 +937,72 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
     int64_t S0 = bs[0], S1 = bs[nk];
     for (int64_t t0 = S0; t0 < S1; t0 += FG_T) {
         int len = (int)(S1 - t0 < FG_T ? S1 - t0 : FG_T);
-        int nj = len * FG_D;
-        // FG_U samples per thread per round: their pointing loads, then their
-        // map gathers, all issued before the first wait, from clamped indices
-        // (out-of-range samples read a valid element and are dropped)
-        // a slot whose whole wave lies past the end issues no loads (the
-        // address path costs per instruction, whatever the exec mask)
-        for (int j0 = threadIdx.x; j0 - (threadIdx.x & 63) < nj; j0 += FG_U * blockDim.x) {
-            // (a skipped slot's registers stay undefined: its products go to the spare slot)
-            const int jb = __builtin_amdgcn_readfirstlane(j0 - (int)(threadIdx.x & 63));
-            PixW r[FG_U];
-            double4 m[FG_U];
+        // one time sample per lane, all FG_D detectors of the block: the
+        // pointing records (one contiguous row per detector), then the map
+        // entry of each detector, loaded only when its pixel differs from the
+        // earlier detectors' (co-pointed detectors share one load).  A wave
+        // whose samples all lie past the end issues no loads (the address
+        // path costs per instruction, whatever the exec mask).
+        for (int tt0 = threadIdx.x; tt0 - (int)(threadIdx.x & 63) < len; tt0 += blockDim.x) {
+            const int tb = __builtin_amdgcn_readfirstlane(tt0 - (int)(threadIdx.x & 63));
+            if (tb >= len) break;   // uniform over the wave
+            int tl = tt0 < len ? tt0 : len - 1;
+            PixW r[FG_D];
 #pragma unroll
-            for (int u = 0; u < FG_U; ++u) {
-                if (jb + u * (int)blockDim.x < nj) {   // uniform over the wave
-                    int j = j0 + u * blockDim.x;
-                    j = j < nj ? j : nj - 1;
-                    int tl = j / FG_D, d = j % FG_D;
-                    d = d < nd ? d : 0;
-                    r[u] = load_rec<RL>(pixw, (int64_t)(d0 + d) * nsamp + t0 + tl);
-                }
+            for (int d = 0; d < FG_D; ++d)
+                r[d] = load_rec<RL>(pixw, (int64_t)(d0 + (d < nd ? d : 0)) * nsamp + t0 + tl);
+            int q[FG_D];
+            bool ld[FG_D];
+#pragma unroll
+            for (int d = 0; d < FG_D; ++d) {
+                q[d] = r[d].pix >= 0 ? (r[d].pix >> dsh) : 0;
+                bool seen = false;
+#pragma unroll
+                for (int e = 0; e < d; ++e) seen = seen || q[e] == q[d];
+                ld[d] = !seen;
             }
+            double2 mxy[FG_D];   // (m.x, m.y); m.y unused for NMAP = 1
+            double mz[FG_D];
 #pragma unroll
-            for (int u = 0; u < FG_U; ++u) {
-                if (jb + u * (int)blockDim.x < nj) {
-                    int q = r[u].pix >= 0 ? (r[u].pix >> dsh) : 0;
-                    m[u] = *(const double4 *)(m4 + (int64_t)q * 4);
+            for (int d = 0; d < FG_D; ++d) {
+                if (ld[d]) {
+                    const double *mp = m4 + (int64_t)q[d] * 4;
+                    if (NMAP == 1) {
+                        mxy[d].x = mp[0];
+                    } else {
+                        mxy[d] = *(const double2 *)mp;
+                        mz[d] = mp[2];
+                    }
                 }
             }
             // keeps the loads above from being sunk into the per-sample code below
             asm volatile("" ::: "memory");
 #pragma unroll
-            for (int u = 0; u < FG_U; ++u) {
-                int j = j0 + u * blockDim.x;
-                int tl = j / FG_D, d = j % FG_D;
-                double v = NMAP == 1 ? m[u].x
-                                     : (double)r[u].w0 * m[u].x + (double)r[u].w1 * m[u].y + (double)r[u].w2 * m[u].z;
-                v = (d < nd && r[u].pix >= 0) ? v : 0.0;
-                vals[j < nj ? d * FG_T + tl : FLAT_TILE] = v;   // past the end: the spare slot
+            for (int d = 1; d < FG_D; ++d) {
+                if (!ld[d]) {
+                    int src = 0;
+#pragma unroll
+                    for (int e = 1; e < d; ++e)
+                        if (q[e] == q[d]) src = e;
+                    double2 cxy = mxy[0];
+                    double cz = mz[0];
+#pragma unroll
+                    for (int e = 1; e < d; ++e)
+                        if (src == e) {
+                            cxy = mxy[e];
+                            cz = mz[e];
+                        }
+                    mxy[d] = cxy;
+                    mz[d] = cz;
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < FG_D; ++d) {
+                double v = NMAP == 1 ? mxy[d].x
+                                     : (double)r[d].w0 * mxy[d].x + (double)r[d].w1 * mxy[d].y +
+                                           (double)r[d].w2 * mz[d];
+                v = (d < nd && r[d].pix >= 0) ? v : 0.0;
+                vals[tt0 < len ? d * FG_T + tt0 : FLAT_TILE] = v;   // past the end: the spare slot
             }
         }
         __syncthreads();
