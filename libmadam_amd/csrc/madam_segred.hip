@@ -48,25 +48,67 @@ __device__ inline int64_t sr_xcd_block(int64_t bid, int64_t nblk)
     return (bid % NX) * per + bid / NX;
 }
 
+// Lane shifts through DPP (VALU, no LDS round trip); a lane whose source is
+// out of range, or whose row is masked off, gets 0, the identity of both
+// scans below.
+template <int CTRL, int ROWMASK>
+__device__ inline int sr_dpp(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWMASK, 0xf, true);
+}
+template <int CTRL, int ROWMASK>
+__device__ inline double sr_dpp(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROWMASK, 0xf, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROWMASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+static constexpr int DPP_ROW_SHR = 0x110;   // + n: row_shr:n
+static constexpr int DPP_WAVE_SHR1 = 0x138, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+// one step of the segmented inclusive scan: (f, v) = (fo, vo) (+) (f, v)
+template <int NC, int CTRL, int ROWMASK>
+__device__ inline void sr_seg_step(double (&v)[NC], int &f)
+{
+    double o[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) o[c] = sr_dpp<CTRL, ROWMASK>(v[c]);
+    int of = sr_dpp<CTRL, ROWMASK>(f);
+    if (!f) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] = o[c] + v[c];
+    }
+    f |= of;
+}
+
+// Segmented inclusive scan over the 64 lanes with the operator
+// (f1, v1) + (f2, v2) = (f1 | f2, f2 ? v2 : v1 + v2): shifts by 1, 2, 4, 8
+// inside rows of 16 lanes, then the row totals across rows (row_bcast:15 into
+// rows 1 and 3, row_bcast:31 into rows 2 and 3).
 template <int NC>
 __device__ inline void sr_seg_scan(double (&v)[NC], bool head, int lane)
 {
-    // inclusive scan with the operator (f1, v1) + (f2, v2) = (f1 | f2, f2 ? v2 : v1 + v2)
+    (void)lane;
     int f = head ? 1 : 0;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        double o[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) o[c] = __shfl_up(v[c], d, 64);
-        int of = __shfl_up(f, d, 64);
-        if (lane >= d) {
-            if (!f) {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) v[c] += o[c];
-            }
-            f |= of;
-        }
-    }
+    sr_seg_step<NC, DPP_ROW_SHR + 1, 0xf>(v, f);
+    sr_seg_step<NC, DPP_ROW_SHR + 2, 0xf>(v, f);
+    sr_seg_step<NC, DPP_ROW_SHR + 4, 0xf>(v, f);
+    sr_seg_step<NC, DPP_ROW_SHR + 8, 0xf>(v, f);
+    sr_seg_step<NC, DPP_ROW_BCAST15, 0xa>(v, f);
+    sr_seg_step<NC, DPP_ROW_BCAST31, 0xc>(v, f);
+}
+
+// inclusive sum over the 64 lanes, same pattern
+__device__ inline int sr_scan_sum(int h)
+{
+    h += sr_dpp<DPP_ROW_SHR + 1, 0xf>(h);
+    h += sr_dpp<DPP_ROW_SHR + 2, 0xf>(h);
+    h += sr_dpp<DPP_ROW_SHR + 4, 0xf>(h);
+    h += sr_dpp<DPP_ROW_SHR + 8, 0xf>(h);
+    h += sr_dpp<DPP_ROW_BCAST15, 0xa>(h);
+    h += sr_dpp<DPP_ROW_BCAST31, 0xc>(h);
+    return h;
 }
 
 // Reduce one tile.  prod[u] are the lane's entry contributions (zero past the
@@ -80,14 +122,9 @@ __device__ inline int sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsi
                                       bool head0, bool cont_last, Fin fin, Part part)
 {
     int nh = __popc(hb);
-    int H = nh;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int o = __shfl_up(H, d, 64);
-        if (lane >= d) H += o;
-    }
+    int H = sr_scan_sum(nh);
     int Hex = H - nh;                    // heads in the lanes before this one
-    int Htot = __shfl(H, 63, 64);
+    int Htot = __builtin_amdgcn_readlane(H, 63);
     int64_t kb = k0 - (head0 ? 1 : 0);   // ordinal of a head h = kb + #heads in [t0, h]
     double pre[NC], cur[NC];
 #pragma unroll
@@ -117,12 +154,9 @@ __device__ inline int sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsi
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = hk ? cur[c] : pre[c];
     sr_seg_scan<NC>(v, hk > 0, lane);
-    double sprev[NC];
+    double sprev[NC];   // the previous lane's scan value (0 in lane 0)
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        sprev[c] = __shfl_up(v[c], 1, 64);
-        if (lane == 0) sprev[c] = 0.0;
-    }
+    for (int c = 0; c < NC; ++c) sprev[c] = sr_dpp<DPP_WAVE_SHR1, 0xf>(v[c]);
     if (hk > 0) {
         int firstu = __ffs(hb) - 1;
         if (lane > 0 || firstu > 0) {   // the segment that ends before this lane's first head
