@@ -556,14 +556,17 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
     double2 *d = work + b * N;
     const double *f = fc + (int64_t)seg[s0 + b].ipsd * fc_stride;
     int tot = L2 * nr;
+    // natural order in and out of LDS (no bank conflicts): the forward row
+    // FFTs are DIF (bit-reversed output), the inverse ones DIT (bit-reversed
+    // input), and the spectral step addresses the bit-reversed positions
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t >> logL2, n = t & (L2 - 1);
         int row = r == 0 ? ra : rb;
-        s[lp(bitrev(n, logL2) * nr + r)] = d[(int64_t)row * L2 + n];
+        s[lp(n * nr + r)] = d[(int64_t)row * L2 + n];
     }
     __syncthreads();
-    lds_fft8(s, L2, logL2, nr - 1, P.sub2, false);   // nr = 1 or 2 rows
-    // X[ra + L1 k2] is at k2 * nr; X[rb + L1 k2] at k2 * nr + 1
+    lds_fft8_dif(s, L2, logL2, nr - 1, P.sub2, false);   // nr = 1 or 2 rows
+    // X[ra + L1 k2] is at bitrev(k2) * nr; X[rb + L1 k2] at bitrev(k2) * nr + 1
     if (ra == 0) {
         for (int k2 = threadIdx.x; k2 <= L2 / 2; k2 += blockDim.x) {
             if (k2 == 0) {
@@ -572,29 +575,29 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
                 double y0 = x0 * f[0], yn = xn * f[N];   // f[0] at [0]
                 s[lp(0)] = make_double2(y0 + yn, y0 - yn);
             } else {
-                conv_pair(s, lp(k2), lp(L2 - k2), (int64_t)L1 * k2, N, P, f[k2], f[L2 - k2]);
+                conv_pair(s, lp(bitrev(k2, logL2)), lp(bitrev(L2 - k2, logL2)), (int64_t)L1 * k2, N, P, f[k2],
+                          f[L2 - k2]);
             }
         }
     } else if (nr == 1) {   // ra == L1 / 2
         for (int k2 = threadIdx.x; k2 < L2 / 2; k2 += blockDim.x)
-            conv_pair(s, lp(k2), lp(L2 - 1 - k2), (int64_t)ra + (int64_t)L1 * k2, N, P, f[(int64_t)ra * L2 + k2],
-                      f[(int64_t)ra * L2 + L2 - 1 - k2]);
+            conv_pair(s, lp(bitrev(k2, logL2)), lp(bitrev(L2 - 1 - k2, logL2)), (int64_t)ra + (int64_t)L1 * k2, N,
+                      P, f[(int64_t)ra * L2 + k2], f[(int64_t)ra * L2 + L2 - 1 - k2]);
     } else {
         for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
             int64_t kA = (int64_t)ra + (int64_t)L1 * k2;
-            int pp = lp(k2 * 2), qq = lp((L2 - 1 - k2) * 2 + 1);
+            int pp = lp(bitrev(k2, logL2) * 2), qq = lp(bitrev(L2 - 1 - k2, logL2) * 2 + 1);
             double fA = f[(int64_t)ra * L2 + k2], fB = f[(int64_t)rb * L2 + L2 - 1 - k2];   // f[kA], f[N - kA]
             if (kA <= N / 2) conv_pair(s, pp, qq, kA, N, P, fA, fB);
             else conv_pair(s, qq, pp, N - kA, N, P, fB, fA);
         }
     }
     __syncthreads();
-    // inverse row FFTs from natural order; the DIF output is bit-reversed
-    lds_fft8_dif(s, L2, logL2, nr - 1, P.sub2, true);
+    lds_fft8(s, L2, logL2, nr - 1, P.sub2, true);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t >> logL2, n = t & (L2 - 1);
         int row = r == 0 ? ra : rb;
-        d[(int64_t)row * L2 + n] = s[lp(bitrev(n, logL2) * nr + r)];
+        d[(int64_t)row * L2 + n] = s[lp(n * nr + r)];
     }
 }
 
