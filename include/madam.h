@@ -74,9 +74,11 @@ int madam_set_profiling(void *ctx, int on);
  * madam_solve are not refreshed */
 int madam_iterate(void *ctx, int niter, int restart);
 int madam_reset_timers(void *ctx);
-/* projection kernels: 0 = segmented 512-entry tiles on the pixel side and LDS
- * tiles on the baseline side (default), 1 = wave per pixel / lane group per
- * baseline, 2 = LDS tiles on both sides, 3 = segmented tiles on both sides */
+/* projection kernels (low 4 bits): 0 = segmented 256-entry tiles on the pixel
+ * side and LDS tiles on the baseline side (default), 1 = wave per pixel / lane
+ * group per baseline, 2 = LDS tiles on both sides, 3 = segmented tiles on both
+ * sides; + 16 = band factorization computes every row (no periodic
+ * continuation of repeating 64-row blocks) */
 int madam_set_kernel_variant(void *ctx, int variant);
 /* pointing expansion of the synthetic ring scan: scan5 = {fsample, spin_period,
  * prec_period, opening_turns, lon0_turns}, dets3[ndet][3] = {psi_turns,

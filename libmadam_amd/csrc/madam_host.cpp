@@ -350,7 +350,8 @@ struct DBuf {
 enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_PRECOND, ST_T_ITER, ST_T_FINAL,
        ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
        ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_NCG,
-       ST_BAND_CHUNKS, ST_BAND_CHUNKS_UNIQUE, ST_BAND_BLOCKS, ST_BAND_BLOCKS_UNIQUE, ST_REC_BYTES, ST_COUNT };
+       ST_BAND_CHUNKS, ST_BAND_CHUNKS_UNIQUE, ST_BAND_BLOCKS, ST_BAND_BLOCKS_UNIQUE, ST_REC_BYTES,
+       ST_BAND_ROWS_FACTORED, ST_COUNT };
 
 struct Context {
     int device = 0;
@@ -1257,7 +1258,7 @@ static void build_preconditioner(Context &c)
     for (;;) {
         std::vector<int> info(c.seg.size(), 0);
         if (band_build(s, c.band, c.seg.data(), c.d_seg.p, (int64_t)c.seg.size(), nband.data(), c.d_invcov.p, c.nof,
-                       c.d_nna.p, info.data()) != 0)
+                       c.d_nna.p, info.data(), !(c.kernel_variant & 16)) != 0)
             fail("band preconditioner wider than 50 baselines is not supported on the GPU path");
         c.band_ready = true;
         bool retry = false;
@@ -1284,6 +1285,7 @@ static void build_preconditioner(Context &c)
     c.stats[ST_BAND_CHUNKS_UNIQUE] = (double)c.band.nchunk_unique;
     c.stats[ST_BAND_BLOCKS] = (double)c.band.nblock;
     c.stats[ST_BAND_BLOCKS_UNIQUE] = (double)c.band.nblock_unique;
+    c.stats[ST_BAND_ROWS_FACTORED] = (double)c.band.nrows_factored;
     c.segcg.clear();
     c.ic0cg.clear();
     for (size_t is = 0; is < c.seg.size(); ++is)
@@ -1352,12 +1354,12 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         done++;
         c.pcg_steps_done++;
         if (prof) { t_it.start(s); t_sc.start(s); }
-        if (c.kernel_variant == 0 || c.kernel_variant == 3)   // segmented tiles (default)
+        if ((c.kernel_variant & 15) == 0 || (c.kernel_variant & 15) == 3)   // segmented tiles (default)
             launch_scatter_seg(s, c.nmap, c.nactive, c.rec_layout == REC16 ? (const void *)c.d_ent.p : c.d_entc.p,
                                c.rec_layout, c.d_pw.p, c.sr_s.heads.p, c.sr_s.k0.p,
                                c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.nzpix, c.d_ccinv_c.p, c.d_m4x.p,
                                c.d_sr_part.p);
-        else if (c.kernel_variant == 2)
+        else if ((c.kernel_variant & 15) == 2)
             launch_scatter_tiles(s, c.nmap, c.npix_cross, c.ndeg, c.d_off.p, c.nactive, c.d_ent.p, c.d_pw.p,
                                  c.d_ccinv6x.p, c.d_m4x.p, c.d_tileinfo.p, c.d_tilescratch.p);
         else
@@ -1368,11 +1370,11 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         else if (p.diagfilter != 0) launch_axpby_nna(s, c.d_ap.p, c.d_nna.p, c.d_p.p, p.diagfilter, nb);
         else launch_fill(s, c.d_ap.p, 0.0, nb);
         if (prof) { t_pr.stop(s); t_ga.start(s); }
-        if (c.kernel_variant == 3 && c.sr_g.ok)
+        if ((c.kernel_variant & 15) == 3 && c.sr_g.ok)
             launch_gather_seg(s, c.nmap, (int64_t)c.ndet * c.nsamp, c.d_pixw.p, c.d_m4x.p, c.ndeg, c.sr_g.heads.p,
                               c.sr_g.k0.p, c.sr_g.fix_end.p, c.sr_g.fix_ord.p, c.d_dwb.p, c.d_nna.p, c.d_p.p,
                               c.d_ap.p, c.d_sr_part.p);
-        else if (c.kernel_variant != 1)
+        else if ((c.kernel_variant & 15) != 1)
             launch_gather_flat(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p,
                                c.rec_layout == REC16 ? (const void *)c.d_pixw.p : c.d_pixc.p, c.rec_layout,
                                c.d_m4x.p, c.ndeg, c.d_dwb.p, c.d_nna.p, c.d_p.p, c.d_ap.p);

@@ -256,10 +256,12 @@ struct BandPrec {
     // partitions (and 64-partition blocks) that are not bitwise repeats of the previous one
     int64_t nchunk_unique = 0, nblock = 0, nblock_unique = 0;
     int64_t nrows = 0;      // baseline rows the factored segments cover
+    int64_t nrows_factored = 0;   // of those, rows the column loop computed (the rest repeat a block)
 };
 
 int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *dseg, int64_t nseg,
-               const int *nband_seg_host, const double *invcov_dev, int64_t nof, const double *nna, int *info_out);
+               const int *nband_seg_host, const double *invcov_dev, int64_t nof, const double *nna, int *info_out,
+               bool periodic = true);
 void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z);
 void band_free(BandPrec &bp);
 

@@ -115,6 +115,33 @@ def test_noise_prior_band_preconditioner():
     check_parity(g, o, c1, c2)
 
 
+@pytest.mark.parametrize("flag_head", [False, True])
+def test_band_factor_periodic_continuation(flag_head):
+    """Once nna is constant to a segment's end, 64-row blocks of the band
+    factor that repeat bitwise are copied instead of recomputed; the solve
+    must be bitwise the one of the full column loop (kernel variant + 16).
+    With flag_head, random samples of the first 40 % of every detector are
+    flagged so that nna varies there and the repeat can only start later."""
+    import libmadam_amd
+    case = ring(prior=True, ndet=2, nsamp=500000, nperiod=2)
+    if flag_head:
+        pix = case["pixels"].reshape(2, -1)
+        rng = np.random.default_rng(3)
+        head = pix[:, :200000]
+        head[rng.random(head.shape) < 0.05] = -1
+    c1, c2 = mc.copy_case(case), mc.copy_case(case)
+    g = libmadam_amd.run(*mc.call_args(c1))
+    f = libmadam_amd.run(*mc.call_args(c2), kernel_variant=16)
+    rows = f["stats"]["band_rows_factored"]
+    print("band rows factored", g["stats"]["band_rows_factored"], "of", rows, "niter", g["niter"], f["niter"])
+    assert g["stats"]["precond_kind"] == 3 and rows == 10000
+    assert g["stats"]["band_rows_factored"] < rows
+    assert g["niter"] == f["niter"]
+    for k in ("aa", "map", "binmap", "yba"):
+        assert np.array_equal(g[k], f[k]), k
+    assert np.array_equal(c1["signal"], c2["signal"])
+
+
 def test_pointing_record_layouts():
     """The projections read 12-byte records {index, w1, w2} when every
     intensity weight is exactly 1, 4-byte records for temperature-only maps,
