@@ -82,21 +82,56 @@ __device__ inline double shift_down_row(double v)
 // pivot, scale the column, rank-1 update, shift the window up by one row.  The
 // diagonal terms nna of the rows entering the window are loaded 64 rows ahead.
 //
-// Periodic continuation.  After column j the window holds A's band entries of
-// rows j+1 .. j+K (the Toeplitz terms and nna on the diagonal) updated by the
-// last K-1 columns of L, so it is a fixed function of those columns and of the
-// window rows' nna.  Once nna is constant (from row const_from[s] on), a
-// 64-column block equal bitwise to the previous one leaves the window as it
-// found it, and every later block whose rows stay clear of the segment's end
-// repeats it exactly.  The wave then records the block (per_src, per_q blocks
-// to fill, done by k_band_fill_periodic) and resumes at the first block
-// that the segment's end truncates.  The values are the same bits as the full
-// column loop would give.
+// Periodic continuation.  The window at the start of column j holds rows
+// j .. j+K-1 of A (Toeplitz terms, nna on the diagonal) minus the products of
+// columns j-K+1 .. j-1 of L, in a fixed order, so it is a function of those
+// columns and of those rows' nna.  If the 64 columns of block j0 equal those of
+// block j0 - 64 bitwise and nna is one value v on rows j0 .. e-1, the window
+// after block j0 equals the one after block j0 - 64, and every block j0 + 64 q'
+// whose rows end before e (j0 + 64 q' + 64 + K <= e) repeats block j0 bitwise.
+// The wave then records (j0, q) for k_band_fill_periodic, which copies the
+// block, and resumes at the first block past them with the window after block
+// j0, the window the full column loop would reach there.  nna runs are tracked
+// as the rows are loaded: rstart is the first row of the run the last loaded
+// row belongs to (rows before K are treated as a run of their own).
+constexpr int BAND_MAXJ = 16;   // recorded repeats per segment
+
+struct NnaRun {
+    int64_t rstart;
+    long long v;
+};
+
+// fold rows r0 .. r0+63 (lane t: row r0 + t, value x if r0 + t < n) into the run
+__device__ inline void nna_run_add(NnaRun &run, int64_t r0, int64_t n, double x, int lane)
+{
+    long long xb = __double_as_longlong(x);
+    long long prev = __shfl_up(xb, 1, 64);
+    if (lane == 0) prev = run.v;
+    uint64_t d = __ballot(r0 + lane < n && xb != prev);
+    if (d) run.rstart = r0 + 63 - __builtin_clzll(d);
+    int64_t last = n - 1 - r0 < 63 ? n - 1 - r0 : 63;
+    if (last >= 0) run.v = __shfl(xb, (int)last, 64);
+}
+
+// first row >= r (up to n) whose nna differs from v
+__device__ inline int64_t nna_run_end(const double *nn, int64_t r, int64_t n, long long v, int lane)
+{
+    for (; r < n; r += 512) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int64_t i = r + 64 * u + lane;
+            long long x = __double_as_longlong(nn[i < n ? i : n - 1]);
+            uint64_t d = __ballot(i < n && x != v);
+            if (d) return r + 64 * u + __builtin_ctzll(d);
+        }
+    }
+    return n;
+}
+
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_t nseg, const int *nband_seg,
                                                       const double *invcov, int64_t nof, const double *nna,
-                                                      const int64_t *const_from, double *Lb, int *info,
-                                                      int64_t *per_src, int64_t *per_q)
+                                                      double *Lb, int *info, int64_t *per_src, int64_t *per_q)
 {
     constexpr int K = KD + 1;
     constexpr bool DPP_SHIFT = K <= 15;   // the window fits one 16-lane DPP row
@@ -129,8 +164,13 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
     // (lane t holds row j0 + K + t); the next block's values are loaded one block
     // ahead, so the only memory wait is once per block.
     double nnext = K + lane < n ? nn[K + lane] : 0.0;
-    const int64_t cf = const_from ? const_from[s] : n;
-    bool periodic_done = false;
+    NnaRun run{K, 0};
+    int njump = 0;
+    int64_t jfirst = 0;   // first block computed since the last repeat (the copies land later)
+    if (per_src) {
+        run.v = __double_as_longlong(__shfl(nnext, 0, 64));
+        nna_run_add(run, K, n, nnext, lane);
+    }
     for (int64_t j0 = 0; j0 < n; j0 += 64) {
         double nbuf = nnext;
         nnext = j0 + 64 + K + lane < n ? nn[j0 + 64 + K + lane] : 0.0;
@@ -176,78 +216,48 @@ __global__ void __launch_bounds__(64) k_band_factor_w(const Segment *seg, int64_
 #pragma unroll
             for (int bb = 0; bb < K; ++bb) W[bb] = Wn[bb];
         }
-        // block j0 against block j0 - 64: the window after block j0 is built from
-        // block j0's columns and the nna of rows j0 + 64 .., that after block j0 - 64
-        // from block j0 - 64's and rows j0 .., so with nna constant from row j0 on
-        // and one whole block ahead clear of the end the two windows are equal
-        if (per_src && !periodic_done && tn == 64 && j0 >= 64 && j0 >= cf && j0 + 128 + K <= n) {
-            bool eq = true;
-            if (lane < K) {
-                const double *cur = Lb + (sg.b0 + j0) * K + lane, *prv = cur - 64 * K;
+        if (!per_src || njump == BAND_MAXJ) continue;
+        nna_run_add(run, j0 + 64 + K, n, nnext, lane);   // rows up to j0 + 127 + K seen
+        if (tn < 64 || j0 < jfirst + 64 || run.rstart > j0 || j0 + 128 + K > n) continue;
+        bool eq = true;
+        if (lane < K) {
+            const double *cur = Lb + (sg.b0 + j0) * K + lane, *prv = cur - 64 * K;
 #pragma unroll 8
-                for (int t = 0; t < 64; ++t)
-                    eq = eq && __double_as_longlong(cur[t * K]) == __double_as_longlong(prv[t * K]);
-            }
-            if (__ballot(!eq) == 0) {   // uniform: every later interior block repeats block j0
-                int64_t q = (n - 64 - K - j0) / 64;   // blocks j0 + 64, ..., j0 + 64 q
-                if (lane == 0) {
-                    per_src[s] = j0;
-                    per_q[s] = q;
-                }
-                periodic_done = true;
-                j0 += 64 * q;
-                nnext = j0 + 64 + K + lane < n ? nn[j0 + 64 + K + lane] : 0.0;
-            }
+            for (int t = 0; t < 64; ++t)
+                eq = eq && __double_as_longlong(cur[t * K]) == __double_as_longlong(prv[t * K]);
         }
+        if (__ballot(!eq) != 0) continue;
+        int64_t e = nna_run_end(nn, j0 + 128 + K, n, run.v, lane);
+        int64_t q = (e - 64 - K - j0) / 64;   // blocks j0 + 64, ..., j0 + 64 q repeat block j0
+        if (q < 1) continue;
+        if (lane == 0) {
+            per_src[s * BAND_MAXJ + njump] = j0;
+            per_q[s * BAND_MAXJ + njump] = q;
+        }
+        ++njump;
+        j0 += 64 * q;
+        jfirst = j0 + 64;
+        // resume at block jfirst: the window's rows jfirst .. jfirst+K-1 lie in
+        // the run; its entering rows go through the run tracker now
+        nnext = jfirst + K + lane < n ? nn[jfirst + K + lane] : 0.0;
+        nna_run_add(run, jfirst + K, n, nnext, lane);
     }
 }
 
-// first row from which nna is constant (bitwise) to the end of each segment;
-// one wave per segment, scanning backwards 512 rows at a time
-__global__ void __launch_bounds__(64) k_band_const_from(const Segment *seg, int64_t nseg, const double *nna,
-                                                        int64_t *cf)
-{
-    int lane = threadIdx.x & 63;
-    int64_t s = blockIdx.x;
-    if (s >= nseg) return;
-    Segment sg = seg[s];
-    int64_t n = sg.noba;
-    const double *nn = nna + sg.b0;
-    int64_t res = 0;
-    if (n > 0) {
-        long long v = __double_as_longlong(nn[n - 1]);
-        for (int64_t e = n; e > 0; e -= 512) {
-            long long badmax = -1;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                int64_t i = e - 1 - (u * 64 + lane);
-                long long x = __double_as_longlong(nn[i >= 0 ? i : 0]);
-                if (i >= 0 && x != v && i > badmax) badmax = i;
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                long long ob = __shfl_xor(badmax, o, 64);
-                badmax = ob > badmax ? ob : badmax;
-            }
-            if (badmax >= 0) {
-                res = badmax + 1;
-                break;
-            }
-        }
-    }
-    if (lane == 0) cf[s] = res;
-}
-
-// Lb rows of block src + 64 q' (q' = 1 .. q) = those of block src, per segment
+// Lb rows of block src + 64 q' (q' = 1 .. q) = those of block src, for every
+// repeat the factorization recorded
 __global__ void k_band_fill_periodic(const Segment *seg, int64_t nseg, int K, const int64_t *per_src,
                                      const int64_t *per_q, double *Lb)
 {
     int blk = 64 * K;
-    for (int64_t s = blockIdx.y; s < nseg; s += gridDim.y) {
-        int64_t q = per_q[s];
-        double *from = Lb + (seg[s].b0 + per_src[s]) * K;
-        for (int64_t qq = 1 + blockIdx.x; qq <= q; qq += gridDim.x)
-            for (int e = threadIdx.x; e < blk; e += blockDim.x) from[qq * blk + e] = from[e];
-    }
+    for (int64_t s = blockIdx.y; s < nseg; s += gridDim.y)
+        for (int k = 0; k < BAND_MAXJ; ++k) {
+            int64_t q = per_q[s * BAND_MAXJ + k];
+            if (q <= 0) break;
+            double *from = Lb + (seg[s].b0 + per_src[s * BAND_MAXJ + k]) * K;
+            for (int64_t qq = 1 + blockIdx.x; qq <= q; qq += gridDim.x)
+                for (int e = threadIdx.x; e < blk; e += blockDim.x) from[qq * blk + e] = from[e];
+        }
 }
 
 // ---------------------------------------------------------------- partitioned solves
@@ -764,33 +774,34 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
     MADAM_CHECK(hipMalloc(&dinfo, sizeof(int) * (nseg > 0 ? nseg : 1)));
     MADAM_CHECK(hipMemcpyAsync(dnb, nband_seg_host, sizeof(int) * nseg, hipMemcpyHostToDevice, s));
     MADAM_CHECK(hipMemsetAsync(dinfo, 0, sizeof(int) * (nseg > 0 ? nseg : 1), s));
-    int64_t *dper = nullptr;   // const_from[nseg], per_src[nseg], per_q[nseg]
-    int64_t ns1 = nseg > 0 ? nseg : 1;
-    MADAM_CHECK(hipMalloc(&dper, sizeof(int64_t) * 3 * ns1));
-    MADAM_CHECK(hipMemsetAsync(dper, 0, sizeof(int64_t) * 3 * ns1, s));
+    int64_t *dper = nullptr;   // per_src[nseg][BAND_MAXJ], per_q[nseg][BAND_MAXJ]
+    int64_t ns1 = nseg > 0 ? nseg : 1, nj = ns1 * BAND_MAXJ;
+    MADAM_CHECK(hipMalloc(&dper, sizeof(int64_t) * 2 * nj));
+    MADAM_CHECK(hipMemsetAsync(dper, 0, sizeof(int64_t) * 2 * nj, s));
     if (nseg > 0) {
-        k_band_const_from<<<(unsigned)nseg, 64, 0, s>>>(dseg, nseg, nna, dper);
         BAND_DISPATCH(KD, (k_band_factor_w<KD><<<(unsigned)nseg, 64, 0, s>>>(dseg, nseg, dnb, invcov_dev, nof, nna,
-                                                                            dper, bp.L, dinfo,
-                                                                            periodic ? dper + ns1 : nullptr,
-                                                                            dper + 2 * ns1)));
+                                                                            bp.L, dinfo, periodic ? dper : nullptr,
+                                                                            dper + nj)));
         if (periodic) {
             unsigned gy = (unsigned)std::min<int64_t>(nseg, 65535);
             unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(256, 8192 / nseg));
-            k_band_fill_periodic<<<dim3(gx, gy), 256, 0, s>>>(dseg, nseg, K, dper + ns1, dper + 2 * ns1, bp.L);
+            k_band_fill_periodic<<<dim3(gx, gy), 256, 0, s>>>(dseg, nseg, K, dper, dper + nj, bp.L);
         }
     }
     MADAM_CHECK(hipGetLastError());
     MADAM_CHECK(hipMemcpyAsync(info_out, dinfo, sizeof(int) * nseg, hipMemcpyDeviceToHost, s));
-    std::vector<int64_t> perq(ns1, 0);
-    MADAM_CHECK(hipMemcpyAsync(perq.data(), dper + 2 * ns1, sizeof(int64_t) * ns1, hipMemcpyDeviceToHost, s));
+    std::vector<int64_t> perq(nj, 0);
+    MADAM_CHECK(hipMemcpyAsync(perq.data(), dper + nj, sizeof(int64_t) * nj, hipMemcpyDeviceToHost, s));
     MADAM_CHECK(hipStreamSynchronize(s));
     (void)hipFree(dnb);
     (void)hipFree(dinfo);
     (void)hipFree(dper);
     bp.nrows_factored = 0;
     for (int64_t i = 0; i < nseg; ++i)
-        if (!hseg[i].skip && nband_seg_host[i] > 0) bp.nrows_factored += hseg[i].noba - 64 * perq[i];
+        if (!hseg[i].skip && nband_seg_host[i] > 0) {
+            bp.nrows_factored += hseg[i].noba;
+            for (int k = 0; k < BAND_MAXJ; ++k) bp.nrows_factored -= 64 * perq[i * BAND_MAXJ + k];
+        }
     // chunk table over the segments that factored
     std::vector<ChunkInfo> ch;
     std::vector<int64_t> first;

@@ -115,26 +115,30 @@ def test_noise_prior_band_preconditioner():
     check_parity(g, o, c1, c2)
 
 
-@pytest.mark.parametrize("flag_head", [False, True])
-def test_band_factor_periodic_continuation(flag_head):
-    """Once nna is constant to a segment's end, 64-row blocks of the band
-    factor that repeat bitwise are copied instead of recomputed; the solve
-    must be bitwise the one of the full column loop (kernel variant + 16).
-    With flag_head, random samples of the first 40 % of every detector are
-    flagged so that nna varies there and the repeat can only start later."""
+@pytest.mark.parametrize("flags", ["none", "head", "bursts"])
+def test_band_factor_periodic_continuation(flags):
+    """Where nna is constant, 64-row blocks of the band factor that repeat
+    bitwise are copied instead of recomputed; the solve must be bitwise the
+    one of the full column loop (kernel variant + 16).  "head" flags random
+    samples of the first 40 % of every detector, so nna varies there and the
+    repeat starts later; "bursts" flags 200-sample bursts every 30000 samples,
+    so each segment has several constant runs, each with its own repeat."""
     import libmadam_amd
-    case = ring(prior=True, ndet=2, nsamp=500000, nperiod=2)
-    if flag_head:
-        pix = case["pixels"].reshape(2, -1)
+    case = ring(prior=True, ndet=4, nsamp=400000, nperiod=2)
+    pix = case["pixels"].reshape(4, -1)
+    if flags == "head":
         rng = np.random.default_rng(3)
-        head = pix[:, :200000]
+        head = pix[:, :160000]
         head[rng.random(head.shape) < 0.05] = -1
+    elif flags == "bursts":
+        for b0 in range(10000, 400000, 30000):
+            pix[:, b0:b0 + 200] = -1
     c1, c2 = mc.copy_case(case), mc.copy_case(case)
     g = libmadam_amd.run(*mc.call_args(c1))
     f = libmadam_amd.run(*mc.call_args(c2), kernel_variant=16)
     rows = f["stats"]["band_rows_factored"]
     print("band rows factored", g["stats"]["band_rows_factored"], "of", rows, "niter", g["niter"], f["niter"])
-    assert g["stats"]["precond_kind"] == 3 and rows == 10000
+    assert g["stats"]["precond_kind"] == 3 and rows == 4 * 8000
     assert g["stats"]["band_rows_factored"] < rows
     assert g["niter"] == f["niter"]
     for k in ("aa", "map", "binmap", "yba"):
