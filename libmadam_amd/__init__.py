@@ -26,7 +26,7 @@ STAT_NAMES = ["niter", "rrinit", "rzinit", "t_pointing", "t_bin", "t_init", "t_p
               "t_scatter", "t_gather", "t_prior", "t_apply_prec", "n_scatter", "n_gather", "precond_kind",
               "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total", "t_step", "ncg",
               "band_chunks", "band_chunks_unique", "band_blocks", "band_blocks_unique", "rec_bytes",
-              "band_rows_factored"]
+              "band_rows_factored", "prec_reused"]
 
 
 class MadamError(RuntimeError):

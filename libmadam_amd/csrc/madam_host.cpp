@@ -351,7 +351,7 @@ enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_
        ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
        ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_NCG,
        ST_BAND_CHUNKS, ST_BAND_CHUNKS_UNIQUE, ST_BAND_BLOCKS, ST_BAND_BLOCKS_UNIQUE, ST_REC_BYTES,
-       ST_BAND_ROWS_FACTORED, ST_COUNT };
+       ST_BAND_ROWS_FACTORED, ST_PREC_REUSED, ST_COUNT };
 
 struct Context {
     int device = 0;
@@ -451,6 +451,13 @@ struct Context {
     DBuf<int> d_cg_act;
     BandPrec band;
     bool band_ready = false;
+    // segments, filters and preconditioner of the last solve, kept while the
+    // configuration, pointing and kernel choice stay and reused when nna is
+    // bitwise the same (they are functions of those alone)
+    bool prec_cached = false;
+    int prec_cached_variant = 0;
+    DBuf<double> d_nna_key;
+    DBuf<int> d_flag;
     // results
     double stats[ST_COUNT] = {0};
     std::vector<double> rr_hist;
@@ -770,6 +777,7 @@ static void configure(Context &c, const char *parstring, long ndet, const char *
     c.configured = true;
     c.pointed = false;
     c.solved = false;
+    c.prec_cached = false;
 }
 
 // ------------------------------------------------------------------ noise filter on the device
@@ -950,6 +958,7 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
     c.stats[ST_NACTIVE] = (double)total;
     c.pointed = true;
     c.tiles_built = false;
+    c.prec_cached = false;
 }
 
 // ------------------------------------------------------------------ solve
@@ -1486,11 +1495,22 @@ static void solve(Context &c, double *signal, int on_device)
         launch_init_a(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, c.d_tod, c.d_m4x.p, c.ndeg,
                       c.d_dwb.p, c.d_yba.p, c.d_nna.p, grp);
         MADAM_CHECK(hipGetLastError());
-        build_segments(c);
+        bool reuse = false;
+        if (c.prec_cached && c.prec_cached_variant == c.kernel_variant) {
+            c.d_flag.alloc(1);
+            MADAM_CHECK(hipMemsetAsync(c.d_flag.p, 0, sizeof(int), s));
+            launch_bits_differ(s, c.d_nna.p, c.d_nna_key.p, c.nbtot, c.d_flag.p);
+            int differ = 1;
+            d2h(&differ, c.d_flag.p, 1, s);
+            MADAM_CHECK(hipStreamSynchronize(s));
+            reuse = differ == 0;
+        }
+        if (!reuse) build_segments(c);
+        c.stats[ST_PREC_REUSED] = reuse ? 1 : 0;
         tm.stop(s);
         c.stats[ST_T_INIT] = tm.ms() * 1e-3;
         double tp = now();
-        if (p.kfilter) {
+        if (!reuse && p.kfilter) {
             build_filters(c);
             // batches whose work buffer (16 B per complex point) stays within
             // 128 MiB, so the row and inverse column passes re-read it from the
@@ -1504,7 +1524,13 @@ static void solve(Context &c, double *signal, int on_device)
             // batch buffer, ipsd ints and the per-segment partial sums
             c.d_fftwork.alloc(fft_convolve_work_doubles(c.fft, c.fft_batch));
         }
-        build_preconditioner(c);
+        if (!reuse) {
+            build_preconditioner(c);
+            c.d_nna_key.alloc(std::max<int64_t>(c.nbtot, 1));
+            MADAM_CHECK(hipMemcpyAsync(c.d_nna_key.p, c.d_nna.p, sizeof(double) * c.nbtot, hipMemcpyDeviceToDevice, s));
+            c.prec_cached = true;
+            c.prec_cached_variant = c.kernel_variant;
+        }
         MADAM_CHECK(hipStreamSynchronize(s));
         c.stats[ST_T_PRECOND] = now() - tp;
         c.stats[ST_PRECOND_KIND] = c.precond_kind;
@@ -2315,7 +2341,10 @@ MADAM_API int madam_iterate(void *ctx, int niter, int restart)
 
 MADAM_API int madam_set_kernel_variant(void *ctx, int v)
 {
-    return guarded([&] { ((Context *)ctx)->kernel_variant = v; });
+    return guarded([&] {
+        ((Context *)ctx)->kernel_variant = v;
+        ((Context *)ctx)->prec_cached = false;
+    });
 }
 
 MADAM_API int madam_reset_timers(void *ctx)

@@ -177,6 +177,7 @@ void launch_find_segments(hipStream_t s, const double *nna, int ndet, int64_t nb
                           const int64_t *noba_pp, int ninterval, int64_t gapmin, Segment *out,
                           unsigned long long *nseg, int64_t cap);
 void launch_fill(hipStream_t s, double *p, double v, int64_t n);
+void launch_bits_differ(hipStream_t s, const double *a, const double *b, int64_t n, int *flag);
 void launch_copy(hipStream_t s, double *dst, const double *src, int64_t n);
 // elementwise: out = a * x
 void launch_diag_precond(hipStream_t s, const double *r, const double *pd, double *z, int64_t n);

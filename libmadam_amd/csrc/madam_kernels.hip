@@ -1398,6 +1398,19 @@ void launch_fill(hipStream_t s, double *p, double v, int64_t n)
     k_fill<<<grid_for(n, TPB), TPB, 0, s>>>(p, v, n);
 }
 
+// flag[0] = 1 if a and b differ bitwise anywhere (flag cleared by the caller)
+__global__ void k_bits_differ(const double *a, const double *b, int64_t n, int *flag)
+{
+    bool d = false;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d = d || __double_as_longlong(a[i]) != __double_as_longlong(b[i]);
+    if (d) flag[0] = 1;
+}
+void launch_bits_differ(hipStream_t s, const double *a, const double *b, int64_t n, int *flag)
+{
+    if (n > 0) k_bits_differ<<<grid_for(n, TPB), TPB, 0, s>>>(a, b, n, flag);
+}
+
 __global__ void k_copy(double *d, const double *a, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) d[i] = a[i];
