@@ -543,46 +543,54 @@ __device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t 
     }
 }
 
-// grid = (L1 / 2 + 1, batch); the pair's rows interleaved in LDS at k2 * nr + r (padded).
-// fc holds the filter spectra in row order (fft_spectrum_rows): f[r + L1 k2] at
-// [r * L2 + k2], f[N] at [N], so a row pair reads two contiguous rows.
+// grid = (L1 / 2, batch): block ra > 0 takes the row pair (ra, L1 - ra); block
+// 0 takes rows 0 and L1 / 2, each paired with itself (L1 / 2 blocks per
+// segment, a multiple of the blocks that fit the GPU at once, so the last
+// round is not a nearly empty one).  The two rows are interleaved in LDS at
+// k2 * 2 + r (padded).  fc holds the filter spectra in row order
+// (fft_spectrum_rows): f[r + L1 k2] at [r * L2 + k2], f[N] at [N], so a row
+// pair reads two contiguous rows.
 __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2, FftPlan P, const double *fc,
                             int64_t fc_stride, const Segment *seg, int64_t s0)
 {
     extern __shared__ double2 s[];
     int64_t b = blockIdx.y;
-    int ra = blockIdx.x, rb = (L1 - ra) % L1;
-    int nr = ra == rb ? 1 : 2;
+    int ra = blockIdx.x, rb = ra == 0 ? L1 / 2 : L1 - ra;
     double2 *d = work + b * N;
     const double *f = fc + (int64_t)seg[s0 + b].ipsd * fc_stride;
-    int tot = L2 * nr;
+    int tot = 2 * L2;
     // natural order in and out of LDS (no bank conflicts): the forward row
     // FFTs are DIF (bit-reversed output), the inverse ones DIT (bit-reversed
     // input), and the spectral step addresses the bit-reversed positions
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t >> logL2, n = t & (L2 - 1);
         int row = r == 0 ? ra : rb;
-        s[lp(n * nr + r)] = d[(int64_t)row * L2 + n];
+        s[lp(n * 2 + r)] = d[(int64_t)row * L2 + n];
     }
     __syncthreads();
-    lds_fft8_dif(s, L2, logL2, nr - 1, P.sub2, false);   // nr = 1 or 2 rows
-    // X[ra + L1 k2] is at bitrev(k2) * nr; X[rb + L1 k2] at bitrev(k2) * nr + 1
+    lds_fft8_dif(s, L2, logL2, 1, P.sub2, false);   // 2 rows
+    // X[ra + L1 k2] is at bitrev(k2) * 2; X[rb + L1 k2] at bitrev(k2) * 2 + 1
     if (ra == 0) {
-        for (int k2 = threadIdx.x; k2 <= L2 / 2; k2 += blockDim.x) {
-            if (k2 == 0) {
+        // row 0: pairs {k2, L2 - k2} (k2 = 0 holds X[0] and X[N / 2]);
+        // row L1 / 2: pairs {k2, L2 - 1 - k2}
+        const int h = L1 / 2;
+        for (int t = threadIdx.x; t <= L2; t += blockDim.x) {
+            if (t == 0) {
                 double2 a = s[lp(0)];
                 double x0 = a.x + a.y, xn = a.x - a.y;
                 double y0 = x0 * f[0], yn = xn * f[N];   // f[0] at [0]
                 s[lp(0)] = make_double2(y0 + yn, y0 - yn);
+            } else if (t <= L2 / 2) {
+                int k2 = t;
+                conv_pair(s, lp(bitrev(k2, logL2) * 2), lp(bitrev(L2 - k2, logL2) * 2), (int64_t)L1 * k2, N, P,
+                          f[k2], f[L2 - k2]);
             } else {
-                conv_pair(s, lp(bitrev(k2, logL2)), lp(bitrev(L2 - k2, logL2)), (int64_t)L1 * k2, N, P, f[k2],
-                          f[L2 - k2]);
+                int k2 = t - (L2 / 2 + 1);
+                conv_pair(s, lp(bitrev(k2, logL2) * 2 + 1), lp(bitrev(L2 - 1 - k2, logL2) * 2 + 1),
+                          (int64_t)h + (int64_t)L1 * k2, N, P, f[(int64_t)h * L2 + k2],
+                          f[(int64_t)h * L2 + L2 - 1 - k2]);
             }
         }
-    } else if (nr == 1) {   // ra == L1 / 2
-        for (int k2 = threadIdx.x; k2 < L2 / 2; k2 += blockDim.x)
-            conv_pair(s, lp(bitrev(k2, logL2)), lp(bitrev(L2 - 1 - k2, logL2)), (int64_t)ra + (int64_t)L1 * k2, N,
-                      P, f[(int64_t)ra * L2 + k2], f[(int64_t)ra * L2 + L2 - 1 - k2]);
     } else {
         for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
             int64_t kA = (int64_t)ra + (int64_t)L1 * k2;
@@ -593,11 +601,11 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
         }
     }
     __syncthreads();
-    lds_fft8(s, L2, logL2, nr - 1, P.sub2, true);
+    lds_fft8(s, L2, logL2, 1, P.sub2, true);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t >> logL2, n = t & (L2 - 1);
         int row = r == 0 ? ra : rb;
-        d[(int64_t)row * L2 + n] = s[lp(n * nr + r)];
+        d[(int64_t)row * L2 + n] = s[lp(n * 2 + r)];
     }
 }
 
@@ -650,23 +658,23 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *wo
     }
 }
 
-// mean[b] of the batch's segment b from its nblk ordered partials, then
-// y[b0 + j] -= mean[b] over the segment
-__global__ void k_conv_seg_mean(const Segment *seg, int64_t s0, int64_t nb, const double *partial_out, int nblk,
-                                double *mean)
-{
-    int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    double t = 0;
-    for (int k = 0; k < nblk; ++k) t += partial_out[b * nblk + k];
-    mean[b] = t / seg[s0 + b].noba;
-}
-
-__global__ void k_conv_sub_mean(double *y, const Segment *seg, int64_t s0, const double *mean)
+// y[b0 + j] -= mean over the batch's segment b, the mean summed by the first
+// wave from the segment's nblk partials (lane l adds partials l, l + 64, ...,
+// then a butterfly over the lanes: the same order in every block)
+__global__ void k_conv_sub_mean(double *y, const Segment *seg, int64_t s0, const double *partial_out, int nblk)
 {
     int64_t b = blockIdx.y;
     Segment sg = seg[s0 + b];
-    double mb = mean[b];
+    __shared__ double smb;
+    if (threadIdx.x < 64) {
+        const double *po = partial_out + b * nblk;
+        double t = 0;
+        for (int k = threadIdx.x; k < nblk; k += 64) t += po[k];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (threadIdx.x == 0) smb = t / sg.noba;
+    }
+    __syncthreads();
+    double mb = smb;
     double *ys = y + sg.b0;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < sg.noba; j += (int64_t)gridDim.x * blockDim.x)
         ys[j] -= mb;
@@ -719,16 +727,14 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
         if (fused) {
             k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 x, dseg, s0, partial, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
-            k_conv_rows<<<dim3((unsigned)(p.L1 / 2 + 1), (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(2 * p.L2)), s>>>(
+            k_conv_rows<<<dim3((unsigned)(p.L1 / 2), (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(2 * p.L2)), s>>>(
                 (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
             k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
                 p);
-            double *means = partial_out + batch * ncb;   // batch doubles
-            k_conv_seg_mean<<<(unsigned)((nb + 63) / 64), 64, 0, s>>>(dseg, s0, nb, partial_out, ncb, means);
             unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
             if (gl > 1024) gl = 1024;
-            k_conv_sub_mean<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, means);
+            k_conv_sub_mean<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
             continue;
         }
         unsigned gl = (unsigned)((nof + FFT_TPB * 4 - 1) / (FFT_TPB * 4));

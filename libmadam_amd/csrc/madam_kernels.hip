@@ -1460,11 +1460,12 @@ void launch_axpby_nna(hipStream_t s, double *ap, const double *nna, const double
 
 // ---------------------------------------------------------------- deterministic masked dots
 
-static constexpr int DOT_BLOCKS_MAX = 1024;
+static constexpr int DOT_BLOCKS_MAX = 2048;
+static constexpr int DOT_CHUNK = 8;   // elements per thread per step (one 8-byte mask load)
 
 int dots_nblocks(int64_t n)
 {
-    int64_t b = (n + TPB * 8 - 1) / (TPB * 8);
+    int64_t b = (n + TPB * DOT_CHUNK - 1) / (TPB * DOT_CHUNK);
     if (b < 1) b = 1;
     if (b > DOT_BLOCKS_MAX) b = DOT_BLOCKS_MAX;
     return (int)b;
@@ -1472,11 +1473,41 @@ int dots_nblocks(int64_t n)
 
 // which = 0: sum a*b (1 value)
 // which = 1: sum a*b, sum c*b, sum a*a (3 values: rz, rz2, rr with a=r, b=z, c=ro)
-__global__ void k_dots(int which, const double *a, const double *b, const double *c, const int8_t *mask, int64_t n,
-                       double *partials)
+// Only elements with mask != 0 count.  Each thread takes chunks of 8 elements
+// and issues all of a chunk's loads (mask as one 8-byte word, the vectors as
+// 16-byte pairs) before the arithmetic; the block partials and their final
+// sum are in a fixed order, so the result depends on n alone.
+__global__ void __launch_bounds__(TPB) k_dots(int which, const double *a, const double *b, const double *c,
+                                              const int8_t *mask, int64_t n, double *partials)
 {
     double s0 = 0, s1 = 0, s2 = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t nch = n / DOT_CHUNK;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (int64_t ch = gt; ch < nch; ch += stride) {
+        uint64_t mk = ((const uint64_t *)mask)[ch];
+        double2 A[DOT_CHUNK / 2], B[DOT_CHUNK / 2], C[DOT_CHUNK / 2];
+#pragma unroll
+        for (int u = 0; u < DOT_CHUNK / 2; ++u) {
+            A[u] = ((const double2 *)a)[ch * (DOT_CHUNK / 2) + u];
+            B[u] = ((const double2 *)b)[ch * (DOT_CHUNK / 2) + u];
+            if (which == 1) C[u] = ((const double2 *)c)[ch * (DOT_CHUNK / 2) + u];
+        }
+#pragma unroll
+        for (int u = 0; u < DOT_CHUNK / 2; ++u)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                bool m = ((mk >> (8 * (2 * u + h))) & 0xff) != 0;
+                double ai = h ? A[u].y : A[u].x, bi = h ? B[u].y : B[u].x;
+                s0 += m ? ai * bi : 0.0;
+                if (which == 1) {
+                    double ci = h ? C[u].y : C[u].x;
+                    s1 += m ? ci * bi : 0.0;
+                    s2 += m ? ai * ai : 0.0;
+                }
+            }
+    }
+    for (int64_t i = nch * DOT_CHUNK + gt; i < n; i += stride) {
         if (!mask[i]) continue;
         double ai = a[i], bi = b[i];
         s0 += ai * bi;
