@@ -492,7 +492,17 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const double *x, 
     int64_t b = blockIdx.y;
     int c0 = blockIdx.x * ncol;
     Segment sg = seg[s0 + b];
-    double mean = seg_mean(partial_in, b, sg.noba);
+    // the mean from the segment's SEG_NB partials: one per lane of the first
+    // wave, a butterfly over the lanes (the same order in every block)
+    __shared__ double smean;
+    static_assert(SEG_NB == 64, "one partial per lane");
+    if (threadIdx.x < 64) {
+        double t = partial_in[b * SEG_NB + threadIdx.x];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (threadIdx.x == 0) smean = t / sg.noba;
+    }
+    __syncthreads();
+    double mean = smean;
     const double *xs = x + sg.b0;
     int64_t noba = sg.noba;
     double2 *d = work + b * N;
@@ -710,23 +720,32 @@ static bool conv_fusable(const FftPlan &p)
 
 void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg, const double *x,
                            double *y, const double *fc, const double *fc_rows, int64_t fc_stride, double *work,
-                           int64_t batch, const Segment *hseg)
+                           int64_t batch, int64_t nseg_max, const Segment *hseg)
 {
     (void)hseg;
+    if (nseg > nseg_max)
+        throw HipError(hipErrorInvalidValue, "fft_convolve_segments: more segments than the work buffer holds",
+                       __FILE__, __LINE__);
     int64_t nof = 2 * p.N;
+    int64_t npart = std::max(batch, nseg_max);
     int *ipsd = (int *)(work + batch * nof);                        // batch ints after the batch buffer
-    double *partial = work + batch * nof + (batch + 1) / 2 + 1;    // batch * SEG_NB doubles
+    double *partial = work + batch * nof + (batch + 1) / 2 + 1;    // npart * SEG_NB doubles
     int logL1 = ilog2(p.L1), logL2 = ilog2(p.L2);
     int ncb = (int)(p.L2 / CONV_NCOL);
-    double *partial_out = partial + batch * SEG_NB;                 // batch * ncb doubles
+    double *partial_out = partial + npart * SEG_NB;                 // batch * ncb doubles
     bool fused = conv_fusable(p);
+    // the input means of every segment at once (one launch fills the GPU)
+    if (fused)
+        for (int64_t q0 = 0; q0 < nseg; q0 += 65535) {
+            int64_t nq = std::min<int64_t>(nseg - q0, 65535);
+            k_seg_partial<<<dim3(SEG_NB, (unsigned)nq), FFT_TPB, 0, s>>>(x, 0, 1, dseg, q0, 1.0, partial + q0 * SEG_NB);
+        }
     for (int64_t s0 = 0; s0 < nseg; s0 += batch) {
         int64_t nb = nseg - s0 < batch ? nseg - s0 : batch;
         dim3 gp(SEG_NB, (unsigned)nb);
-        k_seg_partial<<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
         if (fused) {
             k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
-                x, dseg, s0, partial, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
+                x, dseg, s0, partial + s0 * SEG_NB, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
             k_conv_rows<<<dim3((unsigned)(p.L1 / 2), (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(2 * p.L2)), s>>>(
                 (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
             k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
@@ -737,6 +756,7 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
             k_conv_sub_mean<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
             continue;
         }
+        k_seg_partial<<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
         unsigned gl = (unsigned)((nof + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
         if (gl > 1024) gl = 1024;
         k_seg_load<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(x, dseg, s0, nof, work, partial);
@@ -751,9 +771,9 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
     }
 }
 
-size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch)
+size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch, int64_t nseg_max)
 {
-    return (size_t)batch * 2 * p.N + (size_t)(batch + 1) / 2 + 1 + (size_t)batch * SEG_NB +
+    return (size_t)batch * 2 * p.N + (size_t)(batch + 1) / 2 + 1 + (size_t)std::max(batch, nseg_max) * SEG_NB +
            (size_t)batch * (p.L2 / CONV_NCOL + 1) + 64;
 }
 

@@ -1061,7 +1061,8 @@ static void apply_prior(Context &c, const double *x, double *y)
     if (!c.prior_covers_all) launch_copy(c.stream, y, x, c.nbtot);
     if (c.segconv.empty()) return;
     fft_convolve_segments(c.stream, c.fft, c.d_segconv.p, (int64_t)c.segconv.size(), x, y, c.d_fcov.p,
-                          c.d_fcov_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, c.segconv.data());
+                          c.d_fcov_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, (int64_t)c.segconv.size(),
+                          c.segconv.data());
 }
 
 // apply_CG_preconditioner (src/noise_routines.f90:1305-1409) on every segment
@@ -1102,7 +1103,8 @@ static void cg_apply(Context &c, const double *r, double *z)
         h2d(c.d_segact.p, live.data(), live.size(), s);
         // ap = convolve(p) + nna p
         fft_convolve_segments(s, c.fft, c.d_segact.p, (int64_t)live.size(), c.d_cg_p.p, c.d_cg_ap.p, c.d_fcov.p,
-                              c.d_fcov_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, live.data());
+                              c.d_fcov_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, (int64_t)c.segconv.size(),
+                              live.data());
         launch_cg_nna(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_nna.p, c.d_cg_p.p, c.d_cg_ap.p);
         launch_cg_dot(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_p.p, c.d_cg_ap.p, c.d_cg_part.p, an_d);
         d2h(an.data(), an_d, ns, s);
@@ -1143,7 +1145,8 @@ static void apply_precond(Context &c, const double *r, double *z)
         launch_copy(s, z, r, c.nbtot);
         if (!c.segconv.empty())
             fft_convolve_segments(s, c.fft, c.d_segconv.p, (int64_t)c.segconv.size(), r, z, c.d_fprec.p,
-                                  c.d_fprec_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, c.segconv.data());
+                                  c.d_fprec_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch,
+                                  (int64_t)c.segconv.size(), c.segconv.data());
         break;
     default:
         if (!c.prec_covers_all) launch_copy(s, z, r, c.nbtot);
@@ -1522,7 +1525,7 @@ static void solve(Context &c, double *signal, int on_device)
             int64_t nbatch = (nsc + cap - 1) / cap;
             c.fft_batch = (nsc + nbatch - 1) / nbatch;
             // batch buffer, ipsd ints and the per-segment partial sums
-            c.d_fftwork.alloc(fft_convolve_work_doubles(c.fft, c.fft_batch));
+            c.d_fftwork.alloc(fft_convolve_work_doubles(c.fft, c.fft_batch, nsc));
         }
         if (!reuse) {
             build_preconditioner(c);

@@ -233,10 +233,11 @@ void fft_plan_free(FftPlan &p);
 // fc: filter spectra, nrow x fc_stride; fc_rows: the same in row order (fft_spectrum_rows)
 void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg_dev,
                            const double *x, double *y, const double *fc, const double *fc_rows, int64_t fc_stride,
-                           double *work, int64_t batch, const Segment *hseg);
+                           double *work, int64_t batch, int64_t nseg_max, const Segment *hseg);
 // fT = fc reordered for the fused row pass: f[r + L1 k2] at [r * L2 + k2], f[N] at [N]
 void fft_spectrum_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrow, int64_t stride, double *fT);
-size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch);
+// nseg_max: the most segments one fft_convolve_segments call takes
+size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch, int64_t nseg_max);
 // inverse real FFT of fcov rows (invcov = c2r(fcov) / nof)
 void fft_c2r_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrows, double *out, double *work);
 // noise filter (build_filter) on the device
