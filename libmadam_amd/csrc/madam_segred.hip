@@ -377,7 +377,21 @@ __global__ void k_sr_scatter_fix(int nmap, int64_t ntiles, const int64_t *fix_en
     int64_t jl = fix_end[j];
     if (jl < 0) return;
     double4 a = part[j * 2 + 1];
-    for (int64_t jj = j + 1; jj <= jl; ++jj) {
+    // in tile order; long chains (the most-hit pixels span hundreds of tiles)
+    // load 8 partials ahead of their additions
+    int64_t jj = j + 1;
+    for (; jj + 8 <= jl + 1; jj += 8) {
+        double4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[(jj + u) * 2];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a.x += v[u].x;
+            a.y += v[u].y;
+            a.z += v[u].z;
+        }
+    }
+    for (; jj <= jl; ++jj) {
         double4 v = part[jj * 2];
         a.x += v.x;
         a.y += v.y;
