@@ -922,15 +922,6 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
     int item = threadIdx.x / FLAT_LANES, sub = threadIdx.x % FLAT_LANES;
     int id = item / FG_K, ik = item % FG_K;   // item = (detector, baseline)
     bool own = id < nd && ik < nk;
-    int64_t b = (int64_t)(d0 + id) * nbase + k0 + ik;
-    // the per-baseline terms of the result, loaded before the sample stream
-    double pre_nna = 0.0, pre_p = 0.0, pre_dwb = 0.0, pre_ap = 0.0;
-    if (own && sub == 0) {
-        pre_nna = nna[b];
-        pre_p = p[b];
-        pre_dwb = dwb[b];
-        pre_ap = ap[b];
-    }
     __syncthreads();
     double acc = 0.0;
     const int dsh = __ffsll((unsigned long long)ndeg) - 1;   // ndeg is a power of 4
@@ -1014,7 +1005,12 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
         __syncthreads();
     }
     acc = group_sum<FLAT_LANES>(acc);
-    if (own && sub == 0) ap[b] = pre_ap + (pre_nna * pre_p - acc * pre_dwb);
+    // the per-baseline terms, loaded after the sample stream (holding them
+    // across it would cost the registers of two waves per SIMD)
+    if (own && sub == 0) {
+        int64_t b = (int64_t)(d0 + id) * nbase + k0 + ik;
+        ap[b] = ap[b] + (nna[b] * p[b] - acc * dwb[b]);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_scatter_flat(int nmap, int64_t npc, int64_t ndeg, const int64_t *off,
