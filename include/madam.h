@@ -74,7 +74,7 @@ int madam_set_profiling(void *ctx, int on);
  * madam_solve are not refreshed */
 int madam_iterate(void *ctx, int niter, int restart);
 int madam_reset_timers(void *ctx);
-/* projection kernels (low 4 bits): 0 = segmented 256-entry tiles on the pixel
+/* projection kernels (low 4 bits): 0 = segmented 512-entry tiles on the pixel
  * side and LDS tiles on the baseline side (default), 1 = wave per pixel / lane
  * group per baseline, 2 = LDS tiles on both sides, 3 = segmented tiles on both
  * sides; + 16 = band factorization computes every row (no periodic

@@ -1,4 +1,4 @@
-// Segmented reductions of a sorted entry stream, one wave per 256-entry tile:
+// Segmented reductions of a sorted entry stream, one wave per 512-entry tile:
 // the two per-iteration pointing projections of the PCG.
 //
 //   pixel side    m_q = ccinv_q . sum_{j in pixel q} w_j p[b_j]
@@ -26,7 +26,7 @@
 
 namespace madam {
 
-static constexpr int SR_PER_LANE = 4;
+static constexpr int SR_PER_LANE = 8;
 static constexpr int SR_TILE = 64 * SR_PER_LANE;
 static constexpr int SR_TPB = 256;
 static constexpr int SR_WAVES = SR_TPB / 64;
@@ -242,9 +242,11 @@ __device__ inline unsigned sr_head_bits(const uint8_t *heads, int64_t e0, int64_
     return (byte >> (e0 & 7)) & ((1u << nv) - 1u);
 }
 
-// The transpose goes one component at a time through a 2.2 KB per-wave
-// buffer, the only LDS the kernel uses; with four entries per lane the kernel
-// fits in 72 registers, seven waves per SIMD.
+// The transpose goes one component at a time through a 4.3 KB per-wave
+// buffer, the only LDS the kernel uses.  With eight entries per lane the
+// lane-crossing scan is shared by twice the entries that four gave (the
+// kernel is bound by VALU issue more than by HBM); it takes 94 registers,
+// five waves per SIMD, without spills.
 // The pixels finished in a tile have consecutive ordinals: the lane that
 // completes a sum deposits it in the transpose buffer at slot k - k0, and one
 // uniform pass then has lane l finish ordinal k0 + l, with the inverse matrix
@@ -252,7 +254,7 @@ __device__ inline unsigned sr_head_bits(const uint8_t *heads, int64_t e0, int64_
 // reduction.  Only a tile that finishes more than 64 pixels reads the
 // finishing data of the rest where they complete.
 template <int NC, int WAVES, int RL>
-__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(7, 8))) k_sr_scatter(int64_t nent, const void *ent, const double *pw,
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(5, 8))) k_sr_scatter(int64_t nent, const void *ent, const double *pw,
                                                            const uint8_t *heads, const int64_t *tile_k0,
                                                            const int64_t *fix_end, const int32_t *nzq,
                                                            int64_t nzpix, const double *cc_c, double *m4,
