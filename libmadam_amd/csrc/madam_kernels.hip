@@ -899,9 +899,10 @@ static constexpr int FLAT_LANES = 4;     // lanes per item in the reduction phas
 // the same load instruction, one cache-line access for all of them; the loads
 // of the pointing stream stay in whole 256-byte runs.
 static constexpr int FG_D = 4;                    // detectors per block
-static constexpr int FG_K = FLAT_ITEMS / FG_D;    // baselines per detector per block
+static constexpr int FG_ITEMS = 128;              // (detector, baseline) items per block
+static constexpr int FG_LANES = 256 / FG_ITEMS;   // lanes per item in the reduction phase
+static constexpr int FG_K = FG_ITEMS / FG_D;      // baselines per detector per block
 static constexpr int FG_T = FLAT_TILE / FG_D;     // samples per detector per LDS tile
-static constexpr int FG_U = 4;                    // samples in flight per thread
 
 template <int NMAP, int RL>
 __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, int64_t nbase,
@@ -919,7 +920,7 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
     int nk = (int)(nbase - k0 < FG_K ? nbase - k0 : FG_K);
     int d0 = (int)g * FG_D, nd = ndet - d0 < FG_D ? ndet - d0 : FG_D;
     for (int t = threadIdx.x; t <= nk; t += blockDim.x) bs[t] = bstart[k0 + t];
-    int item = threadIdx.x / FLAT_LANES, sub = threadIdx.x % FLAT_LANES;
+    int item = threadIdx.x / FG_LANES, sub = threadIdx.x % FG_LANES;
     int id = item / FG_K, ik = item % FG_K;   // item = (detector, baseline)
     bool own = id < nd && ik < nk;
     __syncthreads();
@@ -1000,11 +1001,11 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
         if (own) {
             int64_t t1 = t0 + len;
             int64_t a = bs[ik] > t0 ? bs[ik] : t0, e = bs[ik + 1] < t1 ? bs[ik + 1] : t1;
-            for (int64_t t = a + sub; t < e; t += FLAT_LANES) acc += vals[id * FG_T + (t - t0)];
+            for (int64_t t = a + sub; t < e; t += FG_LANES) acc += vals[id * FG_T + (t - t0)];
         }
         __syncthreads();
     }
-    acc = group_sum<FLAT_LANES>(acc);
+    acc = group_sum<FG_LANES>(acc);
     // the per-baseline terms, loaded after the sample stream (holding them
     // across it would cost the registers of two waves per SIMD)
     if (own && sub == 0) {
