@@ -899,7 +899,7 @@ static constexpr int FLAT_LANES = 4;     // lanes per item in the reduction phas
 // the same load instruction, one cache-line access for all of them; the loads
 // of the pointing stream stay in whole 256-byte runs.
 static constexpr int FG_D = 4;                    // detectors per block
-static constexpr int FG_ITEMS = 128;              // (detector, baseline) items per block
+static constexpr int FG_ITEMS = 256;              // (detector, baseline) items per block
 static constexpr int FG_LANES = 256 / FG_ITEMS;   // lanes per item in the reduction phase
 static constexpr int FG_K = FG_ITEMS / FG_D;      // baselines per detector per block
 static constexpr int FG_T = FLAT_TILE / FG_D;     // samples per detector per LDS tile
