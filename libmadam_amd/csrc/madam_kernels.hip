@@ -900,7 +900,8 @@ static constexpr int FLAT_LANES = 4;     // lanes per item in the reduction phas
 // of the pointing stream stay in whole 256-byte runs.
 static constexpr int FG_D = 4;                    // detectors per block
 static constexpr int FG_ITEMS = 256;              // (detector, baseline) items per block
-static constexpr int FG_LANES = 256 / FG_ITEMS;   // lanes per item in the reduction phase
+static constexpr int FG_LANES = FG_ITEMS >= 256 ? 1 : 256 / FG_ITEMS;   // lanes per item (reduction)
+static constexpr int FG_IPT = FG_ITEMS >= 256 ? FG_ITEMS / 256 : 1;     // items per thread
 static constexpr int FG_K = FG_ITEMS / FG_D;      // baselines per detector per block
 static constexpr int FG_T = FLAT_TILE / FG_D;     // samples per detector per LDS tile
 
@@ -920,11 +921,19 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
     int nk = (int)(nbase - k0 < FG_K ? nbase - k0 : FG_K);
     int d0 = (int)g * FG_D, nd = ndet - d0 < FG_D ? ndet - d0 : FG_D;
     for (int t = threadIdx.x; t <= nk; t += blockDim.x) bs[t] = bstart[k0 + t];
-    int item = threadIdx.x / FG_LANES, sub = threadIdx.x % FG_LANES;
-    int id = item / FG_K, ik = item % FG_K;   // item = (detector, baseline)
-    bool own = id < nd && ik < nk;
+    int sub = threadIdx.x % FG_LANES;
+    int id[FG_IPT], ik[FG_IPT];   // items (detector, baseline)
+    bool own[FG_IPT];
+    double acc[FG_IPT];
+#pragma unroll
+    for (int i = 0; i < FG_IPT; ++i) {
+        int item = threadIdx.x / FG_LANES + i * (256 / FG_LANES);
+        id[i] = item / FG_K;
+        ik[i] = item % FG_K;
+        own[i] = id[i] < nd && ik[i] < nk;
+        acc[i] = 0.0;
+    }
     __syncthreads();
-    double acc = 0.0;
     const int dsh = __ffsll((unsigned long long)ndeg) - 1;   // ndeg is a power of 4
     int64_t S0 = bs[0], S1 = bs[nk];
     for (int64_t t0 = S0; t0 < S1; t0 += FG_T) {
@@ -998,19 +1007,24 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
             }
         }
         __syncthreads();
-        if (own) {
-            int64_t t1 = t0 + len;
-            int64_t a = bs[ik] > t0 ? bs[ik] : t0, e = bs[ik + 1] < t1 ? bs[ik + 1] : t1;
-            for (int64_t t = a + sub; t < e; t += FG_LANES) acc += vals[id * FG_T + (t - t0)];
-        }
+#pragma unroll
+        for (int i = 0; i < FG_IPT; ++i)
+            if (own[i]) {
+                int64_t t1 = t0 + len;
+                int64_t a = bs[ik[i]] > t0 ? bs[ik[i]] : t0, e = bs[ik[i] + 1] < t1 ? bs[ik[i] + 1] : t1;
+                for (int64_t t = a + sub; t < e; t += FG_LANES) acc[i] += vals[id[i] * FG_T + (t - t0)];
+            }
         __syncthreads();
     }
-    acc = group_sum<FG_LANES>(acc);
     // the per-baseline terms, loaded after the sample stream (holding them
     // across it would cost the registers of two waves per SIMD)
-    if (own && sub == 0) {
-        int64_t b = (int64_t)(d0 + id) * nbase + k0 + ik;
-        ap[b] = ap[b] + (nna[b] * p[b] - acc * dwb[b]);
+#pragma unroll
+    for (int i = 0; i < FG_IPT; ++i) {
+        double a = group_sum<FG_LANES>(acc[i]);
+        if (own[i] && sub == 0) {
+            int64_t b = (int64_t)(d0 + id[i]) * nbase + k0 + ik[i];
+            ap[b] = ap[b] + (nna[b] * p[b] - a * dwb[b]);
+        }
     }
 }
 
