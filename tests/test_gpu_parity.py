@@ -146,6 +146,24 @@ def test_band_factor_periodic_continuation(flags):
     assert np.array_equal(c1["signal"], c2["signal"])
 
 
+@pytest.mark.parametrize("width", [20, 40])
+def test_band_preconditioner_wide(width):
+    """Wider bands (kd classes 19 and 39: the factor's window no longer fits
+    one 16-lane DPP row) against the oracle, and the periodic continuation of
+    the factorization bitwise against the full column loop at that width."""
+    import libmadam_amd
+    case = ring(prior=True, ndet=4, nsamp=200000, nperiod=2,
+                extra=dict(precond_width_min=width, precond_width_max=width))
+    g, o, c1, c2 = both(case)
+    assert g["stats"]["precond_kind"] == 3 and g["stats"]["nband"] == width
+    check_parity(g, o, c1, c2)
+    f = libmadam_amd.run(*mc.call_args(mc.copy_case(case)), kernel_variant=16)
+    print("band rows factored", g["stats"]["band_rows_factored"], "of", f["stats"]["band_rows_factored"])
+    assert g["stats"]["band_rows_factored"] < f["stats"]["band_rows_factored"]
+    for k in ("aa", "map"):
+        assert np.array_equal(g[k], f[k]), k
+
+
 def test_repeated_solve_reuses_preconditioner():
     """A second solve on the same pointing with new TOD reuses the segments,
     filters and band preconditioner (nna is bitwise unchanged) and gives
