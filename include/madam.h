@@ -66,6 +66,12 @@ int madam_get_sizes(void *ctx, long *out6);
 int madam_get_maps(void *ctx, double *map, double *binmap, int32_t *hits, double *cc, int32_t *mask, double *crit);
 /* baselines [ndet][nbase] */
 int madam_get_baselines(void *ctx, double *aa, double *yba, double *nna);
+/* run statistics, out[0..n): niter, rrinit, rzinit, t_pointing, t_bin, t_init,
+ * t_precond, t_iter, t_final, t_scatter, t_gather, t_prior, t_apply_prec,
+ * n_scatter, n_gather, precond_kind, nband, nseg, nof, rr_final, npix_hit,
+ * nactive, t_total, t_step, ncg, band_chunks, band_chunks_unique, band_blocks,
+ * band_blocks_unique, rec_bytes, band_rows_factored, prec_reused (the order of
+ * STAT_NAMES in libmadam_amd/__init__.py) */
 int madam_get_stats(void *ctx, double *out, int n);
 int madam_get_rr_history(void *ctx, double *out, int n);
 int madam_set_profiling(void *ctx, int on);
