@@ -1471,7 +1471,7 @@ void launch_axpby_nna(hipStream_t s, double *ap, const double *nna, const double
 
 // ---------------------------------------------------------------- deterministic masked dots
 
-static constexpr int DOT_BLOCKS_MAX = 2048;
+static constexpr int DOT_BLOCKS_MAX = 8192;
 static constexpr int DOT_CHUNK = 8;   // elements per thread per step (one 8-byte mask load)
 
 int dots_nblocks(int64_t n)
@@ -1550,27 +1550,40 @@ void launch_dots(hipStream_t s, int which, const double *a, const double *b, con
     k_dots<<<nblocks, TPB, 0, s>>>(which, a, b, c, mask, n, partials);
 }
 
-// sums the block partials in a fixed order into out[0..nval)
-__global__ void k_finalize(const double *partials, int nblocks, int nval, double *out)
+// sums the block partials in a fixed order into out[0..nval): thread t adds
+// blocks t, t + 1024, ... (eight loads in flight per value), then a tree
+static constexpr int FIN_TPB = 1024;
+__global__ void __launch_bounds__(FIN_TPB) k_finalize(const double *partials, int nblocks, int nval, double *out)
 {
-    __shared__ double sh[3][TPB];
+    __shared__ double sh[3][FIN_TPB];
     double t[3] = {0, 0, 0};
-    for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
-        for (int v = 0; v < 3; ++v) t[v] += partials[i * 3 + v];
-    for (int v = 0; v < 3; ++v) sh[v][threadIdx.x] = t[v];
+    for (int i0 = threadIdx.x; i0 < nblocks; i0 += 8 * FIN_TPB) {
+        double v[8][3];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int i = i0 + u * FIN_TPB;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) v[u][q] = i < nblocks ? partials[i * 3 + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) t[q] += v[u][q];
+    }
+    for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] = t[q];
     __syncthreads();
-    for (int o = TPB / 2; o > 0; o >>= 1) {
+    for (int o = FIN_TPB / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o)
-            for (int v = 0; v < 3; ++v) sh[v][threadIdx.x] += sh[v][threadIdx.x + o];
+            for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int v = 0; v < nval; ++v) out[v] = sh[v][0];
+        for (int q = 0; q < nval; ++q) out[q] = sh[q][0];
 }
 
 void launch_finalize(hipStream_t s, const double *partials, int nblocks, int nval, double *out)
 {
-    k_finalize<<<1, TPB, 0, s>>>(partials, nblocks, nval, out);
+    k_finalize<<<1, FIN_TPB, 0, s>>>(partials, nblocks, nval, out);
 }
 
 // scal: [0] rz, [1] pap, [2] rz_new, [3] rz2, [4] rr, [5] beta
