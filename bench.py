@@ -164,8 +164,8 @@ def main():
                          "size, the reference point for the FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: every GPU box here has one GPU, and the multi-GPU sharding is "
-              "described in DESIGN.md, not run", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1): the solve runs on one GPU, and sharding it across "
+              "several GPUs is not built", file=sys.stderr)
         sys.exit(2)
     ndet, nsamp, nside, fs, base, nperiod = a.ndet, 2 ** a.log2nsamp, 512, 100.0, 0.25, 2
     print("bench: workload %d det x 2^%d samples, Nside %d" % (ndet, a.log2nsamp, nside), file=sys.stderr, flush=True)
