@@ -534,12 +534,20 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
 // registers while consecutive steps share it (runs of equal chunks or groups,
 // through cmap / gmap); it is reloaded only where the run changes.  The sums
 // are taken in the same order on every path.
+// v + tr . st: the KD products are independent, and a pairwise tree adds them
+// (depth log2 KD instead of a KD-long dependent chain of FMAs on the path
+// from one step's state to the next)
 template <int KD>
 __device__ inline double chain_step(double v, const double (&tr)[KD], double st)
 {
+    double pr[KD];
 #pragma unroll
-    for (int q = 0; q < KD; ++q) v += tr[q] * readlane_d(st, q);
-    return v;
+    for (int q = 0; q < KD; ++q) pr[q] = tr[q] * readlane_d(st, q);
+#pragma unroll
+    for (int w = 1; w < KD; w *= 2)
+#pragma unroll
+        for (int q = 0; q + w < KD; q += 2 * w) pr[q] += pr[q + w];
+    return v + pr[0];
 }
 
 template <int KD>
