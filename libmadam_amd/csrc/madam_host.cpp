@@ -1401,8 +1401,7 @@ static int pcg_steps(Context &c, int nmax, bool converge)
         apply_precond(c, c.d_r.p, c.d_z.p);
         if (prof) t_pc.stop(s);
         launch_dots(s, 1, c.d_r.p, c.d_z.p, c.d_ro.p, nullptr, c.d_rmask.p, nb, c.d_part.p, nblk);
-        launch_finalize(s, c.d_part.p, nblk, 3, c.d_scal.p + 2);
-        launch_scalars(s, c.d_scal.p);
+        launch_finalize_scalars(s, c.d_part.p, nblk, c.d_scal.p);
         bool last = false;
         if (converge) {
             d2h(hs, c.d_scal.p, 8, s);

@@ -188,6 +188,10 @@ void launch_dots(hipStream_t s, int which, const double *a, const double *b, con
                  const int8_t *mask, int64_t n, double *partials, int nblocks);
 int dots_nblocks(int64_t n);
 void launch_finalize(hipStream_t s, const double *partials, int nblocks, int nval, double *out);
+// launch_finalize of the three r/z dot products into scal[2..4], then, in the
+// same launch, beta = (scal[2] - scal[3]) / scal[0] into scal[5], the old rz
+// into scal[6] and the new rz into scal[0]
+void launch_finalize_scalars(hipStream_t s, const double *partials, int nblocks, double *scal);
 // PCG vector steps, scalars read from device memory
 void launch_pcg_update_r(hipStream_t s, const double *scal, double *aa, double *r, double *ro, const double *p,
                          const double *ap, int64_t n);
@@ -196,7 +200,6 @@ void launch_axpby_nna(hipStream_t s, double *ap, const double *nna, const double
 
 void launch_minmax(hipStream_t s, const int32_t *pix, int64_t n, int *mn, int *mx);
 void launch_restore_pixels(hipStream_t s, int32_t *pix, int64_t n, int32_t dummy);
-void launch_scalars(hipStream_t s, double *scal);
 void launch_fprecond(hipStream_t s, const double *fcov, int64_t nh, const double *wpsd, int nshort, double *out,
                      int64_t npsd);
 

@@ -1553,7 +1553,10 @@ void launch_dots(hipStream_t s, int which, const double *a, const double *b, con
 // sums the block partials in a fixed order into out[0..nval): thread t adds
 // blocks t, t + 1024, ... (eight loads in flight per value), then a tree
 static constexpr int FIN_TPB = 1024;
-__global__ void __launch_bounds__(FIN_TPB) k_finalize(const double *partials, int nblocks, int nval, double *out)
+// with scal (out = scal + 2): then beta = (scal[2] - scal[3]) / scal[0] into
+// scal[5], the old rz into scal[6] and the new rz into scal[0]
+__global__ void __launch_bounds__(FIN_TPB) k_finalize(const double *partials, int nblocks, int nval, double *out,
+                                                      double *scal)
 {
     __shared__ double sh[3][FIN_TPB];
     double t[3] = {0, 0, 0};
@@ -1577,13 +1580,25 @@ __global__ void __launch_bounds__(FIN_TPB) k_finalize(const double *partials, in
             for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         for (int q = 0; q < nval; ++q) out[q] = sh[q][0];
+        if (scal) {
+            double beta = (scal[2] - scal[3]) / scal[0];
+            scal[5] = beta;
+            scal[6] = scal[0];
+            scal[0] = scal[2];
+        }
+    }
 }
 
 void launch_finalize(hipStream_t s, const double *partials, int nblocks, int nval, double *out)
 {
-    k_finalize<<<1, FIN_TPB, 0, s>>>(partials, nblocks, nval, out);
+    k_finalize<<<1, FIN_TPB, 0, s>>>(partials, nblocks, nval, out, nullptr);
+}
+
+void launch_finalize_scalars(hipStream_t s, const double *partials, int nblocks, double *scal)
+{
+    k_finalize<<<1, FIN_TPB, 0, s>>>(partials, nblocks, 3, scal + 2, scal);
 }
 
 // scal: [0] rz, [1] pap, [2] rz_new, [3] rz2, [4] rr, [5] beta
@@ -1679,21 +1694,6 @@ void launch_restore_pixels(hipStream_t s, int32_t *pix, int64_t n, int32_t dummy
 }
 
 // beta = (rz_new - rz2) / rz_old; rz_old <- rz_new   (scal layout as k_update_r)
-__global__ void k_scalars(double *scal)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double beta = (scal[2] - scal[3]) / scal[0];
-        scal[5] = beta;
-        scal[6] = scal[0];
-        scal[0] = scal[2];
-    }
-}
-
-void launch_scalars(hipStream_t s, double *scal)
-{
-    k_scalars<<<1, 64, 0, s>>>(scal);
-}
-
 // fprecond rows: 1 / (nshort * w + fcov)  (src/noise_routines.f90:977-993)
 __global__ void k_fprecond(const double *fcov, int64_t nh, const double *wpsd, int nshort, double *out, int64_t npsd)
 {
