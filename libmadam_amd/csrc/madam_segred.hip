@@ -370,38 +370,64 @@ void segred_compact_cc(hipStream_t s, const int32_t *nzq, int64_t nseg, const do
     MADAM_CHECK(hipGetLastError());
 }
 
-// one thread per tile: finish the segment that starts in tile j and continues
-__global__ void k_sr_scatter_fix(int nmap, int64_t ntiles, const int64_t *fix_end, const int64_t *fix_ord,
-                                 const int32_t *nzq, const double4 *part, const double *cc_c, double *m4)
+// Finish the segments that start in a tile and continue into later ones.
+// One wave per 64 tiles, lane l taking tile j = 64 w + l: a chain of up to
+// SR_FIX_SHORT later tiles is summed by its lane in tile order; a longer one
+// (the most-hit pixels span hundreds of tiles) by the whole wave, lane l
+// adding tiles j + 1 + l, j + 1 + l + 64, ..., then a butterfly over the
+// lanes.  Both orders are fixed, so the sums are reproducible.
+static constexpr int SR_FIX_SHORT = 16;
+__global__ void __launch_bounds__(SR_TPB) k_sr_scatter_fix(int nmap, int64_t ntiles, const int64_t *fix_end,
+                                                           const int64_t *fix_ord, const int32_t *nzq,
+                                                           const double4 *part, const double *cc_c, double *m4)
 {
-    int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (j >= ntiles) return;
-    int64_t jl = fix_end[j];
-    if (jl < 0) return;
-    double4 a = part[j * 2 + 1];
-    // in tile order; long chains (the most-hit pixels span hundreds of tiles)
-    // load 8 partials ahead of their additions
-    int64_t jj = j + 1;
-    for (; jj + 8 <= jl + 1; jj += 8) {
-        double4 v[8];
+    int lane = threadIdx.x & 63;
+    int64_t w = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    int64_t j = w * 64 + lane;
+    if (w * 64 >= ntiles) return;   // uniform over the wave
+    int64_t jl = j < ntiles ? fix_end[j] : -1;
+    bool longc = jl >= 0 && jl - j > SR_FIX_SHORT;
+    if (jl >= 0 && !longc) {
+        double4 a = part[j * 2 + 1];
+        double4 v[SR_FIX_SHORT];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = part[(jj + u) * 2];
+        for (int u = 0; u < SR_FIX_SHORT; ++u)
+            if (j + 1 + u <= jl) v[u] = part[(j + 1 + u) * 2];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            a.x += v[u].x;
-            a.y += v[u].y;
-            a.z += v[u].z;
+        for (int u = 0; u < SR_FIX_SHORT; ++u)
+            if (j + 1 + u <= jl) {
+                a.x += v[u].x;
+                a.y += v[u].y;
+                a.z += v[u].z;
+            }
+        int64_t kk = fix_ord[j];
+        double m[3] = {a.x, a.y, a.z};
+        sr_ccmul_store(nmap, cc_c + kk * 6, m, m4 + (int64_t)nzq[kk] * 4);
+    }
+    uint64_t lm = __ballot(longc);
+    while (lm) {
+        int src = __builtin_ctzll(lm);
+        lm &= lm - 1;
+        int64_t j2 = __shfl(j, src, 64), jl2 = __shfl(jl, src, 64);
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int64_t jj = j2 + 1 + lane; jj <= jl2; jj += 64) {
+            double4 v = part[jj * 2];
+            sx += v.x;
+            sy += v.y;
+            sz += v.z;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o, 64);
+            sy += __shfl_xor(sy, o, 64);
+            sz += __shfl_xor(sz, o, 64);
+        }
+        if (lane == 0) {
+            double4 a = part[j2 * 2 + 1];
+            int64_t kk = fix_ord[j2];
+            double m[3] = {a.x + sx, a.y + sy, a.z + sz};
+            sr_ccmul_store(nmap, cc_c + kk * 6, m, m4 + (int64_t)nzq[kk] * 4);
         }
     }
-    for (; jj <= jl; ++jj) {
-        double4 v = part[jj * 2];
-        a.x += v.x;
-        a.y += v.y;
-        a.z += v.z;
-    }
-    int64_t kk = fix_ord[j];
-    double m[3] = {a.x, a.y, a.z};
-    sr_ccmul_store(nmap, cc_c + kk * 6, m, m4 + (int64_t)nzq[kk] * 4);
 }
 
 // ------------------------------------------------------------ baseline side
@@ -601,7 +627,8 @@ void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const void *ent, 
         else MADAM_SC(3, REC16);
     }
 #undef MADAM_SC
-    k_sr_scatter_fix<<<sr_grid(nt, 1 << 30), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part, cc_c, m4);
+    k_sr_scatter_fix<<<(unsigned)((nt + SR_TPB - 1) / SR_TPB), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part,
+                                                                            cc_c, m4);
 }
 
 void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,

@@ -197,6 +197,15 @@ def test_repeated_solve_reuses_preconditioner():
         m.close()
 
 
+def test_long_pixel_chains():
+    """Nside 2: about 16,000 hits per pixel, so each pixel spans some 32 tiles of
+    the pixel-side reduction and its partial sums are joined by the wave-wide
+    path of the fix-up (chains longer than 16 tiles)."""
+    g, o, c1, c2 = both(ring(nside=2, ndet=4, nsamp=200000))
+    assert g["stats"]["npix_hit"] <= 48
+    check_parity(g, o, c1, c2)
+
+
 def test_pointing_record_layouts():
     """The projections read 12-byte records {index, w1, w2} when every
     intensity weight is exactly 1, 4-byte records for temperature-only maps,
