@@ -197,6 +197,19 @@ def test_repeated_solve_reuses_preconditioner():
         m.close()
 
 
+def test_plain_fp64_4det_2e20_nside64():
+    """The plain FP64 destripe at full size: 4 detectors (0/45/90/135 deg) x
+    2^20 samples at 50 Hz, ring scan with a 60 s spin and the spin axis
+    advancing 1 deg/hour, Nside 64 IQU, 1 s baselines, no noise prior, PCG
+    tolerance 1e-12, rcond cut 1e-3 on both pixel-matrix checks."""
+    case = mc.ring_case(ndet=4, nsamp=2 ** 20, nside=64, fsample=50.0, base_first=1.0, nnz=3, seed=7, prior=False,
+                        pointing=pointing_fn(64, fs=50.0, spin=60.0, prec=360 * 3600.0),
+                        extra_pars=dict(cglimit=1e-12))
+    g, o, c1, c2 = both(case)
+    assert g["stats"]["precond_kind"] == 1
+    check_parity(g, o, c1, c2)
+
+
 def test_long_pixel_chains():
     """Nside 2: about 16,000 hits per pixel, so each pixel spans some 32 tiles of
     the pixel-side reduction and its partial sums are joined by the wave-wide
