@@ -210,6 +210,64 @@ def test_plain_fp64_4det_2e20_nside64():
     check_parity(g, o, c1, c2)
 
 
+def test_full_size_sky_and_offsets_recovered():
+    """16 detectors x 2^24 samples at 100 Hz, Nside 512 IQU, 0.25 s baselines in
+    two periods, no noise prior: a TOD of a known sky plus one random offset
+    per baseline and no noise.  The destriping equations are then solved
+    exactly by the sky and the offsets up to a common constant (the offsets'
+    mean trades against the I monopole), so at full size the solve must
+    return Q and U of the sky and I up to that constant, to the accuracy the
+    1e-12 PCG tolerance leaves; the hits must add up to the samples.  That
+    accuracy scales with the offsets over the sky and with the conditioning
+    of the scan (the oracle leaves 3e-4 of Q and U at Nside 8 and 16 with
+    1 mK offsets; this scan, with 2 uK offsets, must stay below 1e-2), and a
+    wrong sign or index fails it by orders of magnitude."""
+    import torch
+    import libmadam_amd
+    from bench import SCAN, SIGMA, detectors, psd_table
+    ndet, nsamp, nside, fs, nbs = 16, 2 ** 24, 512, 100.0, 25
+    npix, half = 12 * nside * nside, nsamp // 2
+    pix = torch.empty(ndet, nsamp, dtype=torch.int32, device="cuda")
+    w = torch.empty(ndet, nsamp, 3, dtype=torch.float32, device="cuda")
+    libmadam_amd.expand_pointing_device(SCAN | dict(fsample=fs), detectors(ndet), nside, 0, nsamp, pix, w)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    sky = torch.randn(npix, 3, generator=g, device="cuda", dtype=torch.float64)
+    sky *= torch.tensor([100e-6, 5e-6, 5e-6], device="cuda", dtype=torch.float64)
+    nbp = (half + nbs - 1) // nbs   # baselines per period (the last one shorter)
+    offs = torch.randn(ndet, 2 * nbp, generator=g, device="cuda", dtype=torch.float64) * 2e-6
+    i = torch.arange(nsamp, device="cuda")
+    k = (i // half) * nbp + (i % half) // nbs
+    tod = torch.empty(ndet, nsamp, device="cuda", dtype=torch.float64)
+    for d in range(ndet):
+        tod[d] = (sky[pix[d].long()] * w[d].double()).sum(-1) + offs[d][k]
+    torch.cuda.synchronize()
+    pars = dict(base_first=0.25, fsample=fs, nside_map=nside, nside_cross=nside, nside_submap=16, kfilter=False,
+                cglimit=1e-12, iter_max=1000, info=0, pixlim_map=1e-3, pixlim_cross=1e-3)
+    m = libmadam_amd.Madam(0)
+    try:
+        freqs, psd = psd_table(fs)
+        m.configure(pars, ["d%02d" % j for j in range(ndet)], np.full(ndet, 1.0 / SIGMA ** 2),
+                    np.arange(nsamp) / fs, np.array([0, half], np.int64), np.ones(ndet, np.int64), np.zeros(ndet),
+                    freqs, np.tile(psd, ndet), 3)
+        m.set_pointing(pix, w, on_device=True)
+        m.solve(tod, on_device=True)
+        out = m.maps()
+        st = m.stats()
+    finally:
+        m.close()
+    skyh = sky.cpu().numpy()
+    good = out["mask"] > 0
+    print("niter", st["niter"], "rr", st["rr_final"], "good pixels", int(good.sum()))
+    assert int(out["hits"].sum()) == ndet * nsamp
+    assert st["rr_final"] < 1e-12 and good.sum() > 0.2 * npix
+    d = out["map"][good] - skyh[good]
+    d[:, 0] -= d[:, 0].mean()
+    rel = np.sqrt(np.mean(d ** 2, axis=0)) / np.sqrt(np.mean(skyh[good] ** 2, axis=0))
+    print("relative rms error I (less its mean), Q, U", rel)
+    assert rel[0] < 1e-3 and rel[1] < 1e-2 and rel[2] < 1e-2, rel
+
+
 def test_long_pixel_chains():
     """Nside 2: about 16,000 hits per pixel, so each pixel spans some 32 tiles of
     the pixel-side reduction and its partial sums are joined by the wave-wide
