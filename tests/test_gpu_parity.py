@@ -19,6 +19,20 @@ pytestmark = pytest.mark.gpu
 RTOL_RMS = 1e-10
 
 
+@pytest.fixture(scope="module", autouse=True)
+def _torch_gpu_first():
+    """In a process that uses both, torch's GPU runtime is initialised before
+    libmadam.so's, the order bench.py uses (the reverse order leaves torch
+    without a device); only the full-size test builds its inputs with torch."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
+    yield
+
+
 def rel_rms(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
