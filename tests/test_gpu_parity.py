@@ -282,6 +282,17 @@ def test_full_size_sky_and_offsets_recovered():
     assert rel[0] < 1e-3 and rel[1] < 1e-2 and rel[2] < 1e-2, rel
 
 
+def test_odd_baseline_count():
+    """401 baselines (one detector, temperature only, with and without the
+    noise prior): vector lengths that are not a multiple of the dot products'
+    8-element chunks, of the 64-baseline gather blocks or of the band's 64-row
+    chunks."""
+    for prior in (False, True):
+        g, o, c1, c2 = both(ring(ndet=1, nsamp=20050, nnz=1, prior=prior))
+        assert g["yba"].size == 401
+        check_parity(g, o, c1, c2)
+
+
 def test_long_pixel_chains():
     """Nside 2: about 16,000 hits per pixel, so each pixel spans some 32 tiles of
     the pixel-side reduction and its partial sums are joined by the wave-wide
