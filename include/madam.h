@@ -70,8 +70,8 @@ int madam_get_baselines(void *ctx, double *aa, double *yba, double *nna);
  * t_precond, t_iter, t_final, t_scatter, t_gather, t_prior, t_apply_prec,
  * n_scatter, n_gather, precond_kind, nband, nseg, nof, rr_final, npix_hit,
  * nactive, t_total, t_step, ncg, band_chunks, band_chunks_unique, band_blocks,
- * band_blocks_unique, rec_bytes, band_rows_factored, prec_reused (the order of
- * STAT_NAMES in libmadam_amd/__init__.py) */
+ * band_blocks_unique, rec_bytes, band_rows_factored, prec_reused,
+ * pointing_reused, nwide (the order of STAT_NAMES in libmadam_amd/__init__.py) */
 int madam_get_stats(void *ctx, double *out, int n);
 int madam_get_rr_history(void *ctx, double *out, int n);
 int madam_set_profiling(void *ctx, int on);
@@ -80,6 +80,11 @@ int madam_set_profiling(void *ctx, int on);
  * madam_solve are not refreshed */
 int madam_iterate(void *ctx, int niter, int restart);
 int madam_reset_timers(void *ctx);
+/* one operator of the last madam_solve on host vectors x, y of ndet * nbase
+ * baselines: which 0 = noise prior y = C_a^-1 x (cinvmul, src/noise_routines.f90:745-913),
+ * 1 = preconditioner y = M^-1 x (src/noise_routines.f90:1159-1409), 2 = the
+ * destriping operator y = A x of one PCG step (src/smadam_routines.f90:808-872) */
+int madam_apply(void *ctx, int which, const double *x, double *y);
 /* projection kernels (low 4 bits): 0 = segmented 512-entry tiles on the pixel
  * side and LDS tiles on the baseline side (default), 1 = wave per pixel / lane
  * group per baseline, 2 = LDS tiles on both sides, 3 = segmented tiles on both

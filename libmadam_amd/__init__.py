@@ -26,7 +26,7 @@ STAT_NAMES = ["niter", "rrinit", "rzinit", "t_pointing", "t_bin", "t_init", "t_p
               "t_scatter", "t_gather", "t_prior", "t_apply_prec", "n_scatter", "n_gather", "precond_kind",
               "nband", "nseg", "nof", "rr_final", "npix_hit", "nactive", "t_total", "t_step", "ncg",
               "band_chunks", "band_chunks_unique", "band_blocks", "band_blocks_unique", "rec_bytes",
-              "band_rows_factored", "prec_reused"]
+              "band_rows_factored", "prec_reused", "pointing_reused", "nwide"]
 
 
 class MadamError(RuntimeError):
@@ -64,6 +64,8 @@ def lib():
         L.madam_get_rr_history.argtypes = [vp, vp, i32]
         L.madam_iterate.restype = i32
         L.madam_iterate.argtypes = [vp, i32, i32]
+        L.madam_apply.restype = i32
+        L.madam_apply.argtypes = [vp, i32, vp, vp]
         L.madam_set_kernel_variant.restype = i32
         L.madam_set_kernel_variant.argtypes = [vp, i32]
         L.madam_reset_timers.restype = i32
@@ -210,6 +212,14 @@ class Madam:
     def iterate(self, niter, restart=False):
         _check(lib().madam_iterate(self._h, int(niter), int(restart)))
 
+    def apply(self, which, x):
+        """which: 0 noise prior, 1 preconditioner, 2 destriping operator A (on a
+        host vector of ndet * nbase baselines, after solve())."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        y = np.zeros_like(x)
+        _check(lib().madam_apply(self._h, int(which), _ptr(x), _ptr(y)))
+        return y
+
     def set_kernel_variant(self, v):
         _check(lib().madam_set_kernel_variant(self._h, int(v)))
 
@@ -218,6 +228,17 @@ class Madam:
 
     def set_profiling(self, on=True):
         _check(lib().madam_set_profiling(self._h, int(on)))
+
+
+def cache_stats():
+    """Statistics of the context that destripe(mcmode=T) cached (None if empty)."""
+    h = ct.c_void_p()
+    lib().madam_cache_handle(ct.byref(h))
+    if not h.value:
+        return None
+    out = np.zeros(len(STAT_NAMES))
+    _check(lib().madam_get_stats(h, _ptr(out), len(STAT_NAMES)))
+    return dict(zip(STAT_NAMES, out.tolist()))
 
 
 def run(pars, dets, weights, timestamps, pixels, pixweights, signal, periods, npsd, psdstarts, psdfreqs, psdvals,

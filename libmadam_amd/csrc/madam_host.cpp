@@ -351,7 +351,7 @@ enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_
        ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
        ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_NCG,
        ST_BAND_CHUNKS, ST_BAND_CHUNKS_UNIQUE, ST_BAND_BLOCKS, ST_BAND_BLOCKS_UNIQUE, ST_REC_BYTES,
-       ST_BAND_ROWS_FACTORED, ST_PREC_REUSED, ST_COUNT };
+       ST_BAND_ROWS_FACTORED, ST_PREC_REUSED, ST_POINTING_REUSED, ST_NWIDE, ST_COUNT };
 
 struct Context {
     int device = 0;
@@ -431,6 +431,8 @@ struct Context {
     // segments / prior
     std::vector<Segment> seg;
     DBuf<Segment> d_seg;
+    std::vector<Segment> segp;         // seg with the preconditioner's PSD index
+    DBuf<Segment> d_segp;
     std::vector<Segment> segconv;      // segments the prior convolves (PSD found)
     DBuf<Segment> d_segconv;
     // the convolved segments (the band and CG preconditioner segments) cover
@@ -441,7 +443,7 @@ struct Context {
     DBuf<double> d_fcov_rows, d_fprec_rows;   // the spectra in the fused row pass's order
     FftPlan fft{};
     bool fft_ready = false;
-    int64_t fft_batch = 0;
+    int64_t fft_batch = 0, fft_nseg_max = 0;
     int precond_kind = 0;
     // inner-CG preconditioner segments (band width criterion or Cholesky failed)
     std::vector<Segment> segcg;
@@ -719,7 +721,7 @@ static void configure(Context &c, const char *parstring, long ndet, const char *
         int64_t nof = 1;
         while (nof < nof_min) nof *= 2;
         if (nof < 4) nof = 4;
-        if (nof > (1LL << 21)) fail("FFT length %lld exceeds the supported 2^21", (long long)nof);
+        if (nof > (1LL << 22)) fail("FFT length %lld exceeds the supported 2^22", (long long)nof);
         c.nof = nof;
         for (long d = 0; d < ndet; ++d) {
             if (!c.det[d].has_psd) fail("All detectors must have at least one PSD to build the filter");
@@ -844,7 +846,10 @@ static void build_filters(Context &c)
 
 // ------------------------------------------------------------------ pointing
 
-static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
+// dummy_is_flag: pixel 12 * nside_max**2, which a previous call leaves in the
+// caller's array for its flagged samples, flags the sample again (the cached
+// path, src/smadam.F90:713-715)
+static void set_pointing(Context &c, int32_t *pix, float *w, int on_device, bool dummy_is_flag = false)
 {
     if (!c.configured) fail("madam_set_pointing called before configure");
     hipStream_t s = c.stream;
@@ -861,6 +866,7 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
         c.d_pix = c.d_pix_own.p;
         c.d_w = c.d_w_own.p;
     }
+    if (dummy_is_flag) launch_dummy_to_flag(s, c.d_pix, n, (int32_t)(12LL * c.nside_max * c.nside_max));
     // pixel range check (src/smadam.F90:158-171)
     if (n > 0) {
         DBuf<int> mm;
@@ -959,6 +965,7 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device)
     c.pointed = true;
     c.tiles_built = false;
     c.prec_cached = false;
+    c.stats[ST_POINTING_REUSED] = 0;
 }
 
 // ------------------------------------------------------------------ solve
@@ -1103,7 +1110,7 @@ static void cg_apply(Context &c, const double *r, double *z)
         h2d(c.d_segact.p, live.data(), live.size(), s);
         // ap = convolve(p) + nna p
         fft_convolve_segments(s, c.fft, c.d_segact.p, (int64_t)live.size(), c.d_cg_p.p, c.d_cg_ap.p, c.d_fcov.p,
-                              c.d_fcov_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, (int64_t)c.segconv.size(),
+                              c.d_fcov_rows.p, c.nof / 2 + 1, c.d_fftwork.p, c.fft_batch, c.fft_nseg_max,
                               live.data());
         launch_cg_nna(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_nna.p, c.d_cg_p.p, c.d_cg_ap.p);
         launch_cg_dot(s, c.d_segcg.p, ns, c.d_cg_act.p, c.d_cg_p.p, c.d_cg_ap.p, c.d_cg_part.p, an_d);
@@ -1187,6 +1194,15 @@ static void build_segments(Context &c)
         if (c.p.kfilter && psd_index(c, g.det, c.btime[k0]) < 0) g.skip = 1;
     }
     if (!c.seg.empty()) h2d(c.d_seg.p, c.seg.data(), c.seg.size(), s);
+    // the preconditioner looks the PSD up at the trimmed segment's first
+    // baseline (src/noise_routines.f90:1063, 1229), the convolution at the
+    // interval's (src/noise_routines.f90:830)
+    c.segp = c.seg;
+    for (auto &g : c.segp) g.ipsd = c.p.kfilter ? psd_index(c, g.det, c.btime[g.b0 - (int64_t)g.det * c.nbase]) : -1;
+    if (!c.segp.empty()) {
+        c.d_segp.alloc(c.segp.size());
+        h2d(c.d_segp.p, c.segp.data(), c.segp.size(), s);
+    }
     c.segconv.clear();
     c.prior_covers_all = false;
     for (auto &g : c.seg)
@@ -1206,6 +1222,9 @@ static void build_segments(Context &c)
     MADAM_CHECK(hipStreamSynchronize(s));
     c.stats[ST_NSEG] = (double)c.seg.size();
 }
+
+// widest band (nband = kd + 1) of the band-Cholesky kernels' kd classes
+static constexpr int BAND_WIDTH_GPU = 50;
 
 static void build_preconditioner(Context &c)
 {
@@ -1241,14 +1260,19 @@ static void build_preconditioner(Context &c)
     MADAM_CHECK(hipStreamSynchronize(s));
     // band width per PSD and segment (src/noise_routines.f90:1071-1090)
     const double plim = F32(1 - 1e-5f);
-    std::vector<int> nband(c.seg.size(), 0);
+    std::vector<int> nband(c.segp.size(), 0);
     std::vector<double> ptot(c.npsdtot, 0.0);
     for (int ip = 0; ip < c.npsdtot; ++ip)
         for (int64_t i = 0; i < c.nof / 2; ++i) ptot[ip] += ic[(size_t)ip * c.nof + i] * ic[(size_t)ip * c.nof + i];
-    std::vector<int> tries(c.seg.size(), 1);
-    std::vector<char> use_cg(c.seg.size(), 0);
-    for (size_t is = 0; is < c.seg.size(); ++is) {
-        const Segment &g = c.seg[is];
+    std::vector<int> tries(c.segp.size(), 1);
+    std::vector<char> use_cg(c.segp.size(), 0);
+    // Segments whose band would be wider than the factor kernels take
+    // (BAND_WIDTH_GPU baselines) use the inner-CG preconditioner of the same
+    // segment instead: another symmetric positive definite preconditioner, so
+    // the PCG converges to the same solution, in a different number of steps.
+    int64_t nwide = 0;
+    for (size_t is = 0; is < c.segp.size(); ++is) {
+        const Segment &g = c.segp[is];
         if (g.skip || g.ipsd < 0) continue;
         if (p.use_cgprecond) { use_cg[is] = 1; continue; }
         const double *v = ic.data() + (size_t)g.ipsd * c.nof;
@@ -1264,19 +1288,20 @@ static void build_preconditioner(Context &c)
             tr++;
         }
         if (!ok) { use_cg[is] = 1; continue; }   // src/noise_routines.f90:1113-1120
+        if (nb > BAND_WIDTH_GPU) { use_cg[is] = 1; nwide++; continue; }
         nband[is] = nb;
         tries[is] = tr;
     }
     for (;;) {
-        std::vector<int> info(c.seg.size(), 0);
-        if (band_build(s, c.band, c.seg.data(), c.d_seg.p, (int64_t)c.seg.size(), nband.data(), c.d_invcov.p, c.nof,
+        std::vector<int> info(c.segp.size(), 0);
+        if (band_build(s, c.band, c.segp.data(), c.d_segp.p, (int64_t)c.segp.size(), nband.data(), c.d_invcov.p, c.nof,
                        c.d_nna.p, info.data(), !(c.kernel_variant & 16)) != 0)
-            fail("band preconditioner wider than 50 baselines is not supported on the GPU path");
+            fail("band preconditioner wider than %d baselines reached the factor kernels", BAND_WIDTH_GPU);
         c.band_ready = true;
         bool retry = false;
-        for (size_t is = 0; is < c.seg.size(); ++is) {
+        for (size_t is = 0; is < c.segp.size(); ++is) {
             if (info[is] == 0 || nband[is] == 0) continue;
-            const Segment &g = c.seg[is];
+            const Segment &g = c.segp[is];
             if (tries[is] == 1000 || nband[is] == g.noba || nband[is] == p.precond_width_max) {
                 use_cg[is] = 1;   // Cholesky failed at the widest band: inner CG
                 nband[is] = 0;
@@ -1284,6 +1309,12 @@ static void build_preconditioner(Context &c)
             }
             tries[is]++;
             nband[is] = std::min(nband[is] + p.precond_width_min, p.precond_width_max);
+            if (nband[is] > BAND_WIDTH_GPU) {
+                use_cg[is] = 1;
+                nband[is] = 0;
+                nwide++;
+                continue;
+            }
             retry = true;
         }
         if (!retry) break;
@@ -1300,10 +1331,10 @@ static void build_preconditioner(Context &c)
     c.stats[ST_BAND_ROWS_FACTORED] = (double)c.band.nrows_factored;
     c.segcg.clear();
     c.ic0cg.clear();
-    for (size_t is = 0; is < c.seg.size(); ++is)
+    for (size_t is = 0; is < c.segp.size(); ++is)
         if (use_cg[is]) {
-            c.segcg.push_back(c.seg[is]);
-            c.ic0cg.push_back(ic[(size_t)c.seg[is].ipsd * c.nof]);
+            c.segcg.push_back(c.segp[is]);
+            c.ic0cg.push_back(ic[(size_t)c.segp[is].ipsd * c.nof]);
         }
     if (!c.segcg.empty()) {
         c.d_segcg.alloc(c.segcg.size());
@@ -1313,6 +1344,7 @@ static void build_preconditioner(Context &c)
         MADAM_CHECK(hipStreamSynchronize(s));
     }
     c.stats[ST_NCG] = (double)c.segcg.size();
+    c.stats[ST_NWIDE] = (double)nwide;
     {
         int64_t rows = c.band.nrows;
         for (const Segment &g : c.segcg) rows += g.noba;
@@ -1433,6 +1465,49 @@ static int pcg_steps(Context &c, int nmax, bool converge)
     return done;
 }
 
+// y = A x for the destriping operator of the last solve: FᵀZF x (pixel side,
+// ccmultiply, baseline side) plus the noise prior (or diagfilter) term, as one
+// PCG step applies it (src/smadam_routines.f90:808-872).  pw is scratch of nbtot.
+static void apply_operator(Context &c, const double *x, double *y, double *pw)
+{
+    hipStream_t s = c.stream;
+    const Params &p = c.p;
+    int64_t nb = c.nbtot;
+    launch_pw_slots(s, x, c.d_bord.p, c.d_dwb_s.p, pw, nb);
+    launch_scatter_seg(s, c.nmap, c.nactive, c.rec_layout == REC16 ? (const void *)c.d_ent.p : c.d_entc.p,
+                       c.rec_layout, pw, c.sr_s.heads.p, c.sr_s.k0.p, c.sr_s.fix_end.p, c.sr_s.fix_ord.p,
+                       c.d_nzq.p, c.nzpix, c.d_ccinv_c.p, c.d_m4x.p, c.d_sr_part.p);
+    if (p.kfilter) apply_prior(c, x, y);
+    else if (p.diagfilter != 0) launch_axpby_nna(s, y, c.d_nna.p, x, p.diagfilter, nb);
+    else launch_fill(s, y, 0.0, nb);
+    launch_gather_flat(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p,
+                       c.rec_layout == REC16 ? (const void *)c.d_pixw.p : c.d_pixc.p, c.rec_layout, c.d_m4x.p,
+                       c.ndeg, c.d_dwb.p, c.d_nna.p, x, y);
+}
+
+// one operator of the last solve on a host vector of nbtot baselines:
+// which 0 = noise prior C_a^-1 x (cinvmul), 1 = preconditioner M^-1 x,
+// 2 = the destriping operator A x
+static void apply_host(Context &c, int which, const double *hx, double *hy)
+{
+    if (!c.solved || !c.p.kfirst || !c.tiles_built) fail("madam_apply needs a completed madam_solve with destriping");
+    if (which == 0 && !c.p.kfilter) fail("madam_apply: no noise prior (kfilter = F)");
+    if (which < 0 || which > 2) fail("madam_apply: unknown operator %d", which);
+    hipStream_t s = c.stream;
+    int64_t nb = c.nbtot;
+    DBuf<double> x, y, pw;
+    x.alloc(std::max<int64_t>(nb, 1));
+    y.alloc(std::max<int64_t>(nb, 1));
+    pw.alloc(std::max<int64_t>(nb, 1));
+    h2d(x.p, hx, nb, s);
+    if (which == 0) apply_prior(c, x.p, y.p);
+    else if (which == 1) apply_precond(c, x.p, y.p);
+    else apply_operator(c, x.p, y.p, pw.p);
+    MADAM_CHECK(hipGetLastError());
+    d2h(hy, y.p, nb, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+}
+
 static void run_subsets(Context &c);
 
 static void solve(Context &c, double *signal, int on_device)
@@ -1518,13 +1593,15 @@ static void solve(Context &c, double *signal, int on_device)
             // 128 MiB, so the row and inverse column passes re-read it from the
             // 256 MB last-level cache (32 segments of 2^19 points: 2 x 16 is
             // 6 % faster than 1 x 32; smaller batches leave the GPU underfilled)
-            int64_t nsc = std::max<int64_t>((int64_t)c.segconv.size(), 1);
+            // the inner-CG preconditioner convolves its segments with the same buffers
+            int64_t nsc = std::max<int64_t>((int64_t)std::max(c.segconv.size(), c.segp.size()), 1);
             int64_t cap = std::max<int64_t>((128ll << 20) / (16 * c.fft.N), 1);
             cap = std::min<int64_t>(cap, 64);
             int64_t nbatch = (nsc + cap - 1) / cap;
             c.fft_batch = (nsc + nbatch - 1) / nbatch;
             // batch buffer, ipsd ints and the per-segment partial sums
             c.d_fftwork.alloc(fft_convolve_work_doubles(c.fft, c.fft_batch, nsc));
+            c.fft_nseg_max = nsc;
         }
         if (!reuse) {
             build_preconditioner(c);
@@ -2240,6 +2317,33 @@ static void run_subsets(Context &c)
     }
 }
 
+// The caller's pix and weights are bitwise the ones the context's set-up was
+// built from, pix as the reference leaves it in the caller's array (flagged
+// samples as 12 * nside_max**2, which the device copy also holds once they
+// have been written back).
+static bool same_host_pointing(Context &c, const int32_t *pix, const float *w)
+{
+    if (!c.pointed || c.pix_host_dirty || !c.d_pix_own.p || c.d_pix != c.d_pix_own.p || c.d_w != c.d_w_own.p)
+        return false;
+    hipStream_t s = c.stream;
+    int64_t n = (int64_t)c.ndet * c.nsamp;
+    DBuf<int32_t> tp;
+    DBuf<float> tw;
+    DBuf<int> flag;
+    tp.alloc(std::max<int64_t>(n, 1));
+    tw.alloc(std::max<int64_t>(n * c.nnz, 1));
+    flag.alloc(1);
+    MADAM_CHECK(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+    h2d(tp.p, pix, n, s);
+    h2d(tw.p, w, (size_t)n * c.nnz, s);
+    launch_words_differ(s, (const uint32_t *)tp.p, (const uint32_t *)c.d_pix_own.p, n, flag.p);
+    launch_words_differ(s, (const uint32_t *)tw.p, (const uint32_t *)c.d_w_own.p, n * c.nnz, flag.p);
+    int differ = 1;
+    d2h(&differ, flag.p, 1, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    return differ == 0;
+}
+
 static Context *g_cache = nullptr;   // destripe(mcmode=T) keeps its context here
 
 }  // namespace madam
@@ -2339,6 +2443,11 @@ MADAM_API int madam_iterate(void *ctx, int niter, int restart)
         if (restart) pcg_init(c);
         pcg_steps(c, niter, false);
     });
+}
+
+MADAM_API int madam_apply(void *ctx, int which, const double *x, double *y)
+{
+    return guarded([&] { apply_host(*(Context *)ctx, which, x, y); });
 }
 
 MADAM_API int madam_set_kernel_variant(void *ctx, int v)
@@ -2516,8 +2625,13 @@ MADAM_API void destripe_with_cache(int comm, long ndet, long nsamp, long nnz, vo
         std::string path = (outpath && outpath[0]) ? strip(outpath) : c.p.path_output;
         c.subsets_enabled = true;
         c.subsets_path = path;
-        set_pointing(c, (int32_t *)pix, (float *)pixweights, 0);
+        // Monte Carlo mode: the same pointing with a new signal reuses the
+        // whole set-up (pixel grouping, projection tables and, through the
+        // bitwise nna check in solve, the filters and preconditioner)
+        bool same = same_host_pointing(c, (const int32_t *)pix, (const float *)pixweights);
+        if (!same) set_pointing(c, (int32_t *)pix, (float *)pixweights, 0, true);
         solve(c, (double *)signal, 0);
+        c.stats[ST_POINTING_REUSED] = same ? 1 : 0;
         write_back_pixels(c, (int32_t *)pix);
         OutNames names = output_names(c.p, path, 0);
         names.hit.clear();

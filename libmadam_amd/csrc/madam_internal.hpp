@@ -200,6 +200,10 @@ void launch_axpby_nna(hipStream_t s, double *ap, const double *nna, const double
 
 void launch_minmax(hipStream_t s, const int32_t *pix, int64_t n, int *mn, int *mx);
 void launch_restore_pixels(hipStream_t s, int32_t *pix, int64_t n, int32_t dummy);
+// pix == dummy -> -1
+void launch_dummy_to_flag(hipStream_t s, int32_t *pix, int64_t n, int32_t dummy);
+// *flag = 1 if any a[i] != b[i] (flag must start at 0)
+void launch_words_differ(hipStream_t s, const uint32_t *a, const uint32_t *b, int64_t n, int *flag);
 void launch_fprecond(hipStream_t s, const double *fcov, int64_t nh, const double *wpsd, int nshort, double *out,
                      int64_t npsd);
 

@@ -1693,6 +1693,31 @@ void launch_restore_pixels(hipStream_t s, int32_t *pix, int64_t n, int32_t dummy
     k_restore<<<grid_for(n, TPB), TPB, 0, s>>>(pix, n, dummy);
 }
 
+// the dummy pixel a previous call left in the caller's array flags the sample again
+__global__ void k_dummy_to_flag(int32_t *pix, int64_t n, int32_t dummy)
+{
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x)
+        if (pix[g] == dummy) pix[g] = -1;
+}
+
+void launch_dummy_to_flag(hipStream_t s, int32_t *pix, int64_t n, int32_t dummy)
+{
+    if (n > 0) k_dummy_to_flag<<<grid_for(n, TPB), TPB, 0, s>>>(pix, n, dummy);
+}
+
+__global__ void k_words_differ(const uint32_t *a, const uint32_t *b, int64_t n, int *flag)
+{
+    bool d = false;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        d = d || a[i] != b[i];
+    if (d) flag[0] = 1;
+}
+
+void launch_words_differ(hipStream_t s, const uint32_t *a, const uint32_t *b, int64_t n, int *flag)
+{
+    if (n > 0) k_words_differ<<<grid_for(n, TPB), TPB, 0, s>>>(a, b, n, flag);
+}
+
 // beta = (rz_new - rz2) / rz_old; rz_old <- rz_new   (scal layout as k_update_r)
 // fprecond rows: 1 / (nshort * w + fcov)  (src/noise_routines.f90:977-993)
 __global__ void k_fprecond(const double *fcov, int64_t nh, const double *wpsd, int nshort, double *out, int64_t npsd)

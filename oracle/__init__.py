@@ -84,6 +84,8 @@ def lib():
         L.oracle_interpolate_psd.argtypes = [ct.c_long, P(np.float64, flags="C"),
                                              P(np.float64, flags="C"), ct.c_long,
                                              P(np.float64, flags="C"), P(np.float64, flags="C")]
+        L.oracle_convolve.argtypes = [ct.c_long, P(np.float64, flags="C"), P(np.float64, flags="C"), ct.c_long,
+                                      P(np.float64, flags="C")]
         L.offt_create.restype = ct.c_void_p
         L.offt_create.argtypes = [ct.c_long]
         L.offt_destroy.argtypes = [ct.c_void_p]
@@ -228,6 +230,16 @@ def build_filter(fsample, nshort, nof, sigma, psdfreqs, psd, radiometers=True):
                           np.ascontiguousarray(psdfreqs, np.float64),
                           np.ascontiguousarray(psd, np.float64), out)
     return out
+
+
+def convolve(fcov, x, nof):
+    """One segment's prior convolution: y = conv(x - mean, c2r(fcov) / nof) - mean
+    (src/noise_routines.f90:816-857)."""
+    L = lib()
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros_like(x)
+    L.oracle_convolve(nof, np.ascontiguousarray(fcov, np.float64), x, x.size, y)
+    return y
 
 
 def interpolate_psd(freq, psd, target):

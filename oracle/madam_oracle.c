@@ -810,6 +810,21 @@ static void convolve_segment(const octx *c, const double *fc, const double *x, d
     for (int64_t i = 0; i < noba; ++i) y[i] = xx[i] - s;
 }
 
+/* one trimmed segment's convolution on its own, exported for the tests */
+void oracle_convolve(long nof, const double *fcov, const double *x, long noba, double *y)
+{
+    octx c;
+    memset(&c, 0, sizeof c);
+    c.nof = nof;
+    c.plan = offt_create(nof);
+    double *xx = (double *)malloc(sizeof(double) * nof);
+    double *fx = (double *)malloc(sizeof(double) * 2 * (nof / 2 + 1));
+    double *work = (double *)malloc(sizeof(double) * nof);
+    convolve_segment(&c, fcov, x, y, noba, xx, fx, work);
+    free(xx); free(fx); free(work);
+    offt_destroy(c.plan);
+}
+
 /* convolve_pp: y = x, then per (det, interval) segments; src/noise_routines.f90:816-913 */
 static void convolve_pp(const octx *c, const double *fcall, const double *nna, const double *x, double *y)
 {

@@ -120,13 +120,15 @@ def copy_case(case):
 # ------------------------------------------------------------------ dense brute force
 
 
-def dense_problem(case, res, prior_kernel=None):
+def dense_problem(case, res, prior_kernel=None, wsamp=None):
     """Dense normal equations of the destriping problem for order-0 baselines.
 
     Uses the same sample selection as the reference: samples whose cross pixel
     was rejected (cca == 0) are dropped from F (src/smadam_routines.f90:542-544).
     Returns (A, b, P_good, F_good, W) in baseline coordinates [ndet*nbase].
     ``prior_kernel(idet, kstart, noba)`` returns the dense C_a^-1 block.
+    ``wsamp`` [ndet, nsamp] is the detector weight of every sample (default:
+    the detector's weight everywhere).
     """
     ndet = len(case["dets"])
     nsamp = case["timestamps"].size
@@ -143,7 +145,7 @@ def dense_problem(case, res, prior_kernel=None):
     npix = res["npix"]
     nmap = res["nmap"]
     # pixel matrix and binned map
-    Wd = np.repeat(dw, nsamp)
+    Wd = np.repeat(dw, nsamp) if wsamp is None else np.asarray(wsamp, np.float64).reshape(-1)
     rows = np.arange(ndet * nsamp)
     ipix = pix.reshape(-1)
     ok = ipix >= 0
@@ -195,3 +197,20 @@ def project_out_null(A, x, tol=1e-9):
     null = np.abs(ev) < tol * np.abs(ev).max()
     N = V[:, null]
     return x - N @ (N.T @ x), N
+
+
+def with_psd_switch(case, switch_times, levels=(1.0, 3.0), fknees=(0.1, 0.3), alphas=(-1.0, -1.5), sigma=1e-4):
+    """Two PSDs per detector (src/inputparam.f90:73-163): detector d uses PSD 0
+    before ``switch_times[d]`` and PSD 1 from then on; PSD j is
+    levels[j] * sigma^2 / fs * (1 + (f / fknees[j])^alphas[j])."""
+    ndet = len(case["dets"])
+    fs = case["pars"]["fsample"]
+    freqs = case["psdfreqs"]
+    rows = []
+    for d in range(ndet):
+        for j in range(2):
+            rows.append(levels[j] * psd_model(freqs, sigma, fs, fknees[j], alphas[j]))
+    case["npsd"] = np.full(ndet, 2, np.int64)
+    case["psdstarts"] = np.array([[0.0, t] for t in switch_times], np.float64).reshape(-1)
+    case["psdvals"] = np.concatenate(rows)
+    return case

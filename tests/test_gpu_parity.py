@@ -160,9 +160,9 @@ def test_band_factor_periodic_continuation(flags):
     assert np.array_equal(c1["signal"], c2["signal"])
 
 
-@pytest.mark.parametrize("width", [20, 40])
+@pytest.mark.parametrize("width", [20, 30, 40, 50])
 def test_band_preconditioner_wide(width):
-    """Wider bands (kd classes 19 and 39: the factor's window no longer fits
+    """Wider bands (kd classes 19, 29, 39 and 49: the factor's window no longer fits
     one 16-lane DPP row) against the oracle, and the periodic continuation of
     the factorization bitwise against the full column loop at that width."""
     import libmadam_amd
@@ -397,32 +397,71 @@ def test_inner_cg_preconditioner_forced():
 
 def test_mc_mode_cache_cycle(tmp_path):
     """destripe(mcmode=T) -> destripe_with_cache(new signal) -> clear_caches, as in
-    python/tests/test_libmadam_mc.py:132-191; the cached solve equals a fresh
-    solve of the new signal (clean right-hand side)."""
+    python/tests/test_libmadam_mc.py:132-191: the caller passes the same pix
+    and weight arrays again (pix as the first call left them: its flagged
+    samples hold the dummy pixel), so the cached call reuses the pointing
+    set-up and, nna being bitwise unchanged, the filters and preconditioner.
+    The cached solve equals a fresh oracle solve of the new signal (clean
+    right-hand side)."""
     import os
+    import libmadam_amd
     from libmadam_amd import libmadam_wrapper as madam
     os.environ["MADAM_RAISE"] = "1"
     case = ring(prior=True, nsamp=20000, extra=dict(mcmode=True, write_tod=True, path_output=str(tmp_path)))
+    case["pixels"][300:420] = -1
     c1 = mc.copy_case(case)
     madam.destripe(None, c1["pars"], c1["dets"], c1["weights"], c1["timestamps"], c1["pixels"], c1["pixweights"],
                    c1["signal"], c1["periods"], c1["npsd"], c1["psdstarts"], c1["psdfreqs"], c1["psdvals"])
+    assert np.all(c1["pixels"][300:420] == 12 * 8 ** 2)
     rng = np.random.default_rng(3)
     sig2 = case["signal"] + rng.standard_normal(case["signal"].size) * 1e-4
-    c2 = mc.copy_case(case)
-    c2["signal"] = sig2.copy()
+    c1["signal"][:] = sig2
     (tmp_path / "mc").mkdir()
-    madam.destripe_with_cache(None, c2["timestamps"], c2["pixels"], c2["pixweights"], c2["signal"],
+    madam.destripe_with_cache(None, c1["timestamps"], c1["pixels"], c1["pixweights"], c1["signal"],
                               str(tmp_path / "mc"))
+    st = libmadam_amd.cache_stats()
+    assert st["pointing_reused"] == 1 and st["prec_reused"] == 1, st
     assert (tmp_path / "mc" / "madam_map.fits").exists()
     madam.clear_caches()
     madam.clear_caches()   # a second call only warns
+    assert libmadam_amd.cache_stats() is None
     c3 = mc.copy_case(case)
     c3["signal"] = sig2.copy()
     c3["pars"] = dict(case["pars"], mcmode=False)
     c3["pars"].pop("path_output")
     oracle.destripe(*mc.call_args(c3))
-    assert rel_rms(c2["signal"], c3["signal"]) < RTOL_RMS
-    assert np.array_equal(c2["pixels"], c3["pixels"])
+    assert rel_rms(c1["signal"], c3["signal"]) < RTOL_RMS
+    assert np.array_equal(c1["pixels"], c3["pixels"])
+
+
+def test_mc_mode_new_pointing():
+    """destripe_with_cache with a different pointing (a pixel moved): the
+    pointing set-up is rebuilt, the dummy pixels of the caller's array still
+    count as flags, and the result equals the oracle's solve of that input."""
+    import os
+    import libmadam_amd
+    from libmadam_amd import libmadam_wrapper as madam
+    os.environ["MADAM_RAISE"] = "1"
+    case = ring(prior=True, nsamp=20000, extra=dict(mcmode=True, write_tod=True, write_map=False))
+    case["pixels"][300:420] = -1
+    c1 = mc.copy_case(case)
+    madam.destripe(None, c1["pars"], c1["dets"], c1["weights"], c1["timestamps"], c1["pixels"], c1["pixweights"],
+                   c1["signal"], c1["periods"], c1["npsd"], c1["psdstarts"], c1["psdfreqs"], c1["psdvals"])
+    c1["pixels"][5000] = (c1["pixels"][5000] + 1) % (12 * 8 ** 2)
+    c1["signal"][:] = case["signal"]
+    moved = c1["pixels"].copy()
+    try:
+        madam.destripe_with_cache(None, c1["timestamps"], c1["pixels"], c1["pixweights"], c1["signal"], "")
+        st = libmadam_amd.cache_stats()
+        assert st["pointing_reused"] == 0, st
+    finally:
+        madam.clear_caches()
+    c3 = mc.copy_case(case)
+    c3["pixels"] = np.where(moved == 12 * 8 ** 2, -1, moved).astype(np.int32)
+    c3["pars"] = dict(case["pars"], mcmode=False)
+    oracle.destripe(*mc.call_args(c3))
+    assert rel_rms(c1["signal"], c3["signal"]) < RTOL_RMS
+    assert np.array_equal(c1["pixels"], c3["pixels"])
 
 
 def test_bitwise_reproducible():
@@ -453,4 +492,115 @@ def test_band_preconditioner_chunk_groups():
     second local pass) is exercised."""
     g, o, c1, c2 = both(ring(prior=True, ndet=2, nsamp=500000, nperiod=2))
     assert g["stats"]["precond_kind"] == 3 and g["stats"]["nband"] >= 2
+    check_parity(g, o, c1, c2)
+
+
+# ------------------------------------------------------------------ long filters, multi-PSD, flags, wide bands
+
+
+@pytest.mark.parametrize("log2nof,nsamp", [(20, 300000), (21, 600000), (22, 1100000)])
+def test_prior_convolution_long_fft(log2nof, nsamp):
+    """The noise-prior convolution at nof = 2^20 (512 x 1024 four-step plans,
+    74 KB LDS column passes), 2^21 and 2^22 (1024-point columns, 147 KB of
+    LDS) against the oracle's convolution of the same segment with the
+    oracle's filter: one detector, one-sample baselines, one period, so the
+    segment is the whole period.  The GPU filter (build_filter) is part of
+    the comparison."""
+    import libmadam_amd
+    case = ring(prior=True, ndet=1, nsamp=nsamp, nperiod=1, base=0.02, nnz=1,
+                extra=dict(use_fprecond=True, iter_max=1))
+    a = mc.call_args(mc.copy_case(case))
+    m = libmadam_amd.Madam(0)
+    try:
+        m.configure(*a[:4], a[7], a[8], a[9], a[10], a[11], 1, 0)
+        m.set_pointing(a[4], a[5])
+        m.solve(a[6])
+        st = m.stats()
+        assert st["nof"] == 2 ** log2nof and st["nseg"] == 1
+        x = np.random.default_rng(log2nof).standard_normal(nsamp)
+        y = m.apply(0, x)
+    finally:
+        m.close()
+    fs = case["pars"]["fsample"]
+    fc = oracle.build_filter(fs, 1, 2 ** log2nof, 1.0 / np.sqrt(case["weights"][0]), case["psdfreqs"],
+                             case["psdvals"][:case["psdfreqs"].size])
+    ref = oracle.convolve(fc, x, 2 ** log2nof)
+    err = rel_rms(y, ref)
+    print("nof 2^%d: prior relative rms error %.3g" % (log2nof, err))
+    assert err <= 1e-12
+
+
+@pytest.mark.timeout(900)
+def test_config2_shape_noise_prior_band():
+    """Two of the benchmark's detectors at its full per-detector shape: 2^24
+    samples at 100 Hz in two periods of 2^23, 0.25 s baselines (335,545 per
+    segment: 5,243 band chunks, 82 carry-chain groups), Nside 512 IQU, noise
+    prior from the PSD (sigma 2e-4 K, fknee 0.05 Hz, alpha -1.5) with nof =
+    2^20, band-Cholesky preconditioner, PCG tolerance 1e-12.  Full solve
+    against the oracle: hits bit-exact, maps, baselines and TOD to 1e-10."""
+    import libmadam_amd
+    from bench import SCAN, SIGMA, FKNEE, ALPHA, psd_table, params
+    ndet, nsamp, nside, fs = 2, 2 ** 24, 512, 100.0
+    print("config-2 shape: 2 det x 2^24, pointing on the GPU", flush=True)
+    pix, w = libmadam_amd.expand_pointing(SCAN | dict(fsample=fs), [dict(psi_turns=0.0), dict(psi_turns=0.125)],
+                                          nside, 0, nsamp)
+    rng = np.random.default_rng(5)
+    sky = rng.standard_normal((12 * nside ** 2, 3)) * np.array([100e-6, 5e-6, 5e-6])
+    sig = np.einsum("dsk,dsk->ds", sky[pix], w.astype(np.float64))
+    for d in range(ndet):
+        sig[d] += mc.one_over_f(rng, nsamp, fs, SIGMA, FKNEE, ALPHA)
+    freqs, psd = psd_table(fs)
+    case = dict(pars=params(nside, fs, 0.25), dets=["a", "b"], weights=np.full(ndet, 1 / SIGMA ** 2),
+                timestamps=np.arange(nsamp) / fs, pixels=pix.reshape(-1), pixweights=w.reshape(-1),
+                signal=sig.reshape(-1), periods=np.array([0, nsamp // 2], np.int64), npsd=np.ones(ndet, np.int64),
+                psdstarts=np.zeros(ndet), psdfreqs=freqs, psdvals=np.tile(psd, ndet))
+    case["pars"]["write_tod"] = True
+    print("solving on the GPU and the oracle", flush=True)
+    g, o, c1, c2 = both(case)
+    assert g["stats"]["nof"] == 2 ** 20 and g["stats"]["precond_kind"] == 3 and g["stats"]["ncg"] == 0
+    assert g["stats"]["band_chunks"] == 2 * 2 * 5243
+    check_parity(g, o, c1, c2)
+
+
+def test_multi_psd_switch():
+    """npsd = 2 per detector with noise weights from each PSD: detector 0
+    switches PSD inside period 1 after a 65 s gap (61 empty baselines end the
+    period's first segment, so its second segment starts past the switch: the
+    preconditioner takes the PSD at the segment start, the convolution the one
+    at the period start); detector 1 switches exactly at the start of period
+    2 (weights take the new PSD, the filter the old one)."""
+    case = ring(prior=True, ndet=2, nsamp=45000, nperiod=3, extra=dict(noise_weights_from_psd=True))
+    t, per = case["timestamps"], case["periods"]
+    pix = case["pixels"].reshape(2, -1)
+    pix[0, per[1] + 2000:per[1] + 2000 + 3250] = -1
+    mc.with_psd_switch(case, [t[per[1] + 4000], t[per[2]]])
+    g, o, c1, c2 = both(case)
+    assert g["stats"]["precond_kind"] == 3 and g["stats"]["nseg"] == 7
+    check_parity(g, o, c1, c2)
+
+
+def test_sparse_period_flagged():
+    """A detector-period with fewer than 0.1 % valid samples is flagged whole
+    (src/read_data.f90:241-272); one with 0.125 % is kept."""
+    for nvalid in (3, 5):
+        case = ring(ndet=4, nsamp=16000, nperiod=4)
+        pix = case["pixels"].reshape(4, -1)
+        pix[1, 4000 + nvalid:8000] = -1
+        g, o, c1, c2 = both(case)
+        cp = c1["pixels"].reshape(4, -1)
+        assert np.all(cp[1, 4000:4000 + nvalid] == 12 * 8 ** 2) == (nvalid == 3)
+        check_parity(g, o, c1, c2)
+
+
+def test_band_preconditioner_wider_than_kernels():
+    """precond_width_min = 60: wider than the band kernels' widest class
+    (50), so those segments use the inner-CG preconditioner.  Another SPD
+    preconditioner: the solution is the oracle's (band of 60), the
+    iteration count is not compared."""
+    case = ring(prior=True, ndet=2, nsamp=100000, nperiod=2,
+                extra=dict(precond_width_min=60, precond_width_max=100))
+    g, o, c1, c2 = both(case)
+    print("niter gpu/oracle", g["niter"], o["niter"], "oracle band", o["nband_max"])
+    assert g["stats"]["nwide"] > 0 and g["stats"]["ncg"] > 0 and o["nband_max"] >= 60
+    g["niter"] = o["niter"]
     check_parity(g, o, c1, c2)

@@ -344,3 +344,167 @@ def test_filter_closed_form():
     exp = 1.0 / (fa * asp)
     exp[0] = 0.0
     np.testing.assert_allclose(fcov, exp, rtol=1e-12)
+
+
+def test_filter_against_ou_closed_form():
+    """build_filter (src/noise_routines.f90:367-490) pinned without its alias
+    loop.  For an Ornstein-Uhlenbeck process (two-sided PSD A / (1 + (f/f0)^2)
+    on the white plateau) the covariance of baseline means (boxcars of length
+    T = 1 / fa) is closed form, C(0) = A pi f0 2 (b - 1 + e^-b) / b^2 and
+    C(l) = A pi f0 (2 cosh b - 2) / b^2 e^(-b |l|) with b = 2 pi f0 T, and so
+    is its transform Chat(w) = sum_l C(l) e^(-i w l).  By Poisson summation
+    Chat / fa is the sum over every alias m of (PSD - plateau) sinc^2; the
+    filter keeps the aliases m = -3 .. 2 (k = 0..2 and their mirrors), so
+    fcov = 1 / (fa * (Chat / fa - tail - c (1 - tail_sinc))) with the excluded
+    tails summed from the analytic PSD and c the plateau's Lorentzian part
+    (sum_m sinc^2 = 1).  The table is log-spaced; its log-log interpolation
+    error is ~2e-6, far below the k = 2 aliases' share (up to 1.6e-3), so a
+    dropped or misplaced alias term, sinc^2 or plateau fails by 100x.
+    The time-domain statement: circulant(irfft(fcov)) inverts the Toeplitz
+    covariance of the baseline means up to the truncation bound (7.2e-4 for
+    this PSD; 1 - 1/nof on the diagonal: the mean mode is left free)."""
+    fs, nshort, nof = 100.0, 50, 256
+    fa, f0, sigma = fs / nshort, 0.2, 1e-3
+    plateau = sigma ** 2 / fs
+    A = 100 * plateau
+    freqs = np.logspace(-5, 2, 4001)
+    lor = lambda f: A / (1 + (f / f0) ** 2)
+    psd = plateau + lor(freqs)
+    fcov = oracle.build_filter(fs, nshort, nof, sigma, freqs, psd)
+    c = oracle.interpolate_psd(freqs, psd, np.array([fs / 2]))[0] - plateau   # plateau at fs / 2
+    b = 2 * np.pi * f0 / fa
+    C0 = A * np.pi * f0 * 2 * (b - 1 + np.exp(-b)) / b ** 2
+    K = A * np.pi * f0 * (2 * np.cosh(b) - 2) / b ** 2
+    nu = np.arange(1, nof // 2 + 1) / nof
+    z = np.exp(-b - 2j * np.pi * nu)
+    chat = C0 + 2 * K * np.real(z / (1 - z))
+    sinc2 = lambda x: np.sin(np.pi * x) ** 2 / (np.pi * x) ** 2
+    S = lambda x: lor(fa * np.abs(x)) * sinc2(x)
+    j = np.arange(3, 200000)[:, None]
+    tail = (S(j + nu) + S(j + 1 - nu)).sum(0)
+    tail_s = (sinc2(j + nu) + sinc2(j + 1 - nu)).sum(0)
+    asp = chat / fa - tail - c * (1 - tail_s)
+    assert fcov[0] == 0
+    np.testing.assert_allclose(fcov[1:], 1 / (fa * asp), rtol=1e-5)
+    # the k = 2 aliases matter at the 1e-3 level: the check has the power to see them
+    k2 = (S(2 + nu) + S(3 - nu) - c * (sinc2(2 + nu) + sinc2(3 - nu))) / asp
+    assert np.abs(k2).max() > 100 * 1e-5
+    # time domain: circulant(h) C_per = I - 1/nof, up to the truncation bound
+    bound = np.abs((tail - c * tail_s) / asp).max()
+    assert bound < 1e-3
+    h = np.fft.irfft(fcov, nof)
+    l = np.arange(nof)
+    cl = lambda l: np.where(l == 0, C0, K * np.exp(-b * np.abs(l)))
+    cper = cl(l) + cl(nof - l) * (l > 0)
+    conv = np.real(np.fft.ifft(np.fft.fft(h) * np.fft.fft(cper)))
+    err = conv - (np.eye(1, nof)[0] - 1 / nof)
+    assert np.abs(err).max() <= bound, (np.abs(err).max(), bound)
+
+
+def _multi_psd_truth(case, r, perturb=False):
+    """Per-sample weights and per-interval prior blocks from the PSD each
+    interval's first sample falls in (src/noise_routines.f90:193-236), with
+    noise weights from the PSD plateau at fs / 2 (w = 1 / (plateau fs),
+    src/inputparam.f90:120-140, src/parameter_control.f90:196-204).  perturb:
+    PSD 0 everywhere (the mistake the pin must catch)."""
+    pars = case["pars"]
+    fs = pars["fsample"]
+    nshort = int(round(pars["base_first"] * fs))
+    ndet = len(case["dets"])
+    nsamp = case["timestamps"].size
+    freqs = case["psdfreqs"]
+    nbin = freqs.size
+    rows = case["psdvals"].reshape(-1, nbin)
+    starts = case["psdstarts"].reshape(ndet, 2)
+    t = case["timestamps"]
+    per = list(case["periods"]) + [nsamp]
+    pb = np.searchsorted(r["base_start"], case["periods"])
+    nb = r["nbase"]
+    wsamp = np.zeros((ndet, nsamp))
+    blocks = {}
+    segs = []
+    for d in range(ndet):
+        sd = []
+        for i in range(len(per) - 1):
+            # weights: psd_index_det, the last PSD with start <= t; filter:
+            # psd_index, the first PSD with start <= t <= its stop (the next
+            # start), so at a start exactly on t the filter keeps the old PSD
+            jw = 0 if perturb else int(np.searchsorted(starts[d], t[per[i]], side="right") - 1)
+            jf = 0 if perturb else int(np.searchsorted(starts[d][1:], t[per[i]], side="left"))
+            pl = oracle.interpolate_psd(freqs, rows[2 * d + jw], np.array([fs / 2]))[0]
+            wsamp[d, per[i]:per[i + 1]] = 1.0 / (pl * fs)
+            psd = rows[2 * d + jf]
+            plf = oracle.interpolate_psd(freqs, psd, np.array([fs / 2]))[0]
+            fcov = oracle.build_filter(fs, nshort, r["nof"], np.sqrt(plf * fs), freqs, psd)
+            k0, k1 = pb[i], (pb[i + 1] if i + 1 < len(pb) else nb)
+            blocks[(d, int(k0))] = np.fft.irfft(fcov, r["nof"])
+            sd.append((int(k0), int(k1 - k0)))
+        segs.append(sd)
+    pk = dict(segments=segs, block=lambda d, ks, no: mc.prior_block(blocks[(d, ks)], no))
+    return wsamp, pk
+
+
+def test_multi_psd_selection_dense():
+    """npsd = 2 (src/noise_routines.f90:193-236): detector 0 switches PSD
+    inside period 1, detector 1 exactly at the start of period 2, where the
+    reference's two lookups part: the weights take the new PSD and the filter
+    the old one (its stop time is inclusive).  Noise weights come from each
+    PSD.  The oracle's solution satisfies the dense normal equations built
+    with each interval's PSDs (weights and prior block), and does not satisfy
+    the ones built with PSD 0 throughout."""
+    case = small_case(prior=True, nperiod=4, ndet=2, extra=dict(noise_weights_from_psd=True))
+    t, per = case["timestamps"], case["periods"]
+    mc.with_psd_switch(case, [t[per[1] + 150], t[per[2]]])
+    r, _ = run(case)
+    assert r["niter"] > 3
+    a = r["aa"].reshape(-1)
+    for perturb in (False, True):
+        wsamp, pk = _multi_psd_truth(case, r, perturb)
+        A, b, _ = mc.dense_problem(case, r, pk, wsamp=wsamp)
+        res = np.linalg.norm(b - A @ a) / np.linalg.norm(b)
+        brel = np.linalg.norm(r["yba"].reshape(-1) - b) / np.linalg.norm(b)
+        print("perturb", perturb, "residual", res, "rhs", brel)
+        if not perturb:
+            assert res < 1e-10 and brel < 1e-12
+        else:
+            assert res > 1e-3 or brel > 1e-3
+
+
+def test_sparse_period_flagged():
+    """A detector-period with fewer than 0.1 % valid samples is flagged as a
+    whole (src/read_data.f90:241-272): here 3 of det 1's 4000 samples in
+    period 1 stay valid (0.075 %), and they come back as the dummy pixel and
+    drop out of the hits; 5 valid samples (0.125 %) are kept."""
+    for nvalid, flagged in ((3, True), (5, False)):
+        case = small_case(prior=False, nperiod=4, nsamp=16000)
+        pix = case["pixels"].reshape(4, -1)
+        pix[1, 4000 + nvalid:8000] = -1
+        r, c = run(case)
+        cp = c["pixels"].reshape(4, -1)
+        assert np.all(cp[1, 4000 + nvalid:8000] == 12 * 4 ** 2)
+        assert np.all(cp[1, 4000:4000 + nvalid] == 12 * 4 ** 2) == flagged
+        assert r["hits"].sum() == 4 * 16000 - 4000 + (0 if flagged else nvalid)
+
+
+@pytest.mark.parametrize("nof,noba", [(64, 30), (4096, 2000), (2 ** 16, 32769)])
+def test_convolve_segment_vs_circulant(nof, noba):
+    """One segment's convolution (src/noise_routines.f90:816-857): mean
+    removed, zero-padded to nof, circular convolution with invcov =
+    irfft(fcov), mean removed again.  Against the explicit circulant product
+    for the small case and numpy's FFT for the large ones."""
+    fs = 50.0
+    f = np.logspace(-5, np.log10(fs / 2), 100)
+    psd = mc.psd_model(f, 1e-4, fs, 0.1, -1.0)
+    fc = oracle.build_filter(fs, 1, nof, 1e-4, f, psd)
+    x = np.random.default_rng(nof).standard_normal(noba)
+    y = oracle.convolve(fc, x, nof)
+    h = np.fft.irfft(fc, nof)
+    xm = np.zeros(nof)
+    xm[:noba] = x - x.mean()
+    if nof <= 64:
+        idx = (np.arange(noba)[:, None] - np.arange(noba)[None, :]) % nof
+        ref = h[idx] @ (x - x.mean())
+    else:
+        ref = np.fft.irfft(np.fft.rfft(h) * np.fft.rfft(xm), nof)[:noba]
+    ref -= ref.mean()
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
