@@ -80,11 +80,34 @@ int madam_set_profiling(void *ctx, int on);
  * madam_solve are not refreshed */
 int madam_iterate(void *ctx, int niter, int restart);
 int madam_reset_timers(void *ctx);
+/* Batched solve: nrhs (1..64) signals that share this handle's pointing
+ * (Monte Carlo realisations of one scan: the reference solves them one
+ * destripe_with_cache call at a time, src/smadam.F90:593-918), stored
+ * [nrhs][ndet][nsamp] as float64 (tod_f32 = 0) or float32 (tod_f32 = 1).
+ * The set-up (pixel matrices, nna, segments, filters, preconditioner) is that
+ * of the first signal and serves all of them.  precision 0: one PCG over all
+ * right-hand sides in FP64, each with the reference's recurrence and exit
+ * tests (src/smadam_routines.f90:677-924); precision 1: FP32 operator and
+ * vectors (FP64 dot products and scalars) inside an FP64 iterative refinement
+ * to the same cglimit on the FP64 residual.  The signals are not modified. */
+int madam_solve_batch(void *ctx, int nrhs, const void *signals, int tod_f32, int on_device, int precision);
+/* destriped and binned maps [npix][nmap] of right-hand side irhs (NULL skips) */
+int madam_get_batch_maps(void *ctx, int irhs, double *map, double *binmap);
+/* baselines and right-hand side [ndet][nbase] of right-hand side irhs */
+int madam_get_batch_baselines(void *ctx, int irhs, double *aa, double *yba);
+/* out[0..n): last PCG's steps, status (1 converged, 2 diverging, 4 iteration
+ * limit, 8 nothing to solve, 16 stalled), its rr / rr0, refinement passes,
+ * the FP64 residual's rr / rrinit, t_setup, t_solve, PCG steps over the
+ * batch, their time, entries per pixel-side work item (autotuned) */
+int madam_get_batch_info(void *ctx, int irhs, double *out, int n);
 /* one operator of the last madam_solve on host vectors x, y of ndet * nbase
  * baselines: which 0 = noise prior y = C_a^-1 x (cinvmul, src/noise_routines.f90:745-913),
  * 1 = preconditioner y = M^-1 x (src/noise_routines.f90:1159-1409), 2 = the
  * destriping operator y = A x of one PCG step (src/smadam_routines.f90:808-872) */
 int madam_apply(void *ctx, int which, const double *x, double *y);
+/* the noise filter of global PSD ipsd, fcov[nof / 2 + 1] (build_filter,
+ * src/noise_routines.f90:367-490), after a madam_solve with kfilter = T */
+int madam_get_filter(void *ctx, int ipsd, double *fcov);
 /* projection kernels (low 4 bits): 0 = segmented 512-entry tiles on the pixel
  * side and LDS tiles on the baseline side (default), 1 = wave per pixel / lane
  * group per baseline, 2 = LDS tiles on both sides, 3 = segmented tiles on both

@@ -64,6 +64,16 @@ def lib():
         L.madam_get_rr_history.argtypes = [vp, vp, i32]
         L.madam_iterate.restype = i32
         L.madam_iterate.argtypes = [vp, i32, i32]
+        L.madam_solve_batch.restype = i32
+        L.madam_solve_batch.argtypes = [vp, i32, vp, i32, i32, i32]
+        L.madam_get_batch_maps.restype = i32
+        L.madam_get_batch_maps.argtypes = [vp, i32, vp, vp]
+        L.madam_get_batch_baselines.restype = i32
+        L.madam_get_batch_baselines.argtypes = [vp, i32, vp, vp]
+        L.madam_get_batch_info.restype = i32
+        L.madam_get_batch_info.argtypes = [vp, i32, vp, i32]
+        L.madam_get_filter.restype = i32
+        L.madam_get_filter.argtypes = [vp, i32, vp]
         L.madam_apply.restype = i32
         L.madam_apply.argtypes = [vp, i32, vp, vp]
         L.madam_set_kernel_variant.restype = i32
@@ -219,6 +229,44 @@ class Madam:
         y = np.zeros_like(x)
         _check(lib().madam_apply(self._h, int(which), _ptr(x), _ptr(y)))
         return y
+
+    def solve_batch(self, signals, precision=0, on_device=False):
+        """Solve nrhs signals [nrhs][ndet][nsamp] (float64 or float32) sharing this
+        handle's pointing; precision 0 = FP64 PCG, 1 = FP32 operator inside FP64
+        iterative refinement."""
+        if on_device:
+            nrhs = int(signals.shape[0])
+            f32 = str(signals.dtype) in ("torch.float32", "float32")
+        else:
+            signals = np.ascontiguousarray(signals)
+            assert signals.dtype in (np.float64, np.float32)
+            nrhs = signals.shape[0]
+            f32 = signals.dtype == np.float32
+        self._keep.append(signals)
+        self.nrhs = nrhs
+        _check(lib().madam_solve_batch(self._h, nrhs, _ptr(signals), int(f32), int(on_device), int(precision)))
+
+    def batch_result(self, irhs):
+        """map, binmap [npix][nmap], aa, yba [ndet][nbase] and the solve info of right-hand side irhs."""
+        sz = self.sizes()
+        npix, nmap = sz["npix"], sz["nmap"]
+        m = np.zeros((npix, nmap))
+        b = np.zeros((npix, nmap))
+        _check(lib().madam_get_batch_maps(self._h, int(irhs), _ptr(m), _ptr(b)))
+        aa = np.zeros((sz["ndet"], sz["nbase"]))
+        yba = np.zeros((sz["ndet"], sz["nbase"]))
+        _check(lib().madam_get_batch_baselines(self._h, int(irhs), _ptr(aa), _ptr(yba)))
+        info = np.zeros(10)
+        _check(lib().madam_get_batch_info(self._h, int(irhs), _ptr(info), 10))
+        keys = ["steps", "status", "rr_inner", "passes", "rr_final", "t_setup", "t_solve", "pcg_steps", "t_pcg",
+                "piece"]
+        return dict(map=m, binmap=b, aa=aa, yba=yba, info=dict(zip(keys, info.tolist())))
+
+    def filter(self, ipsd=0):
+        """The noise filter fcov[nof / 2 + 1] of global PSD ipsd (after solve())."""
+        out = np.zeros(int(self.stats()["nof"]) // 2 + 1)
+        _check(lib().madam_get_filter(self._h, int(ipsd), _ptr(out)))
+        return out
 
     def set_kernel_variant(self, v):
         _check(lib().madam_set_kernel_variant(self._h, int(v)))

@@ -304,8 +304,18 @@ __device__ inline double *vi_chunk(double *V, int64_t c)
 // To and from the interleaved layout, one 64-chunk block per workgroup through
 // a padded LDS tile: both sides access whole 512-byte rows.
 static constexpr int BAND_TP = BAND_CH + 1;
-__global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_t nch, const double *v, double *vi)
+// Strides of one right-hand side (grid dimension y) in the solve's buffers:
+// v = interleaved vectors (ri, zi), s = per-chunk carry states, g = per-group states
+struct BandRhs {
+    int64_t v, s, g;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_t nch, const T *v, double *vi,
+                                                     int64_t vstride, BandRhs st)
 {
+    v += blockIdx.y * vstride;
+    vi += blockIdx.y * st.v;
     __shared__ double tile[64 * BAND_TP];
     __shared__ int64_t b0s[64];
     __shared__ int ns[64];
@@ -318,7 +328,7 @@ __global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_
     __syncthreads();
     for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
         int j = t / BAND_CH, i = t % BAND_CH;   // chunk j of the block, row i
-        tile[j * BAND_TP + i] = i < ns[j] ? v[b0s[j] + i] : 0.0;
+        tile[j * BAND_TP + i] = i < ns[j] ? (double)v[b0s[j] + i] : 0.0;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
@@ -327,8 +337,12 @@ __global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_
     }
 }
 
-__global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int64_t nch, const double *vi, double *v)
+template <typename T>
+__global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int64_t nch, const double *vi, T *v,
+                                                       int64_t vstride, BandRhs st)
 {
+    v += blockIdx.y * vstride;
+    vi += blockIdx.y * st.v;
     __shared__ double tile[64 * BAND_TP];
     __shared__ int64_t b0s[64];
     __shared__ int ns[64];
@@ -345,7 +359,7 @@ __global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int6
     __syncthreads();
     for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
         int j = t / BAND_CH, i = t % BAND_CH;
-        if (i < ns[j]) v[b0s[j] + i] = tile[j * BAND_TP + i];
+        if (i < ns[j]) v[b0s[j] + i] = (T)tile[j * BAND_TP + i];
     }
 }
 
@@ -503,8 +517,11 @@ __global__ void __launch_bounds__(64) k_band_response(const ChunkInfo *ch, int64
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                   const double *Lb, const int32_t *uidx,
-                                                  const double *__restrict__ Lu, const double *r, double *s0)
+                                                  const double *__restrict__ Lu, const double *r, double *s0,
+                                                  BandRhs st)
 {
+    r += blockIdx.y * st.v;
+    s0 += blockIdx.y * st.s;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
     ChunkInfo ci = ch[c];
@@ -553,9 +570,14 @@ __device__ inline double chain_step(double v, const double (&tr)[KD], double st)
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, const int32_t *gn, int64_t ngrp,
                                                          const double *s0, const double *T, const int32_t *cmap,
-                                                         int dir, const double *Sg, double *sin, double *cg)
+                                                         int dir, const double *Sg, double *sin, double *cg,
+                                                         BandRhs rs)
 {
     __shared__ double s0l[BAND_G * KD];
+    s0 += blockIdx.y * rs.s;
+    if (Sg) Sg += blockIdx.y * rs.g;
+    if (sin) sin += blockIdx.y * rs.s;
+    if (cg) cg += blockIdx.y * rs.g;
     int lane = threadIdx.x & 63;
     int64_t g = blockIdx.x;
     if (g >= ngrp) return;
@@ -590,10 +612,13 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
 // chain order, staging cg in windows of 64 groups
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, int64_t nseg, const double *cg,
-                                                       const double *Gm, const int32_t *gmap, int dir, double *Sg)
+                                                       const double *Gm, const int32_t *gmap, int dir, double *Sg,
+                                                       BandRhs rs)
 {
     constexpr int W = 64;
     __shared__ double cgl[W * KD];
+    cg += blockIdx.y * rs.g;
+    Sg += blockIdx.y * rs.g;
     int lane = threadIdx.x & 63;
     int64_t s = blockIdx.x;
     if (s >= nseg) return;
@@ -671,8 +696,12 @@ template <int KD>
 __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                        const double *Lb, const int32_t *uidx,
                                                        const double *__restrict__ Lu, const double *r,
-                                                       const double *sinf, double *z, double *s0b)
+                                                       const double *sinf, double *z, double *s0b, BandRhs st)
 {
+    r += blockIdx.y * st.v;
+    z += blockIdx.y * st.v;
+    sinf += blockIdx.y * st.s;
+    s0b += blockIdx.y * st.s;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
     ChunkInfo ci = ch[c];
@@ -701,8 +730,11 @@ __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int6
 template <int KD>
 __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                   const double *Lb, const int32_t *uidx,
-                                                  const double *__restrict__ Lu, const double *sinb, double *z)
+                                                  const double *__restrict__ Lu, const double *sinb, double *z,
+                                                  BandRhs st)
 {
+    sinb += blockIdx.y * st.s;
+    z += blockIdx.y * st.v;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= nch) return;
     ChunkInfo ci = ch[c];
@@ -740,6 +772,7 @@ struct BandImpl {
     // uniform blocks: uidx[w] = slot of the block's compact factor copy in Lu, or -1
     int32_t *uidx = nullptr;
     double *Lu = nullptr;
+    int rcap = 1;   // right-hand sides the solve buffers (ri, zi, states) hold
 };
 
 static int kd_class(int kd)
@@ -969,33 +1002,66 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
     return 0;
 }
 
-void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z)
+// grow the solve buffers (interleaved vectors and carry states) to nrhs right-hand sides
+void band_reserve(const BandPrec &bp, int nrhs)
+{
+    BandImpl *im = (BandImpl *)bp.impl;
+    if (!im || bp.nchunk == 0 || nrhs <= im->rcap) return;
+    int KD = bp.kd;
+    int64_t nc = bp.nchunk, ng = im->ngrp > 0 ? im->ngrp : 1;
+    for (double **b : {&im->ri, &im->zi}) {
+        (void)hipFree(*b);
+        MADAM_CHECK(hipMalloc(b, sizeof(double) * im->NCP * BAND_CH * nrhs));
+    }
+    for (double **b : {&im->s0f, &im->s0b, &im->sinf, &im->sinb}) {
+        (void)hipFree(*b);
+        MADAM_CHECK(hipMalloc(b, sizeof(double) * nc * KD * nrhs));
+    }
+    for (double **b : {&im->cg, &im->Sg}) {
+        (void)hipFree(*b);
+        MADAM_CHECK(hipMalloc(b, sizeof(double) * ng * KD * nrhs));
+    }
+    im->rcap = nrhs;
+}
+
+// z = M^-1 r on every factored segment, for nrhs right-hand sides at a
+// stride of vstride elements (grid dimension y of every launch)
+template <typename T>
+void band_apply(hipStream_t s, const BandPrec &bp, const T *r, T *z, int nrhs, int64_t vstride)
 {
     BandImpl *im = (BandImpl *)bp.impl;
     if (!im || bp.nchunk == 0) return;
+    if (nrhs > im->rcap)
+        throw HipError(hipErrorInvalidValue, "band_apply: more right-hand sides than band_reserve made room for",
+                       __FILE__, __LINE__);
     int KD = bp.kd;
-    unsigned g = (unsigned)((bp.nchunk + 63) / 64);
-    unsigned gs = (unsigned)im->nseg_used;   // one 64-lane wave per segment
-    unsigned gg = (unsigned)im->ngrp;        // one wave per chunk group
+    unsigned R = (unsigned)nrhs;
+    dim3 g((unsigned)((bp.nchunk + 63) / 64), R);
+    dim3 gs((unsigned)im->nseg_used, R);   // one 64-lane wave per segment
+    dim3 gg((unsigned)im->ngrp, R);        // one wave per chunk group
+    BandRhs st{im->NCP * BAND_CH, bp.nchunk * KD, (im->ngrp > 0 ? im->ngrp : 1) * KD};
     BAND_DISPATCH(KD, ({
-        k_band_to_int<<<g, 256, 0, s>>>(im->ch, bp.nchunk, r, im->ri);
-        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->ri, im->s0f);
+        k_band_to_int<T><<<g, 256, 0, s>>>(im->ch, bp.nchunk, r, im->ri, vstride, st);
+        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->ri, im->s0f, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, nullptr,
-                                                 nullptr, im->cg);
-        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, im->gmap, 0, im->Sg);
+                                                 nullptr, im->cg, st);
+        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, im->gmap, 0, im->Sg, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, im->Sg,
-                                                 im->sinf, nullptr);
+                                                 im->sinf, nullptr, st);
         k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->ri,
-                                              im->sinf, im->zi, im->s0b);
+                                              im->sinf, im->zi, im->s0b, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, nullptr,
-                                                 nullptr, im->cg);
-        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, im->gmap, 1, im->Sg);
+                                                 nullptr, im->cg, st);
+        k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, im->gmap, 1, im->Sg, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, im->Sg,
-                                                 im->sinb, nullptr);
-        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->sinb, im->zi);
-        k_band_from_int<<<g, 256, 0, s>>>(im->ch, bp.nchunk, im->zi, z);
+                                                 im->sinb, nullptr, st);
+        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->sinb, im->zi, st);
+        k_band_from_int<T><<<g, 256, 0, s>>>(im->ch, bp.nchunk, im->zi, z, vstride, st);
     }));
 }
+
+template void band_apply<double>(hipStream_t, const BandPrec &, const double *, double *, int, int64_t);
+template void band_apply<float>(hipStream_t, const BandPrec &, const float *, float *, int, int64_t);
 
 void band_free(BandPrec &bp)
 {

@@ -1,0 +1,627 @@
+// Batched and mixed-precision destriping solves on one context.
+//
+// A batch is nrhs right-hand sides that share the context's pointing: Monte
+// Carlo realisations of one scan, the reference's destripe_with_cache cycle
+// (src/smadam.F90:593-918) done for many signals at once.  The set-up of the
+// first signal (pixel matrices, projection tables, nna, segments, filters,
+// preconditioner) serves all of them; each signal adds its own binned map and
+// right-hand side yba (src/smadam_routines.f90:252-671).
+//
+// precision 0: one PCG over all right-hand sides in FP64, the recurrence of
+//   src/smadam_routines.f90:677-924 per right-hand side, each with its own
+//   scalars and exit tests (a finished right-hand side is masked out).
+// precision 1: the same PCG with an FP32 operator and FP32 vectors (FP64 dot
+//   products and scalars) inside an FP64 iterative refinement:
+//   r = yba - A64 a; solve A32 d = r to the reduction the tolerance still
+//   needs; a += d; until rr / rrinit < cglimit on the FP64 residual.
+//
+// One PCG step is the same sequence of launches every time, so it is
+// captured once into a HIP graph and replayed (unless the inner-CG
+// preconditioner fallback, which synchronises with the host, is in use).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "madam_context.hpp"
+
+namespace madam {
+
+struct BatchState {
+    int nrhs = 0, R = 0, prec = 0;
+    // pixel-side work items (rebuilt when the pointing changes)
+    DBuf<BsItem> items;
+    DBuf<BsFix> fix;
+    BsTables bt;
+    int piece = 0;            // entries per work item (the autotuner's choice)
+    bool tables_ok = false;
+    const void *tables_ent = nullptr;   // the CSR the tables were built for
+    // FP32 copies of the shared per-baseline / per-pixel terms
+    DBuf<float> dwb_s32, dwb32, nna32, cc_c32;
+    // FP64 per right-hand side (RHS-major)
+    DBuf<double> yba, a64, r64, ap64, rhs4, map4, binmap4;
+    DBuf<double> pw64, m64, part64;
+    // inner solve vectors (float or double by precision)
+    DBuf<char> x, r, ro, z, p, ap, pw, m, part;
+    DBuf<double> dots;
+    DBuf<BatchScal> sc, sc64;
+    DBuf<int> active, active64, was_active, ones;
+    // the prior's segments repeated for every right-hand side
+    std::vector<Segment> segR;
+    DBuf<Segment> d_segR;
+    DBuf<double> fftwork;
+    int64_t fft_nseg = 0;
+    // per right-hand side results
+    std::vector<BatchScal> hsc, hsc64;
+    std::vector<int> inner_steps;
+    // the captured PCG step, per vector type
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    bool use_graph = true;
+    double t_setup = 0, t_solve = 0, t_inner = 0;
+    int64_t n_inner = 0, n_outer = 0;
+
+    void drop_graphs()
+    {
+        for (int i = 0; i < 2; ++i) {
+            if (gexec[i]) (void)hipGraphExecDestroy(gexec[i]);
+            if (graph[i]) (void)hipGraphDestroy(graph[i]);
+            gexec[i] = nullptr;
+            graph[i] = nullptr;
+        }
+    }
+    ~BatchState() { drop_graphs(); }
+};
+
+void batch_release(Context &c)
+{
+    delete c.batch;
+    c.batch = nullptr;
+}
+
+template <typename T>
+static T *tp(DBuf<char> &b)
+{
+    return (T *)b.p;
+}
+
+// ------------------------------------------------------------------ pixel-side work items
+
+// Each non-empty cross pixel is one item, or, beyond `piece` entries, pieces
+// of `piece` entries whose sums the fix-up pass adds in order.
+static void build_items(Context &c, BatchState &B, int piece)
+{
+    int64_t nz = c.nzpix;
+    std::vector<int64_t> st(nz + 1);
+    d2h(st.data(), c.d_segstart_s.p, nz + 1, c.stream);
+    MADAM_CHECK(hipStreamSynchronize(c.stream));
+    std::vector<BsItem> items;
+    std::vector<BsFix> fix;
+    items.reserve(nz + nz / 8);
+    int32_t nparts = 0;
+    for (int64_t k = 0; k < nz; ++k) {
+        int64_t e0 = st[k], n = st[k + 1] - st[k];
+        if (n <= piece) {
+            items.push_back(BsItem{e0, (int32_t)n, (int32_t)k, -1, 0});
+            continue;
+        }
+        BsFix f{(int32_t)k, nparts, 0};
+        for (int64_t a = 0; a < n; a += piece) {
+            int32_t m = (int32_t)std::min<int64_t>(piece, n - a);
+            items.push_back(BsItem{e0 + a, m, (int32_t)k, nparts++, 0});
+            f.nparts++;
+        }
+        fix.push_back(f);
+    }
+    B.items.alloc(std::max<size_t>(items.size(), 1));
+    B.fix.alloc(std::max<size_t>(fix.size(), 1));
+    h2d(B.items.p, items.data(), items.size(), c.stream);
+    h2d(B.fix.p, fix.data(), fix.size(), c.stream);
+    MADAM_CHECK(hipStreamSynchronize(c.stream));
+    B.bt.items = B.items.p;
+    B.bt.nitems = (int64_t)items.size();
+    B.bt.fix = B.fix.p;
+    B.bt.nfix = (int64_t)fix.size();
+    B.bt.nparts = nparts;
+    B.piece = piece;
+}
+
+static const void *ent_records(const Context &c)
+{
+    return c.rec_layout == REC16 ? (const void *)c.d_ent.p : (const void *)c.d_entc.p;
+}
+
+static const void *sample_records(const Context &c)
+{
+    return c.rec_layout == REC16 ? (const void *)c.d_pixw.p : (const void *)c.d_pixc.p;
+}
+
+// ------------------------------------------------------------------ operator pieces
+
+template <typename T>
+struct Ops;   // the per-type buffers of the shared terms
+
+template <>
+struct Ops<double> {
+    static const double *dwb_s(Context &c, BatchState &) { return c.d_dwb_s.p; }
+    static const double *dwb(Context &c, BatchState &) { return c.d_dwb.p; }
+    static const double *nna(Context &c, BatchState &) { return c.d_nna.p; }
+    static const double *cc(Context &c, BatchState &) { return c.d_ccinv_c.p; }
+};
+template <>
+struct Ops<float> {
+    static const float *dwb_s(Context &, BatchState &B) { return B.dwb_s32.p; }
+    static const float *dwb(Context &, BatchState &B) { return B.dwb32.p; }
+    static const float *nna(Context &, BatchState &B) { return B.nna32.p; }
+    static const float *cc(Context &, BatchState &B) { return B.cc_c32.p; }
+};
+
+// y = A x for the R right-hand sides (src/smadam_routines.f90:808-872): pixel
+// side with the ccmultiply, prior, baseline side
+template <typename T>
+static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T *m, T *part, const int *active)
+{
+    hipStream_t s = c.stream;
+    int R = B.R;
+    int64_t nb = c.nbtot;
+    batch_pw<T>(s, R, x, nb, c.d_bord.p, Ops<T>::dwb_s(c, B), active, pw);
+    batch_scatter<T>(s, R, c.rec_layout, c.nmap, B.bt, ent_records(c), pw, Ops<T>::cc(c, B), c.d_nzq.p, m, part);
+    if (c.p.kfilter) {
+        if (!c.prior_covers_all) batch_copy<T>(s, y, x, nb * R);
+        if (!B.segR.empty())
+            fft_convolve_segments<T>(s, c.fft, B.d_segR.p, (int64_t)B.segR.size(), x, y, c.d_fcov.p, c.d_fcov_rows.p,
+                                     c.nof / 2 + 1, B.fftwork.p, c.fft_batch, B.fft_nseg, B.segR.data());
+    } else {
+        batch_fill<T>(s, y, (T)0, nb * R);
+    }
+    batch_gather<T>(s, R, c.rec_layout, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, sample_records(c), m, c.ndeg,
+                    Ops<T>::dwb(c, B), Ops<T>::nna(c, B), x, y, active);
+}
+
+// z = M^-1 r for the R right-hand sides (src/noise_routines.f90:1159-1409)
+template <typename T>
+static void batch_precond(Context &c, BatchState &B, const T *r, T *z)
+{
+    hipStream_t s = c.stream;
+    int R = B.R;
+    int64_t nb = c.nbtot;
+    switch (c.precond_kind) {
+    case 0: batch_copy<T>(s, z, r, nb * R); break;
+    case 1: batch_diag<T>(s, R, r, c.d_prec.p, z, nb); break;
+    case 2:
+        batch_copy<T>(s, z, r, nb * R);
+        if (!B.segR.empty())
+            fft_convolve_segments<T>(s, c.fft, B.d_segR.p, (int64_t)B.segR.size(), r, z, c.d_fprec.p,
+                                     c.d_fprec_rows.p, c.nof / 2 + 1, B.fftwork.p, c.fft_batch, B.fft_nseg,
+                                     B.segR.data());
+        break;
+    default:
+        if (!c.prec_covers_all) batch_copy<T>(s, z, r, nb * R);
+        band_apply<T>(s, c.band, r, z, R, nb);
+        if (!c.segcg.empty()) {
+            // the inner-CG fallback segments, one right-hand side at a time in FP64
+            DBuf<double> r1, z1;
+            r1.alloc(nb);
+            z1.alloc(nb);
+            for (int k = 0; k < B.nrhs; ++k) {
+                batch_cast<T, double>(s, r + (int64_t)k * nb, r1.p, nb);
+                batch_cast<T, double>(s, z + (int64_t)k * nb, z1.p, nb);
+                cg_apply(c, r1.p, z1.p);
+                batch_cast<double, T>(s, z1.p, z + (int64_t)k * nb, nb);
+            }
+        }
+        break;
+    }
+}
+
+// ------------------------------------------------------------------ one PCG over the batch
+
+template <typename T>
+struct Vec {
+    T *x, *r, *ro, *z, *p, *ap, *pw, *m, *part;
+};
+
+// one PCG step (src/smadam_routines.f90:785-924) for every active right-hand side
+template <typename T>
+static void pcg_step(Context &c, BatchState &B, const Vec<T> &v, double lim, int iter_min, int iter_max, int stall)
+{
+    hipStream_t s = c.stream;
+    int R = B.R;
+    int64_t nb = c.nbtot;
+    batch_operator<T>(c, B, v.p, v.ap, v.pw, v.m, v.part, B.active.p);
+    batch_dots<T>(s, 0, R, v.p, v.ap, nullptr, c.d_rmask.p, nb, B.dots.p);
+    batch_final(s, R, B.dots.p, 0, B.sc.p, B.active.p, lim, iter_min, iter_max);
+    batch_update_r<T>(s, R, B.sc.p, B.active.p, v.x, v.r, v.ro, v.p, v.ap, nb);
+    batch_precond<T>(c, B, v.r, v.z);
+    batch_dots<T>(s, 1, R, v.r, v.z, v.ro, c.d_rmask.p, nb, B.dots.p);
+    batch_final(s, R, B.dots.p, 1, B.sc.p, B.active.p, lim, iter_min, iter_max, nullptr, stall);
+    batch_update_p<T>(s, R, B.sc.p, B.active.p, v.p, v.z, nb);
+}
+
+static int count_active(Context &c, BatchState &B)
+{
+    std::vector<int> a(B.R);
+    d2h(a.data(), B.active.p, B.R, c.stream);
+    MADAM_CHECK(hipStreamSynchronize(c.stream));
+    int n = 0;
+    for (int v : a) n += v != 0;
+    return n;
+}
+
+// x = A^-1 b on the right-hand sides allowed by `allow` (null = all): PCG from
+// x = 0 until each right-hand side's exit test (lim < 0: its own BatchScal::lim)
+template <typename T>
+static int64_t batch_pcg(Context &c, BatchState &B, const Vec<T> &v, const T *b, const int *allow, double lim,
+                         int iter_min, int iter_max, int stall)
+{
+    hipStream_t s = c.stream;
+    int R = B.R;
+    int64_t nb = c.nbtot;
+    batch_copy<T>(s, v.r, b, nb * R);
+    batch_fill<T>(s, v.x, (T)0, nb * R);
+    batch_precond<T>(c, B, v.r, v.z);
+    batch_copy<T>(s, v.p, v.z, nb * R);
+    batch_dots<T>(s, 1, R, v.r, v.z, v.r, c.d_rmask.p, nb, B.dots.p);
+    batch_final(s, R, B.dots.p, 2, B.sc.p, B.active.p, lim, iter_min, iter_max, allow);
+    int gi = std::is_same<T, float>::value ? 1 : 0;
+    bool graph = B.use_graph && c.segcg.empty();
+    int64_t steps = 0;
+    double t0 = now();
+    while (count_active(c, B) > 0) {
+        if (graph) {
+            if (!B.gexec[gi]) {
+                MADAM_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                pcg_step<T>(c, B, v, lim, iter_min, iter_max, stall);
+                MADAM_CHECK(hipStreamEndCapture(s, &B.graph[gi]));
+                MADAM_CHECK(hipGraphInstantiate(&B.gexec[gi], B.graph[gi], nullptr, nullptr, 0));
+            }
+            MADAM_CHECK(hipGraphLaunch(B.gexec[gi], s));
+        } else {
+            pcg_step<T>(c, B, v, lim, iter_min, iter_max, stall);
+        }
+        MADAM_CHECK(hipGetLastError());
+        steps++;
+    }
+    B.t_inner += now() - t0;
+    B.n_inner += steps;
+    return steps;
+}
+
+// ------------------------------------------------------------------ set-up and solve
+
+template <typename T>
+static Vec<T> alloc_vectors(Context &c, BatchState &B)
+{
+    int64_t n = c.nbtot * B.R;
+    size_t e = sizeof(T);
+    for (DBuf<char> *b : {&B.x, &B.r, &B.ro, &B.z, &B.p, &B.ap}) b->alloc(std::max<int64_t>(n, 1) * e);
+    B.pw.alloc(std::max<int64_t>(n, 1) * e);
+    B.m.alloc((size_t)std::max<int64_t>(c.npix_cross, 1) * c.nmap * B.R * e);
+    B.part.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * B.R * e);
+    return Vec<T>{tp<T>(B.x), tp<T>(B.r), tp<T>(B.ro), tp<T>(B.z), tp<T>(B.p), tp<T>(B.ap),
+                  tp<T>(B.pw), tp<T>(B.m), tp<T>(B.part)};
+}
+
+// Autotuner: entries per pixel-side work item.  Short items balance the
+// long pixels (pole and caustic pixels hold thousands of entries) at the cost
+// of more fix-up partial sums; the best cut depends on the scan and on R.  The
+// candidates are timed on this pointing with HIP events and the fastest kept.
+template <typename T>
+static void tune_pieces(Context &c, BatchState &B, const Vec<T> &v)
+{
+    static const int cand[] = {256, 1024, 4096};
+    hipStream_t s = c.stream;
+    hipEvent_t e0, e1;
+    MADAM_CHECK(hipEventCreate(&e0));
+    MADAM_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bestp = cand[1];
+    for (int piece : cand) {
+        build_items(c, B, piece);
+        B.part.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * B.R * sizeof(T));
+        T *part = tp<T>(B.part);
+        // a warm-up launch, then two timed
+        batch_scatter<T>(s, B.R, c.rec_layout, c.nmap, B.bt, ent_records(c), v.pw, Ops<T>::cc(c, B), c.d_nzq.p, v.m,
+                         part);
+        MADAM_CHECK(hipEventRecord(e0, s));
+        for (int k = 0; k < 2; ++k)
+            batch_scatter<T>(s, B.R, c.rec_layout, c.nmap, B.bt, ent_records(c), v.pw, Ops<T>::cc(c, B), c.d_nzq.p,
+                             v.m, part);
+        MADAM_CHECK(hipEventRecord(e1, s));
+        MADAM_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        MADAM_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bestp = piece;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    build_items(c, B, bestp);
+}
+
+static void make_segments(Context &c, BatchState &B)
+{
+    B.segR.clear();
+    int64_t ns = (int64_t)c.segconv.size();
+    for (int r = 0; r < B.R; ++r)
+        for (int64_t i = 0; i < ns; ++i) {
+            Segment g = c.segconv[i];
+            g.b0 += (int64_t)r * c.nbtot;
+            B.segR.push_back(g);
+        }
+    if (!B.segR.empty()) {
+        B.d_segR.alloc(B.segR.size());
+        h2d(B.d_segR.p, B.segR.data(), B.segR.size(), c.stream);
+    }
+    B.fft_nseg = std::max<int64_t>((int64_t)B.segR.size(), 1);
+    if (c.p.kfilter) B.fftwork.alloc(fft_convolve_work_doubles(c.fft, c.fft_batch, B.fft_nseg));
+}
+
+// binned map and right-hand side of one signal (src/smadam_routines.f90:252-671),
+// with the shared pixel matrices of the set-up
+template <typename TT>
+static void bin_signal(Context &c, const TT *tod, double *rhs4, double *yba, DBuf<double> &cc6, DBuf<double> &rhs4x,
+                       DBuf<double> &m4x, DBuf<double> &nna)
+{
+    hipStream_t s = c.stream;
+    launch_bin<TT>(s, c.nmap, c.npix_map, c.d_off.p, c.d_ent.p, c.d_src.p, tod, c.d_dwb_s.p, c.d_sdwb_s.p, cc6.p,
+                   rhs4);
+    const double *rx = rhs4;
+    if (c.ndeg > 1) {
+        launch_degrade(s, c.nmap, c.npix_cross, c.ndeg, cc6.p, rhs4, cc6.p + c.npix_map * 6, rhs4x.p);
+        rx = rhs4x.p;
+    }
+    launch_ccmultiply(s, c.nmap, c.npix_cross, c.d_ccinv6x.p, rx, m4x.p);
+    launch_init_a<TT>(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, c.d_pixw.p, tod, m4x.p, c.ndeg, c.d_dwb.p,
+                      yba, nna.p, group_for(c));
+}
+
+template <typename TT>
+static void solve_batch_t(Context &c, int nrhs, const TT *signals, int on_device, int precision)
+{
+    if (!c.pointed) fail("madam_solve_batch called before the pointing was set");
+    if (!c.p.kfirst) fail("madam_solve_batch needs destriping (kfirst = T)");
+    if (c.p.diagfilter != 0 && !c.p.kfilter) fail("madam_solve_batch does not take diagfilter");
+    if (precision != 0 && precision != 1) fail("madam_solve_batch: precision must be 0 (FP64) or 1 (mixed)");
+    int R = batch_width(nrhs);
+    if (nrhs < 1 || R < 0) fail("madam_solve_batch: nrhs = %d outside 1..64", nrhs);
+    hipStream_t s = c.stream;
+    double tstart = now();
+    int64_t n = (int64_t)c.ndet * c.nsamp, nb = c.nbtot, np = c.npix_map;
+    if (!c.batch) c.batch = new BatchState();
+    BatchState &B = *c.batch;
+    if (B.R != R || B.prec != precision || !B.tables_ok || B.tables_ent != ent_records(c)) B.drop_graphs();
+    B.nrhs = nrhs;
+    B.prec = precision;
+    B.t_inner = 0;
+    B.n_inner = 0;
+    B.n_outer = 0;
+    const char *ge = getenv("MADAM_BATCH_GRAPH");
+    B.use_graph = !(ge && ge[0] == '0');
+    // the signals on the device, one at a time when they are in host memory
+    DBuf<TT> tod1;
+    auto signal = [&](int k) -> const TT * {
+        if (on_device) return signals + (int64_t)k * n;
+        tod1.alloc(std::max<int64_t>(n, 1));
+        h2d(tod1.p, signals + (int64_t)k * n, n, s);
+        return tod1.p;
+    };
+    // shared set-up with the first signal
+    solve_setup<TT>(c, signal(0));
+    bool rebuilt_prec = c.stats[ST_PREC_REUSED] == 0;
+    if (rebuilt_prec || B.R != R) B.drop_graphs();
+    B.R = R;
+    B.yba.alloc((size_t)std::max<int64_t>(nb, 1) * R);
+    B.rhs4.alloc((size_t)np * 4 * R);
+    batch_fill<double>(s, B.yba.p, 0.0, nb * R);
+    batch_copy<double>(s, B.yba.p, c.d_yba.p, nb);
+    batch_copy<double>(s, B.rhs4.p, c.d_rhs4.p, np * 4);
+    {
+        DBuf<double> cc6, rhs4x, m4x, nna;
+        cc6.alloc((size_t)(np + c.npix_cross) * 6);
+        rhs4x.alloc((size_t)c.npix_cross * 4);
+        m4x.alloc((size_t)c.npix_cross * 4);
+        nna.alloc(std::max<int64_t>(nb, 1));
+        for (int k = 1; k < nrhs; ++k)
+            bin_signal<TT>(c, signal(k), B.rhs4.p + (int64_t)k * np * 4, B.yba.p + (int64_t)k * nb, cc6, rhs4x, m4x,
+                           nna);
+        MADAM_CHECK(hipStreamSynchronize(s));
+    }
+    if (c.precond_kind == 3) band_reserve(c.band, R);
+    make_segments(c, B);
+    // per-RHS state
+    B.dots.alloc(batch_partials_doubles(R));
+    B.sc.alloc(R);
+    B.sc64.alloc(R);
+    B.active.alloc(R);
+    B.active64.alloc(R);
+    B.was_active.alloc(R);
+    B.ones.alloc(R);
+    {
+        std::vector<int> one(R, 0);
+        for (int k = 0; k < nrhs; ++k) one[k] = 1;
+        h2d(B.ones.p, one.data(), R, s);   // padding right-hand sides beyond nrhs stay inactive
+    }
+    B.a64.alloc((size_t)std::max<int64_t>(nb, 1) * R);
+    const Params &p = c.p;
+    if (precision == 0) {
+        Vec<double> v = alloc_vectors<double>(c, B);
+        if (!B.tables_ok || B.tables_ent != ent_records(c)) {
+            tune_pieces<double>(c, B, v);
+            B.part.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * R * sizeof(double));
+            v.part = tp<double>(B.part);
+            B.tables_ok = true;
+            B.tables_ent = ent_records(c);
+            B.drop_graphs();
+        }
+        v.x = B.a64.p;
+        B.drop_graphs();   // the solution vector is bound into the captured step
+        double t0 = now();
+        batch_pcg<double>(c, B, v, B.yba.p, B.ones.p, p.cglimit, p.iter_min, p.iter_max, 0);
+        B.hsc.resize(R);
+        d2h(B.hsc.data(), B.sc.p, R, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        B.n_outer = 1;
+        B.t_solve = now() - t0;
+        for (int k = 0; k < nrhs; ++k)
+            if (B.hsc[k].status == 2) fail("CG is diverging (right-hand side %d)", k);
+    } else {
+        // FP32 copies of the shared terms
+        B.dwb_s32.alloc(std::max<int64_t>(nb, 1));
+        B.dwb32.alloc(std::max<int64_t>(nb, 1));
+        B.nna32.alloc(std::max<int64_t>(nb, 1));
+        B.cc_c32.alloc((size_t)std::max<int64_t>(c.nzpix, 1) * 6);
+        batch_cast<double, float>(s, c.d_dwb_s.p, B.dwb_s32.p, nb);
+        batch_cast<double, float>(s, c.d_dwb.p, B.dwb32.p, nb);
+        batch_cast<double, float>(s, c.d_nna.p, B.nna32.p, nb);
+        batch_cast<double, float>(s, c.d_ccinv_c.p, B.cc_c32.p, c.nzpix * 6);
+        Vec<float> v = alloc_vectors<float>(c, B);
+        if (!B.tables_ok || B.tables_ent != ent_records(c)) {
+            tune_pieces<float>(c, B, v);
+            B.tables_ok = true;
+            B.tables_ent = ent_records(c);
+            B.drop_graphs();
+        }
+        B.part.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * R * sizeof(float));
+        v.part = tp<float>(B.part);
+        // the FP64 operator's own buffers (its pieces in the same tables)
+        B.pw64.alloc((size_t)std::max<int64_t>(nb, 1) * R);
+        B.m64.alloc((size_t)std::max<int64_t>(c.npix_cross, 1) * c.nmap * R);
+        B.part64.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * R);
+        B.r64.alloc((size_t)std::max<int64_t>(nb, 1) * R);
+        B.ap64.alloc((size_t)std::max<int64_t>(nb, 1) * R);
+        double t0 = now();
+        batch_fill<double>(s, B.a64.p, 0.0, nb * R);
+        batch_copy<double>(s, B.r64.p, B.yba.p, nb * R);
+        // rrinit per right-hand side: outer stage 2 on r = yba (src/smadam_routines.f90:737-756)
+        batch_dots<double>(s, 1, R, B.r64.p, B.r64.p, B.r64.p, c.d_rmask.p, nb, B.dots.p);
+        batch_final(s, R, B.dots.p, 2, B.sc64.p, B.active64.p, p.cglimit, 0, 64, B.ones.p);
+        const int max_outer = 20;
+        float *r32 = tp<float>(B.ro);   // the inner right-hand side lives in the PCG's ro buffer until it starts
+        DBuf<float> b32;
+        b32.alloc((size_t)std::max<int64_t>(nb, 1) * R);
+        (void)r32;
+        for (int outer = 0; outer < max_outer; ++outer) {
+            std::vector<int> a(R);
+            d2h(a.data(), B.active64.p, R, s);
+            MADAM_CHECK(hipStreamSynchronize(s));
+            int na = 0;
+            for (int k = 0; k < R; ++k) na += a[k] != 0;
+            if (na == 0) break;
+            // inner FP32 solve of A d = r to the reduction still needed
+            batch_cast<double, float>(s, B.r64.p, b32.p, nb * R);
+            batch_inner_limits(s, R, B.sc64.p, B.sc.p, p.cglimit, 1e-12);
+            MADAM_CHECK(hipMemcpyAsync(B.was_active.p, B.active64.p, sizeof(int) * R, hipMemcpyDeviceToDevice, s));
+            batch_pcg<float>(c, B, v, b32.p, B.was_active.p, -1.0, p.iter_min, p.iter_max, 50);
+            // a += d, then the FP64 residual r = yba - A64 a
+            batch_accumulate<float>(s, R, B.a64.p, v.x, B.was_active.p, nb);
+            batch_operator<double>(c, B, B.a64.p, B.ap64.p, B.pw64.p, B.m64.p, B.part64.p, B.ones.p);
+            batch_residual(s, B.yba.p, B.ap64.p, B.r64.p, nb * R);
+            batch_dots<double>(s, 1, R, B.r64.p, B.r64.p, B.r64.p, c.d_rmask.p, nb, B.dots.p);
+            batch_final(s, R, B.dots.p, 3, B.sc64.p, B.active64.p, p.cglimit, 0, max_outer);
+            B.n_outer++;
+        }
+        B.hsc.resize(R);
+        B.hsc64.resize(R);
+        d2h(B.hsc.data(), B.sc.p, R, s);
+        d2h(B.hsc64.data(), B.sc64.p, R, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        B.t_solve = now() - t0;
+    }
+    // final maps per right-hand side (src/smadam_routines.f90:1489-1539, src/map_routines.f90:218-357)
+    c.d_ccinv6.alloc(np * 6);
+    c.d_mask.alloc(np);
+    c.d_crit.alloc(np);
+    MADAM_CHECK(hipMemsetAsync(c.d_crit.p, 0, sizeof(double) * np, s));
+    launch_invert(s, c.nmap, np, c.d_cc6.p, c.d_ccinv6.p, c.d_crit.p, c.d_mask.p, p.pixmode_map, p.pixlim_map,
+                  p.incomplete_matrices, p.allow_decoupling, nullptr);
+    B.map4.alloc((size_t)np * 4 * R);
+    B.binmap4.alloc((size_t)np * 4 * R);
+    {
+        DBuf<double> aa_s, rhsd4;
+        aa_s.alloc(std::max<int64_t>(nb, 1));
+        rhsd4.alloc(np * 4);
+        for (int k = 0; k < nrhs; ++k) {
+            const double *rhs = B.rhs4.p + (int64_t)k * np * 4;
+            MADAM_CHECK(hipMemcpyAsync(rhsd4.p, rhs, sizeof(double) * np * 4, hipMemcpyDeviceToDevice, s));
+            launch_gather_order(s, B.a64.p + (int64_t)k * nb, c.d_bord.p, aa_s.p, nb);
+            launch_subtract_baselines(s, c.nmap, np, c.d_off.p, c.d_ent.p, aa_s.p, c.d_dwb_s.p, rhsd4.p);
+            launch_ccmultiply(s, c.nmap, np, c.d_ccinv6.p, rhsd4.p, B.map4.p + (int64_t)k * np * 4);
+            launch_ccmultiply(s, c.nmap, np, c.d_ccinv6.p, rhs, B.binmap4.p + (int64_t)k * np * 4);
+        }
+    }
+    MADAM_CHECK(hipGetLastError());
+    MADAM_CHECK(hipStreamSynchronize(s));
+    B.t_setup = now() - tstart - B.t_solve;
+}
+
+void solve_batch(Context &c, int nrhs, const void *signals, int tod_f32, int on_device, int precision)
+{
+    if (tod_f32) solve_batch_t<float>(c, nrhs, (const float *)signals, on_device, precision);
+    else solve_batch_t<double>(c, nrhs, (const double *)signals, on_device, precision);
+}
+
+static BatchState &batch_of(Context &c, int irhs)
+{
+    if (!c.batch || c.batch->nrhs == 0) fail("no batched solution available");
+    if (irhs < 0 || irhs >= c.batch->nrhs) fail("right-hand side %d outside 0..%d", irhs, c.batch->nrhs - 1);
+    return *c.batch;
+}
+
+void batch_get_maps(Context &c, int irhs, double *map, double *binmap)
+{
+    BatchState &B = batch_of(c, irhs);
+    int64_t np = c.npix_map;
+    int nm = c.nmap;
+    std::vector<double> m4(np * 4), b4(np * 4);
+    d2h(m4.data(), B.map4.p + (int64_t)irhs * np * 4, np * 4, c.stream);
+    d2h(b4.data(), B.binmap4.p + (int64_t)irhs * np * 4, np * 4, c.stream);
+    MADAM_CHECK(hipStreamSynchronize(c.stream));
+    for (int64_t q = 0; q < np; ++q)
+        for (int a = 0; a < nm; ++a) {
+            if (map) map[q * nm + a] = m4[q * 4 + a];
+            if (binmap) binmap[q * nm + a] = b4[q * 4 + a];
+        }
+}
+
+void batch_get_baselines(Context &c, int irhs, double *aa, double *yba)
+{
+    BatchState &B = batch_of(c, irhs);
+    int64_t nb = c.nbtot;
+    if (aa) d2h(aa, B.a64.p + (int64_t)irhs * nb, nb, c.stream);
+    if (yba) d2h(yba, B.yba.p + (int64_t)irhs * nb, nb, c.stream);
+    MADAM_CHECK(hipStreamSynchronize(c.stream));
+}
+
+// out: steps of the (last inner) PCG, its status, its rr / rr0, refinement
+// passes, the FP64 residual's rr / rrinit (mixed precision; FP64: the PCG's),
+// t_setup, t_solve, inner PCG steps over the batch, t_inner, piece
+void batch_get_info(Context &c, int irhs, double *out, int n)
+{
+    BatchState &B = batch_of(c, irhs);
+    const BatchScal &S = B.hsc[irhs];
+    double v[10];
+    v[0] = S.steps;
+    v[1] = S.status;
+    v[2] = S.rr0 > 0 ? S.rr / S.rr0 : 0;
+    if (B.prec == 1) {
+        const BatchScal &O = B.hsc64[irhs];
+        v[3] = O.steps;
+        v[4] = O.rr0 > 0 ? O.rr / O.rr0 : 0;
+    } else {
+        v[3] = 1;
+        v[4] = v[2];
+    }
+    v[5] = B.t_setup;
+    v[6] = B.t_solve;
+    v[7] = (double)B.n_inner;
+    v[8] = B.t_inner;
+    v[9] = B.piece;
+    for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];
+}
+
+}  // namespace madam

@@ -226,15 +226,16 @@ __global__ void k_c2r_prep(double2 *data, int64_t N, int L1, int L2, FftPlan P, 
 static constexpr int SEG_NB = 64;
 
 // partial[i * SEG_NB + blk] = sum over the block's stride of scale * v[j], j < noba
-__global__ void k_seg_partial(const double *v, int64_t vstride, int use_b0, const Segment *seg, int64_t s0,
+template <typename V>
+__global__ void k_seg_partial(const V *v, int64_t vstride, int use_b0, const Segment *seg, int64_t s0,
                               double scale, double *partial)
 {
     int64_t i = blockIdx.y;
     Segment sg = seg[s0 + i];
-    const double *x = use_b0 ? v + sg.b0 : v + i * vstride;
+    const V *x = use_b0 ? v + sg.b0 : v + i * vstride;
     double acc = 0;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < sg.noba; j += (int64_t)SEG_NB * blockDim.x)
-        acc += x[j] * scale;
+        acc += (double)x[j] * scale;
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     __shared__ double sh[FFT_TPB / 64];
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
@@ -254,20 +255,22 @@ __device__ inline double seg_mean(const double *partial, int64_t i, int64_t noba
 }
 
 // work[i][j] = x[j] - mean for j < noba, 0 up to nof; grid (blocks, batch)
-__global__ void k_seg_load(const double *x, const Segment *seg, int64_t s0, int64_t nof, double *work,
+template <typename T>
+__global__ void k_seg_load(const T *x, const Segment *seg, int64_t s0, int64_t nof, double *work,
                            const double *partial)
 {
     int64_t i = blockIdx.y;
     Segment sg = seg[s0 + i];
     double mean = seg_mean(partial, i, sg.noba);
-    const double *xs = x + sg.b0;
+    const T *xs = x + sg.b0;
     double *w = work + i * nof;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nof; j += (int64_t)gridDim.x * blockDim.x)
-        w[j] = j < sg.noba ? xs[j] - mean : 0.0;
+        w[j] = j < sg.noba ? (double)xs[j] - mean : 0.0;
 }
 
 // y[b0 + j] = work[i][j] / nof - mean(work / nof), j < noba
-__global__ void k_seg_store(const double *work, const Segment *seg, int64_t s0, int64_t nof, double *y,
+template <typename T>
+__global__ void k_seg_store(const double *work, const Segment *seg, int64_t s0, int64_t nof, T *y,
                             const double *partial)
 {
     int64_t i = blockIdx.y;
@@ -276,7 +279,7 @@ __global__ void k_seg_store(const double *work, const Segment *seg, int64_t s0, 
     double mean = seg_mean(partial, i, sg.noba);
     const double *w = work + i * nof;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < sg.noba; j += (int64_t)gridDim.x * blockDim.x)
-        y[sg.b0 + j] = w[j] * inv - mean;
+        y[sg.b0 + j] = (T)(w[j] * inv - mean);
 }
 
 __global__ void k_seg_ipsd(const Segment *seg, int64_t s0, int64_t nb, int *ipsd)
@@ -484,7 +487,8 @@ __host__ __device__ inline int conv_lds_elems(int n) { return n + (n >> 3) + 1; 
 static constexpr int CONV_COL_TPB = 512;
 
 // grid = (L2 / CONV_NCOL, batch)
-__global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const double *x, const Segment *seg, int64_t s0, const double *partial_in,
+template <typename T>
+__global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const T *x, const Segment *seg, int64_t s0, const double *partial_in,
                                double2 *work, int64_t N, int L1, int logL1, int L2, FftPlan P)
 {
     extern __shared__ double2 s[];
@@ -503,7 +507,7 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const double *x, 
     }
     __syncthreads();
     double mean = smean;
-    const double *xs = x + sg.b0;
+    const T *xs = x + sg.b0;
     int64_t noba = sg.noba;
     double2 *d = work + b * N;
     int tot = L1 * ncol;
@@ -511,8 +515,8 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const double *x, 
         int col = t % ncol, n1 = t / ncol;
         int64_t jj = 2 * ((int64_t)n1 * L2 + c0 + col);
         double2 v;
-        v.x = jj < noba ? xs[jj] - mean : 0.0;
-        v.y = jj + 1 < noba ? xs[jj + 1] - mean : 0.0;
+        v.x = jj < noba ? (double)xs[jj] - mean : 0.0;
+        v.y = jj + 1 < noba ? (double)xs[jj + 1] - mean : 0.0;
         s[lp(bitrev(n1, logL1) * ncol + col)] = v;
     }
     __syncthreads();
@@ -620,7 +624,8 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
 }
 
 // grid = (L2 / CONV_NCOL, batch)
-__global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *work, const Segment *seg, int64_t s0, double *y, double *partial_out,
+template <typename T>
+__global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *work, const Segment *seg, int64_t s0, T *y, double *partial_out,
                                int64_t N, int L1, int logL1, int L2, FftPlan P)
 {
     extern __shared__ double2 s[];
@@ -641,19 +646,19 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *wo
     __syncthreads();
     lds_fft8(s, L1, logL1, 3, P.sub1, true);
     double acc = 0;
-    double *ys = y + sg.b0;
+    T *ys = y + sg.b0;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, n1 = t / ncol;
         int64_t jj = 2 * ((int64_t)n1 * L2 + c0 + col);
         double2 v = s[lp(n1 * ncol + col)];
         if (jj < noba) {
             double a = v.x * inv;
-            ys[jj] = a;
+            ys[jj] = (T)a;
             acc += a;
         }
         if (jj + 1 < noba) {
             double a = v.y * inv;
-            ys[jj + 1] = a;
+            ys[jj + 1] = (T)a;
             acc += a;
         }
     }
@@ -671,7 +676,8 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *wo
 // y[b0 + j] -= mean over the batch's segment b, the mean summed by the first
 // wave from the segment's nblk partials (lane l adds partials l, l + 64, ...,
 // then a butterfly over the lanes: the same order in every block)
-__global__ void k_conv_sub_mean(double *y, const Segment *seg, int64_t s0, const double *partial_out, int nblk)
+template <typename T>
+__global__ void k_conv_sub_mean(T *y, const Segment *seg, int64_t s0, const double *partial_out, int nblk)
 {
     int64_t b = blockIdx.y;
     Segment sg = seg[s0 + b];
@@ -685,9 +691,9 @@ __global__ void k_conv_sub_mean(double *y, const Segment *seg, int64_t s0, const
     }
     __syncthreads();
     double mb = smb;
-    double *ys = y + sg.b0;
+    T *ys = y + sg.b0;
     for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < sg.noba; j += (int64_t)gridDim.x * blockDim.x)
-        ys[j] -= mb;
+        ys[j] = (T)((double)ys[j] - mb);
 }
 
 // fT[r * L2 + k2] = f[r + L1 * k2], fT[N] = f[N], per spectrum row
@@ -718,9 +724,10 @@ static bool conv_fusable(const FftPlan &p)
     return p.L2 >= CONV_NCOL && p.L1 >= 2 && p.L1 * CONV_NCOL <= 8192 && p.L2 <= 2048;
 }
 
-void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg, const double *x,
-                           double *y, const double *fc, const double *fc_rows, int64_t fc_stride, double *work,
-                           int64_t batch, int64_t nseg_max, const Segment *hseg)
+template <typename T>
+void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg, const T *x, T *y,
+                           const double *fc, const double *fc_rows, int64_t fc_stride, double *work, int64_t batch,
+                           int64_t nseg_max, const Segment *hseg)
 {
     (void)hseg;
     if (nseg > nseg_max)
@@ -738,38 +745,45 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
     if (fused)
         for (int64_t q0 = 0; q0 < nseg; q0 += 65535) {
             int64_t nq = std::min<int64_t>(nseg - q0, 65535);
-            k_seg_partial<<<dim3(SEG_NB, (unsigned)nq), FFT_TPB, 0, s>>>(x, 0, 1, dseg, q0, 1.0, partial + q0 * SEG_NB);
+            k_seg_partial<T><<<dim3(SEG_NB, (unsigned)nq), FFT_TPB, 0, s>>>(x, 0, 1, dseg, q0, 1.0, partial + q0 * SEG_NB);
         }
     for (int64_t s0 = 0; s0 < nseg; s0 += batch) {
         int64_t nb = nseg - s0 < batch ? nseg - s0 : batch;
         dim3 gp(SEG_NB, (unsigned)nb);
         if (fused) {
-            k_conv_col_fwd<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
+            k_conv_col_fwd<T><<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 x, dseg, s0, partial + s0 * SEG_NB, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
             k_conv_rows<<<dim3((unsigned)(p.L1 / 2), (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(2 * p.L2)), s>>>(
                 (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
-            k_conv_col_inv<<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
+            k_conv_col_inv<T><<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
                 (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
                 p);
             unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
             if (gl > 1024) gl = 1024;
-            k_conv_sub_mean<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
+            k_conv_sub_mean<T><<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
             continue;
         }
-        k_seg_partial<<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
+        k_seg_partial<T><<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
         unsigned gl = (unsigned)((nof + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
         if (gl > 1024) gl = 1024;
-        k_seg_load<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(x, dseg, s0, nof, work, partial);
+        k_seg_load<T><<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(x, dseg, s0, nof, work, partial);
         k_seg_ipsd<<<(unsigned)((nb + 255) / 256), 256, 0, s>>>(dseg, s0, nb, ipsd);
         fft_forward(s, p, (double2 *)work, nb);
         dim3 gs((unsigned)((p.N / 2 + 1 + FFT_TPB - 1) / FFT_TPB), (unsigned)nb);
         k_spectral<<<gs, FFT_TPB, 0, s>>>((double2 *)work, p.N, (int)p.L1, (int)p.L2, p, fc,
                                            fc_stride, ipsd);
         fft_inverse(s, p, (double2 *)work, nb);
-        k_seg_partial<<<gp, FFT_TPB, 0, s>>>(work, nof, 0, dseg, s0, 1 / (double)nof, partial);
-        k_seg_store<<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(work, dseg, s0, nof, y, partial);
+        k_seg_partial<double><<<gp, FFT_TPB, 0, s>>>(work, nof, 0, dseg, s0, 1 / (double)nof, partial);
+        k_seg_store<T><<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(work, dseg, s0, nof, y, partial);
     }
 }
+
+template void fft_convolve_segments<double>(hipStream_t, const FftPlan &, const Segment *, int64_t, const double *,
+                                            double *, const double *, const double *, int64_t, double *, int64_t,
+                                            int64_t, const Segment *);
+template void fft_convolve_segments<float>(hipStream_t, const FftPlan &, const Segment *, int64_t, const float *,
+                                           float *, const double *, const double *, int64_t, double *, int64_t,
+                                           int64_t, const Segment *);
 
 size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch, int64_t nseg_max)
 {

@@ -106,8 +106,10 @@ void launch_pack(hipStream_t s, const int32_t *pix, const float *w, int nnz, int
 void launch_hist(hipStream_t s, const PixW *pixw, uint64_t n, int32_t *counts);
 void launch_exclusive_scan_i32_to_i64(hipStream_t s, const int32_t *in, int64_t n, int64_t *out, int64_t *tmp,
                                       int64_t tmp_len);
+// TT: the TOD's type (double, or float for the mixed-precision solves)
+template <typename TT>
 void launch_bin(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent, const uint32_t *src,
-                const double *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4);
+                const TT *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4);
 void launch_degrade(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const double *cc6, const double *rhs4,
                     double *cc6x, double *rhs4x);
 void launch_bin_subset(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent,
@@ -117,8 +119,9 @@ void launch_invert(hipStream_t s, int nmap, int64_t npix, const double *cc6, dou
                    int32_t *mask, int pixmode, double pixlim, int incomplete, int allow_decoupling,
                    unsigned long long *ngood);
 void launch_ccmultiply(hipStream_t s, int nmap, int64_t npix, const double *ccinv6, const double *rhs4, double *out4);
+template <typename TT>
 void launch_init_a(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
-                   const PixW *pixw, const double *tod, const double *m4, int64_t ndeg, const double *dwb,
+                   const PixW *pixw, const TT *tod, const double *m4, int64_t ndeg, const double *dwb,
                    double *yba, double *nna, int group);
 void launch_scatter_pixels(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
                            const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4);
@@ -238,9 +241,11 @@ void fft_plan_init(FftPlan &p, int64_t nof);
 void fft_plan_free(FftPlan &p);
 // convolve every segment: y = x outside segments is the caller's job (copy first)
 // fc: filter spectra, nrow x fc_stride; fc_rows: the same in row order (fft_spectrum_rows)
-void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg_dev,
-                           const double *x, double *y, const double *fc, const double *fc_rows, int64_t fc_stride,
-                           double *work, int64_t batch, int64_t nseg_max, const Segment *hseg);
+// T (float or double) is the type of x and y; the transforms run in FP64
+template <typename T>
+void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg_dev, const T *x, T *y,
+                           const double *fc, const double *fc_rows, int64_t fc_stride, double *work, int64_t batch,
+                           int64_t nseg_max, const Segment *hseg);
 // fT = fc reordered for the fused row pass: f[r + L1 k2] at [r * L2 + k2], f[N] at [N]
 void fft_spectrum_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrow, int64_t stride, double *fT);
 // nseg_max: the most segments one fft_convolve_segments call takes
@@ -271,7 +276,12 @@ struct BandPrec {
 int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *dseg, int64_t nseg,
                const int *nband_seg_host, const double *invcov_dev, int64_t nof, const double *nna, int *info_out,
                bool periodic = true);
-void band_apply(hipStream_t s, const BandPrec &bp, const double *r, double *z);
+// z = M^-1 r for nrhs right-hand sides at a stride of vstride elements (T =
+// float or double; the factor and the solves are FP64)
+template <typename T>
+void band_apply(hipStream_t s, const BandPrec &bp, const T *r, T *z, int nrhs = 1, int64_t vstride = 0);
+// make room for nrhs right-hand sides in band_apply (allocates; call outside graph capture)
+void band_reserve(const BandPrec &bp, int nrhs);
 void band_free(BandPrec &bp);
 
 // ------------------------------------------------------------------ pointing.hip
@@ -284,5 +294,8 @@ struct ScanDet {
 };
 void launch_expand_pointing(hipStream_t s, ScanParams sp, const ScanDet *dets, int ndet, int64_t nside,
                             int64_t i0, int64_t n, int64_t ld, int32_t *pix, float *w);
+
+// the message madam_last_error() returns (madam_host.cpp)
+void set_last_error(const char *msg);
 
 }  // namespace madam

@@ -307,8 +307,9 @@ void launch_exclusive_scan_i32_to_i64(hipStream_t s, const int32_t *in, int64_t 
 
 // pixel_matrix + bin_tod, one wave per map pixel over its CSR range.
 // cc6 = upper triangle (11,12,13,22,23,33), rhs4 = (T, Q, U, 0).
+template <typename TT>
 __global__ void k_bin(int nmap, int64_t npix, const int64_t *off, const CsrEnt *ent, const uint32_t *src,
-                      const double *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4)
+                      const TT *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4)
 {
     int lane = threadIdx.x & 63;
     int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -318,7 +319,7 @@ __global__ void k_bin(int nmap, int64_t npix, const int64_t *off, const CsrEnt *
         double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0, r0 = 0, r1 = 0, r2 = 0;
         for (int64_t j = j0 + lane; j < j1; j += 64) {
             CsrEnt e = ent[j];
-            double d = tod[src[j]];
+            double d = (double)tod[src[j]];
             double dw = dwb[e.b];
             if (nmap == 3) {
                 double sq = sdwb[e.b];
@@ -347,12 +348,17 @@ __global__ void k_bin(int nmap, int64_t npix, const int64_t *off, const CsrEnt *
     }
 }
 
+template <typename TT>
 void launch_bin(hipStream_t s, int nmap, int64_t npix_map, const int64_t *off, const CsrEnt *ent, const uint32_t *src,
-                const double *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4)
+                const TT *tod, const double *dwb, const double *sdwb, double *cc6, double *rhs4)
 {
-    k_bin<<<grid_for(npix_map, TPB / 64, 1 << 20), TPB, 0, s>>>(nmap, npix_map, off, ent, src, tod, dwb, sdwb,
-                                                               cc6, rhs4);
+    k_bin<TT><<<grid_for(npix_map, TPB / 64, 1 << 20), TPB, 0, s>>>(nmap, npix_map, off, ent, src, tod, dwb, sdwb,
+                                                                   cc6, rhs4);
 }
+template void launch_bin<double>(hipStream_t, int, int64_t, const int64_t *, const CsrEnt *, const uint32_t *,
+                                 const double *, const double *, const double *, double *, double *);
+template void launch_bin<float>(hipStream_t, int, int64_t, const int64_t *, const CsrEnt *, const uint32_t *,
+                                const float *, const double *, const double *, double *, double *);
 
 // pixel_matrix + count_hits + bin_tod over the samples of one subset (detector
 // set x survey spans, src/smadam.F90:954-1257): smask[g] != 0 selects sample g
@@ -704,9 +710,9 @@ void launch_ccmultiply(hipStream_t s, int nmap, int64_t npix, const double *ccin
 // ---------------------------------------------------------------- time-ordered per-baseline reductions
 
 // initialize_a: yba, nna for order-0 baselines.  G lanes per baseline.
-template <int G>
+template <int G, typename TT>
 __global__ void k_init_a(int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart, const PixW *pixw,
-                         const double *tod, const double *m4, int64_t ndeg, const double *dwb, double *yba,
+                         const TT *tod, const double *m4, int64_t ndeg, const double *dwb, double *yba,
                          double *nna)
 {
     int lane = threadIdx.x & (G - 1);
@@ -719,7 +725,7 @@ __global__ void k_init_a(int nmap, int ndet, int64_t nsamp, int64_t nbase, const
         int64_t k = b - (int64_t)det * nbase;
         int64_t i0 = bstart[k], i1 = bstart[k + 1];
         const PixW *P = pixw + (int64_t)det * nsamp;
-        const double *T = tod + (int64_t)det * nsamp;
+        const TT *T = tod + (int64_t)det * nsamp;
         double y = 0, n = 0;
         for (int64_t i = i0 + lane; i < i1; i += G) {
             PixW r = P[i];
@@ -728,7 +734,7 @@ __global__ void k_init_a(int nmap, int ndet, int64_t nsamp, int64_t nbase, const
             double4 m = *(const double4 *)L;
             if (m.x == 0.0 && m.y == 0.0 && m.z == 0.0) continue;
             double dot = nmap == 1 ? m.x : (m.x * (double)r.w0 + m.y * (double)r.w1 + m.z * (double)r.w2);
-            y += T[i] - dot;
+            y += (double)T[i] - dot;
             n += 1.0;
         }
         y = group_sum<G>(y);
@@ -752,13 +758,24 @@ __global__ void k_init_a(int nmap, int ndet, int64_t nsamp, int64_t nbase, const
     default: KERNEL<64><<<grid, TPB, 0, s>>>(__VA_ARGS__); break;                   \
     }
 
+template <typename TT>
 void launch_init_a(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
-                   const PixW *pixw, const double *tod, const double *m4, int64_t ndeg, const double *dwb,
+                   const PixW *pixw, const TT *tod, const double *m4, int64_t ndeg, const double *dwb,
                    double *yba, double *nna, int group)
 {
     int grid = (int)(((int64_t)ndet * nbase + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
-    DISPATCH_G(group, k_init_a, nmap, ndet, nsamp, nbase, bstart, pixw, tod, m4, ndeg, dwb, yba, nna);
+    switch (group) {
+    case 4: k_init_a<4, TT><<<grid, TPB, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, tod, m4, ndeg, dwb, yba, nna); break;
+    case 8: k_init_a<8, TT><<<grid, TPB, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, tod, m4, ndeg, dwb, yba, nna); break;
+    case 16: k_init_a<16, TT><<<grid, TPB, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, tod, m4, ndeg, dwb, yba, nna); break;
+    case 32: k_init_a<32, TT><<<grid, TPB, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, tod, m4, ndeg, dwb, yba, nna); break;
+    default: k_init_a<64, TT><<<grid, TPB, 0, s>>>(nmap, ndet, nsamp, nbase, bstart, pixw, tod, m4, ndeg, dwb, yba, nna); break;
+    }
 }
+template void launch_init_a<double>(hipStream_t, int, int, int64_t, int64_t, const int64_t *, const PixW *,
+                                    const double *, const double *, int64_t, const double *, double *, double *, int);
+template void launch_init_a<float>(hipStream_t, int, int, int64_t, int64_t, const int64_t *, const PixW *,
+                                   const float *, const double *, int64_t, const double *, double *, double *, int);
 
 // map_to_baseline_order0: ap[b] += nna[b] p[b] - dw[b] * sum_i w_i . m[cross(pix_i)]
 template <int G>
