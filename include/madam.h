@@ -64,6 +64,9 @@ int madam_restore_pixels(void *ctx, int32_t *host_pix);
 int madam_get_sizes(void *ctx, long *out6);
 /* maps [npix][nmap], cc [npix][nmap][nmap] inverse pixel matrices; NULL skips */
 int madam_get_maps(void *ctx, double *map, double *binmap, int32_t *hits, double *cc, int32_t *mask, double *crit);
+/* hit counts [npix] of the pointing set with madam_set_pointing (count_hits,
+ * src/smadam_routines.f90:349-422) */
+int madam_get_hits(void *ctx, int32_t *hits);
 /* baselines [ndet][nbase] */
 int madam_get_baselines(void *ctx, double *aa, double *yba, double *nna);
 /* run statistics, out[0..n): niter, rrinit, rzinit, t_pointing, t_bin, t_init,
@@ -91,6 +94,10 @@ int madam_reset_timers(void *ctx);
  * vectors (FP64 dot products and scalars) inside an FP64 iterative refinement
  * to the same cglimit on the FP64 residual.  The signals are not modified. */
 int madam_solve_batch(void *ctx, int nrhs, const void *signals, int tod_f32, int on_device, int precision);
+/* benchmark: exactly niter steps of the last batch's PCG (the FP32 inner one
+ * for precision 1) on every right-hand side, restarted from 0, without the
+ * exit tests; the batch's results are not refreshed */
+int madam_iterate_batch(void *ctx, int niter);
 /* destriped and binned maps [npix][nmap] of right-hand side irhs (NULL skips) */
 int madam_get_batch_maps(void *ctx, int irhs, double *map, double *binmap);
 /* baselines and right-hand side [ndet][nbase] of right-hand side irhs */
@@ -98,7 +105,10 @@ int madam_get_batch_baselines(void *ctx, int irhs, double *aa, double *yba);
 /* out[0..n): last PCG's steps, status (1 converged, 2 diverging, 4 iteration
  * limit, 8 nothing to solve, 16 stalled), its rr / rr0, refinement passes,
  * the FP64 residual's rr / rrinit, t_setup, t_solve, PCG steps over the
- * batch, their time, entries per pixel-side work item (autotuned) */
+ * batch, their time, entries per pixel-side work item (autotuned); with
+ * madam_set_profiling on, the summed HIP-event times of the steps' pixel
+ * side, prior, baseline side, preconditioner and whole step, and the number
+ * of steps timed (n <= 16) */
 int madam_get_batch_info(void *ctx, int irhs, double *out, int n);
 /* one operator of the last madam_solve on host vectors x, y of ndet * nbase
  * baselines: which 0 = noise prior y = C_a^-1 x (cinvmul, src/noise_routines.f90:745-913),
@@ -119,6 +129,37 @@ int madam_set_kernel_variant(void *ctx, int variant);
  * open_offset_turns, spin_offset_turns}; outputs pix[ndet][n], weights[ndet][n][3] */
 int madam_expand_pointing(const double *scan5, const double *dets3, int ndet, long nside, long i0, long n,
                           int32_t *pix, float *weights, int on_device);
+/* ------------------------------------------------------------------ request loop
+ * Destripe jobs served on one GPU (no counterpart in the reference, whose
+ * destripe() serves one job per call and keeps one cached configuration in
+ * Monte Carlo mode, src/smadam.F90:440-448, 593-951).  Jobs are admitted by
+ * an estimate of their device memory against hbm_budget_bytes, first-fit in
+ * arrival order, up to max_inflight at once, each on its own stream; jobs
+ * with equal non-zero scan_key share one resident set-up (pointing, pixel
+ * matrices, filters, preconditioner: the key must identify the pointing and
+ * the parameters), and those waiting together are solved as one batch of
+ * right-hand sides.  The job's host buffers are read in place until it is
+ * done.  precision as madam_solve_batch; signal float64 or float32 (tod_f32). */
+void *madam_server_create(int device, double hbm_budget_bytes, int max_inflight);
+void madam_server_destroy(void *srv);
+/* returns the job id, or -1 when the job alone exceeds the budget */
+long madam_server_submit(void *srv, const char *parstring, long ndet, const double *detweights, long nsamp, long nnz,
+                         const double *timestamps, const int32_t *pix, const float *pixweights, const void *signal,
+                         int tod_f32, long nperiod, const long *periods, const long *npsd, long npsdtot,
+                         const double *psdstarts, long npsdbin, const double *psdfreqs, long npsdval,
+                         const double *psdvals, unsigned long long scan_key, int precision);
+/* 0 done, 1 failed (madam_server_error), 2 not finished within timeout_s (< 0: wait) */
+int madam_server_wait(void *srv, long id, double timeout_s);
+const char *madam_server_error(void *srv, long id);
+/* map, binmap [npix][nmap], hits [npix], aa [ndet][nbase] (NULL skips);
+ * sizes6 = {npix, nmap, nbase, ndet, batch size, resident scan};
+ * times4 = {submitted, started, done (seconds, steady clock), PCG steps} */
+int madam_server_result(void *srv, long id, double *map, double *binmap, int32_t *hits, double *aa, long *sizes6,
+                        double *times4);
+int madam_server_release(void *srv, long id);
+/* out = {done, failed, batches, jobs in batches, jobs on resident scans,
+ * evictions, peak estimated bytes, most jobs in flight, resident scans} */
+int madam_server_stats(void *srv, double *out, int n);
 const char *madam_last_error(void);
 /* status of the last destripe / destripe_with_cache call (0 ok); only
  * meaningful when MADAM_RAISE is set, otherwise errors end the process like

@@ -66,12 +66,32 @@ def lib():
         L.madam_iterate.argtypes = [vp, i32, i32]
         L.madam_solve_batch.restype = i32
         L.madam_solve_batch.argtypes = [vp, i32, vp, i32, i32, i32]
+        L.madam_iterate_batch.restype = i32
+        L.madam_iterate_batch.argtypes = [vp, i32]
         L.madam_get_batch_maps.restype = i32
         L.madam_get_batch_maps.argtypes = [vp, i32, vp, vp]
         L.madam_get_batch_baselines.restype = i32
         L.madam_get_batch_baselines.argtypes = [vp, i32, vp, vp]
         L.madam_get_batch_info.restype = i32
         L.madam_get_batch_info.argtypes = [vp, i32, vp, i32]
+        L.madam_get_hits.restype = i32
+        L.madam_get_hits.argtypes = [vp, vp]
+        L.madam_server_create.restype = vp
+        L.madam_server_create.argtypes = [i32, ct.c_double, i32]
+        L.madam_server_destroy.argtypes = [vp]
+        L.madam_server_submit.restype = lg
+        L.madam_server_submit.argtypes = [vp, cp, lg, vp, lg, lg, vp, vp, vp, vp, i32, lg, vp, vp, lg, vp, lg, vp,
+                                          lg, vp, ct.c_ulonglong, i32]
+        L.madam_server_wait.restype = i32
+        L.madam_server_wait.argtypes = [vp, lg, ct.c_double]
+        L.madam_server_error.restype = cp
+        L.madam_server_error.argtypes = [vp, lg]
+        L.madam_server_result.restype = i32
+        L.madam_server_result.argtypes = [vp, lg, vp, vp, vp, vp, vp, vp]
+        L.madam_server_release.restype = i32
+        L.madam_server_release.argtypes = [vp, lg]
+        L.madam_server_stats.restype = i32
+        L.madam_server_stats.argtypes = [vp, vp, i32]
         L.madam_get_filter.restype = i32
         L.madam_get_filter.argtypes = [vp, i32, vp]
         L.madam_apply.restype = i32
@@ -246,6 +266,9 @@ class Madam:
         self.nrhs = nrhs
         _check(lib().madam_solve_batch(self._h, nrhs, _ptr(signals), int(f32), int(on_device), int(precision)))
 
+    def iterate_batch(self, niter):
+        _check(lib().madam_iterate_batch(self._h, int(niter)))
+
     def batch_result(self, irhs):
         """map, binmap [npix][nmap], aa, yba [ndet][nbase] and the solve info of right-hand side irhs."""
         sz = self.sizes()
@@ -256,10 +279,10 @@ class Madam:
         aa = np.zeros((sz["ndet"], sz["nbase"]))
         yba = np.zeros((sz["ndet"], sz["nbase"]))
         _check(lib().madam_get_batch_baselines(self._h, int(irhs), _ptr(aa), _ptr(yba)))
-        info = np.zeros(10)
-        _check(lib().madam_get_batch_info(self._h, int(irhs), _ptr(info), 10))
         keys = ["steps", "status", "rr_inner", "passes", "rr_final", "t_setup", "t_solve", "pcg_steps", "t_pcg",
-                "piece"]
+                "piece", "t_scatter", "t_prior", "t_gather", "t_prec", "t_step", "n_prof"]
+        info = np.zeros(len(keys))
+        _check(lib().madam_get_batch_info(self._h, int(irhs), _ptr(info), len(keys)))
         return dict(map=m, binmap=b, aa=aa, yba=yba, info=dict(zip(keys, info.tolist())))
 
     def filter(self, ipsd=0):
@@ -276,6 +299,85 @@ class Madam:
 
     def set_profiling(self, on=True):
         _check(lib().madam_set_profiling(self._h, int(on)))
+
+
+class Server:
+    """Request loop on one GPU (madam_server_*): submit destripe jobs, wait for
+    them, read their maps.  Jobs with the same non-zero scan_key share one
+    resident set-up and are batched when they wait together."""
+
+    STAT_NAMES = ["done", "failed", "batches", "batched_jobs", "resident_jobs", "evictions", "peak_bytes",
+                  "max_inflight", "resident_scans"]
+
+    def __init__(self, device=0, hbm_budget=200e9, max_inflight=4):
+        self._h = lib().madam_server_create(int(device), float(hbm_budget), int(max_inflight))
+        self._jobs = {}
+
+    def close(self):
+        if self._h:
+            lib().madam_server_destroy(self._h)
+            self._h = None
+            self._jobs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, pars, weights, timestamps, pixels, pixweights, signal, periods, npsd, psdstarts, psdfreqs,
+               psdvals, nnz=3, scan_key=0, precision=0):
+        """signal [ndet][nsamp] float64 or float32; the arrays are kept until the job is released."""
+        w = np.ascontiguousarray(weights, np.float64)
+        ts = np.ascontiguousarray(timestamps, np.float64)
+        pix = np.ascontiguousarray(pixels, np.int32)
+        pw = np.ascontiguousarray(pixweights, np.float32)
+        sig = np.ascontiguousarray(signal)
+        assert sig.dtype in (np.float64, np.float32)
+        per = np.ascontiguousarray(periods, np.int64)
+        nps = np.ascontiguousarray(npsd, np.int64)
+        pst = np.ascontiguousarray(psdstarts, np.float64)
+        pfr = np.ascontiguousarray(psdfreqs, np.float64)
+        pv = np.ascontiguousarray(psdvals, np.float64).reshape(-1)
+        keep = (w, ts, pix, pw, sig, per, nps, pst, pfr, pv)
+        ndet, nsamp = w.size, ts.size
+        npsdbin = pfr.size
+        jid = lib().madam_server_submit(self._h, dict2parstring(pars), ndet, _ptr(w), nsamp, int(nnz), _ptr(ts),
+                                        _ptr(pix), _ptr(pw), _ptr(sig), int(sig.dtype == np.float32), per.size,
+                                        _ptr(per), _ptr(nps), pv.size // max(npsdbin, 1), _ptr(pst), npsdbin,
+                                        _ptr(pfr), pv.size, _ptr(pv), int(scan_key), int(precision))
+        if jid < 0:
+            raise MadamError("job exceeds the server's HBM budget")
+        self._jobs[jid] = (keep, ndet)
+        return jid
+
+    def wait(self, jid, timeout=-1.0):
+        rc = lib().madam_server_wait(self._h, int(jid), float(timeout))
+        if rc == 1:
+            raise MadamError(lib().madam_server_error(self._h, int(jid)).decode())
+        return rc == 0
+
+    def result(self, jid):
+        sz = np.zeros(6, np.int64)
+        tm = np.zeros(4)
+        _check(lib().madam_server_result(self._h, int(jid), None, None, None, None, _ptr(sz), _ptr(tm)))
+        npix, nmap, nbase, ndet = int(sz[0]), int(sz[1]), int(sz[2]), int(sz[3])
+        m = np.zeros((npix, nmap))
+        b = np.zeros((npix, nmap))
+        h = np.zeros(npix, np.int32)
+        aa = np.zeros((ndet, nbase))
+        _check(lib().madam_server_result(self._h, int(jid), _ptr(m), _ptr(b), _ptr(h), _ptr(aa), None, None))
+        return dict(map=m, binmap=b, hits=h, aa=aa, batch=int(sz[4]), resident=int(sz[5]), t_submit=tm[0],
+                    t_start=tm[1], t_done=tm[2], latency=tm[2] - tm[0], steps=tm[3])
+
+    def release(self, jid):
+        lib().madam_server_release(self._h, int(jid))
+        self._jobs.pop(jid, None)
+
+    def stats(self):
+        out = np.zeros(len(self.STAT_NAMES))
+        lib().madam_server_stats(self._h, _ptr(out), len(out))
+        return dict(zip(self.STAT_NAMES, out.tolist()))
 
 
 def cache_stats():

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmadam.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["madam_kernels.hip", "madam_fft.hip", "madam_band.hip", "madam_pointing.hip", "madam_csr.hip",
-           "madam_segred.hip", "madam_batch.hip", "madam_fits.cpp", "madam_host.cpp", "madam_batch_host.cpp"]
+           "madam_segred.hip", "madam_batch.hip", "madam_fits.cpp", "madam_host.cpp", "madam_batch_host.cpp", "madam_serve.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MADAM_OFFLOAD_ARCH", "gfx950")
 

@@ -228,9 +228,11 @@ __global__ void __launch_bounds__(BTPB) k_bgather(int ndet, int64_t nsamp, int64
     int r = lane % R, g = (lane / R) % G, kk = lane / (R * G);
     int64_t nruns = (nbase + BG_RUN - 1) / BG_RUN;
     int64_t wv = bx_block(blockIdx.x, gridDim.x) * (BTPB / 64) + (threadIdx.x >> 6);
-    int64_t det = wv / nruns;
-    if (det >= ndet) return;   // uniform over the wave
-    int64_t k0 = (wv - det * nruns) * BG_RUN;
+    // time-major: the waves of one time window, all detectors, run together
+    // (co-pointed detectors read the same map pixels)
+    int64_t run = wv / ndet, det = wv - run * ndet;
+    if (run >= nruns) return;   // uniform over the wave
+    int64_t k0 = run * BG_RUN;
     int64_t k1 = k0 + BG_RUN < nbase ? k0 + BG_RUN : nbase;
     int64_t nbtot = (int64_t)ndet * nbase;
     bool act = active[r] != 0;
@@ -240,7 +242,26 @@ __global__ void __launch_bounds__(BTPB) k_bgather(int ndet, int64_t nsamp, int64
         bool valid = k < k1;
         int64_t i0 = valid ? bstart[k] : 0, i1 = valid ? bstart[k + 1] : 0;
         T acc = 0;
-        for (int64_t i = i0 + g; i < i1; i += G) {
+        int64_t i = i0 + g;
+        // four samples per step, their records and then their map values in flight together
+        for (; i + 3 * G < i1; i += 4 * G) {
+            PixW e[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) e[u] = load_rec<RL>(rec, rbase + i + u * G);
+            T mv[4][NMAP];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const T *mq = m + (int64_t)((e[u].pix >= 0 ? e[u].pix : 0) >> dsh) * NMAP * R + r;
+#pragma unroll
+                for (int c = 0; c < NMAP; ++c) mv[u][c] = mq[c * R];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                T v = NMAP == 1 ? mv[u][0] : (T)e[u].w0 * mv[u][0] + (T)e[u].w1 * mv[u][1] + (T)e[u].w2 * mv[u][2];
+                acc += e[u].pix >= 0 ? v : (T)0;
+            }
+        }
+        for (; i < i1; i += G) {
             PixW e = load_rec<RL>(rec, rbase + i);
             if (e.pix >= 0) {
                 const T *mq = m + (int64_t)(e.pix >> dsh) * NMAP * R + r;

@@ -40,6 +40,7 @@ struct BatchState {
     const void *tables_ent = nullptr;   // the CSR the tables were built for
     // FP32 copies of the shared per-baseline / per-pixel terms
     DBuf<float> dwb_s32, dwb32, nna32, cc_c32;
+    DBuf<float> m4f;   // the one-RHS FP32 map, 4 per pixel (the FP64 one is the context's m4x)
     // FP64 per right-hand side (RHS-major)
     DBuf<double> yba, a64, r64, ap64, rhs4, map4, binmap4;
     DBuf<double> pw64, m64, part64;
@@ -59,9 +60,15 @@ struct BatchState {
     // the captured PCG step, per vector type
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
+    // the same step without exit tests, for madam_iterate_batch
+    hipGraphExec_t gexec_free = nullptr;
+    hipGraph_t graph_free = nullptr;
     bool use_graph = true;
     double t_setup = 0, t_solve = 0, t_inner = 0;
     int64_t n_inner = 0, n_outer = 0;
+    // profiling (Context::profile): HIP-event times of the parts of each PCG step, summed
+    double t_scatter = 0, t_prior = 0, t_gather = 0, t_prec = 0, t_step = 0;
+    int64_t n_prof = 0;
 
     void drop_graphs()
     {
@@ -71,6 +78,10 @@ struct BatchState {
             gexec[i] = nullptr;
             graph[i] = nullptr;
         }
+        if (gexec_free) (void)hipGraphExecDestroy(gexec_free);
+        if (graph_free) (void)hipGraphDestroy(graph_free);
+        gexec_free = nullptr;
+        graph_free = nullptr;
     }
     ~BatchState() { drop_graphs(); }
 };
@@ -79,6 +90,15 @@ void batch_release(Context &c)
 {
     delete c.batch;
     c.batch = nullptr;
+}
+
+// a new pointing: the work items and the captured steps refer to the old one
+void batch_invalidate(Context &c)
+{
+    if (!c.batch) return;
+    c.batch->tables_ok = false;
+    c.batch->drop_graphs();
+    c.batch->nrhs = 0;
 }
 
 template <typename T>
@@ -149,6 +169,7 @@ struct Ops<double> {
     static const double *dwb(Context &c, BatchState &) { return c.d_dwb.p; }
     static const double *nna(Context &c, BatchState &) { return c.d_nna.p; }
     static const double *cc(Context &c, BatchState &) { return c.d_ccinv_c.p; }
+    static double *m4(Context &c, BatchState &) { return c.d_m4x.p; }
 };
 template <>
 struct Ops<float> {
@@ -156,18 +177,38 @@ struct Ops<float> {
     static const float *dwb(Context &, BatchState &B) { return B.dwb32.p; }
     static const float *nna(Context &, BatchState &B) { return B.nna32.p; }
     static const float *cc(Context &, BatchState &B) { return B.cc_c32.p; }
+    static float *m4(Context &, BatchState &B) { return B.m4f.p; }
+};
+
+// events around the parts of a PCG step when profiling
+struct StepTimers {
+    Timer sc, pr, ga, pc, st;
 };
 
 // y = A x for the R right-hand sides (src/smadam_routines.f90:808-872): pixel
 // side with the ccmultiply, prior, baseline side
 template <typename T>
-static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T *m, T *part, const int *active)
+static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T *m, T *part, const int *active,
+                           StepTimers *tm = nullptr)
 {
     hipStream_t s = c.stream;
     int R = B.R;
     int64_t nb = c.nbtot;
+    // one right-hand side: the single solve's segmented-tile pixel side and
+    // flat baseline side in the operator's type; several: the batched kernels
+    bool one = R == 1;
     batch_pw<T>(s, R, x, nb, c.d_bord.p, Ops<T>::dwb_s(c, B), active, pw);
-    batch_scatter<T>(s, R, c.rec_layout, c.nmap, B.bt, ent_records(c), pw, Ops<T>::cc(c, B), c.d_nzq.p, m, part);
+    if (tm) tm->sc.start(s);
+    if (one)
+        launch_scatter_seg<T>(s, c.nmap, c.nactive, ent_records(c), c.rec_layout, pw, c.sr_s.heads.p, c.sr_s.k0.p,
+                              c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.nzpix, Ops<T>::cc(c, B),
+                              Ops<T>::m4(c, B), c.d_sr_part.p);
+    else
+        batch_scatter<T>(s, R, c.rec_layout, c.nmap, B.bt, ent_records(c), pw, Ops<T>::cc(c, B), c.d_nzq.p, m, part);
+    if (tm) {
+        tm->sc.stop(s);
+        tm->pr.start(s);
+    }
     if (c.p.kfilter) {
         if (!c.prior_covers_all) batch_copy<T>(s, y, x, nb * R);
         if (!B.segR.empty())
@@ -176,8 +217,17 @@ static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T
     } else {
         batch_fill<T>(s, y, (T)0, nb * R);
     }
-    batch_gather<T>(s, R, c.rec_layout, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, sample_records(c), m, c.ndeg,
-                    Ops<T>::dwb(c, B), Ops<T>::nna(c, B), x, y, active);
+    if (tm) {
+        tm->pr.stop(s);
+        tm->ga.start(s);
+    }
+    if (one)
+        launch_gather_flat<T>(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, sample_records(c), c.rec_layout,
+                              Ops<T>::m4(c, B), c.ndeg, Ops<T>::dwb(c, B), Ops<T>::nna(c, B), x, y);
+    else
+        batch_gather<T>(s, R, c.rec_layout, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, sample_records(c), m,
+                        c.ndeg, Ops<T>::dwb(c, B), Ops<T>::nna(c, B), x, y, active);
+    if (tm) tm->ga.stop(s);
 }
 
 // z = M^-1 r for the R right-hand sides (src/noise_routines.f90:1159-1409)
@@ -225,19 +275,32 @@ struct Vec {
 
 // one PCG step (src/smadam_routines.f90:785-924) for every active right-hand side
 template <typename T>
-static void pcg_step(Context &c, BatchState &B, const Vec<T> &v, double lim, int iter_min, int iter_max, int stall)
+static void pcg_step(Context &c, BatchState &B, const Vec<T> &v, double lim, int iter_min, int iter_max, int stall,
+                     StepTimers *tm = nullptr)
 {
     hipStream_t s = c.stream;
     int R = B.R;
     int64_t nb = c.nbtot;
-    batch_operator<T>(c, B, v.p, v.ap, v.pw, v.m, v.part, B.active.p);
+    if (tm) tm->st.start(s);
+    batch_operator<T>(c, B, v.p, v.ap, v.pw, v.m, v.part, B.active.p, tm);
     batch_dots<T>(s, 0, R, v.p, v.ap, nullptr, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 0, B.sc.p, B.active.p, lim, iter_min, iter_max);
     batch_update_r<T>(s, R, B.sc.p, B.active.p, v.x, v.r, v.ro, v.p, v.ap, nb);
+    if (tm) tm->pc.start(s);
     batch_precond<T>(c, B, v.r, v.z);
+    if (tm) tm->pc.stop(s);
     batch_dots<T>(s, 1, R, v.r, v.z, v.ro, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 1, B.sc.p, B.active.p, lim, iter_min, iter_max, nullptr, stall);
     batch_update_p<T>(s, R, B.sc.p, B.active.p, v.p, v.z, nb);
+    if (tm) {
+        tm->st.stop(s);
+        B.t_scatter += tm->sc.ms() * 1e-3;
+        B.t_prior += tm->pr.ms() * 1e-3;
+        B.t_gather += tm->ga.ms() * 1e-3;
+        B.t_prec += tm->pc.ms() * 1e-3;
+        B.t_step += tm->st.ms() * 1e-3;
+        B.n_prof++;
+    }
 }
 
 static int count_active(Context &c, BatchState &B)
@@ -266,11 +329,15 @@ static int64_t batch_pcg(Context &c, BatchState &B, const Vec<T> &v, const T *b,
     batch_dots<T>(s, 1, R, v.r, v.z, v.r, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 2, B.sc.p, B.active.p, lim, iter_min, iter_max, allow);
     int gi = std::is_same<T, float>::value ? 1 : 0;
-    bool graph = B.use_graph && c.segcg.empty();
+    // profiling times every part with events, so the steps run as launched
+    bool graph = B.use_graph && c.segcg.empty() && !c.profile;
+    StepTimers tmr;
     int64_t steps = 0;
     double t0 = now();
     while (count_active(c, B) > 0) {
-        if (graph) {
+        if (c.profile) {
+            pcg_step<T>(c, B, v, lim, iter_min, iter_max, stall, &tmr);
+        } else if (graph) {
             if (!B.gexec[gi]) {
                 MADAM_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 pcg_step<T>(c, B, v, lim, iter_min, iter_max, stall);
@@ -400,6 +467,8 @@ static void solve_batch_t(Context &c, int nrhs, const TT *signals, int on_device
     B.t_inner = 0;
     B.n_inner = 0;
     B.n_outer = 0;
+    B.t_scatter = B.t_prior = B.t_gather = B.t_prec = B.t_step = 0;
+    B.n_prof = 0;
     const char *ge = getenv("MADAM_BATCH_GRAPH");
     B.use_graph = !(ge && ge[0] == '0');
     // the signals on the device, one at a time when they are in host memory
@@ -450,7 +519,7 @@ static void solve_batch_t(Context &c, int nrhs, const TT *signals, int on_device
     const Params &p = c.p;
     if (precision == 0) {
         Vec<double> v = alloc_vectors<double>(c, B);
-        if (!B.tables_ok || B.tables_ent != ent_records(c)) {
+        if (R > 1 && (!B.tables_ok || B.tables_ent != ent_records(c))) {
             tune_pieces<double>(c, B, v);
             B.part.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * R * sizeof(double));
             v.part = tp<double>(B.part);
@@ -480,7 +549,8 @@ static void solve_batch_t(Context &c, int nrhs, const TT *signals, int on_device
         batch_cast<double, float>(s, c.d_nna.p, B.nna32.p, nb);
         batch_cast<double, float>(s, c.d_ccinv_c.p, B.cc_c32.p, c.nzpix * 6);
         Vec<float> v = alloc_vectors<float>(c, B);
-        if (!B.tables_ok || B.tables_ent != ent_records(c)) {
+        B.m4f.alloc((size_t)std::max<int64_t>(c.npix_cross, 1) * 4);
+        if (R > 1 && (!B.tables_ok || B.tables_ent != ent_records(c))) {
             tune_pieces<float>(c, B, v);
             B.tables_ok = true;
             B.tables_ent = ent_records(c);
@@ -565,6 +635,67 @@ void solve_batch(Context &c, int nrhs, const void *signals, int tod_f32, int on_
     else solve_batch_t<double>(c, nrhs, (const double *)signals, on_device, precision);
 }
 
+// Benchmark / continuation: exactly niter PCG steps of the last batch's PCG
+// (the inner FP32 one for precision 1) on every right-hand side, restarted
+// from x = 0 on its right-hand side, without the exit tests (src/smadam_routines.f90:785-924).
+template <typename T>
+static void iterate_t(Context &c, BatchState &B, const T *b, int niter)
+{
+    hipStream_t s = c.stream;
+    int R = B.R;
+    int64_t nb = c.nbtot;
+    Vec<T> v{tp<T>(B.x), tp<T>(B.r), tp<T>(B.ro), tp<T>(B.z), tp<T>(B.p), tp<T>(B.ap), tp<T>(B.pw), tp<T>(B.m),
+             tp<T>(B.part)};
+    batch_copy<T>(s, v.r, b, nb * R);   // x: the scratch solution vector, the batch's own stays
+    batch_fill<T>(s, v.x, (T)0, nb * R);
+    batch_precond<T>(c, B, v.r, v.z);
+    batch_copy<T>(s, v.p, v.z, nb * R);
+    batch_dots<T>(s, 1, R, v.r, v.z, v.r, c.d_rmask.p, nb, B.dots.p);
+    batch_final(s, R, B.dots.p, 2, B.sc.p, B.active.p, 0.0, 0, 1 << 30, B.ones.p);
+    const int big = 1 << 30;
+    StepTimers tmr;
+    bool graph = B.use_graph && c.segcg.empty() && !c.profile;
+    for (int k = 0; k < niter; ++k) {
+        if (c.profile) {
+            pcg_step<T>(c, B, v, 0.0, 0, big, 0, &tmr);
+        } else if (graph) {
+            if (!B.gexec_free) {
+                MADAM_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                pcg_step<T>(c, B, v, 0.0, 0, big, 0);
+                MADAM_CHECK(hipStreamEndCapture(s, &B.graph_free));
+                MADAM_CHECK(hipGraphInstantiate(&B.gexec_free, B.graph_free, nullptr, nullptr, 0));
+            }
+            MADAM_CHECK(hipGraphLaunch(B.gexec_free, s));
+        } else {
+            pcg_step<T>(c, B, v, 0.0, 0, big, 0);
+        }
+    }
+    MADAM_CHECK(hipGetLastError());
+    MADAM_CHECK(hipStreamSynchronize(s));
+}
+
+void batch_iterate(Context &c, int niter)
+{
+    if (!c.batch || c.batch->nrhs == 0) fail("madam_iterate_batch needs a completed madam_solve_batch");
+    BatchState &B = *c.batch;
+    if (B.prec == 0) {
+        iterate_t<double>(c, B, B.yba.p, niter);
+    } else {
+        DBuf<float> b32;
+        b32.alloc((size_t)std::max<int64_t>(c.nbtot, 1) * B.R);
+        batch_cast<double, float>(c.stream, B.yba.p, b32.p, c.nbtot * B.R);
+        iterate_t<float>(c, B, b32.p, niter);
+    }
+}
+
+void batch_reset_timers(Context &c)
+{
+    if (!c.batch) return;
+    BatchState &B = *c.batch;
+    B.t_scatter = B.t_prior = B.t_gather = B.t_prec = B.t_step = 0;
+    B.n_prof = 0;
+}
+
 static BatchState &batch_of(Context &c, int irhs)
 {
     if (!c.batch || c.batch->nrhs == 0) fail("no batched solution available");
@@ -599,12 +730,14 @@ void batch_get_baselines(Context &c, int irhs, double *aa, double *yba)
 
 // out: steps of the (last inner) PCG, its status, its rr / rr0, refinement
 // passes, the FP64 residual's rr / rrinit (mixed precision; FP64: the PCG's),
-// t_setup, t_solve, inner PCG steps over the batch, t_inner, piece
+// t_setup, t_solve, inner PCG steps over the batch, t_inner, piece; with
+// profiling, the summed event times of the steps' pixel side, prior,
+// baseline side, preconditioner and whole step, and the steps timed
 void batch_get_info(Context &c, int irhs, double *out, int n)
 {
     BatchState &B = batch_of(c, irhs);
     const BatchScal &S = B.hsc[irhs];
-    double v[10];
+    double v[16];
     v[0] = S.steps;
     v[1] = S.status;
     v[2] = S.rr0 > 0 ? S.rr / S.rr0 : 0;
@@ -621,7 +754,13 @@ void batch_get_info(Context &c, int irhs, double *out, int n)
     v[7] = (double)B.n_inner;
     v[8] = B.t_inner;
     v[9] = B.piece;
-    for (int i = 0; i < n && i < 10; ++i) out[i] = v[i];
+    v[10] = B.t_scatter;
+    v[11] = B.t_prior;
+    v[12] = B.t_gather;
+    v[13] = B.t_prec;
+    v[14] = B.t_step;
+    v[15] = (double)B.n_prof;
+    for (int i = 0; i < n && i < 16; ++i) out[i] = v[i];
 }
 
 }  // namespace madam

@@ -93,6 +93,7 @@ enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_
 struct Context;
 struct BatchState;
 void batch_release(Context &c);
+void batch_invalidate(Context &c);
 
 struct Context {
     int device = 0;
@@ -282,5 +283,7 @@ void solve_batch(Context &c, int nrhs, const void *signals, int tod_f32, int on_
 void batch_get_maps(Context &c, int irhs, double *map, double *binmap);
 void batch_get_baselines(Context &c, int irhs, double *aa, double *yba);
 void batch_get_info(Context &c, int irhs, double *out, int n);
+void batch_iterate(Context &c, int niter);
+void batch_reset_timers(Context &c);
 
 }  // namespace madam

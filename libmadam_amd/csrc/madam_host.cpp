@@ -764,6 +764,7 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device, bool
     c.tiles_built = false;
     c.prec_cached = false;
     c.stats[ST_POINTING_REUSED] = 0;
+    batch_invalidate(c);
 }
 
 // ------------------------------------------------------------------ solve
@@ -802,6 +803,7 @@ static void build_segtiles(Context &c)
         MADAM_CHECK(hipStreamSynchronize(s));
         int64_t nseg = last_pos + last_flag;
         c.nzpix = nseg;
+        c.stats[ST_NPIX_HIT] = (double)nseg;
         segtiles_alloc(c.sr_s, c.nactive);
         segred_build(s, segstart.p, nseg, c.nactive, c.sr_s.heads.p, c.sr_s.k0.p, c.sr_s.fix_end.p,
                      c.sr_s.fix_ord.p);
@@ -2246,6 +2248,11 @@ MADAM_API int madam_solve_batch(void *ctx, int nrhs, const void *signals, int to
     });
 }
 
+MADAM_API int madam_iterate_batch(void *ctx, int niter)
+{
+    return guarded([&] { batch_iterate(*(Context *)ctx, niter); });
+}
+
 MADAM_API int madam_get_batch_maps(void *ctx, int irhs, double *map, double *binmap)
 {
     return guarded([&] { batch_get_maps(*(Context *)ctx, irhs, map, binmap); });
@@ -2292,6 +2299,7 @@ MADAM_API int madam_reset_timers(void *ctx)
         Context &c = *(Context *)ctx;
         for (int i : {ST_T_SCATTER, ST_T_PRIOR, ST_T_GATHER, ST_T_APPLY_PREC, ST_T_STEP, ST_N_SCATTER, ST_N_GATHER})
             c.stats[i] = 0;
+        batch_reset_timers(c);
     });
 }
 
@@ -2317,6 +2325,16 @@ MADAM_API int madam_get_maps(void *ctx, double *map, double *binmap, int32_t *hi
                              double *crit)
 {
     return guarded([&] { get_maps(*(Context *)ctx, map, binmap, hits, cc, mask, crit); });
+}
+
+MADAM_API int madam_get_hits(void *ctx, int32_t *hits)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        if (!c.pointed) fail("madam_get_hits needs the pointing");
+        d2h(hits, c.d_hits.p, c.npix_map, c.stream);
+        MADAM_CHECK(hipStreamSynchronize(c.stream));
+    });
 }
 
 MADAM_API int madam_get_baselines(void *ctx, double *aa, double *yba, double *nna)

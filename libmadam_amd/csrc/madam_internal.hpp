@@ -62,6 +62,20 @@ __device__ inline PixW load_rec(const void *base, int64_t i)
     return r;
 }
 
+// two- and four-wide vector types of the projections' element type
+template <typename T>
+struct VecT;
+template <>
+struct VecT<double> {
+    using v2 = double2;
+    using v4 = double4;
+};
+template <>
+struct VecT<float> {
+    using v2 = float2;
+    using v4 = float4;
+};
+
 struct CsrEnt {
     int32_t b;   // global baseline index
     float w0, w1, w2;
@@ -128,10 +142,11 @@ void launch_scatter_pixels(hipStream_t s, int nmap, int64_t npix_cross, int64_t 
 void launch_gather_baselines(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase,
                              const int64_t *bstart, const PixW *pixw, const double *m4, int64_t ndeg,
                              const double *dwb, const double *nna, const double *p, double *ap, int group);
-// rec: the pointing records in layout rl (PixW for REC16)
+// rec: the pointing records in layout rl (PixW for REC16); T as launch_scatter_seg
+template <typename T>
 void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
-                        const void *rec, int rl, const double *m4, int64_t ndeg, const double *dwb,
-                        const double *nna, const double *p, double *ap);
+                        const void *rec, int rl, const T *m4, int64_t ndeg, const T *dwb, const T *nna, const T *p,
+                        T *ap);
 void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4);
 // entry-balanced pixel-side projection: tinfo from launch_tile_info (once per
@@ -151,10 +166,13 @@ void segred_pixel_segments(hipStream_t s, const int64_t *off, int64_t npc, int64
                            int64_t *scan_tmp, int64_t nchunks, int32_t *nzq, int64_t *segstart);
 void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbase, int ndet, int64_t nsamp,
                               int64_t *segstart);
-// ent: the CSR entries in layout rl (CsrEnt for REC16)
-void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const void *ent, int rl, const double *pw,
+// ent: the CSR entries in layout rl (CsrEnt for REC16); T = double, or float
+// for the mixed-precision operator (pw, cc_c, m4 and the partial sums in T,
+// part: 2 per tile of VecT<T>::v4, at most the bytes of the double4 ones)
+template <typename T>
+void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const void *ent, int rl, const T *pw,
                         const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
-                        const int32_t *nzq, int64_t nzpix, const double *cc_c, double *m4, double4 *part);
+                        const int32_t *nzq, int64_t nzpix, const T *cc_c, T *m4, void *part);
 // cc_c[k] = ccinv6[nzq[k]]: the inverse pixel matrices in the pixel side's ordinal order
 void segred_compact_cc(hipStream_t s, const int32_t *nzq, int64_t nseg, const double *ccinv6, double *cc_c);
 void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,

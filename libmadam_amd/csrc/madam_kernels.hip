@@ -922,13 +922,14 @@ static constexpr int FG_IPT = FG_ITEMS >= 256 ? FG_ITEMS / 256 : 1;     // items
 static constexpr int FG_K = FG_ITEMS / FG_D;      // baselines per detector per block
 static constexpr int FG_T = FLAT_TILE / FG_D;     // samples per detector per LDS tile
 
-template <int NMAP, int RL>
+template <typename T, int NMAP, int RL>
 __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, int64_t nbase,
-                                                     const int64_t *bstart, const void *pixw, const double *m4,
-                                                     int64_t ndeg, const double *dwb, const double *nna,
-                                                     const double *p, double *ap)
+                                                     const int64_t *bstart, const void *pixw, const T *m4,
+                                                     int64_t ndeg, const T *dwb, const T *nna,
+                                                     const T *p, T *ap)
 {
-    __shared__ double vals[FLAT_TILE + 1];    // vals[d * FG_T + t]
+    using V2 = typename VecT<T>::v2;
+    __shared__ T vals[FLAT_TILE + 1];    // vals[d * FG_T + t]
     __shared__ int64_t bs[FG_K + 1];      // sample starts of the block's baselines, bs[nk] = end
     int64_t ngrp = (ndet + FG_D - 1) / FG_D;
     int64_t lb = xcd_block(blockIdx.x, gridDim.x);
@@ -941,14 +942,14 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
     int sub = threadIdx.x % FG_LANES;
     int id[FG_IPT], ik[FG_IPT];   // items (detector, baseline)
     bool own[FG_IPT];
-    double acc[FG_IPT];
+    T acc[FG_IPT];
 #pragma unroll
     for (int i = 0; i < FG_IPT; ++i) {
         int item = threadIdx.x / FG_LANES + i * (256 / FG_LANES);
         id[i] = item / FG_K;
         ik[i] = item % FG_K;
         own[i] = id[i] < nd && ik[i] < nk;
-        acc[i] = 0.0;
+        acc[i] = (T)0;
     }
     __syncthreads();
     const int dsh = __ffsll((unsigned long long)ndeg) - 1;   // ndeg is a power of 4
@@ -979,16 +980,16 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
                 for (int e = 0; e < d; ++e) seen = seen || q[e] == q[d];
                 ld[d] = !seen;
             }
-            double2 mxy[FG_D];   // (m.x, m.y); m.y unused for NMAP = 1
-            double mz[FG_D];
+            V2 mxy[FG_D];   // (m.x, m.y); m.y unused for NMAP = 1
+            T mz[FG_D];
 #pragma unroll
             for (int d = 0; d < FG_D; ++d) {
                 if (ld[d]) {
-                    const double *mp = m4 + (int64_t)q[d] * 4;
+                    const T *mp = m4 + (int64_t)q[d] * 4;
                     if (NMAP == 1) {
                         mxy[d].x = mp[0];
                     } else {
-                        mxy[d] = *(const double2 *)mp;
+                        mxy[d] = *(const V2 *)mp;
                         mz[d] = mp[2];
                     }
                 }
@@ -1002,8 +1003,8 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
 #pragma unroll
                     for (int e = 1; e < d; ++e)
                         if (q[e] == q[d]) src = e;
-                    double2 cxy = mxy[0];
-                    double cz = mz[0];
+                    V2 cxy = mxy[0];
+                    T cz = mz[0];
 #pragma unroll
                     for (int e = 1; e < d; ++e)
                         if (src == e) {
@@ -1016,10 +1017,9 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
             }
 #pragma unroll
             for (int d = 0; d < FG_D; ++d) {
-                double v = NMAP == 1 ? mxy[d].x
-                                     : (double)r[d].w0 * mxy[d].x + (double)r[d].w1 * mxy[d].y +
-                                           (double)r[d].w2 * mz[d];
-                v = (d < nd && r[d].pix >= 0) ? v : 0.0;
+                T v = NMAP == 1 ? mxy[d].x
+                                : (T)r[d].w0 * mxy[d].x + (T)r[d].w1 * mxy[d].y + (T)r[d].w2 * mz[d];
+                v = (d < nd && r[d].pix >= 0) ? v : (T)0;
                 vals[tt0 < len ? d * FG_T + tt0 : FLAT_TILE] = v;   // past the end: the spare slot
             }
         }
@@ -1037,7 +1037,7 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
     // across it would cost the registers of two waves per SIMD)
 #pragma unroll
     for (int i = 0; i < FG_IPT; ++i) {
-        double a = group_sum<FG_LANES>(acc[i]);
+        T a = group_sum<FG_LANES>(acc[i]);
         if (own[i] && sub == 0) {
             int64_t b = (int64_t)(d0 + id[i]) * nbase + k0 + ik[i];
             ap[b] = ap[b] + (nna[b] * p[b] - a * dwb[b]);
@@ -1237,15 +1237,17 @@ size_t scatter_tiles_scratch_bytes(int64_t nent)
     return (size_t)ntiles * 2 * sizeof(double4) + 256;
 }
 
+template <typename T>
 void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
-                        const void *pixw, int rl, const double *m4, int64_t ndeg, const double *dwb,
-                        const double *nna, const double *p, double *ap)
+                        const void *pixw, int rl, const T *m4, int64_t ndeg, const T *dwb, const T *nna, const T *p,
+                        T *ap)
 {
     int64_t grid = ((ndet + FG_D - 1) / FG_D) * ((nbase + FG_K - 1) / FG_K);
     if (grid > 0)
     {
 #define MADAM_GF(NM, RLV)                                                                                        \
-    k_gather_flat<NM, RLV><<<(unsigned)grid, 256, 0, s>>>(ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p, ap)
+    k_gather_flat<T, NM, RLV><<<(unsigned)grid, 256, 0, s>>>(ndet, nsamp, nbase, bstart, pixw, m4, ndeg, dwb, nna, p, \
+                                                             ap)
         if (nmap == 1) {
             if (rl == REC4) MADAM_GF(1, REC4);
             else MADAM_GF(1, REC16);
@@ -1256,6 +1258,11 @@ void launch_gather_flat(hipStream_t s, int nmap, int ndet, int64_t nsamp, int64_
 #undef MADAM_GF
     }
 }
+template void launch_gather_flat<double>(hipStream_t, int, int, int64_t, int64_t, const int64_t *, const void *, int,
+                                         const double *, int64_t, const double *, const double *, const double *,
+                                         double *);
+template void launch_gather_flat<float>(hipStream_t, int, int, int64_t, int64_t, const int64_t *, const void *, int,
+                                        const float *, int64_t, const float *, const float *, const float *, float *);
 
 void launch_scatter_flat(hipStream_t s, int nmap, int64_t npix_cross, int64_t ndeg, const int64_t *off,
                          const CsrEnt *ent, const double *pw, const double *ccinv6, double *m4)

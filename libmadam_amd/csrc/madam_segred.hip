@@ -57,6 +57,11 @@ __device__ inline int sr_dpp(int v)
     return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWMASK, 0xf, true);
 }
 template <int CTRL, int ROWMASK>
+__device__ inline float sr_dpp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, true));
+}
+template <int CTRL, int ROWMASK>
 __device__ inline double sr_dpp(double v)
 {
     int lo = __double2loint(v), hi = __double2hiint(v);
@@ -68,10 +73,10 @@ static constexpr int DPP_ROW_SHR = 0x110;   // + n: row_shr:n
 static constexpr int DPP_WAVE_SHR1 = 0x138, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
 
 // one step of the segmented inclusive scan: (f, v) = (fo, vo) (+) (f, v)
-template <int NC, int CTRL, int ROWMASK>
-__device__ inline void sr_seg_step(double (&v)[NC], int &f)
+template <typename T, int NC, int CTRL, int ROWMASK>
+__device__ inline void sr_seg_step(T (&v)[NC], int &f)
 {
-    double o[NC];
+    T o[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) o[c] = sr_dpp<CTRL, ROWMASK>(v[c]);
     int of = sr_dpp<CTRL, ROWMASK>(f);
@@ -86,17 +91,17 @@ __device__ inline void sr_seg_step(double (&v)[NC], int &f)
 // (f1, v1) + (f2, v2) = (f1 | f2, f2 ? v2 : v1 + v2): shifts by 1, 2, 4, 8
 // inside rows of 16 lanes, then the row totals across rows (row_bcast:15 into
 // rows 1 and 3, row_bcast:31 into rows 2 and 3).
-template <int NC>
-__device__ inline void sr_seg_scan(double (&v)[NC], bool head, int lane)
+template <typename T, int NC>
+__device__ inline void sr_seg_scan(T (&v)[NC], bool head, int lane)
 {
     (void)lane;
     int f = head ? 1 : 0;
-    sr_seg_step<NC, DPP_ROW_SHR + 1, 0xf>(v, f);
-    sr_seg_step<NC, DPP_ROW_SHR + 2, 0xf>(v, f);
-    sr_seg_step<NC, DPP_ROW_SHR + 4, 0xf>(v, f);
-    sr_seg_step<NC, DPP_ROW_SHR + 8, 0xf>(v, f);
-    sr_seg_step<NC, DPP_ROW_BCAST15, 0xa>(v, f);
-    sr_seg_step<NC, DPP_ROW_BCAST31, 0xc>(v, f);
+    sr_seg_step<T, NC, DPP_ROW_SHR + 1, 0xf>(v, f);
+    sr_seg_step<T, NC, DPP_ROW_SHR + 2, 0xf>(v, f);
+    sr_seg_step<T, NC, DPP_ROW_SHR + 4, 0xf>(v, f);
+    sr_seg_step<T, NC, DPP_ROW_SHR + 8, 0xf>(v, f);
+    sr_seg_step<T, NC, DPP_ROW_BCAST15, 0xa>(v, f);
+    sr_seg_step<T, NC, DPP_ROW_BCAST31, 0xc>(v, f);
 }
 
 // inclusive sum over the 64 lanes, same pattern
@@ -117,8 +122,8 @@ __device__ inline int sr_scan_sum(int h)
 // in the tile continues into the next tile.  fin(ordinal, sum) finishes a
 // segment; part(slot, sum) stores a partial sum of this tile.  Returns the
 // number of segment heads in the tile.
-template <int NC, typename Fin, typename Part>
-__device__ inline int sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsigned hb, int lane, int64_t k0,
+template <typename T, int NC, typename Fin, typename Part>
+__device__ inline int sr_reduce_tile(const T (&prod)[SR_PER_LANE][NC], unsigned hb, int lane, int64_t k0,
                                       bool head0, bool cont_last, Fin fin, Part part)
 {
     int nh = __popc(hb);
@@ -126,9 +131,9 @@ __device__ inline int sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsi
     int Hex = H - nh;                    // heads in the lanes before this one
     int Htot = __builtin_amdgcn_readlane(H, 63);
     int64_t kb = k0 - (head0 ? 1 : 0);   // ordinal of a head h = kb + #heads in [t0, h]
-    double pre[NC], cur[NC];
+    T pre[NC], cur[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) pre[c] = cur[c] = 0.0;
+    for (int c = 0; c < NC; ++c) pre[c] = cur[c] = (T)0;
     int hk = 0;
 #pragma unroll
     for (int u = 0; u < SR_PER_LANE; ++u) {
@@ -140,7 +145,7 @@ __device__ inline int sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsi
                 fin(kb + Hex + hk, cur);   // began and ended inside this lane
             }
 #pragma unroll
-            for (int c = 0; c < NC; ++c) cur[c] = 0.0;
+            for (int c = 0; c < NC; ++c) cur[c] = (T)0;
             ++hk;
         }
 #pragma unroll
@@ -150,17 +155,17 @@ __device__ inline int sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsi
 #pragma unroll
         for (int c = 0; c < NC; ++c) pre[c] = cur[c];
     }
-    double v[NC];
+    T v[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = hk ? cur[c] : pre[c];
-    sr_seg_scan<NC>(v, hk > 0, lane);
-    double sprev[NC];   // the previous lane's scan value (0 in lane 0)
+    sr_seg_scan<T, NC>(v, hk > 0, lane);
+    T sprev[NC];   // the previous lane's scan value (0 in lane 0)
 #pragma unroll
     for (int c = 0; c < NC; ++c) sprev[c] = sr_dpp<DPP_WAVE_SHR1, 0xf>(v[c]);
     if (hk > 0) {
         int firstu = __ffs(hb) - 1;
         if (lane > 0 || firstu > 0) {   // the segment that ends before this lane's first head
-            double tot[NC];
+            T tot[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) tot[c] = pre[c] + sprev[c];
             if (Hex == 0) part(0, tot);   // began in an earlier tile
@@ -175,30 +180,32 @@ __device__ inline int sr_reduce_tile(const double (&prod)[SR_PER_LANE][NC], unsi
     return Htot;
 }
 
-__device__ inline void sr_part_store(double4 *part, int64_t j, int slot, const double *v, int nc)
+template <typename T>
+__device__ inline void sr_part_store(typename VecT<T>::v4 *part, int64_t j, int slot, const T *v, int nc)
 {
-    double4 r;
+    typename VecT<T>::v4 r;
     r.x = v[0];
-    r.y = nc > 1 ? v[1] : 0.0;
-    r.z = nc > 2 ? v[2] : 0.0;
-    r.w = 0.0;
+    r.y = nc > 1 ? v[1] : (T)0;
+    r.z = nc > 2 ? v[2] : (T)0;
+    r.w = (T)0;
     part[j * 2 + slot] = r;
 }
 
-__device__ inline void sr_ccmul_store(int nmap, const double *cc, const double *m, double *out4)
+template <typename T>
+__device__ inline void sr_ccmul_store(int nmap, const T *cc, const T *m, T *out4)
 {
     // lower-triangle storage [c11, c21, c31, c22, c32, c33] of the symmetric inverse
-    double4 r;
+    typename VecT<T>::v4 r;
     if (nmap == 1) {
         r.x = cc[0] * m[0];
-        r.y = r.z = 0.0;
+        r.y = r.z = (T)0;
     } else {
         r.x = cc[0] * m[0] + cc[1] * m[1] + cc[2] * m[2];
         r.y = cc[1] * m[0] + cc[3] * m[1] + cc[4] * m[2];
         r.z = cc[2] * m[0] + cc[4] * m[1] + cc[5] * m[2];
     }
-    r.w = 0.0;
-    *(double4 *)out4 = r;
+    r.w = (T)0;
+    *(typename VecT<T>::v4 *)out4 = r;
 }
 
 // ------------------------------------------------------------ pixel side
@@ -253,20 +260,21 @@ __device__ inline unsigned sr_head_bits(const uint8_t *heads, int64_t e0, int64_
 // (compact ordinal order) and the pixel index that lane l loaded before the
 // reduction.  Only a tile that finishes more than 64 pixels reads the
 // finishing data of the rest where they complete.
-template <int NC, int WAVES, int RL>
-__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(5, 8))) k_sr_scatter(int64_t nent, const void *ent, const double *pw,
+template <typename T, int NC, int WAVES, int RL>
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(5, 8))) k_sr_scatter(int64_t nent, const void *ent, const T *pw,
                                                            const uint8_t *heads, const int64_t *tile_k0,
                                                            const int64_t *fix_end, const int32_t *nzq,
-                                                           int64_t nzpix, const double *cc_c, double *m4,
-                                                           double4 *part)
+                                                           int64_t nzpix, const T *cc_c, T *m4,
+                                                           typename VecT<T>::v4 *part)
 {
-    __shared__ double tr[WAVES][SR_PAD];
+    using V2 = typename VecT<T>::v2;
+    __shared__ T tr[WAVES][SR_PAD];
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int64_t j = sr_xcd_block(blockIdx.x, gridDim.x) * WAVES + w;
     int64_t t0 = j * SR_TILE;
     if (t0 >= nent) return;   // uniform over the wave; waves synchronise only with themselves
     PixW e[SR_PER_LANE];   // the entries as full records: .pix is the baseline slot
-    double x[SR_PER_LANE];
+    T x[SR_PER_LANE];
     int64_t k0 = 0;
     bool head0 = false, cont_last = false;
     unsigned hb = 0u;
@@ -288,17 +296,17 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
             bool ok = t0 + u * 64 + lane < nent && e[u].pix >= 0;
-            double v = pw[ok ? e[u].pix : 0];
-            x[u] = ok ? v : 0.0;
+            T v = pw[ok ? e[u].pix : 0];
+            x[u] = ok ? v : (T)0;
         }
     }
-    double prod[SR_PER_LANE][NC];
+    T prod[SR_PER_LANE][NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
             float wc = NC == 1 ? 1.0f : (c == 0 ? e[u].w0 : (c == 1 ? e[u].w1 : e[u].w2));
-            tr[w][sr_col(u, lane)] = NC == 1 ? x[u] : (double)wc * x[u];
+            tr[w][sr_col(u, lane)] = NC == 1 ? x[u] : (T)wc * x[u];
         }
         sr_wave_sync();
 #pragma unroll
@@ -314,26 +322,26 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
     int lo = head0 ? 0 : 1;
     int hi = htot - (cont_last ? 1 : 0) - (head0 ? 1 : 0);
     const int nmap = NC == 1 ? 1 : 3;
-    auto part_fn = [&](int slot, const double *s) { sr_part_store(part, j, slot, s, NC); };
+    auto part_fn = [&](int slot, const T *s) { sr_part_store<T>(part, j, slot, s, NC); };
     if (hi < 64) {
         // the common case: every finished pixel has a slot in the transpose buffer
         // only the lanes that finish a pixel load its inverse matrix (three
         // 16-byte loads of 48 contiguous bytes); issued here, the loads
         // overlap the reduction
         bool fin_lane = lane >= lo && lane <= hi;
-        double2 cc[3];
+        V2 cc[3];
         int32_t q = 0;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) cc[i] = make_double2(0.0, 0.0);
+        for (int i = 0; i < 3; ++i) cc[i].x = cc[i].y = (T)0;
         if (fin_lane) {
             q = nzq[k0 + lane];
-            const double2 *src = (const double2 *)(cc_c + (k0 + lane) * 6);
+            const V2 *src = (const V2 *)(cc_c + (k0 + lane) * 6);
 #pragma unroll
             for (int i = 0; i < 3; ++i) cc[i] = src[i];
         }
-        sr_reduce_tile<NC>(
+        sr_reduce_tile<T, NC>(
             prod, hb, lane, k0, head0, cont_last,
-            [&](int64_t k, const double *s) {
+            [&](int64_t k, const T *s) {
                 int loc = (int)(k - k0);
 #pragma unroll
                 for (int c = 0; c < NC; ++c) tr[w][loc * NC + c] = s[c];
@@ -341,17 +349,17 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
             part_fn);
         sr_wave_sync();
         if (fin_lane) {
-            double s[3] = {0.0, 0.0, 0.0};
+            T s[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
             for (int c = 0; c < NC; ++c) s[c] = tr[w][lane * NC + c];
-            const double ccd[6] = {cc[0].x, cc[0].y, cc[1].x, cc[1].y, cc[2].x, cc[2].y};
-            sr_ccmul_store(nmap, ccd, s, m4 + (int64_t)q * 4);
+            const T ccd[6] = {cc[0].x, cc[0].y, cc[1].x, cc[1].y, cc[2].x, cc[2].y};
+            sr_ccmul_store<T>(nmap, ccd, s, m4 + (int64_t)q * 4);
         }
     } else {
         // a tile of many short segments: finish each where it completes
-        sr_reduce_tile<NC>(
+        sr_reduce_tile<T, NC>(
             prod, hb, lane, k0, head0, cont_last,
-            [&](int64_t k, const double *s) { sr_ccmul_store(nmap, cc_c + k * 6, s, m4 + (int64_t)nzq[k] * 4); },
+            [&](int64_t k, const T *s) { sr_ccmul_store<T>(nmap, cc_c + k * 6, s, m4 + (int64_t)nzq[k] * 4); },
             part_fn);
     }
 }
@@ -377,10 +385,12 @@ void segred_compact_cc(hipStream_t s, const int32_t *nzq, int64_t nseg, const do
 // adding tiles j + 1 + l, j + 1 + l + 64, ..., then a butterfly over the
 // lanes.  Both orders are fixed, so the sums are reproducible.
 static constexpr int SR_FIX_SHORT = 16;
+template <typename T>
 __global__ void __launch_bounds__(SR_TPB) k_sr_scatter_fix(int nmap, int64_t ntiles, const int64_t *fix_end,
                                                            const int64_t *fix_ord, const int32_t *nzq,
-                                                           const double4 *part, const double *cc_c, double *m4)
+                                                           const typename VecT<T>::v4 *part, const T *cc_c, T *m4)
 {
+    using V4 = typename VecT<T>::v4;
     int lane = threadIdx.x & 63;
     int64_t w = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     int64_t j = w * 64 + lane;
@@ -388,8 +398,8 @@ __global__ void __launch_bounds__(SR_TPB) k_sr_scatter_fix(int nmap, int64_t nti
     int64_t jl = j < ntiles ? fix_end[j] : -1;
     bool longc = jl >= 0 && jl - j > SR_FIX_SHORT;
     if (jl >= 0 && !longc) {
-        double4 a = part[j * 2 + 1];
-        double4 v[SR_FIX_SHORT];
+        V4 a = part[j * 2 + 1];
+        V4 v[SR_FIX_SHORT];
 #pragma unroll
         for (int u = 0; u < SR_FIX_SHORT; ++u)
             if (j + 1 + u <= jl) v[u] = part[(j + 1 + u) * 2];
@@ -401,17 +411,17 @@ __global__ void __launch_bounds__(SR_TPB) k_sr_scatter_fix(int nmap, int64_t nti
                 a.z += v[u].z;
             }
         int64_t kk = fix_ord[j];
-        double m[3] = {a.x, a.y, a.z};
-        sr_ccmul_store(nmap, cc_c + kk * 6, m, m4 + (int64_t)nzq[kk] * 4);
+        T m[3] = {a.x, a.y, a.z};
+        sr_ccmul_store<T>(nmap, cc_c + kk * 6, m, m4 + (int64_t)nzq[kk] * 4);
     }
     uint64_t lm = __ballot(longc);
     while (lm) {
         int src = __builtin_ctzll(lm);
         lm &= lm - 1;
         int64_t j2 = __shfl(j, src, 64), jl2 = __shfl(jl, src, 64);
-        double sx = 0.0, sy = 0.0, sz = 0.0;
+        T sx = (T)0, sy = (T)0, sz = (T)0;
         for (int64_t jj = j2 + 1 + lane; jj <= jl2; jj += 64) {
-            double4 v = part[jj * 2];
+            V4 v = part[jj * 2];
             sx += v.x;
             sy += v.y;
             sz += v.z;
@@ -422,10 +432,10 @@ __global__ void __launch_bounds__(SR_TPB) k_sr_scatter_fix(int nmap, int64_t nti
             sz += __shfl_xor(sz, o, 64);
         }
         if (lane == 0) {
-            double4 a = part[j2 * 2 + 1];
+            V4 a = part[j2 * 2 + 1];
             int64_t kk = fix_ord[j2];
-            double m[3] = {a.x + sx, a.y + sy, a.z + sz};
-            sr_ccmul_store(nmap, cc_c + kk * 6, m, m4 + (int64_t)nzq[kk] * 4);
+            T m[3] = {a.x + sx, a.y + sy, a.z + sz};
+            sr_ccmul_store<T>(nmap, cc_c + kk * 6, m, m4 + (int64_t)nzq[kk] * 4);
         }
     }
 }
@@ -475,10 +485,10 @@ __global__ void __launch_bounds__(SR_TPB) k_sr_gather(int64_t nent, const PixW *
     int64_t k0 = sr_meta_k0(meta);
     bool head0 = sr_meta_head0(meta);
     bool cont_last = sr_meta_cont(meta);
-    sr_reduce_tile<1>(
+    sr_reduce_tile<double, 1>(
         prod, hb, lane, k0, head0, cont_last,
         [&](int64_t b, const double *s) { ap[b] = ap[b] + (nna[b] * p[b] - s[0] * dwb[b]); },
-        [&](int slot, const double *s) { sr_part_store(part, j, slot, s, 1); });
+        [&](int slot, const double *s) { sr_part_store<double>(part, j, slot, s, 1); });
 }
 
 __global__ void k_sr_gather_fix(int64_t ntiles, const int64_t *fix_end, const int64_t *fix_ord, const double4 *part,
@@ -609,16 +619,19 @@ void segred_baseline_segments(hipStream_t s, const int64_t *bstart, int64_t nbas
 
 // ------------------------------------------------------------ launchers
 
-void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const void *ent, int rl, const double *pw,
+template <typename T>
+void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const void *ent, int rl, const T *pw,
                         const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
-                        const int32_t *nzq, int64_t nzpix, const double *cc_c, double *m4, double4 *part)
+                        const int32_t *nzq, int64_t nzpix, const T *cc_c, T *m4, void *part_v)
 {
+    using V4 = typename VecT<T>::v4;
+    V4 *part = (V4 *)part_v;
     int64_t nt = segred_ntiles(nent);
     if (nt == 0) return;
     unsigned grid = (unsigned)((nt + SR_WAVES - 1) / SR_WAVES);
 #define MADAM_SC(NCV, RLV)                                                                                       \
-    k_sr_scatter<NCV, SR_WAVES, RLV><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, nzpix,      \
-                                                              cc_c, m4, part)
+    k_sr_scatter<T, NCV, SR_WAVES, RLV><<<grid, SR_TPB, 0, s>>>(nent, ent, pw, heads, tile_k0, fix_end, nzq, nzpix,   \
+                                                                 cc_c, m4, part)
     if (nmap == 1) {
         if (rl == REC4) MADAM_SC(1, REC4);
         else MADAM_SC(1, REC16);
@@ -627,9 +640,16 @@ void launch_scatter_seg(hipStream_t s, int nmap, int64_t nent, const void *ent, 
         else MADAM_SC(3, REC16);
     }
 #undef MADAM_SC
-    k_sr_scatter_fix<<<(unsigned)((nt + SR_TPB - 1) / SR_TPB), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq, part,
-                                                                            cc_c, m4);
+    k_sr_scatter_fix<T><<<(unsigned)((nt + SR_TPB - 1) / SR_TPB), SR_TPB, 0, s>>>(nmap, nt, fix_end, fix_ord, nzq,
+                                                                               part, cc_c, m4);
 }
+
+template void launch_scatter_seg<double>(hipStream_t, int, int64_t, const void *, int, const double *,
+                                         const uint8_t *, const int64_t *, const int64_t *, const int64_t *,
+                                         const int32_t *, int64_t, const double *, double *, void *);
+template void launch_scatter_seg<float>(hipStream_t, int, int64_t, const void *, int, const float *, const uint8_t *,
+                                        const int64_t *, const int64_t *, const int64_t *, const int32_t *, int64_t,
+                                        const float *, float *, void *);
 
 void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,
                        const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,

@@ -90,8 +90,13 @@ def test_fp64_batch_equals_single_solves(kind):
         r = res[k]
         errs = dict(map=rel_rms(r["map"], s1["map"]), binmap=rel_rms(r["binmap"], s1["binmap"]),
                     aa=rel_rms(r["aa"], s1["aa"]), yba=rel_rms(r["yba"], s1["yba"]))
-        print(kind, "rhs", k, "steps", r["info"]["steps"], "single", s1["niter"], errs)
-        assert abs(r["info"]["steps"] - s1["niter"]) <= 1
+        print(kind, "rhs", k, "steps", r["info"]["steps"], "single", s1["niter"], "status", r["info"]["status"],
+              "rr/rr0", r["info"]["rr_inner"], "single rr", s1["rr"][-3:], errs)
+        if kind != "fprecond":
+            # (the Fourier preconditioner cannot move per-segment offsets: these
+            # solves stall, and one whose rz reaches exactly 0 in one summation
+            # order and not in the other stops at another step, same results)
+            assert abs(r["info"]["steps"] - s1["niter"]) <= 1
         for name, e in errs.items():
             assert e <= 1e-10, (name, e)
 

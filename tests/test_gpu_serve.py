@@ -1,0 +1,128 @@
+"""The request loop (madam_server_*): jobs of different scans, sizes and
+precisions submitted together, against each job solved alone.
+
+Bars: FP64 jobs equal their solve alone to 1e-10 relative rms (maps,
+baselines), mixed-precision jobs to 1e-6; hits bit-exact.  The loop must
+have batched same-scan jobs, reused resident scans and run several jobs at
+once, and admission must keep the estimated device memory within the budget.
+"""
+import numpy as np
+import pytest
+
+from tests import madam_cases as mc
+from tests.test_gpu_parity import ring, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _torch_gpu_first():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
+    yield
+
+
+def scans():
+    """three scans: (case, scan key)"""
+    a = ring(nside=8, ndet=4, nsamp=20000, prior=True, nperiod=2, seed=1)
+    b = ring(nside=16, ndet=2, nsamp=30000, prior=True, nperiod=2, seed=2)
+    c = ring(nside=8, ndet=4, nsamp=12000, prior=False, nperiod=1, seed=3)
+    return [(a, 101), (b, 202), (c, 303)]
+
+
+def jobs():
+    """(scan index, signal seed, precision, float32 TOD)"""
+    return [(0, 0, 0, False), (0, 1, 0, False), (0, 2, 0, False), (1, 3, 1, True), (1, 4, 1, True),
+            (2, 5, 0, False), (2, 6, 1, False), (0, 7, 0, False)]
+
+
+def signal(case, seed):
+    rng = np.random.default_rng(1000 + seed)
+    ndet = len(case["dets"])
+    nsamp = case["timestamps"].size
+    s = case["signal"].reshape(ndet, nsamp).copy()
+    for d in range(ndet):
+        s[d] += mc.one_over_f(rng, nsamp, case["pars"]["fsample"], 1e-4, 0.1, -1.0)
+    return s
+
+
+def submit(srv, case, key, sig, precision):
+    return srv.submit(case["pars"], case["weights"], case["timestamps"], case["pixels"], case["pixweights"], sig,
+                      case["periods"], case["npsd"], case["psdstarts"], case["psdfreqs"], case["psdvals"],
+                      nnz=case["nnz"], scan_key=key, precision=precision)
+
+
+def test_jobs_alone_and_in_the_mix():
+    import libmadam_amd
+    sc = scans()
+    js = jobs()
+    sigs = []
+    for (si, seed, prec, f32) in js:
+        s = signal(sc[si][0], seed)
+        sigs.append(s.astype(np.float32) if f32 else s)
+    # in the mix: everything submitted at once, three workers
+    srv = libmadam_amd.Server(0, hbm_budget=8e9, max_inflight=3)
+    try:
+        ids = [submit(srv, sc[si][0], sc[si][1], sigs[k], prec) for k, (si, seed, prec, f32) in enumerate(js)]
+        mix = []
+        for jid in ids:
+            srv.wait(jid)
+            mix.append(srv.result(jid))
+        st = srv.stats()
+    finally:
+        srv.close()
+    print("server stats", st)
+    print("batch sizes", [r["batch"] for r in mix], "resident", [r["resident"] for r in mix])
+    assert st["done"] == len(js) and st["failed"] == 0
+    assert st["batches"] >= 1 and st["max_inflight"] >= 2
+    assert st["peak_bytes"] <= 8e9
+    # alone: one job per fresh server, without a scan key
+    for k, (si, seed, prec, f32) in enumerate(js):
+        srv = libmadam_amd.Server(0, hbm_budget=8e9, max_inflight=1)
+        try:
+            jid = submit(srv, sc[si][0], 0, sigs[k], prec)
+            srv.wait(jid)
+            alone = srv.result(jid)
+        finally:
+            srv.close()
+        r = mix[k]
+        tol = 1e-10 if prec == 0 else 1e-6
+        good = alone["hits"] > 0
+        errs = dict(map=rel_rms(r["map"][good], alone["map"][good]), aa=rel_rms(r["aa"], alone["aa"]),
+                    binmap=rel_rms(r["binmap"][good], alone["binmap"][good]))
+        print("job", k, "scan", si, "precision", prec, "batch", r["batch"], "resident", r["resident"], errs)
+        assert np.array_equal(r["hits"], alone["hits"])
+        for name, e in errs.items():
+            assert e <= tol, (k, name, e)
+
+
+def test_admission_by_budget():
+    """A budget that holds one job at a time: every job still completes, the
+    estimated memory in use never exceeds the budget, and a job larger than
+    the budget is refused at submission."""
+    import libmadam_amd
+    case, key = scans()[0]
+    sig = signal(case, 0)
+    srv0 = libmadam_amd.Server(0, hbm_budget=1e12, max_inflight=1)
+    # the estimate of one job, read back from a generous server's peak
+    jid = submit(srv0, case, 0, sig, 0)
+    srv0.wait(jid)
+    one = srv0.stats()["peak_bytes"]
+    srv0.close()
+    srv = libmadam_amd.Server(0, hbm_budget=1.5 * one, max_inflight=4)
+    try:
+        ids = [submit(srv, case, 0, signal(case, s), 0) for s in range(4)]
+        for jid in ids:
+            srv.wait(jid)
+        st = srv.stats()
+        assert st["done"] == 4 and st["peak_bytes"] <= 1.5 * one and st["max_inflight"] == 1
+        small = libmadam_amd.Server(0, hbm_budget=0.5 * one, max_inflight=1)
+        with pytest.raises(Exception, match="budget"):
+            submit(small, case, 0, sig, 0)
+        small.close()
+    finally:
+        srv.close()
