@@ -129,6 +129,17 @@ int madam_set_kernel_variant(void *ctx, int variant);
  * open_offset_turns, spin_offset_turns}; outputs pix[ndet][n], weights[ndet][n][3] */
 int madam_expand_pointing(const double *scan5, const double *dets3, int ndet, long nside, long i0, long n,
                           int32_t *pix, float *weights, int on_device);
+/* No counterpart in the reference.  madam_expand_pointing for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): pix / weights are device buffers
+ * of the caller on the current device, the detector table is staged in a
+ * fixed per-GPU array of the library (ndet <= 4096, 96 KiB per GPU, held
+ * until the process ends), the work is queued on `stream` (a hipStream_t of
+ * the current device; NULL = the null stream) and nothing is allocated or
+ * freed.  Calls that overlap on one GPU share that array and race; under
+ * graph capture dets3 must stay valid while the graph is replayed. */
+int madam_expand_pointing_noalloc(const double *scan5, const double *dets3, int ndet, long nside, long i0, long n,
+                                  int32_t *pix, float *weights, void *stream);
 /* ------------------------------------------------------------------ request loop
  * Destripe jobs served on one GPU (no counterpart in the reference, whose
  * destripe() serves one job per call and keeps one cached configuration in

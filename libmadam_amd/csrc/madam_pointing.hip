@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
+#include <mutex>
 
 #include "madam_internal.hpp"
 
@@ -214,4 +216,79 @@ void launch_expand_pointing(hipStream_t s, ScanParams sp, const ScanDet *dets, i
     k_expand_pointing<<<(unsigned)g, 256, 0, s>>>(sp, dets, ndet, nside, i0, n, ld, pix, w);
 }
 
+// ---------------------------------------------------------------- the entry point that allocates nothing
+//
+// madam_expand_pointing stages the detector table in device memory it
+// allocates per call.  The variant below stages it in `scratch`, a fixed
+// array in the library's device image instead: each GPU the library runs on
+// holds its own copy (NOALLOC_MAX_DETS * 24 bytes = 96 KiB) until the process
+// ends.  Host code never passes `scratch` itself to a kernel (its host-side
+// name is not a device address): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+
+static constexpr int NOALLOC_MAX_DETS = 4096;
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) ScanDet scratch[NOALLOC_MAX_DETS];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[NOALLOC_MAX_DEVICES];   // scratch's device address, per device
+
+// scratch's address on the current device (looked up on the device's first call)
+static hipError_t scratch_on_current_device(void **out, int *dev_out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    *dev_out = dev;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return hipSuccess;
+}
+
 }  // namespace madam
+
+using namespace madam;
+
+extern "C" __attribute__((visibility("default"))) int madam_expand_pointing_noalloc(
+    const double *scan5, const double *dets3, int ndet, long nside, long i0, long n, int32_t *pix, float *weights,
+    void *stream)
+{
+    char msg[256];
+    if (ndet < 1 || ndet > NOALLOC_MAX_DETS) {
+        snprintf(msg, sizeof msg, "madam_expand_pointing_noalloc: ndet = %d outside 1..%d (the staging array's size)",
+                 ndet, NOALLOC_MAX_DETS);
+        set_last_error(msg);
+        return 1;
+    }
+    if (!scan5 || !dets3 || !pix || !weights || n < 0 || nside < 1 || (nside & (nside - 1)) || nside > (1L << 29)) {
+        set_last_error("madam_expand_pointing_noalloc: null pointer, n < 0 or nside not a power of two");
+        return 1;
+    }
+    void *sp = nullptr;
+    int dev = -1;
+    hipError_t e = scratch_on_current_device(&sp, &dev);
+    if (e == hipSuccess) {
+        hipStream_t s = (hipStream_t)stream;
+        e = hipMemcpyAsync(sp, dets3, sizeof(ScanDet) * ndet, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && n > 0) {
+            ScanParams p{scan5[0], scan5[1], scan5[2], scan5[3], scan5[4]};
+            launch_expand_pointing(s, p, (const ScanDet *)sp, ndet, nside, i0, n, n, pix, weights);
+            e = hipGetLastError();
+        }
+    }
+    if (e != hipSuccess) {
+        snprintf(msg, sizeof msg, "madam_expand_pointing_noalloc: HIP error %d (%s) on device %d", (int)e,
+                 hipGetErrorString(e), dev);
+        set_last_error(msg);
+        return 1;
+    }
+    return 0;
+}
