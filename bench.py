@@ -353,8 +353,15 @@ def run_cfg4(a):
         m.iterate_batch(a.steps)
         torch.cuda.synchronize()
         t_step = (time.perf_counter() - t0) / a.steps
-        log("group %d: %d det x %d samples x %d RHS: step %.3f ms, solve %.2f s" % (gi, nd, ns, R, t_step * 1e3,
-                                                                                    t_solve))
+        m.set_profiling(True)
+        m.reset_timers()
+        m.iterate_batch(a.prof_steps)
+        prof = m.batch_result(0)["info"]
+        m.set_profiling(False)
+        nprof = max(prof["n_prof"], 1)
+        brk = {k: prof[k] / nprof * 1e3 for k in ("t_scatter", "t_prior", "t_gather", "t_prec", "t_step")}
+        log("group %d: %d det x %d samples x %d RHS: step %.3f ms, solve %.2f s, breakdown %s"
+            % (gi, nd, ns, R, t_step * 1e3, t_solve, brk))
         if gi == 0:
             # the FP64 single-RHS path on the first realisation, same pointing
             tod64 = sig[0].double().contiguous()
@@ -371,9 +378,12 @@ def run_cfg4(a):
         del sig, pix, w
         torch.cuda.empty_cache()
         out.append(dict(ndet=nd, nsamp=ns, nrhs=R, ms_per_step=t_step * 1e3, solve_s=t_solve,
-                        sample_rhs_per_s=nd * ns * R / t_step, pcg_steps=info["pcg_steps"], passes=info["passes"]))
+                        sample_rhs_per_s=nd * ns * R / t_step, pcg_steps=info["pcg_steps"], passes=info["passes"],
+                        step_breakdown_ms=brk, piece=info["piece"]))
         tot_work += nd * ns * R
         tot_time += t_step
+        if a.groups and gi + 1 >= a.groups:
+            break
     value = tot_work / tot_time
     return {
         "metric": METRIC, "value": value, "unit": "sample-RHS/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -532,8 +542,9 @@ def main():
     ap.add_argument("--jobs", type=int, default=2000)
     ap.add_argument("--log2min", type=float, default=12.0)
     ap.add_argument("--log2max", type=float, default=16.0)
-    ap.add_argument("--rate", type=float, default=100.0)
+    ap.add_argument("--rate", type=float, default=12.0)
     ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--groups", type=int, default=0, help="cfg4: run only the first N detector subsets")
     ap.add_argument("--distinct", action="store_true", help="every detector on its own boresight")
     ap.add_argument("--small", action="store_true", help="a reduced shape for quick checks (not a bench number)")
     ap.add_argument("--no-cpu", action="store_true")

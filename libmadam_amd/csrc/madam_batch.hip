@@ -18,14 +18,18 @@
 // pieces whose sums a fix-up pass adds in piece order.
 //
 // Baseline side (map_to_baseline, src/smadam_routines.f90:1235-1376): one wave
-// per run of BG_RUN baselines of one detector.  Lane (k, g, r) takes baseline
-// k of a step of KB, samples g, g + G, ... of it, right-hand side r; the
-// samples' pointing records are read once for all R right-hand sides.
+// per run of consecutive baselines of one detector.  Lane (k, g, r) takes
+// baseline k of a step of KB, samples g, g + G, ... of it, right-hand side r;
+// the samples' pointing records are read once for all R right-hand sides.
+// For R >= 8 (k_bgather_w) the waves of the co-pointed time windows of a few
+// detectors share one XCD, so that its L2 holds the map rows they read.
 //
 // Every sum is taken in a fixed order, so results are bitwise reproducible.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
+#include <cstdlib>
 
 #include "madam_batch.hpp"
 
@@ -71,17 +75,38 @@ int batch_width(int nrhs)
 
 // ---------------------------------------------------------------- pixel side
 
-// pw[slot * R + r] = p[r * nbtot + bord[slot]] * dwb_s[slot] (0 for finished right-hand sides)
-template <typename T, int R>
-__global__ void k_bpw(const T *p, int64_t nbtot, const int32_t *bord, const T *dwb_s, const int *active, T *pw)
+// pw[bslot[b] * R + r] = p[r * nbtot + b] * dwb[b] (0 for finished right-hand
+// sides): a tile of BPW_TB baselines x R is read row by row (RHS-major,
+// coalesced), turned over in LDS and written one slot row of R values at a time
+static constexpr int BPW_TB = 64;
+
+// one right-hand side: pw[slot] = p[bord[slot]] * dwb_s[slot], written in order
+template <typename T>
+__global__ void k_bpw1(const T *p, int64_t nbtot, const int32_t *bord, const T *dwb_s, const int *active, T *pw)
 {
-    int64_t n = nbtot * R;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t slot = i / R;
-        int r = (int)(i - slot * R);
-        T v = 0;
-        if (active[r]) v = p[r * nbtot + bord[slot]] * dwb_s[slot];
-        pw[i] = v;
+    bool act = active[0] != 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nbtot; i += (int64_t)gridDim.x * blockDim.x)
+        pw[i] = act ? p[bord[i]] * dwb_s[i] : (T)0;
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(BTPB) k_bpw(const T *p, int64_t nbtot, const int32_t *bslot, const T *dwb,
+                                              const int *active, T *pw)
+{
+    __shared__ T tile[R][BPW_TB + 1];
+    int64_t b0 = (int64_t)blockIdx.x * BPW_TB;
+    int nbt = (int)(b0 + BPW_TB <= nbtot ? BPW_TB : nbtot - b0);
+    for (int t = threadIdx.x; t < R * BPW_TB; t += BTPB) {
+        int r = t / BPW_TB, j = t - r * BPW_TB;
+        if (j < nbt) tile[r][j] = p[(int64_t)r * nbtot + b0 + j];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * BPW_TB; t += BTPB) {
+        int j = t / R, r = t - j * R;
+        if (j < nbt) {
+            int64_t b = b0 + j;
+            pw[(int64_t)bslot[b] * R + r] = active[r] ? tile[r][j] * dwb[b] : (T)0;
+        }
     }
 }
 
@@ -98,46 +123,119 @@ __device__ inline void bs_finish(const T *cc, T a0, T a1, T a2, T *mq)
     }
 }
 
+// The entries are taken in chunks of CH = U * G.  A chunk's records are
+// wave-uniform (scalar loads, one copy per wave rather than one per lane);
+// each lane takes its entries' values from them, and the next chunk's records
+// are loaded while the current chunk's baseline values are in flight.  Record
+// reads run past the item's end (the arrays carry zero records past their
+// end, and every record holds a real slot), and those entries weigh 0.
 template <typename T, int R, int RL, int NMAP>
-__global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *items, int64_t nitems, const void *ent, const T *pw,
+__global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *__restrict__ items, int64_t nitems,
+                                                   const void *__restrict__ ent, const T *__restrict__ pw,
                                                    const T *cc_c, const int32_t *nzq, T *m, T *part)
 {
     constexpr int G = 64 / R;
+    constexpr int U = R >= 64 ? 8 : (R >= 8 ? 16 / G : 4);
+    constexpr int CH = U * G;
+    constexpr int RB = RL == REC16 ? 16 : (RL == REC12 ? 12 : 4);
+    constexpr int ND = CH * RB / 4;
     int lane = threadIdx.x & 63;
     int r = lane % R, g = lane / R;
-    int64_t it = bx_block(blockIdx.x, gridDim.x) * (BTPB / 64) + (threadIdx.x >> 6);
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int64_t it = bx_block(blockIdx.x, gridDim.x) * (BTPB / 64) + wib;
     if (it >= nitems) return;   // uniform over the wave
     BsItem in = items[it];
     T a0 = 0, a1 = 0, a2 = 0;
-    int j = g;
-    // four entries per step, every load issued before the arithmetic
-    for (; j + 3 * G < in.n; j += 4 * G) {
-        PixW e[4];
-        T v[4];
+    if (in.n > 0) {
+        const char *ente = (const char *)ent + (size_t)in.e0 * RB;
+        const int n = in.n;
+        uint32_t cw[ND];
+        auto load_chunk = [&](int jb) {
+            const uint32_t *w = (const uint32_t *)(ente + (size_t)jb * RB);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) e[u] = load_rec<RL>(ent, in.e0 + j + u * G);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = pw[(int64_t)e[u].pix * R + r];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (NMAP == 1) {
-                a0 += v[u];
+            for (int k = 0; k < ND; ++k) cw[k] = w[k];
+        };
+        auto rec_at = [&](int k) -> PixW {
+            PixW q;
+            if constexpr (RL == REC16) {
+                q.pix = (int32_t)cw[4 * k];
+                q.w0 = __uint_as_float(cw[4 * k + 1]);
+                q.w1 = __uint_as_float(cw[4 * k + 2]);
+                q.w2 = __uint_as_float(cw[4 * k + 3]);
+            } else if constexpr (RL == REC12) {
+                q.pix = (int32_t)cw[3 * k];
+                q.w0 = 1.0f;
+                q.w1 = __uint_as_float(cw[3 * k + 1]);
+                q.w2 = __uint_as_float(cw[3 * k + 2]);
             } else {
-                a0 += (T)e[u].w0 * v[u];
-                a1 += (T)e[u].w1 * v[u];
-                a2 += (T)e[u].w2 * v[u];
+                q.pix = (int32_t)cw[k];
+                q.w0 = 1.0f;
+                q.w1 = q.w2 = 0.0f;
             }
-        }
-    }
-    for (; j < in.n; j += G) {
-        PixW e = load_rec<RL>(ent, in.e0 + j);
-        T v = pw[(int64_t)e.pix * R + r];
-        if (NMAP == 1) {
-            a0 += v;
+            return q;
+        };
+        if constexpr (R < 8) {
+            // narrow rows: one record per lane per entry, the next chunk's
+            // records in flight with the current baseline values
+            auto load_lane = [&](PixW *e, int jb) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) e[u] = load_rec<RL>(ente, (int64_t)(uint32_t)(jb + u * G + g));
+            };
+            PixW ea[U], eb[U];
+            auto step = [&](const PixW *cur, PixW *nxt, int j0) -> bool {
+                bool more = j0 + CH < n;
+                load_lane(nxt, more ? j0 + CH : j0);
+                __builtin_amdgcn_sched_barrier(0);
+                T v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) v[u] = pw[(int64_t)cur[u].pix * R + r];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    T fv = (j0 + u * G + g < n) ? v[u] : (T)0;
+                    if (NMAP == 1) {
+                        a0 += fv;
+                    } else {
+                        a0 += (T)cur[u].w0 * fv;
+                        a1 += (T)cur[u].w1 * fv;
+                        a2 += (T)cur[u].w2 * fv;
+                    }
+                }
+                return more;
+            };
+            load_lane(ea, 0);
+            for (int j0 = 0;; j0 += 2 * CH) {
+                if (!step(ea, eb, j0)) break;
+                if (!step(eb, ea, j0 + CH)) break;
+            }
         } else {
-            a0 += (T)e.w0 * v;
-            a1 += (T)e.w1 * v;
-            a2 += (T)e.w2 * v;
+        load_chunk(0);
+        for (int j0 = 0;; j0 += CH) {
+            bool more = j0 + CH < n;
+            PixW e[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                e[u] = rec_at(u * G);
+#pragma unroll
+                for (int gg = 1; gg < G; ++gg)
+                    if (g == gg) e[u] = rec_at(u * G + gg);
+            }
+            T v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = pw[(int64_t)e[u].pix * R + r];
+            if (more) load_chunk(j0 + CH);   // in flight with the baseline values
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                T fv = (j0 + u * G + g < n) ? v[u] : (T)0;
+                if (NMAP == 1) {
+                    a0 += fv;
+                } else {
+                    a0 += (T)e[u].w0 * fv;
+                    a1 += (T)e[u].w1 * fv;
+                    a2 += (T)e[u].w2 * fv;
+                }
+            }
+            if (!more) break;
+        }
         }
     }
 #pragma unroll
@@ -205,18 +303,52 @@ void batch_scatter(hipStream_t s, int R, int rl, int nmap, const BsTables &bt, c
     }));
 }
 
+// key[k] = the earliest baseline (index in its detector's timeline) among the
+// CSR entries of non-empty cross pixel k: the order in which a scan first
+// reaches the pixels, so that the pixel side's concurrent waves read the
+// baseline slots of one stretch of the scan
+__global__ void k_pixel_first_time(const int64_t *segstart, int64_t nz, const void *ent, int rec_b,
+                                   const int32_t *bord, int64_t nbase, int64_t *key)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nz; k += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t = INT64_MAX;
+        for (int64_t e = segstart[k]; e < segstart[k + 1]; ++e) {
+            int32_t slot = *(const int32_t *)((const char *)ent + e * rec_b);
+            int64_t b = bord[slot];
+            int64_t kk = b % nbase;
+            t = kk < t ? kk : t;
+        }
+        key[k] = t;
+    }
+}
+
+void batch_pixel_keys(hipStream_t s, const int64_t *segstart, int64_t nz, const void *ent, int rec_b,
+                      const int32_t *bord, int64_t nbase, int64_t *key)
+{
+    if (nz <= 0) return;
+    k_pixel_first_time<<<bgrid(nz, BTPB), BTPB, 0, s>>>(segstart, nz, ent, rec_b, bord, nbase, key);
+}
+
 template <typename T>
-void batch_pw(hipStream_t s, int R, const T *p, int64_t nbtot, const int32_t *bord, const T *dwb_s,
-              const int *active, T *pw)
+void batch_pw(hipStream_t s, int R, const T *p, int64_t nbtot, const int32_t *bord, const int32_t *bslot,
+              const T *dwb_s, const T *dwb, const int *active, T *pw)
 {
     if (nbtot <= 0) return;
-    BATCH_DISPATCH_R(R, (k_bpw<T, R><<<bgrid(nbtot * R, BTPB), BTPB, 0, s>>>(p, nbtot, bord, dwb_s, active, pw)));
+    if (R == 1) {
+        k_bpw1<T><<<bgrid(nbtot, BTPB), BTPB, 0, s>>>(p, nbtot, bord, dwb_s, active, pw);
+        return;
+    }
+    unsigned g = (unsigned)((nbtot + BPW_TB - 1) / BPW_TB);
+    BATCH_DISPATCH_R(R, (k_bpw<T, R><<<g, BTPB, 0, s>>>(p, nbtot, bslot, dwb, active, pw)));
 }
 
 // ---------------------------------------------------------------- baseline side
 
-static constexpr int BG_RUN = 64;   // baselines per wave
+static constexpr int BG_RUN = 64;   // baselines per wave of k_bgather
+static constexpr int BG_RUN_MAX = 63;   // baselines per wave of k_bgather_w (one lane each, plus the end)
 
+// time-major wave order: the waves of one time window, all detectors, run
+// together (co-pointed detectors read the same map pixels)
 template <typename T, int R, int RL, int NMAP>
 __global__ void __launch_bounds__(BTPB) k_bgather(int ndet, int64_t nsamp, int64_t nbase, const int64_t *bstart,
                                                   const void *rec, const T *m, int dsh, const T *dwb, const T *nna,
@@ -228,8 +360,6 @@ __global__ void __launch_bounds__(BTPB) k_bgather(int ndet, int64_t nsamp, int64
     int r = lane % R, g = (lane / R) % G, kk = lane / (R * G);
     int64_t nruns = (nbase + BG_RUN - 1) / BG_RUN;
     int64_t wv = bx_block(blockIdx.x, gridDim.x) * (BTPB / 64) + (threadIdx.x >> 6);
-    // time-major: the waves of one time window, all detectors, run together
-    // (co-pointed detectors read the same map pixels)
     int64_t run = wv / ndet, det = wv - run * ndet;
     if (run >= nruns) return;   // uniform over the wave
     int64_t k0 = run * BG_RUN;
@@ -277,15 +407,220 @@ __global__ void __launch_bounds__(BTPB) k_bgather(int ndet, int64_t nsamp, int64
     }
 }
 
+// R >= 8: one baseline at a time per wave, G = 64 / R sample groups.  The
+// baselines of a run are cut into chunks of CH = U * G samples.  A chunk's
+// records are wave-uniform (scalar loads, one copy per wave rather than one
+// per lane); each lane takes its samples' values from them, and the next
+// chunk's records are loaded while the current chunk's map values are in
+// flight.  Record reads run past a baseline's end (up to CH - 1 records, the
+// arrays carry zero records past their end); those samples weigh 0, as do
+// flagged ones (which read the row of pixel 0: the rows of pixels without
+// hits are zero, the others finite).  The run's baseline starts sit one per
+// lane.  Sums are taken in chunk order, samples g, g + G, ... per group, then
+// over the groups by xor shuffles.
+//
+// ndg > 0: detector groups per XCD split.  Workgroups go round-robin over the
+// 8 XCDs; XCD x takes detector group x % ndg (of ndg, detectors contiguous)
+// over time slice x / ndg (of 8 / ndg), time-major inside, so that the L2 of
+// each XCD holds the map pixels of a few detectors only.  ndg = 0: every XCD
+// takes a contiguous slice of the time-major (run, detector) order.
+//
+// The baseline sums of a run go to the wave's LDS rows (run_len x (R + 1),
+// dynamic shared memory, BTPB / 64 waves), and the run ends with one pass that
+// reads p, ap, nna, dwb and writes ap as R contiguous row segments of the run.
+template <typename T, int R, int RL, int NMAP, int U>
+__global__ void __launch_bounds__(BTPB) k_bgather_w(int ndet, int64_t nsamp, int64_t nbase, int run_len, int ndg,
+                                                    const int64_t *__restrict__ bstart, const void *__restrict__ rec,
+                                                    const T *__restrict__ m, int dsh, const T *dwb, const T *nna,
+                                                    const T *p, T *ap, const int *active)
+{
+    extern __shared__ __align__(16) unsigned char bg_lds[];
+    constexpr int G = 64 / R;
+    constexpr int CH = U * G;   // samples per chunk
+    constexpr int RB = RL == REC16 ? 16 : (RL == REC12 ? 12 : 4);
+    int lane = threadIdx.x & 63;
+    int r = lane % R, g = lane / R;
+    int64_t nruns = (nbase + run_len - 1) / run_len;
+    int64_t run, det;
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // the wave in the block (uniform)
+    T *stage = reinterpret_cast<T *>(bg_lds) + wib * run_len * (R + 1);
+    if (ndg > 0) {
+        int xcd = blockIdx.x & 7, ntg = 8 / ndg;
+        int dg = xcd % ndg, tg = xcd / ndg;
+        int d0 = (int)((int64_t)dg * ndet / ndg), d1 = (int)((int64_t)(dg + 1) * ndet / ndg);
+        int64_t r0 = (int64_t)tg * nruns / ntg, r1 = (int64_t)(tg + 1) * nruns / ntg;
+        int64_t lw = (int64_t)(blockIdx.x >> 3) * (BTPB / 64) + wib;
+        int nd = d1 - d0;
+        if (nd <= 0) return;
+        run = r0 + lw / nd;
+        det = d0 + lw % nd;
+        if (run >= r1) return;   // uniform over the wave
+    } else {
+        int64_t wv = bx_block(blockIdx.x, gridDim.x) * (BTPB / 64) + wib;
+        run = wv / ndet;
+        det = wv - run * ndet;
+        if (run >= nruns) return;   // uniform over the wave
+    }
+    int64_t k0 = run * run_len;
+    int nk = (int)(k0 + run_len < nbase ? run_len : nbase - k0);
+    // sample indices inside the detector's stream fit 32 bits (nsamp < 2^31)
+    int32_t bsl = (int32_t)bstart[k0 + (lane < nk ? lane : nk)];   // run_len <= 63: lane nk holds the end
+    int64_t nbtot = (int64_t)ndet * nbase;
+    const char *recd = (const char *)rec + (size_t)det * nsamp * RB;
+    // the baseline bounds are read with readlane: the chunk control is scalar
+    int kc = 0;
+    int32_t ib = __builtin_amdgcn_readlane(bsl, 0), ie = __builtin_amdgcn_readlane(bsl, 1);
+    int32_t i = ib;
+    T acc = 0;
+    // the chunk's records (uniform), read as CH * RB / 4 contiguous dwords
+    constexpr int ND = CH * RB / 4;
+    uint32_t cw[ND];
+    auto load_chunk = [&](int32_t i0) {
+        const uint32_t *w = (const uint32_t *)(recd + (size_t)(uint32_t)i0 * RB);
+#pragma unroll
+        for (int k = 0; k < ND; ++k) cw[k] = w[k];
+    };
+    auto rec_at = [&](int k) -> PixW {
+        PixW q;
+        if constexpr (RL == REC16) {
+            q.pix = (int32_t)cw[4 * k];
+            q.w0 = __uint_as_float(cw[4 * k + 1]);
+            q.w1 = __uint_as_float(cw[4 * k + 2]);
+            q.w2 = __uint_as_float(cw[4 * k + 3]);
+        } else if constexpr (RL == REC12) {
+            q.pix = (int32_t)cw[3 * k];
+            q.w0 = 1.0f;
+            q.w1 = __uint_as_float(cw[3 * k + 1]);
+            q.w2 = __uint_as_float(cw[3 * k + 2]);
+        } else {
+            q.pix = (int32_t)cw[k];
+            q.w0 = 1.0f;
+            q.w1 = q.w2 = 0.0f;
+        }
+        return q;
+    };
+    load_chunk(i);
+    for (;;) {
+        bool last = i + CH >= ie;
+        int nkc = kc;
+        int32_t ni = i + CH, nib = ib, nie = ie;
+        if (last) {
+            nkc = kc + 1;
+            nib = ie;
+            nie = __builtin_amdgcn_readlane(bsl, nkc + 1 < 64 ? nkc + 1 : 63);
+            ni = nib;
+        }
+        bool more = nkc < nk;
+        // this lane's samples u * G + g of the chunk
+        PixW e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            e[u] = rec_at(u * G);
+#pragma unroll
+            for (int gg = 1; gg < G; ++gg)
+                if (g == gg) e[u] = rec_at(u * G + gg);
+        }
+        T mv[U][NMAP];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const T *mq = m + (int64_t)((e[u].pix >= 0 ? e[u].pix : 0) >> dsh) * NMAP * R + r;
+#pragma unroll
+            for (int c = 0; c < NMAP; ++c) mv[u][c] = mq[c * R];
+        }
+        if (more) load_chunk(ni);   // in flight with the map values
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            T f = (i + u * G + g < ie && e[u].pix >= 0) ? (T)1 : (T)0;
+            T v = NMAP == 1 ? mv[u][0] : (T)e[u].w0 * mv[u][0] + (T)e[u].w1 * mv[u][1] + (T)e[u].w2 * mv[u][2];
+            acc += f * v;
+        }
+        if (last) {
+#pragma unroll
+            for (int o = R; o < 64; o <<= 1) acc += __shfl_xor(acc, o, 64);
+            if (g == 0) stage[kc * (R + 1) + r] = acc;
+            acc = 0;
+        }
+        if (!more) break;
+        i = ni;
+        kc = nkc;
+        ib = nib;
+        ie = nie;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ap = ap + (nna p - sum dwb) over the run, row segment by row segment
+    const int64_t bb0 = det * nbase + k0;
+    for (int t = lane; t < nk * R; t += 64) {
+        int r2 = t / nk, j = t - r2 * nk;
+        if (active[r2]) {
+            int64_t b = bb0 + j, ix = (int64_t)r2 * nbtot + b;
+            ap[ix] = ap[ix] + (nna[b] * p[ix] - stage[j * (R + 1) + r2] * dwb[b]);
+        }
+    }
+}
+
+static int bg_run_len()
+{
+    static int v = [] {
+        const char *e = getenv("MADAM_BG_RUN");
+        int x = e ? atoi(e) : 32;
+        return x < 1 ? 1 : (x > BG_RUN_MAX ? BG_RUN_MAX : x);
+    }();
+    return v;
+}
+
+static bool bg_xcd_split()
+{
+    static bool v = [] {
+        const char *e = getenv("MADAM_BG_XCD");
+        return e ? atoi(e) != 0 : true;
+    }();
+    return v;
+}
+
 template <typename T>
 void batch_gather(hipStream_t s, int R, int rl, int nmap, int ndet, int64_t nsamp, int64_t nbase,
                   const int64_t *bstart, const void *rec, const T *m, int64_t ndeg, const T *dwb, const T *nna,
                   const T *p, T *ap, const int *active)
 {
-    int64_t nw = (int64_t)ndet * ((nbase + BG_RUN - 1) / BG_RUN);
-    if (nw <= 0) return;
+    if (ndet <= 0 || nbase <= 0) return;
     int dsh = 0;
     while ((1ll << dsh) < ndeg) ++dsh;
+    if (R >= 8) {
+        int run_len = bg_run_len();
+        int64_t nruns = (nbase + run_len - 1) / run_len;
+        int64_t nw = (int64_t)ndet * nruns;
+        unsigned g = (unsigned)((nw + BTPB / 64 - 1) / (BTPB / 64));
+        int ndg = 0;
+        if (bg_xcd_split()) {
+            ndg = ndet >= 8 ? 8 : ndet >= 4 ? 4 : ndet >= 2 ? 2 : 1;
+            int ntg = 8 / ndg;
+            // the largest (detector group x time slice) rectangle, in waves, on every XCD
+            int64_t w = (int64_t)((ndet + ndg - 1) / ndg) * ((nruns + ntg - 1) / ntg);
+            g = (unsigned)(8 * ((w + BTPB / 64 - 1) / (BTPB / 64)));
+        }
+#define BGW_LAUNCH(RR, UU)                                                                                           \
+    do {                                                                                                             \
+        size_t lds = sizeof(T) * (BTPB / 64) * run_len * (RR + 1);                                                   \
+        if (nmap == 1)                                                                                               \
+            k_bgather_w<T, RR, REC4, 1, UU><<<g, BTPB, lds, s>>>(ndet, nsamp, nbase, run_len, ndg, bstart, rec, m, dsh, \
+                                                              dwb, nna, p, ap, active);                                   \
+        else if (rl == REC12)                                                                                        \
+            k_bgather_w<T, RR, REC12, 3, UU><<<g, BTPB, lds, s>>>(ndet, nsamp, nbase, run_len, ndg, bstart, rec, m,     \
+                                                               dsh, dwb, nna, p, ap, active);                             \
+        else                                                                                                         \
+            k_bgather_w<T, RR, REC16, 3, UU><<<g, BTPB, lds, s>>>(ndet, nsamp, nbase, run_len, ndg, bstart, rec, m,     \
+                                                               dsh, dwb, nna, p, ap, active);                             \
+    } while (0)
+        switch (R) {   // 16 records per chunk (8 at R = 64)
+        case 8: BGW_LAUNCH(8, 2); break;
+        case 16: BGW_LAUNCH(16, 4); break;
+        case 32: BGW_LAUNCH(32, 8); break;
+        default: BGW_LAUNCH(64, 8); break;
+        }
+#undef BGW_LAUNCH
+        return;
+    }
+    int64_t nw = (int64_t)ndet * ((nbase + BG_RUN - 1) / BG_RUN);
     unsigned g = (unsigned)((nw + BTPB / 64 - 1) / (BTPB / 64));
     BATCH_DISPATCH_R(R, ({
         if (nmap == 1)
@@ -611,7 +946,8 @@ __global__ void k_bmap_unpack(const T *m, int64_t npix, int nmap, int R, int r, 
 #define BATCH_INST(T)                                                                                              \
     template void batch_scatter<T>(hipStream_t, int, int, int, const BsTables &, const void *, const T *, const T *, \
                                    const int32_t *, T *, T *);                                                     \
-    template void batch_pw<T>(hipStream_t, int, const T *, int64_t, const int32_t *, const T *, const int *, T *); \
+    template void batch_pw<T>(hipStream_t, int, const T *, int64_t, const int32_t *, const int32_t *, const T *,   \
+                               const T *, const int *, T *); \
     template void batch_gather<T>(hipStream_t, int, int, int, int, int64_t, int64_t, const int64_t *, const void *, \
                                   const T *, int64_t, const T *, const T *, const T *, T *, const int *);          \
     template void batch_dots<T>(hipStream_t, int, int, const T *, const T *, const T *, const int8_t *, int64_t,    \

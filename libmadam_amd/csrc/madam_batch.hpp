@@ -43,8 +43,11 @@ struct BatchScal {
 int batch_width(int nrhs);
 
 template <typename T>
-void batch_pw(hipStream_t s, int R, const T *p, int64_t nbtot, const int32_t *bord, const T *dwb_s,
-              const int *active, T *pw);
+void batch_pw(hipStream_t s, int R, const T *p, int64_t nbtot, const int32_t *bord, const int32_t *bslot,
+              const T *dwb_s, const T *dwb, const int *active, T *pw);
+// the earliest baseline (index in its detector) among each non-empty pixel's entries
+void batch_pixel_keys(hipStream_t s, const int64_t *segstart, int64_t nz, const void *ent, int rec_b,
+                      const int32_t *bord, int64_t nbase, int64_t *key);
 template <typename T>
 void batch_scatter(hipStream_t s, int R, int rl, int nmap, const BsTables &bt, const void *ent, const T *pw,
                    const T *cc_c, const int32_t *nzq, T *m, T *part);

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -38,9 +39,12 @@ struct BatchState {
     int piece = 0;            // entries per work item (the autotuner's choice)
     bool tables_ok = false;
     const void *tables_ent = nullptr;   // the CSR the tables were built for
+    std::vector<int64_t> item_order;    // pixels in first-hit order (build_items)
+    const void *order_ent = nullptr;
     // FP32 copies of the shared per-baseline / per-pixel terms
     DBuf<float> dwb_s32, dwb32, nna32, cc_c32;
     DBuf<float> m4f;   // the one-RHS FP32 map, 4 per pixel (the FP64 one is the context's m4x)
+    DBuf<float> fcov_rows32, fprec_rows32;   // the filter spectra in row order, FP32 (empty: FP64 transforms)
     // FP64 per right-hand side (RHS-major)
     DBuf<double> yba, a64, r64, ap64, rhs4, map4, binmap4;
     DBuf<double> pw64, m64, part64;
@@ -97,6 +101,7 @@ void batch_invalidate(Context &c)
 {
     if (!c.batch) return;
     c.batch->tables_ok = false;
+    c.batch->item_order.clear();
     c.batch->drop_graphs();
     c.batch->nrhs = 0;
 }
@@ -109,19 +114,44 @@ static T *tp(DBuf<char> &b)
 
 // ------------------------------------------------------------------ pixel-side work items
 
+static const void *ent_records(const Context &c);
+
 // Each non-empty cross pixel is one item, or, beyond `piece` entries, pieces
 // of `piece` entries whose sums the fix-up pass adds in order.
+//
+// The items are listed in the order in which the scan first reaches their
+// pixels (the earliest baseline among a pixel's entries, ties by pixel): the
+// waves that run together then read the baseline slots of one stretch of the
+// scan, which the L2 holds.  The order of the items changes no sum.
 static void build_items(Context &c, BatchState &B, int piece)
 {
     int64_t nz = c.nzpix;
     std::vector<int64_t> st(nz + 1);
     d2h(st.data(), c.d_segstart_s.p, nz + 1, c.stream);
+    std::vector<int64_t> &order = B.item_order;
+    if ((int64_t)order.size() != nz || B.order_ent != ent_records(c)) {
+        order.resize(nz);
+        for (int64_t k = 0; k < nz; ++k) order[k] = k;
+        const char *eo = getenv("MADAM_BS_ORDER");
+        if (!(eo && atoi(eo) == 0) && nz > 0) {
+            DBuf<int64_t> dkey;
+            dkey.alloc(nz);
+            batch_pixel_keys(c.stream, c.d_segstart_s.p, nz, ent_records(c), rec_bytes(c.rec_layout), c.d_bord.p,
+                             c.nbase, dkey.p);
+            MADAM_CHECK(hipGetLastError());
+            std::vector<int64_t> key(nz);
+            d2h(key.data(), dkey.p, nz, c.stream);
+            MADAM_CHECK(hipStreamSynchronize(c.stream));
+            std::stable_sort(order.begin(), order.end(), [&key](int64_t a, int64_t b) { return key[a] < key[b]; });
+        }
+        B.order_ent = ent_records(c);
+    }
     MADAM_CHECK(hipStreamSynchronize(c.stream));
     std::vector<BsItem> items;
     std::vector<BsFix> fix;
     items.reserve(nz + nz / 8);
     int32_t nparts = 0;
-    for (int64_t k = 0; k < nz; ++k) {
+    for (int64_t k : order) {
         int64_t e0 = st[k], n = st[k + 1] - st[k];
         if (n <= piece) {
             items.push_back(BsItem{e0, (int32_t)n, (int32_t)k, -1, 0});
@@ -170,6 +200,8 @@ struct Ops<double> {
     static const double *nna(Context &c, BatchState &) { return c.d_nna.p; }
     static const double *cc(Context &c, BatchState &) { return c.d_ccinv_c.p; }
     static double *m4(Context &c, BatchState &) { return c.d_m4x.p; }
+    static const float *fcov32(BatchState &) { return nullptr; }
+    static const float *fprec32(BatchState &) { return nullptr; }
 };
 template <>
 struct Ops<float> {
@@ -178,6 +210,8 @@ struct Ops<float> {
     static const float *nna(Context &, BatchState &B) { return B.nna32.p; }
     static const float *cc(Context &, BatchState &B) { return B.cc_c32.p; }
     static float *m4(Context &, BatchState &B) { return B.m4f.p; }
+    static const float *fcov32(BatchState &B) { return B.fcov_rows32.p; }
+    static const float *fprec32(BatchState &B) { return B.fprec_rows32.p; }
 };
 
 // events around the parts of a PCG step when profiling
@@ -197,7 +231,7 @@ static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T
     // one right-hand side: the single solve's segmented-tile pixel side and
     // flat baseline side in the operator's type; several: the batched kernels
     bool one = R == 1;
-    batch_pw<T>(s, R, x, nb, c.d_bord.p, Ops<T>::dwb_s(c, B), active, pw);
+    batch_pw<T>(s, R, x, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), active, pw);
     if (tm) tm->sc.start(s);
     if (one)
         launch_scatter_seg<T>(s, c.nmap, c.nactive, ent_records(c), c.rec_layout, pw, c.sr_s.heads.p, c.sr_s.k0.p,
@@ -213,7 +247,8 @@ static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T
         if (!c.prior_covers_all) batch_copy<T>(s, y, x, nb * R);
         if (!B.segR.empty())
             fft_convolve_segments<T>(s, c.fft, B.d_segR.p, (int64_t)B.segR.size(), x, y, c.d_fcov.p, c.d_fcov_rows.p,
-                                     c.nof / 2 + 1, B.fftwork.p, c.fft_batch, B.fft_nseg, B.segR.data());
+                                     c.nof / 2 + 1, B.fftwork.p, c.fft_batch, B.fft_nseg, B.segR.data(),
+                                     Ops<T>::fcov32(B));
     } else {
         batch_fill<T>(s, y, (T)0, nb * R);
     }
@@ -245,7 +280,8 @@ static void batch_precond(Context &c, BatchState &B, const T *r, T *z)
         if (!B.segR.empty())
             fft_convolve_segments<T>(s, c.fft, B.d_segR.p, (int64_t)B.segR.size(), r, z, c.d_fprec.p,
                                      c.d_fprec_rows.p, c.nof / 2 + 1, B.fftwork.p, c.fft_batch, B.fft_nseg,
-                                     B.segR.data());
+                                     B.segR.data(),
+                                     Ops<T>::fprec32(B));
         break;
     default:
         if (!c.prec_covers_all) batch_copy<T>(s, z, r, nb * R);
@@ -365,7 +401,11 @@ static Vec<T> alloc_vectors(Context &c, BatchState &B)
     size_t e = sizeof(T);
     for (DBuf<char> *b : {&B.x, &B.r, &B.ro, &B.z, &B.p, &B.ap}) b->alloc(std::max<int64_t>(n, 1) * e);
     B.pw.alloc(std::max<int64_t>(n, 1) * e);
-    B.m.alloc((size_t)std::max<int64_t>(c.npix_cross, 1) * c.nmap * B.R * e);
+    size_t mb = (size_t)std::max<int64_t>(c.npix_cross, 1) * c.nmap * B.R * e;
+    B.m.alloc(mb);
+    // the rows of pixels without hits are never written: zero, so that the
+    // baseline side may weight them by 0
+    MADAM_CHECK(hipMemsetAsync(B.m.p, 0, mb, c.stream));
     B.part.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * B.R * e);
     return Vec<T>{tp<T>(B.x), tp<T>(B.r), tp<T>(B.ro), tp<T>(B.z), tp<T>(B.p), tp<T>(B.ap),
                   tp<T>(B.pw), tp<T>(B.m), tp<T>(B.part)};
@@ -380,6 +420,13 @@ static void tune_pieces(Context &c, BatchState &B, const Vec<T> &v)
 {
     static const int cand[] = {256, 1024, 4096};
     hipStream_t s = c.stream;
+    // MADAM_BS_PIECE pins the cut (the tuned cut can differ from run to run,
+    // and with it the rounding of the long pixels' sums)
+    const char *pe = getenv("MADAM_BS_PIECE");
+    if (pe && atoi(pe) > 0) {
+        build_items(c, B, atoi(pe));
+        return;
+    }
     hipEvent_t e0, e1;
     MADAM_CHECK(hipEventCreate(&e0));
     MADAM_CHECK(hipEventCreate(&e1));
@@ -548,6 +595,25 @@ static void solve_batch_t(Context &c, int nrhs, const TT *signals, int on_device
         batch_cast<double, float>(s, c.d_dwb.p, B.dwb32.p, nb);
         batch_cast<double, float>(s, c.d_nna.p, B.nna32.p, nb);
         batch_cast<double, float>(s, c.d_ccinv_c.p, B.cc_c32.p, c.nzpix * 6);
+        // the FP32 operator's noise prior and Fourier preconditioner transform in
+        // FP32 (MADAM_FFT32=0 keeps them in FP64)
+        const char *e32 = getenv("MADAM_FFT32");
+        bool fft32 = !(e32 && atoi(e32) == 0);
+        int64_t nspec = (int64_t)(c.nof / 2 + 1) * c.npsdtot;
+        const float *old_f = B.fcov_rows32.p, *old_p = B.fprec_rows32.p;
+        if (fft32 && c.p.kfilter && c.d_fcov_rows.p) {
+            B.fcov_rows32.alloc((size_t)nspec);
+            batch_cast<double, float>(s, c.d_fcov_rows.p, B.fcov_rows32.p, nspec);
+        } else {
+            B.fcov_rows32.release();
+        }
+        if (fft32 && c.precond_kind == 2 && c.d_fprec_rows.p) {
+            B.fprec_rows32.alloc((size_t)nspec);
+            batch_cast<double, float>(s, c.d_fprec_rows.p, B.fprec_rows32.p, nspec);
+        } else {
+            B.fprec_rows32.release();
+        }
+        if (B.fcov_rows32.p != old_f || B.fprec_rows32.p != old_p) B.drop_graphs();   // bound into the captured step
         Vec<float> v = alloc_vectors<float>(c, B);
         B.m4f.alloc((size_t)std::max<int64_t>(c.npix_cross, 1) * 4);
         if (R > 1 && (!B.tables_ok || B.tables_ent != ent_records(c))) {
@@ -561,6 +627,8 @@ static void solve_batch_t(Context &c, int nrhs, const TT *signals, int on_device
         // the FP64 operator's own buffers (its pieces in the same tables)
         B.pw64.alloc((size_t)std::max<int64_t>(nb, 1) * R);
         B.m64.alloc((size_t)std::max<int64_t>(c.npix_cross, 1) * c.nmap * R);
+        MADAM_CHECK(hipMemsetAsync(B.m64.p, 0, sizeof(double) * (size_t)std::max<int64_t>(c.npix_cross, 1) * c.nmap * R,
+                                   s));
         B.part64.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * R);
         B.r64.alloc((size_t)std::max<int64_t>(nb, 1) * R);
         B.ap64.alloc((size_t)std::max<int64_t>(nb, 1) * R);

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "madam_internal.hpp"
@@ -37,9 +38,44 @@ __device__ inline double2 cmulc(double2 a, double2 b)   // a * conj(b)
 {
     return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
+__device__ inline float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cmulc(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// complex type of the fused passes' work buffer: double2 (FP64) or float2 (FP32)
+template <typename C>
+struct CxT;
+template <>
+struct CxT<double2> {
+    using S = double;
+    __device__ static double2 mk(double a, double b) { return make_double2(a, b); }
+    __device__ static const double2 *sub1(const FftPlan &P) { return P.sub1; }
+    __device__ static const double2 *sub2(const FftPlan &P) { return P.sub2; }
+    __device__ static const double2 *na(const FftPlan &P) { return P.na; }
+    __device__ static const double2 *nb(const FftPlan &P) { return P.nb; }
+    __device__ static const double2 *fa(const FftPlan &P) { return P.fa; }
+    __device__ static const double2 *fb(const FftPlan &P) { return P.fb; }
+};
+template <>
+struct CxT<float2> {
+    using S = float;
+    __device__ static float2 mk(float a, float b) { return make_float2(a, b); }
+    __device__ static const float2 *sub1(const FftPlan &P) { return P.sub1f; }
+    __device__ static const float2 *sub2(const FftPlan &P) { return P.sub2f; }
+    __device__ static const float2 *na(const FftPlan &P) { return P.naf; }
+    __device__ static const float2 *nb(const FftPlan &P) { return P.nbf; }
+    __device__ static const float2 *fa(const FftPlan &P) { return P.faf; }
+    __device__ static const float2 *fb(const FftPlan &P) { return P.fbf; }
+};
 
 // W^m from a two-level table: a[m % 1024] * b[m / 1024]
-__device__ inline double2 tw2(const double2 *a, const double2 *b, int64_t m)
+template <typename C>
+__device__ inline C tw2(const C *a, const C *b, int64_t m)
 {
     return cmul(a[m & 1023], b[m >> 10]);
 }
@@ -305,26 +341,32 @@ void fft_plan_init(FftPlan &p, int64_t nof)
     if (l2 > 11) { l2 = 11; l1 = logN - 11; }
     p.L1 = 1LL << l1;
     p.L2 = 1LL << l2;
-    // exp(-2 pi i num / den), computed in long double and rounded once
-    auto W = [](long double num, long double den, double *out) {
+    // exp(-2 pi i num / den), computed in long double and rounded once (to
+    // FP64, and separately to FP32 for the FP32 tables)
+    std::vector<long double> h;
+    auto W = [&h](long double num, long double den) {
         long double a = -2.0L * 3.14159265358979323846264338327950288L * num / den;
-        out[0] = (double)cosl(a);
-        out[1] = (double)sinl(a);
+        h.push_back(cosl(a));
+        h.push_back(sinl(a));
     };
     int64_t n1 = std::max<int64_t>(p.L1 / 2, 1), n2 = std::max<int64_t>(p.L2 / 2, 1);
     int64_t nna = 1024, nnb = p.N / 1024 + 1;
     int64_t nfa = 1024, nfb = (p.N / 2) / 1024 + 1;
     int64_t tot = n1 + n2 + nna + nnb + nfa + nfb;
-    std::vector<double> h(2 * tot);
-    double *q = h.data();
-    for (int64_t j = 0; j < n1; ++j, q += 2) W(j, p.L1, q);
-    for (int64_t j = 0; j < n2; ++j, q += 2) W(j, p.L2, q);
-    for (int64_t j = 0; j < nna; ++j, q += 2) W(j, p.N, q);
-    for (int64_t j = 0; j < nnb; ++j, q += 2) W(1024.0L * j, p.N, q);
-    for (int64_t j = 0; j < nfa; ++j, q += 2) W(j, nof, q);
-    for (int64_t j = 0; j < nfb; ++j, q += 2) W(1024.0L * j, nof, q);
-    MADAM_CHECK(hipMalloc(&p.tw_mem, sizeof(double) * h.size()));
-    MADAM_CHECK(hipMemcpy(p.tw_mem, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    h.reserve(2 * tot);
+    for (int64_t j = 0; j < n1; ++j) W(j, p.L1);
+    for (int64_t j = 0; j < n2; ++j) W(j, p.L2);
+    for (int64_t j = 0; j < nna; ++j) W(j, p.N);
+    for (int64_t j = 0; j < nnb; ++j) W(1024.0L * j, p.N);
+    for (int64_t j = 0; j < nfa; ++j) W(j, nof);
+    for (int64_t j = 0; j < nfb; ++j) W(1024.0L * j, nof);
+    std::vector<double> hd(h.begin(), h.end());
+    std::vector<float> hf(h.size());
+    for (size_t i = 0; i < h.size(); ++i) hf[i] = (float)h[i];
+    MADAM_CHECK(hipMalloc(&p.tw_mem, sizeof(double) * hd.size()));
+    MADAM_CHECK(hipMemcpy(p.tw_mem, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
+    MADAM_CHECK(hipMalloc(&p.tw32_mem, sizeof(float) * hf.size()));
+    MADAM_CHECK(hipMemcpy(p.tw32_mem, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice));
     const double2 *b = (const double2 *)p.tw_mem;
     p.sub1 = b;
     p.sub2 = p.sub1 + n1;
@@ -332,13 +374,23 @@ void fft_plan_init(FftPlan &p, int64_t nof)
     p.nb = p.na + nna;
     p.fa = p.nb + nnb;
     p.fb = p.fa + nfa;
+    const float2 *bf = (const float2 *)p.tw32_mem;
+    p.sub1f = bf;
+    p.sub2f = p.sub1f + n1;
+    p.naf = p.sub2f + n2;
+    p.nbf = p.naf + nna;
+    p.faf = p.nbf + nnb;
+    p.fbf = p.faf + nfa;
 }
 
 void fft_plan_free(FftPlan &p)
 {
     if (p.tw_mem) (void)hipFree(p.tw_mem);
+    if (p.tw32_mem) (void)hipFree(p.tw32_mem);
     p.tw_mem = nullptr;
+    p.tw32_mem = nullptr;
     p.sub1 = p.sub2 = p.na = p.nb = p.fa = p.fb = nullptr;
+    p.sub1f = p.sub2f = p.naf = p.nbf = p.faf = p.fbf = nullptr;
 }
 
 static void fft_forward(hipStream_t s, const FftPlan &p, double2 *d, int64_t batch)
@@ -390,7 +442,8 @@ __device__ inline int lp(int idx) { return idx + (idx >> 3); }
 // DIT, bit-reversed input order, natural output order; twL[j] = W_L^j (j < L / 2).
 // 2^logncol interleaved transforms; every length is a power of two, so the
 // index arithmetic is shifts and masks.
-__device__ void lds_fft8(double2 *s, int L, int logL, int logncol, const double2 *twL, bool inv)
+template <typename C>
+__device__ void lds_fft8(C *s, int L, int logL, int logncol, const C *twL, bool inv)
 {
     const int ncol = 1 << logncol;
     for (int st = 0; st < logL; st += 3) {
@@ -401,7 +454,7 @@ __device__ void lds_fft8(double2 *s, int L, int logL, int logncol, const double2
             int col = t & (ncol - 1), it = t >> logncol;
             int j = it & (h0 - 1), grp = it >> st;
             int base = grp * (E * h0) + j;
-            double2 a[8];
+            C a[8];
 #pragma unroll
             for (int m = 0; m < 8; ++m)
                 if (m < E) a[m] = s[lp((base + m * h0) * ncol + col)];
@@ -412,11 +465,11 @@ __device__ void lds_fft8(double2 *s, int L, int logL, int logncol, const double2
                 for (int m0 = 0; m0 < 8; ++m0) {
                     if (m0 >= E || (m0 & (1 << tt))) continue;
                     int r = m0 & ((1 << tt) - 1);
-                    double2 w = twL[(j + r * h0) << (logL - st - tt - 1)];   // * L / (2 h)
+                    C w = twL[(j + r * h0) << (logL - st - tt - 1)];   // * L / (2 h)
                     if (inv) w.y = -w.y;
-                    double2 b = cmul(a[m0 + (1 << tt)], w), x = a[m0];
-                    a[m0] = make_double2(x.x + b.x, x.y + b.y);
-                    a[m0 + (1 << tt)] = make_double2(x.x - b.x, x.y - b.y);
+                    C b = cmul(a[m0 + (1 << tt)], w), x = a[m0];
+                    a[m0] = CxT<C>::mk(x.x + b.x, x.y + b.y);
+                    a[m0 + (1 << tt)] = CxT<C>::mk(x.x - b.x, x.y - b.y);
                 }
             }
 #pragma unroll
@@ -428,7 +481,8 @@ __device__ void lds_fft8(double2 *s, int L, int logL, int logncol, const double2
 }
 
 // DIF, natural input order, bit-reversed output order
-__device__ void lds_fft8_dif(double2 *s, int L, int logL, int logncol, const double2 *twL, bool inv)
+template <typename C>
+__device__ void lds_fft8_dif(C *s, int L, int logL, int logncol, const C *twL, bool inv)
 {
     const int ncol = 1 << logncol;
     for (int hi = logL - 1; hi >= 0; hi -= 3) {
@@ -440,7 +494,7 @@ __device__ void lds_fft8_dif(double2 *s, int L, int logL, int logncol, const dou
             int col = t & (ncol - 1), it = t >> logncol;
             int j = it & (h0 - 1), grp = it >> lo;
             int base = grp * (E * h0) + j;
-            double2 a[8];
+            C a[8];
 #pragma unroll
             for (int m = 0; m < 8; ++m)
                 if (m < E) a[m] = s[lp((base + m * h0) * ncol + col)];
@@ -451,11 +505,11 @@ __device__ void lds_fft8_dif(double2 *s, int L, int logL, int logncol, const dou
                 for (int m0 = 0; m0 < 8; ++m0) {
                     if (m0 >= E || (m0 & (1 << tt))) continue;
                     int r = m0 & ((1 << tt) - 1);
-                    double2 w = twL[(j + r * h0) << (logL - lo - tt - 1)];   // * L / (2 h)
+                    C w = twL[(j + r * h0) << (logL - lo - tt - 1)];   // * L / (2 h)
                     if (inv) w.y = -w.y;
-                    double2 x = a[m0], y = a[m0 + (1 << tt)];
-                    a[m0] = make_double2(x.x + y.x, x.y + y.y);
-                    a[m0 + (1 << tt)] = cmul(make_double2(x.x - y.x, x.y - y.y), w);
+                    C x = a[m0], y = a[m0 + (1 << tt)];
+                    a[m0] = CxT<C>::mk(x.x + y.x, x.y + y.y);
+                    a[m0 + (1 << tt)] = cmul(CxT<C>::mk(x.x - y.x, x.y - y.y), w);
                 }
             }
 #pragma unroll
@@ -487,11 +541,14 @@ __host__ __device__ inline int conv_lds_elems(int n) { return n + (n >> 3) + 1; 
 static constexpr int CONV_COL_TPB = 512;
 
 // grid = (L2 / CONV_NCOL, batch)
-template <typename T>
+// (C: the work type, double2 or float2; T: the type of x)
+template <typename T, typename C>
 __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const T *x, const Segment *seg, int64_t s0, const double *partial_in,
-                               double2 *work, int64_t N, int L1, int logL1, int L2, FftPlan P)
+                               C *work, int64_t N, int L1, int logL1, int L2, FftPlan P)
 {
-    extern __shared__ double2 s[];
+    using S = typename CxT<C>::S;
+    extern __shared__ __align__(16) unsigned char conv_lds[];
+    C *s = reinterpret_cast<C *>(conv_lds);
     const int ncol = CONV_NCOL;
     int64_t b = blockIdx.y;
     int c0 = blockIdx.x * ncol;
@@ -509,22 +566,22 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const T *x, const
     double mean = smean;
     const T *xs = x + sg.b0;
     int64_t noba = sg.noba;
-    double2 *d = work + b * N;
+    C *d = work + b * N;
     int tot = L1 * ncol;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, n1 = t / ncol;
         int64_t jj = 2 * ((int64_t)n1 * L2 + c0 + col);
-        double2 v;
-        v.x = jj < noba ? (double)xs[jj] - mean : 0.0;
-        v.y = jj + 1 < noba ? (double)xs[jj + 1] - mean : 0.0;
+        C v;
+        v.x = jj < noba ? (S)((double)xs[jj] - mean) : (S)0;
+        v.y = jj + 1 < noba ? (S)((double)xs[jj + 1] - mean) : (S)0;
         s[lp(bitrev(n1, logL1) * ncol + col)] = v;
     }
     __syncthreads();
-    lds_fft8(s, L1, logL1, 3, P.sub1, false);   // CONV_NCOL = 8 columns
+    lds_fft8<C>(s, L1, logL1, 3, CxT<C>::sub1(P), false);   // CONV_NCOL = 8 columns
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
-        double2 w = tw2(P.na, P.nb, (n2 * k1) & (N - 1))   /* N is a power of two */;
+        C w = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * k1) & (N - 1))   /* N is a power of two */;
         d[(int64_t)k1 * L2 + n2] = cmul(s[lp(k1 * ncol + col)], w);
     }
 }
@@ -532,28 +589,29 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const T *x, const
 // spectral step on one pair {k, N - k}: X[k] at s[ik], X[N - k] at s[im] (padded
 // LDS indices), 0 < k <= N / 2
 // fk = f[k], fm = f[N - k]
-__device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t N, const FftPlan &P, double fk,
-                                 double fm)
+template <typename C>
+__device__ inline void conv_pair(C *s, int ik, int im, int64_t k, int64_t N, const FftPlan &P, typename CxT<C>::S fk,
+                                 typename CxT<C>::S fm)
 {
-    double2 a = s[ik], bc = s[im];
+    C a = s[ik], bc = s[im];
     bc.y = -bc.y;
-    double2 E = make_double2(0.5 * (a.x + bc.x), 0.5 * (a.y + bc.y));
-    double2 O = make_double2(0.5 * (a.y - bc.y), -0.5 * (a.x - bc.x));
-    double2 W = tw2(P.fa, P.fb, k);
-    double2 WO = cmul(W, O);
-    double2 Xk = make_double2(E.x + WO.x, E.y + WO.y);
-    double2 Xm = make_double2(E.x - WO.x, -(E.y - WO.y));
-    double2 Yk = make_double2(Xk.x * fk, Xk.y * fk);
-    double2 Ym = make_double2(Xm.x * fm, Xm.y * fm);
-    double2 s1 = make_double2(Yk.x + Ym.x, Yk.y - Ym.y);
-    double2 d1 = make_double2(Yk.x - Ym.x, Yk.y + Ym.y);
-    double2 t1 = cmulc(d1, W);
-    s[ik] = make_double2(s1.x - t1.y, s1.y + t1.x);
+    C E = CxT<C>::mk((typename CxT<C>::S)0.5 * (a.x + bc.x), (typename CxT<C>::S)0.5 * (a.y + bc.y));
+    C O = CxT<C>::mk((typename CxT<C>::S)0.5 * (a.y - bc.y), -(typename CxT<C>::S)0.5 * (a.x - bc.x));
+    C W = tw2(CxT<C>::fa(P), CxT<C>::fb(P), k);
+    C WO = cmul(W, O);
+    C Xk = CxT<C>::mk(E.x + WO.x, E.y + WO.y);
+    C Xm = CxT<C>::mk(E.x - WO.x, -(E.y - WO.y));
+    C Yk = CxT<C>::mk(Xk.x * fk, Xk.y * fk);
+    C Ym = CxT<C>::mk(Xm.x * fm, Xm.y * fm);
+    C s1 = CxT<C>::mk(Yk.x + Ym.x, Yk.y - Ym.y);
+    C d1 = CxT<C>::mk(Yk.x - Ym.x, Yk.y + Ym.y);
+    C t1 = cmulc(d1, W);
+    s[ik] = CxT<C>::mk(s1.x - t1.y, s1.y + t1.x);
     if (ik != im) {
-        double2 s2 = make_double2(Ym.x + Yk.x, Ym.y - Yk.y);
-        double2 d2 = make_double2(Ym.x - Yk.x, Ym.y + Yk.y);
-        double2 t2 = cmul(d2, W);
-        s[im] = make_double2(s2.x + t2.y, s2.y - t2.x);
+        C s2 = CxT<C>::mk(Ym.x + Yk.x, Ym.y - Yk.y);
+        C d2 = CxT<C>::mk(Ym.x - Yk.x, Ym.y + Yk.y);
+        C t2 = cmul(d2, W);
+        s[im] = CxT<C>::mk(s2.x + t2.y, s2.y - t2.x);
     }
 }
 
@@ -564,14 +622,17 @@ __device__ inline void conv_pair(double2 *s, int ik, int im, int64_t k, int64_t 
 // k2 * 2 + r (padded).  fc holds the filter spectra in row order
 // (fft_spectrum_rows): f[r + L1 k2] at [r * L2 + k2], f[N] at [N], so a row
 // pair reads two contiguous rows.
-__global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2, FftPlan P, const double *fc,
+template <typename C>
+__global__ void k_conv_rows(C *work, int64_t N, int L1, int L2, int logL2, FftPlan P, const typename CxT<C>::S *fc,
                             int64_t fc_stride, const Segment *seg, int64_t s0)
 {
-    extern __shared__ double2 s[];
+    using S = typename CxT<C>::S;
+    extern __shared__ __align__(16) unsigned char conv_lds[];
+    C *s = reinterpret_cast<C *>(conv_lds);
     int64_t b = blockIdx.y;
     int ra = blockIdx.x, rb = ra == 0 ? L1 / 2 : L1 - ra;
-    double2 *d = work + b * N;
-    const double *f = fc + (int64_t)seg[s0 + b].ipsd * fc_stride;
+    C *d = work + b * N;
+    const S *f = fc + (int64_t)seg[s0 + b].ipsd * fc_stride;
     int tot = 2 * L2;
     // natural order in and out of LDS (no bank conflicts): the forward row
     // FFTs are DIF (bit-reversed output), the inverse ones DIT (bit-reversed
@@ -582,7 +643,7 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
         s[lp(n * 2 + r)] = d[(int64_t)row * L2 + n];
     }
     __syncthreads();
-    lds_fft8_dif(s, L2, logL2, 1, P.sub2, false);   // 2 rows
+    lds_fft8_dif<C>(s, L2, logL2, 1, CxT<C>::sub2(P), false);   // 2 rows
     // X[ra + L1 k2] is at bitrev(k2) * 2; X[rb + L1 k2] at bitrev(k2) * 2 + 1
     if (ra == 0) {
         // row 0: pairs {k2, L2 - k2} (k2 = 0 holds X[0] and X[N / 2]);
@@ -590,10 +651,10 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
         const int h = L1 / 2;
         for (int t = threadIdx.x; t <= L2; t += blockDim.x) {
             if (t == 0) {
-                double2 a = s[lp(0)];
-                double x0 = a.x + a.y, xn = a.x - a.y;
-                double y0 = x0 * f[0], yn = xn * f[N];   // f[0] at [0]
-                s[lp(0)] = make_double2(y0 + yn, y0 - yn);
+                C a = s[lp(0)];
+                S x0 = a.x + a.y, xn = a.x - a.y;
+                S y0 = x0 * f[0], yn = xn * f[N];   // f[0] at [0]
+                s[lp(0)] = CxT<C>::mk(y0 + yn, y0 - yn);
             } else if (t <= L2 / 2) {
                 int k2 = t;
                 conv_pair(s, lp(bitrev(k2, logL2) * 2), lp(bitrev(L2 - k2, logL2) * 2), (int64_t)L1 * k2, N, P,
@@ -609,13 +670,13 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
         for (int k2 = threadIdx.x; k2 < L2; k2 += blockDim.x) {
             int64_t kA = (int64_t)ra + (int64_t)L1 * k2;
             int pp = lp(bitrev(k2, logL2) * 2), qq = lp(bitrev(L2 - 1 - k2, logL2) * 2 + 1);
-            double fA = f[(int64_t)ra * L2 + k2], fB = f[(int64_t)rb * L2 + L2 - 1 - k2];   // f[kA], f[N - kA]
+            S fA = f[(int64_t)ra * L2 + k2], fB = f[(int64_t)rb * L2 + L2 - 1 - k2];   // f[kA], f[N - kA]
             if (kA <= N / 2) conv_pair(s, pp, qq, kA, N, P, fA, fB);
             else conv_pair(s, qq, pp, N - kA, N, P, fB, fA);
         }
     }
     __syncthreads();
-    lds_fft8(s, L2, logL2, 1, P.sub2, true);
+    lds_fft8<C>(s, L2, logL2, 1, CxT<C>::sub2(P), true);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int r = t >> logL2, n = t & (L2 - 1);
         int row = r == 0 ? ra : rb;
@@ -624,40 +685,41 @@ __global__ void k_conv_rows(double2 *work, int64_t N, int L1, int L2, int logL2,
 }
 
 // grid = (L2 / CONV_NCOL, batch)
-template <typename T>
-__global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const double2 *work, const Segment *seg, int64_t s0, T *y, double *partial_out,
+template <typename T, typename C>
+__global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const C *work, const Segment *seg, int64_t s0, T *y, double *partial_out,
                                int64_t N, int L1, int logL1, int L2, FftPlan P)
 {
-    extern __shared__ double2 s[];
+    extern __shared__ __align__(16) unsigned char conv_lds[];
+    C *s = reinterpret_cast<C *>(conv_lds);
     const int ncol = CONV_NCOL;
     int64_t b = blockIdx.y;
     int c0 = blockIdx.x * ncol;
     Segment sg = seg[s0 + b];
     int64_t noba = sg.noba;
-    const double2 *d = work + b * N;
+    const C *d = work + b * N;
     double inv = 1 / (double)(2 * N);
     int tot = L1 * ncol;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, k1 = t / ncol;
         int64_t n2 = c0 + col;
-        double2 w = tw2(P.na, P.nb, (n2 * k1) & (N - 1))   /* N is a power of two */;
+        C w = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * k1) & (N - 1))   /* N is a power of two */;
         s[lp(bitrev(k1, logL1) * ncol + col)] = cmulc(d[(int64_t)k1 * L2 + n2], w);
     }
     __syncthreads();
-    lds_fft8(s, L1, logL1, 3, P.sub1, true);
+    lds_fft8<C>(s, L1, logL1, 3, CxT<C>::sub1(P), true);
     double acc = 0;
     T *ys = y + sg.b0;
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
         int col = t % ncol, n1 = t / ncol;
         int64_t jj = 2 * ((int64_t)n1 * L2 + c0 + col);
-        double2 v = s[lp(n1 * ncol + col)];
+        C v = s[lp(n1 * ncol + col)];
         if (jj < noba) {
-            double a = v.x * inv;
+            double a = (double)v.x * inv;
             ys[jj] = (T)a;
             acc += a;
         }
         if (jj + 1 < noba) {
-            double a = v.y * inv;
+            double a = (double)v.y * inv;
             ys[jj + 1] = (T)a;
             acc += a;
         }
@@ -727,7 +789,7 @@ static bool conv_fusable(const FftPlan &p)
 template <typename T>
 void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg, const T *x, T *y,
                            const double *fc, const double *fc_rows, int64_t fc_stride, double *work, int64_t batch,
-                           int64_t nseg_max, const Segment *hseg)
+                           int64_t nseg_max, const Segment *hseg, const float *fc_rows32)
 {
     (void)hseg;
     if (nseg > nseg_max)
@@ -751,13 +813,26 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
         int64_t nb = nseg - s0 < batch ? nseg - s0 : batch;
         dim3 gp(SEG_NB, (unsigned)nb);
         if (fused) {
-            k_conv_col_fwd<T><<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
-                x, dseg, s0, partial + s0 * SEG_NB, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
-            k_conv_rows<<<dim3((unsigned)(p.L1 / 2), (unsigned)nb), FFT_TPB, sizeof(double2) * conv_lds_elems((int)(2 * p.L2)), s>>>(
-                (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
-            k_conv_col_inv<T><<<dim3((unsigned)ncb, (unsigned)nb), CONV_COL_TPB, sizeof(double2) * conv_lds_elems((int)(p.L1 * CONV_NCOL)), s>>>(
-                (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2,
-                p);
+            bool f32 = std::is_same<T, float>::value && fc_rows32 != nullptr;
+            dim3 gc((unsigned)ncb, (unsigned)nb), gr((unsigned)(p.L1 / 2), (unsigned)nb);
+            size_t col_elems = conv_lds_elems((int)(p.L1 * CONV_NCOL)), row_elems = conv_lds_elems((int)(2 * p.L2));
+            if (f32) {
+                // FP32 work: the same passes on float2, half the bytes and LDS
+                float2 *w32 = (float2 *)work;
+                k_conv_col_fwd<T, float2><<<gc, CONV_COL_TPB, sizeof(float2) * col_elems, s>>>(
+                    x, dseg, s0, partial + s0 * SEG_NB, w32, p.N, (int)p.L1, logL1, (int)p.L2, p);
+                k_conv_rows<float2><<<gr, FFT_TPB, sizeof(float2) * row_elems, s>>>(
+                    w32, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows32, fc_stride, dseg, s0);
+                k_conv_col_inv<T, float2><<<gc, CONV_COL_TPB, sizeof(float2) * col_elems, s>>>(
+                    w32, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2, p);
+            } else {
+                k_conv_col_fwd<T, double2><<<gc, CONV_COL_TPB, sizeof(double2) * col_elems, s>>>(
+                    x, dseg, s0, partial + s0 * SEG_NB, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
+                k_conv_rows<double2><<<gr, FFT_TPB, sizeof(double2) * row_elems, s>>>(
+                    (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
+                k_conv_col_inv<T, double2><<<gc, CONV_COL_TPB, sizeof(double2) * col_elems, s>>>(
+                    (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2, p);
+            }
             unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
             if (gl > 1024) gl = 1024;
             k_conv_sub_mean<T><<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
@@ -780,10 +855,10 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
 
 template void fft_convolve_segments<double>(hipStream_t, const FftPlan &, const Segment *, int64_t, const double *,
                                             double *, const double *, const double *, int64_t, double *, int64_t,
-                                            int64_t, const Segment *);
+                                            int64_t, const Segment *, const float *);
 template void fft_convolve_segments<float>(hipStream_t, const FftPlan &, const Segment *, int64_t, const float *,
                                            float *, const double *, const double *, int64_t, double *, int64_t,
-                                           int64_t, const Segment *);
+                                           int64_t, const Segment *, const float *);
 
 size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch, int64_t nseg_max)
 {

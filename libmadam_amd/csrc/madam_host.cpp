@@ -706,7 +706,10 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device, bool
     }
     c.pix_host_dirty = true;
     // pack and build the CSR by map pixel
-    c.d_pixw.alloc(n);
+    // every record array carries REC_PAD zero records past its end: the batched
+    // projections read whole chunks of records without clamping their index
+    c.d_pixw.alloc(n + REC_PAD);
+    MADAM_CHECK(hipMemsetAsync(c.d_pixw.p + n, 0, sizeof(PixW) * REC_PAD, s));
     launch_pack(s, c.d_pix, c.d_w, c.nnz, c.nmap, c.ndet, c.nsamp, c.d_ivs.p, c.nival, c.d_dw_ival.p, c.p.nsubchunk,
                 c.pass, c.d_pixw.p);
     c.d_hits.alloc(c.npix_map);
@@ -720,7 +723,8 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device, bool
     d2h(&total, c.d_off.p + c.npix_map, 1, s);
     MADAM_CHECK(hipStreamSynchronize(s));
     c.nactive = total;
-    c.d_ent.alloc(total > 0 ? total : 1);
+    c.d_ent.alloc(total + REC_PAD);
+    MADAM_CHECK(hipMemsetAsync(c.d_ent.p + total, 0, sizeof(c.d_ent.p[0]) * REC_PAD, s));
     c.d_src.alloc(total > 0 ? total : 1);
     c.d_bord.alloc(std::max<int64_t>(c.nbtot, 1));
     c.d_bslot.alloc(std::max<int64_t>(c.nbtot, 1));
@@ -746,8 +750,10 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device, bool
         c.rec_layout = rl;
         c.stats[ST_REC_BYTES] = rec_bytes(rl);
         if (rl != REC16) {
-            c.d_pixc.alloc((size_t)n * rec_bytes(rl));
-            c.d_entc.alloc((size_t)std::max<int64_t>(total, 1) * rec_bytes(rl));
+            c.d_pixc.alloc((size_t)(n + REC_PAD) * rec_bytes(rl));
+            c.d_entc.alloc((size_t)(total + REC_PAD) * rec_bytes(rl));
+            MADAM_CHECK(hipMemsetAsync(c.d_pixc.p + (size_t)n * rec_bytes(rl), 0, (size_t)REC_PAD * rec_bytes(rl), s));
+            MADAM_CHECK(hipMemsetAsync(c.d_entc.p + (size_t)total * rec_bytes(rl), 0, (size_t)REC_PAD * rec_bytes(rl), s));
             launch_compact_records(s, c.d_pixw.p, n, rl, c.d_pixc.p);
             launch_compact_records(s, (const PixW *)c.d_ent.p, total, rl, c.d_entc.p);
             MADAM_CHECK(hipGetLastError());

@@ -36,6 +36,8 @@ struct Rec12 {
     float w1, w2;
 };
 inline int rec_bytes(int rl) { return rl == REC16 ? 16 : (rl == REC12 ? 12 : 4); }
+// zero records after the end of every record array (pixel 0, weights 0)
+static constexpr int REC_PAD = 128;
 // number of active records (index >= 0) whose w0 is not exactly 1; *count must start at 0
 void launch_count_nonunit_w0(hipStream_t s, const PixW *rec, int64_t n, unsigned long long *count);
 // compact copy of 16-byte records (PixW or CsrEnt) in layout rl
@@ -253,17 +255,22 @@ struct FftPlan {
     int64_t L1, L2;  // N = L1 * L2, column pass length L1, row pass length L2
     double *tw_mem;
     const double2 *sub1, *sub2, *na, *nb, *fa, *fb;
+    float *tw32_mem;   // the same tables rounded to FP32 (the FP32-work convolution)
+    const float2 *sub1f, *sub2f, *naf, *nbf, *faf, *fbf;
 };
 
 void fft_plan_init(FftPlan &p, int64_t nof);
 void fft_plan_free(FftPlan &p);
 // convolve every segment: y = x outside segments is the caller's job (copy first)
 // fc: filter spectra, nrow x fc_stride; fc_rows: the same in row order (fft_spectrum_rows)
-// T (float or double) is the type of x and y; the transforms run in FP64
+// T (float or double) is the type of x and y; the transforms run in FP64,
+// except for T = float with fc_rows32 (the row-order spectra in FP32) given:
+// then the fused passes work in FP32 (the operator of the mixed-precision
+// inner solve; segment means stay FP64)
 template <typename T>
 void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg, int64_t nseg_dev, const T *x, T *y,
                            const double *fc, const double *fc_rows, int64_t fc_stride, double *work, int64_t batch,
-                           int64_t nseg_max, const Segment *hseg);
+                           int64_t nseg_max, const Segment *hseg, const float *fc_rows32 = nullptr);
 // fT = fc reordered for the fused row pass: f[r + L1 k2] at [r * L2 + k2], f[N] at [N]
 void fft_spectrum_rows(hipStream_t s, const FftPlan &p, const double *fc, int64_t nrow, int64_t stride, double *fT);
 // nseg_max: the most segments one fft_convolve_segments call takes

@@ -107,17 +107,21 @@ def test_fp64_batch_graph_replay_bitwise():
     results of the same launches issued eagerly."""
     case = ring(prior=True, nsamp=30000, nperiod=2)
     sig = signals_for(case, 3)
-    old = os.environ.get("MADAM_BATCH_GRAPH")
+    old = {k: os.environ.get(k) for k in ("MADAM_BATCH_GRAPH", "MADAM_BS_PIECE")}
     try:
+        # the two solves run on separate contexts: pin the pixel-side cut,
+        # which the autotuner may otherwise choose differently for each
+        os.environ["MADAM_BS_PIECE"] = "256"
         os.environ["MADAM_BATCH_GRAPH"] = "1"
         g, _ = batch(case, sig, 0)
         os.environ["MADAM_BATCH_GRAPH"] = "0"
         e, _ = batch(case, sig, 0)
     finally:
-        if old is None:
-            os.environ.pop("MADAM_BATCH_GRAPH", None)
-        else:
-            os.environ["MADAM_BATCH_GRAPH"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     for k in range(3):
         assert g[k]["info"]["steps"] == e[k]["info"]["steps"]
         for name in ("map", "aa", "binmap"):
