@@ -439,6 +439,46 @@ static void fft_inverse(hipStream_t s, const FftPlan &p, double2 *d, int64_t bat
 // early stages do not collide on the same banks.
 __device__ inline int lp(int idx) { return idx + (idx >> 3); }
 
+// The twiddles of one work item of the fused-stage FFTs, from one table read:
+// with b = W^(j / (h0 E)) (E = 2^T points per item, W = exp(-2 pi i), conjugated
+// for the inverse), stage tt (half-length h0 2^tt) and butterfly r < 2^tt use
+// W^((j + r h0) / (h0 2^(tt + 1))) = b^(2^(T - 1 - tt)) * W^(r / 2^(tt + 1)),
+// the powers of b by repeated squaring and the second factors exact constants.
+template <typename C>
+__device__ inline C rot_const(C x, int num, int den, bool inv)
+{
+    // x * exp(-+ 2 pi i num / den), den in {2, 4, 8}, 0 <= num < den / 2
+    using S = typename CxT<C>::S;
+    const S h = (S)0.70710678118654752440;
+    int k = num * (8 / den);   // in eighths of a turn: 0..3
+    S sg = inv ? (S)1 : (S)-1;   // sign of the angle
+    switch (k) {
+    case 0: return x;
+    case 1: {   // (h, sg h)
+        return CxT<C>::mk(h * (x.x - sg * x.y), h * (x.y + sg * x.x));
+    }
+    case 2:     // sg i
+        return CxT<C>::mk(-sg * x.y, sg * x.x);
+    default: {  // (-h, sg h)
+        return CxT<C>::mk(-h * (x.x + sg * x.y), h * (sg * x.x - x.y));
+    }
+    }
+}
+
+template <typename C>
+__device__ inline void item_twiddles(C b, int T, bool inv, C w[3][4])
+{
+    if (inv) b.y = -b.y;
+    C bt = b;
+#pragma unroll
+    for (int tt = 2; tt >= 0; --tt) {
+        if (tt >= T) continue;
+#pragma unroll
+        for (int r = 0; r < (1 << tt); ++r) w[tt][r] = rot_const(bt, r, 2 << tt, inv);
+        bt = cmul(bt, bt);
+    }
+}
+
 // DIT, bit-reversed input order, natural output order; twL[j] = W_L^j (j < L / 2).
 // 2^logncol interleaved transforms; every length is a power of two, so the
 // index arithmetic is shifts and masks.
@@ -458,6 +498,8 @@ __device__ void lds_fft8(C *s, int L, int logL, int logncol, const C *twL, bool 
 #pragma unroll
             for (int m = 0; m < 8; ++m)
                 if (m < E) a[m] = s[lp((base + m * h0) * ncol + col)];
+            C w[3][4];
+            item_twiddles(twL[j << (logL - st - T)], T, inv, w);
 #pragma unroll
             for (int tt = 0; tt < 3; ++tt) {
                 if (tt >= T) break;
@@ -465,9 +507,7 @@ __device__ void lds_fft8(C *s, int L, int logL, int logncol, const C *twL, bool 
                 for (int m0 = 0; m0 < 8; ++m0) {
                     if (m0 >= E || (m0 & (1 << tt))) continue;
                     int r = m0 & ((1 << tt) - 1);
-                    C w = twL[(j + r * h0) << (logL - st - tt - 1)];   // * L / (2 h)
-                    if (inv) w.y = -w.y;
-                    C b = cmul(a[m0 + (1 << tt)], w), x = a[m0];
+                    C b = cmul(a[m0 + (1 << tt)], w[tt][r]), x = a[m0];
                     a[m0] = CxT<C>::mk(x.x + b.x, x.y + b.y);
                     a[m0 + (1 << tt)] = CxT<C>::mk(x.x - b.x, x.y - b.y);
                 }
@@ -498,6 +538,8 @@ __device__ void lds_fft8_dif(C *s, int L, int logL, int logncol, const C *twL, b
 #pragma unroll
             for (int m = 0; m < 8; ++m)
                 if (m < E) a[m] = s[lp((base + m * h0) * ncol + col)];
+            C w[3][4];
+            item_twiddles(twL[j << (logL - lo - T)], T, inv, w);
 #pragma unroll
             for (int tt = 2; tt >= 0; --tt) {
                 if (tt >= T) continue;
@@ -505,11 +547,9 @@ __device__ void lds_fft8_dif(C *s, int L, int logL, int logncol, const C *twL, b
                 for (int m0 = 0; m0 < 8; ++m0) {
                     if (m0 >= E || (m0 & (1 << tt))) continue;
                     int r = m0 & ((1 << tt) - 1);
-                    C w = twL[(j + r * h0) << (logL - lo - tt - 1)];   // * L / (2 h)
-                    if (inv) w.y = -w.y;
                     C x = a[m0], y = a[m0 + (1 << tt)];
                     a[m0] = CxT<C>::mk(x.x + y.x, x.y + y.y);
-                    a[m0 + (1 << tt)] = cmul(CxT<C>::mk(x.x - y.x, x.y - y.y), w);
+                    a[m0 + (1 << tt)] = cmul(CxT<C>::mk(x.x - y.x, x.y - y.y), w[tt][r]);
                 }
             }
 #pragma unroll
@@ -539,6 +579,7 @@ __host__ __device__ inline int conv_lds_elems(int n) { return n + (n >> 3) + 1; 
 // The column passes hold L1 * CONV_NCOL points (74 KB of LDS at L1 = 512), so
 // only two blocks fit on a CU: they run 512 threads, one 8-point work item each.
 static constexpr int CONV_COL_TPB = 512;
+static_assert(CONV_COL_TPB % CONV_NCOL == 0, "a column pass thread keeps one column");
 
 // grid = (L2 / CONV_NCOL, batch)
 // (C: the work type, double2 or float2; T: the type of x)
@@ -578,11 +619,17 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const T *x, const
     }
     __syncthreads();
     lds_fft8<C>(s, L1, logL1, 3, CxT<C>::sub1(P), false);   // CONV_NCOL = 8 columns
-    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-        int col = t % ncol, k1 = t / ncol;
+    // W_N^(n2 k1): a thread keeps its column and steps k1 by blockDim / ncol,
+    // the twiddle by the matching power (two table reads per thread)
+    {
+        int col = threadIdx.x % ncol, kst = blockDim.x / ncol;
         int64_t n2 = c0 + col;
-        C w = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * k1) & (N - 1))   /* N is a power of two */;
-        d[(int64_t)k1 * L2 + n2] = cmul(s[lp(k1 * ncol + col)], w);
+        C w = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * (int64_t)(threadIdx.x / ncol)) & (N - 1));
+        C ws = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * kst) & (N - 1));   /* N is a power of two */
+        for (int k1 = threadIdx.x / ncol; k1 < L1; k1 += kst) {
+            d[(int64_t)k1 * L2 + n2] = cmul(s[lp(k1 * ncol + col)], w);
+            w = cmul(w, ws);
+        }
     }
 }
 
@@ -699,11 +746,15 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const C *work, co
     const C *d = work + b * N;
     double inv = 1 / (double)(2 * N);
     int tot = L1 * ncol;
-    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-        int col = t % ncol, k1 = t / ncol;
+    {
+        int col = threadIdx.x % ncol, kst = blockDim.x / ncol;
         int64_t n2 = c0 + col;
-        C w = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * k1) & (N - 1))   /* N is a power of two */;
-        s[lp(bitrev(k1, logL1) * ncol + col)] = cmulc(d[(int64_t)k1 * L2 + n2], w);
+        C w = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * (int64_t)(threadIdx.x / ncol)) & (N - 1));
+        C ws = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * kst) & (N - 1));   /* N is a power of two */
+        for (int k1 = threadIdx.x / ncol; k1 < L1; k1 += kst) {
+            s[lp(bitrev(k1, logL1) * ncol + col)] = cmulc(d[(int64_t)k1 * L2 + n2], w);
+            w = cmul(w, ws);
+        }
     }
     __syncthreads();
     lds_fft8<C>(s, L1, logL1, 3, CxT<C>::sub1(P), true);
