@@ -292,13 +292,10 @@ __device__ inline const double *li_chunk(const double *Li, const int32_t *bmap, 
 
 // Vectors of the sweeps in the factor's interleaved layout: row i of chunk c
 // at V[((c / 64) * CH + i) * 64 + c % 64].
-__device__ inline const double *vi_chunk(const double *V, int64_t c)
+template <typename V>
+__device__ inline V *vi_chunk(V *v, int64_t c)
 {
-    return V + (c >> 6) * (int64_t)BAND_CH * 64 + (c & 63);
-}
-__device__ inline double *vi_chunk(double *V, int64_t c)
-{
-    return V + (c >> 6) * (int64_t)BAND_CH * 64 + (c & 63);
+    return v + (c >> 6) * (int64_t)BAND_CH * 64 + (c & 63);
 }
 
 // To and from the interleaved layout, one 64-chunk block per workgroup through
@@ -310,13 +307,13 @@ struct BandRhs {
     int64_t v, s, g;
 };
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_t nch, const T *v, double *vi,
+template <typename T, typename V>
+__global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_t nch, const T *v, V *vi,
                                                      int64_t vstride, BandRhs st)
 {
     v += blockIdx.y * vstride;
     vi += blockIdx.y * st.v;
-    __shared__ double tile[64 * BAND_TP];
+    __shared__ V tile[64 * BAND_TP];
     __shared__ int64_t b0s[64];
     __shared__ int ns[64];
     int64_t w = blockIdx.x;
@@ -328,7 +325,7 @@ __global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_
     __syncthreads();
     for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
         int j = t / BAND_CH, i = t % BAND_CH;   // chunk j of the block, row i
-        tile[j * BAND_TP + i] = i < ns[j] ? (double)v[b0s[j] + i] : 0.0;
+        tile[j * BAND_TP + i] = i < ns[j] ? (V)v[b0s[j] + i] : (V)0;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
@@ -337,13 +334,13 @@ __global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int64_t nch, const double *vi, T *v,
+template <typename T, typename V>
+__global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int64_t nch, const V *vi, T *v,
                                                        int64_t vstride, BandRhs st)
 {
     v += blockIdx.y * vstride;
     vi += blockIdx.y * st.v;
-    __shared__ double tile[64 * BAND_TP];
+    __shared__ V tile[64 * BAND_TP];
     __shared__ int64_t b0s[64];
     __shared__ int ns[64];
     int64_t w = blockIdx.x;
@@ -425,8 +422,8 @@ struct BandPF {
 // factor, MS = 1 for a block's compact copy read with scalar loads; r and
 // y are the chunk's columns of vectors in the same interleaved layout (row i
 // at [i * 64]), so the lanes of a wave read and write consecutive addresses.
-template <int KD, int MS = 64>
-__device__ inline void fwd_sweep(const double *lic, int n, const double *r, double *acc, double *y, bool store,
+template <int KD, int MS = 64, typename V = double>
+__device__ inline void fwd_sweep(const double *lic, int n, const V *r, double *acc, V *y, bool store,
                                  bool zero_rhs)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
@@ -439,7 +436,7 @@ __device__ inline void fwd_sweep(const double *lic, int n, const double *r, doub
                 const double *row = lic + (int64_t)i * K * MS;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * MS];
-                rh[u] = zero_rhs ? 0.0 : r[(int64_t)i * 64];
+                rh[u] = zero_rhs ? 0.0 : (double)r[(int64_t)i * 64];
             }
         }
 #pragma unroll
@@ -450,15 +447,15 @@ __device__ inline void fwd_sweep(const double *lic, int n, const double *r, doub
 #pragma unroll
                 for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - rw[u][m] * v;
                 acc[KD - 1] = -rw[u][KD] * v;
-                if (store) y[(int64_t)i * 64] = v;
+                if (store) y[(int64_t)i * 64] = (V)v;
             }
         }
     }
 }
 
 // backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
-template <int KD, int MS = 64>
-__device__ inline void bwd_sweep(const double *lic, int n, const double *y, double *xs, double *x, bool store,
+template <int KD, int MS = 64, typename V = double>
+__device__ inline void bwd_sweep(const double *lic, int n, const V *y, double *xs, V *x, bool store,
                                  bool zero_rhs)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
@@ -471,7 +468,7 @@ __device__ inline void bwd_sweep(const double *lic, int n, const double *y, doub
                 const double *row = lic + (int64_t)i * K * MS;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * MS];
-                rh[u] = zero_rhs ? 0.0 : y[(int64_t)i * 64];
+                rh[u] = zero_rhs ? 0.0 : (double)y[(int64_t)i * 64];
             }
         }
 #pragma unroll
@@ -485,7 +482,7 @@ __device__ inline void bwd_sweep(const double *lic, int n, const double *y, doub
 #pragma unroll
                 for (int m = KD - 1; m > 0; --m) xs[m] = xs[m - 1];
                 xs[0] = xi;
-                if (store) x[(int64_t)i * 64] = xi;
+                if (store) x[(int64_t)i * 64] = (V)xi;
             }
         }
     }
@@ -504,20 +501,20 @@ __global__ void __launch_bounds__(64) k_band_response(const ChunkInfo *ch, int64
     double st[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    fwd_sweep<KD>(li_chunk(Lb, nullptr, c, KD + 1), ci.n, nullptr, st, nullptr, false, true);
+    fwd_sweep<KD, 64, double>(li_chunk(Lb, nullptr, c, KD + 1), ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tf[(c * KD + m) * KD + j] = st[m];
 #pragma unroll
     for (int m = 0; m < KD; ++m) st[m] = (m == j) ? 1.0 : 0.0;
-    bwd_sweep<KD>(li_chunk(Lb, nullptr, c, KD + 1), ci.n, nullptr, st, nullptr, false, true);
+    bwd_sweep<KD, 64, double>(li_chunk(Lb, nullptr, c, KD + 1), ci.n, nullptr, st, nullptr, false, true);
 #pragma unroll
     for (int m = 0; m < KD; ++m) Tb[(c * KD + m) * KD + j] = st[m];
 }
 
-template <int KD>
+template <int KD, typename V>
 __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                   const double *Lb, const int32_t *uidx,
-                                                  const double *__restrict__ Lu, const double *r, double *s0,
+                                                  const double *__restrict__ Lu, const V *r, double *s0,
                                                   BandRhs st)
 {
     r += blockIdx.y * st.v;
@@ -530,10 +527,10 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
     for (int m = 0; m < KD; ++m) acc[m] = 0;
     int32_t ui = uidx[blockIdx.x];
     if (ui >= 0)
-        fwd_sweep<KD, 1>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ci.n), vi_chunk(r, c),
-                         acc, nullptr, false, false);
+        fwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ci.n),
+                            vi_chunk(r, c), acc, (V *)nullptr, false, false);
     else
-        fwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(r, c), acc, nullptr, false, false);
+        fwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(r, c), acc, (V *)nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
 }
@@ -692,11 +689,11 @@ __global__ void __launch_bounds__(64) k_band_group_mats(const int64_t *gfirst, c
 
 // final forward sweep with true carries writes y into z, then the zero-carry
 // backward sweep of the same chunk gives its backward partial state
-template <int KD>
+template <int KD, typename V>
 __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                        const double *Lb, const int32_t *uidx,
-                                                       const double *__restrict__ Lu, const double *r,
-                                                       const double *sinf, double *z, double *s0b, BandRhs st)
+                                                       const double *__restrict__ Lu, const V *r,
+                                                       const double *sinf, V *z, double *s0b, BandRhs st)
 {
     r += blockIdx.y * st.v;
     z += blockIdx.y * st.v;
@@ -712,25 +709,25 @@ __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int6
     if (ui >= 0) {
         const double *lu = Lu + (int64_t)ui * BAND_CH * (KD + 1);
         int n = __builtin_amdgcn_readfirstlane(ci.n);
-        fwd_sweep<KD, 1>(lu, n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
+        fwd_sweep<KD, 1, V>(lu, n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
 #pragma unroll
         for (int m = 0; m < KD; ++m) acc[m] = 0;
-        bwd_sweep<KD, 1>(lu, n, vi_chunk(z, c), acc, nullptr, false, false);
+        bwd_sweep<KD, 1, V>(lu, n, (const V *)vi_chunk(z, c), acc, (V *)nullptr, false, false);
     } else {
         const double *lic = li_chunk(Lb, bmap, c, KD + 1);
-        fwd_sweep<KD>(lic, ci.n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
+        fwd_sweep<KD, 64, V>(lic, ci.n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
 #pragma unroll
         for (int m = 0; m < KD; ++m) acc[m] = 0;
-        bwd_sweep<KD>(lic, ci.n, vi_chunk(z, c), acc, nullptr, false, false);
+        bwd_sweep<KD, 64, V>(lic, ci.n, (const V *)vi_chunk(z, c), acc, (V *)nullptr, false, false);
     }
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
 }
 
-template <int KD>
+template <int KD, typename V>
 __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                   const double *Lb, const int32_t *uidx,
-                                                  const double *__restrict__ Lu, const double *sinb, double *z,
+                                                  const double *__restrict__ Lu, const double *sinb, V *z,
                                                   BandRhs st)
 {
     sinb += blockIdx.y * st.s;
@@ -743,10 +740,11 @@ __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t n
     for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
     int32_t ui = uidx[blockIdx.x];
     if (ui >= 0)
-        bwd_sweep<KD, 1>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ci.n), vi_chunk(z, c),
-                         xs, vi_chunk(z, c), true, false);
+        bwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ci.n),
+                            (const V *)vi_chunk(z, c), xs, vi_chunk(z, c), true, false);
     else
-        bwd_sweep<KD>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(z, c), xs, vi_chunk(z, c), true, false);
+        bwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), ci.n, (const V *)vi_chunk(z, c), xs, vi_chunk(z, c), true,
+                             false);
 }
 
 // ---------------------------------------------------------------- host side
@@ -1040,23 +1038,27 @@ void band_apply(hipStream_t s, const BandPrec &bp, const T *r, T *z, int nrhs, i
     dim3 gs((unsigned)im->nseg_used, R);   // one 64-lane wave per segment
     dim3 gg((unsigned)im->ngrp, R);        // one wave per chunk group
     BandRhs st{im->NCP * BAND_CH, bp.nchunk * KD, (im->ngrp > 0 ? im->ngrp : 1) * KD};
+    // the interleaved vectors in the type of r and z (FP32 halves their
+    // traffic); the sweeps compute in FP64
+    using V = T;
+    V *ri = (V *)im->ri, *zi = (V *)im->zi;
     BAND_DISPATCH(KD, ({
-        k_band_to_int<T><<<g, 256, 0, s>>>(im->ch, bp.nchunk, r, im->ri, vstride, st);
-        k_band_fwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->ri, im->s0f, st);
+        k_band_to_int<T, V><<<g, 256, 0, s>>>(im->ch, bp.nchunk, r, ri, vstride, st);
+        k_band_fwd0<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, ri, im->s0f, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, nullptr,
                                                  nullptr, im->cg, st);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, im->gmap, 0, im->Sg, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, im->Sg,
                                                  im->sinf, nullptr, st);
-        k_band_fwd1_bwd0<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->ri,
-                                              im->sinf, im->zi, im->s0b, st);
+        k_band_fwd1_bwd0<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, ri, im->sinf,
+                                                 zi, im->s0b, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, nullptr,
                                                  nullptr, im->cg, st);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, im->gmap, 1, im->Sg, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, im->Sg,
                                                  im->sinb, nullptr, st);
-        k_band_bwd1<KD><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->sinb, im->zi, st);
-        k_band_from_int<T><<<g, 256, 0, s>>>(im->ch, bp.nchunk, im->zi, z, vstride, st);
+        k_band_bwd1<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->sinb, zi, st);
+        k_band_from_int<T, V><<<g, 256, 0, s>>>(im->ch, bp.nchunk, zi, z, vstride, st);
     }));
 }
 
