@@ -123,12 +123,13 @@ __device__ inline void bs_finish(const T *cc, T a0, T a1, T a2, T *mq)
     }
 }
 
-// The entries are taken in chunks of CH = U * G.  A chunk's records are
-// wave-uniform (scalar loads, one copy per wave rather than one per lane);
-// each lane takes its entries' values from them, and the next chunk's records
-// are loaded while the current chunk's baseline values are in flight.  Record
-// reads run past the item's end (the arrays carry zero records past their
-// end, and every record holds a real slot), and those entries weigh 0.
+// The entries are taken in chunks of CH = U * G, the next chunk's records in
+// flight with the current chunk's baseline values.  R >= 8: the chunk's
+// records go through the wave's LDS slots (one record per lane in, broadcast
+// reads out), with the weights of entries past the item's end zeroed there,
+// so that the per-entry selects and masks stay off the VALU (the kernel is
+// bound by VALU issue).  Record reads run past the item's end (the arrays
+// carry zero records past their end, and every record holds a real slot).
 template <typename T, int R, int RL, int NMAP>
 __global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *__restrict__ items, int64_t nitems,
                                                    const void *__restrict__ ent, const T *__restrict__ pw,
@@ -138,7 +139,6 @@ __global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *__restrict__ it
     constexpr int U = R >= 64 ? 8 : (R >= 8 ? 16 / G : 4);
     constexpr int CH = U * G;
     constexpr int RB = RL == REC16 ? 16 : (RL == REC12 ? 12 : 4);
-    constexpr int ND = CH * RB / 4;
     int lane = threadIdx.x & 63;
     int r = lane % R, g = lane / R;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -149,31 +149,6 @@ __global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *__restrict__ it
     if (in.n > 0) {
         const char *ente = (const char *)ent + (size_t)in.e0 * RB;
         const int n = in.n;
-        uint32_t cw[ND];
-        auto load_chunk = [&](int jb) {
-            const uint32_t *w = (const uint32_t *)(ente + (size_t)jb * RB);
-#pragma unroll
-            for (int k = 0; k < ND; ++k) cw[k] = w[k];
-        };
-        auto rec_at = [&](int k) -> PixW {
-            PixW q;
-            if constexpr (RL == REC16) {
-                q.pix = (int32_t)cw[4 * k];
-                q.w0 = __uint_as_float(cw[4 * k + 1]);
-                q.w1 = __uint_as_float(cw[4 * k + 2]);
-                q.w2 = __uint_as_float(cw[4 * k + 3]);
-            } else if constexpr (RL == REC12) {
-                q.pix = (int32_t)cw[3 * k];
-                q.w0 = 1.0f;
-                q.w1 = __uint_as_float(cw[3 * k + 1]);
-                q.w2 = __uint_as_float(cw[3 * k + 2]);
-            } else {
-                q.pix = (int32_t)cw[k];
-                q.w0 = 1.0f;
-                q.w1 = q.w2 = 0.0f;
-            }
-            return q;
-        };
         if constexpr (R < 8) {
             // narrow rows: one record per lane per entry, the next chunk's
             // records in flight with the current baseline values
@@ -208,34 +183,45 @@ __global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *__restrict__ it
                 if (!step(eb, ea, j0 + CH)) break;
             }
         } else {
-        load_chunk(0);
-        for (int j0 = 0;; j0 += CH) {
-            bool more = j0 + CH < n;
-            PixW e[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                e[u] = rec_at(u * G);
-#pragma unroll
-                for (int gg = 1; gg < G; ++gg)
-                    if (g == gg) e[u] = rec_at(u * G + gg);
-            }
-            T v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = pw[(int64_t)e[u].pix * R + r];
-            if (more) load_chunk(j0 + CH);   // in flight with the baseline values
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                T fv = (j0 + u * G + g < n) ? v[u] : (T)0;
-                if (NMAP == 1) {
-                    a0 += fv;
-                } else {
-                    a0 += (T)e[u].w0 * fv;
-                    a1 += (T)e[u].w1 * fv;
-                    a2 += (T)e[u].w2 * fv;
+            // lane j < CH loads entry record j of the chunk (one coalesced
+            // load, one chunk ahead) and writes it to the wave's LDS slot j
+            // with the weights zeroed past the item's end; lane (g, r) reads
+            // the records of its entries u * G + g (broadcast reads)
+            __shared__ int4 srec[BTPB / 64][CH];
+            int4 *rl = srec[wib];
+            PixW rr{0, 0.0f, 0.0f, 0.0f};
+            if (lane < CH) rr = load_rec<RL>(ente, (int64_t)(uint32_t)lane);
+            for (int j0 = 0;; j0 += CH) {
+                bool more = j0 + CH < n;
+                if (lane < CH) {
+                    bool ok = j0 + lane < n;
+                    rl[lane] = make_int4(ok ? rr.pix : 0, __float_as_int(ok ? rr.w0 : 0.0f),
+                                         __float_as_int(ok ? rr.w1 : 0.0f), __float_as_int(ok ? rr.w2 : 0.0f));
                 }
+                __builtin_amdgcn_wave_barrier();
+                PixW e[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    int4 q = rl[u * G + g];
+                    e[u].pix = q.x;
+                    e[u].w0 = __int_as_float(q.y);
+                    e[u].w1 = __int_as_float(q.z);
+                    e[u].w2 = __int_as_float(q.w);
+                }
+                T v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) v[u] = pw[(int64_t)e[u].pix * R + r];
+                if (more && lane < CH) rr = load_rec<RL>(ente, (int64_t)(uint32_t)(j0 + CH + lane));   // in flight
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    a0 += (T)e[u].w0 * v[u];
+                    if (NMAP == 3) {
+                        a1 += (T)e[u].w1 * v[u];
+                        a2 += (T)e[u].w2 * v[u];
+                    }
+                }
+                if (!more) break;
             }
-            if (!more) break;
-        }
         }
     }
 #pragma unroll
@@ -472,34 +458,15 @@ __global__ void __launch_bounds__(BTPB) k_bgather_w(int ndet, int64_t nsamp, int
     int32_t ib = __builtin_amdgcn_readlane(bsl, 0), ie = __builtin_amdgcn_readlane(bsl, 1);
     int32_t i = ib;
     T acc = 0;
-    // the chunk's records (uniform), read as CH * RB / 4 contiguous dwords
-    constexpr int ND = CH * RB / 4;
-    uint32_t cw[ND];
-    auto load_chunk = [&](int32_t i0) {
-        const uint32_t *w = (const uint32_t *)(recd + (size_t)(uint32_t)i0 * RB);
-#pragma unroll
-        for (int k = 0; k < ND; ++k) cw[k] = w[k];
-    };
-    auto rec_at = [&](int k) -> PixW {
-        PixW q;
-        if constexpr (RL == REC16) {
-            q.pix = (int32_t)cw[4 * k];
-            q.w0 = __uint_as_float(cw[4 * k + 1]);
-            q.w1 = __uint_as_float(cw[4 * k + 2]);
-            q.w2 = __uint_as_float(cw[4 * k + 3]);
-        } else if constexpr (RL == REC12) {
-            q.pix = (int32_t)cw[3 * k];
-            q.w0 = 1.0f;
-            q.w1 = __uint_as_float(cw[3 * k + 1]);
-            q.w2 = __uint_as_float(cw[3 * k + 2]);
-        } else {
-            q.pix = (int32_t)cw[k];
-            q.w0 = 1.0f;
-            q.w1 = q.w2 = 0.0f;
-        }
-        return q;
-    };
-    load_chunk(i);
+    // The chunk's records: lane j < CH loads record j (one coalesced load,
+    // one chunk ahead) and writes it to the wave's LDS slot j as
+    // {pix, w0, w1, w2} with the pixel and weights zeroed for samples past the
+    // baseline's end and flagged ones; lane (g, r) then reads the records of
+    // its samples u * G + g (broadcast reads).  The per-sample selects and
+    // masks stay off the VALU.
+    int4 *rl = reinterpret_cast<int4 *>(reinterpret_cast<T *>(bg_lds) + (BTPB / 64) * run_len * (R + 1)) + wib * CH;
+    PixW rr{0, 0.0f, 0.0f, 0.0f};
+    if (lane < CH) rr = load_rec<RL>(recd, (int64_t)(uint32_t)(i + lane));
     for (;;) {
         bool last = i + CH >= ie;
         int nkc = kc;
@@ -511,28 +478,39 @@ __global__ void __launch_bounds__(BTPB) k_bgather_w(int ndet, int64_t nsamp, int
             ni = nib;
         }
         bool more = nkc < nk;
-        // this lane's samples u * G + g of the chunk
+        if (lane < CH) {
+            bool ok = i + lane < ie && rr.pix >= 0;
+            rl[lane] = make_int4(ok ? rr.pix : 0, __float_as_int(ok ? rr.w0 : 0.0f), __float_as_int(ok ? rr.w1 : 0.0f),
+                                 __float_as_int(ok ? rr.w2 : 0.0f));
+        }
+        __builtin_amdgcn_wave_barrier();
         PixW e[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            e[u] = rec_at(u * G);
-#pragma unroll
-            for (int gg = 1; gg < G; ++gg)
-                if (g == gg) e[u] = rec_at(u * G + gg);
+            int4 q = rl[u * G + g];
+            e[u].pix = q.x;
+            e[u].w0 = __int_as_float(q.y);
+            e[u].w1 = __int_as_float(q.z);
+            e[u].w2 = __int_as_float(q.w);
         }
         T mv[U][NMAP];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const T *mq = m + (int64_t)((e[u].pix >= 0 ? e[u].pix : 0) >> dsh) * NMAP * R + r;
+            const T *mq = m + (int64_t)(e[u].pix >> dsh) * NMAP * R + r;
 #pragma unroll
             for (int c = 0; c < NMAP; ++c) mv[u][c] = mq[c * R];
         }
-        if (more) load_chunk(ni);   // in flight with the map values
+        // the next chunk's records, in flight with the map values
+        if (more && lane < CH) rr = load_rec<RL>(recd, (int64_t)(uint32_t)(ni + lane));
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            T f = (i + u * G + g < ie && e[u].pix >= 0) ? (T)1 : (T)0;
-            T v = NMAP == 1 ? mv[u][0] : (T)e[u].w0 * mv[u][0] + (T)e[u].w1 * mv[u][1] + (T)e[u].w2 * mv[u][2];
-            acc += f * v;
+            if (NMAP == 1) {
+                acc += (T)e[u].w0 * mv[u][0];
+            } else {
+                acc += (T)e[u].w0 * mv[u][0];
+                acc += (T)e[u].w1 * mv[u][1];
+                acc += (T)e[u].w2 * mv[u][2];
+            }
         }
         if (last) {
 #pragma unroll
@@ -600,7 +578,7 @@ void batch_gather(hipStream_t s, int R, int rl, int nmap, int ndet, int64_t nsam
         }
 #define BGW_LAUNCH(RR, UU)                                                                                           \
     do {                                                                                                             \
-        size_t lds = sizeof(T) * (BTPB / 64) * run_len * (RR + 1);                                                   \
+        size_t lds = sizeof(T) * (BTPB / 64) * run_len * (RR + 1) + (BTPB / 64) * 16 * sizeof(int4);                 \
         if (nmap == 1)                                                                                               \
             k_bgather_w<T, RR, REC4, 1, UU><<<g, BTPB, lds, s>>>(ndet, nsamp, nbase, run_len, ndg, bstart, rec, m, dsh, \
                                                               dwb, nna, p, ap, active);                                   \
