@@ -114,12 +114,16 @@ def rel_err(a, b):
 # ------------------------------------------------------------------ algorithmic bytes of the projections
 
 
-def bytes_fp64(st, ndet, nsamp, npix, nbtot):
-    """the FP64 path's projections (k_gather_flat, k_sr_scatter): per launch"""
+def bytes_fp64(st, ndet, nsamp, npix, nbtot, s=8):
+    """the one-RHS projections (k_gather_flat, k_sr_scatter) in value size s (8: FP64, 4: the mixed
+    solve's FP32 operator), per launch: one record per sample / CSR entry, per hit pixel the map
+    (3 values, read once) and on the pixel side the head / ordinal bits (8 B), the inverse pixel
+    matrix (6 values) and the map written (4 values), per baseline nna, p, dwb, ap r/w (5 values)
+    and pw (1 value)"""
     rec = st.get("rec_bytes") or 16
     nact = st["nactive"]
-    return dict(gather=ndet * nsamp * rec + npix * 24 + nbtot * 5 * 8,
-                scatter=nact * rec + nbtot * 8 + npix * (8 + 48 + 32))
+    return dict(gather=ndet * nsamp * rec + npix * 3 * s + nbtot * 5 * s,
+                scatter=nact * rec + nbtot * s + npix * (8 + 6 * s + 4 * s))
 
 
 def bytes_batch(st, ndet, nsamp, npix_hit, nbtot, R, s):
@@ -227,8 +231,9 @@ def run_cfg3(a):
     npix_hit = int(st["npix_hit"]) if st.get("npix_hit") else npix
     nprof = max(prof["n_prof"], 1)
     times = dict(scatter=prof["t_scatter"] / nprof, gather=prof["t_gather"] / nprof)
-    roof, kern = roofline_from(times, bytes_batch(st, ndet, nsamp, npix, nbtot, 1, 4),
-                               dict(scatter="k_bscatter<float, 1, 1, 3>", gather="k_bgather<float, 1, 1, 3>"))
+    # one right-hand side runs the FP32 instantiations of the FP64 path's kernels
+    roof, kern = roofline_from(times, bytes_fp64(st, ndet, nsamp, npix_hit, nbtot, 4),
+                               dict(scatter="k_sr_scatter<float, 3, 4, 1>", gather="k_gather_flat<float, 3, 1>"))
     value = N / t_mixed
     line = {
         "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -296,8 +301,9 @@ def run_cfg2(a):
     nbtot = ndet * ((nsamp // nperiod - 1) // int(round(base * fs)) + 1) * nperiod
     ng = max(st["n_gather"], 1)
     times = dict(gather=st["t_gather"] / ng, scatter=st["t_scatter"] / ng)
-    roof, kern = roofline_from(times, bytes_fp64(st, ndet, nsamp, npix, nbtot),
-                               dict(gather="gather_baselines", scatter="scatter_pixels"))
+    npix_hit = int(st["npix_hit"]) if st.get("npix_hit") else npix
+    roof, kern = roofline_from(times, bytes_fp64(st, ndet, nsamp, npix_hit, nbtot),
+                               dict(gather="k_gather_flat<double, 3, 1>", scatter="k_sr_scatter<double, 3, 4, 1>"))
     lat_s = sorted(lat[1:] or lat)
     return {
         "metric": METRIC, "value": N / (ms_step * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": a.steps,
@@ -360,6 +366,15 @@ def run_cfg4(a):
         m.set_profiling(False)
         nprof = max(prof["n_prof"], 1)
         brk = {k: prof[k] / nprof * 1e3 for k in ("t_scatter", "t_prior", "t_gather", "t_prec", "t_step")}
+        if gi == 0:
+            # the roofline of the largest subset's projections
+            st = m.stats()
+            nbt = nd * ((ns // nperiod - 1) // int(round(base * fs)) + 1) * nperiod
+            nph = int(st["npix_hit"]) if st.get("npix_hit") else 12 * nside * nside
+            roof4, kern4 = roofline_from(dict(scatter=prof["t_scatter"] / nprof, gather=prof["t_gather"] / nprof),
+                                         bytes_batch(st, nd, ns, nph, nbt, R, 4),
+                                         dict(scatter="k_bscatter<float, %d, 1, 3>" % R,
+                                              gather="k_bgather_w<float, %d, 1, 3, %d>" % (R, min(16 * R // 64, 8))))
         log("group %d: %d det x %d samples x %d RHS: step %.3f ms, solve %.2f s, breakdown %s"
             % (gi, nd, ns, R, t_step * 1e3, t_solve, brk))
         if gi == 0:
@@ -393,6 +408,7 @@ def run_cfg4(a):
                                "in subsets %s (det, samples, RHS), mixed precision"
                                % (ndet0, int(np.log2(nsamp0)), nside, groups),
                    "parallelism": "1 GPU"},
+        "roofline": roof4, "kernels": kern4,
         "groups": out, "fp64_single_rhs_samples_per_s": fp64_rate,
         "batched_over_fp64_single": value / fp64_rate if fp64_rate else None,
     }
