@@ -126,23 +126,31 @@ def bytes_fp64(st, ndet, nsamp, npix, nbtot, s=8):
                 scatter=nact * rec + nbtot * s + npix * (8 + 6 * s + 4 * s))
 
 
-def bytes_batch(st, ndet, nsamp, npix_hit, nbtot, R, s):
-    """the batched kernels (k_bgather, k_bscatter) in type size s for R right-hand sides: the
-    pointing records once, per right-hand side its vectors and map values"""
+def bytes_batch(st, ndet, nsamp, npix_hit, nbtot, R, s, info=None):
+    """the batched kernels (k_bgather_w, k_bscatter) in type size s for R right-hand sides: the
+    pointing records once (16-byte run-length records when the FP32 operator uses them), per
+    right-hand side its vectors and map values"""
     rec = st.get("rec_bytes") or 16
     nact = st["nactive"]
-    return dict(gather=ndet * nsamp * rec + npix_hit * 3 * s * R + nbtot * (2 * s + 8 + 3 * s * R),
-                scatter=nact * rec + nbtot * s * R + npix_hit * (6 * s + 4 + 24 + 3 * s * R))
+    g_rec = ndet * nsamp * rec
+    s_rec = nact * rec
+    if info and info.get("rle_records_gather"):
+        g_rec = info["rle_records_gather"] * 16
+        s_rec = info["rle_records_scatter"] * 16
+    return dict(gather=g_rec + npix_hit * 3 * s * R + nbtot * (2 * s + 8 + 3 * s * R),
+                scatter=s_rec + nbtot * s * R + npix_hit * (6 * s + 4 + 24 + 3 * s * R))
 
 
-def roofline_from(times, nbytes, traffic_key):
+def roofline_from(times, nbytes, traffic_key, workload):
+    """the longest of the projection kernels: algorithmic bytes per launch / mean launch time; traffic =
+    its HBM bytes per launch measured on this workload (profiles/pmc_traffic.json), or null"""
     kern = {k: dict(avg_ms=times[k] * 1e3, alg_bytes=nbytes[k], gbs=nbytes[k] / times[k] / 1e9) for k in times}
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
     traffic = None
     tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tf):
         try:
-            traffic = json.load(open(tf)).get(traffic_key[dom])
+            traffic = json.load(open(tf)).get(workload, {}).get(traffic_key[dom])
         except Exception:
             traffic = None
     roof = {"bound": "hbm", "kernel": traffic_key[dom], "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS,
@@ -233,7 +241,8 @@ def run_cfg3(a):
     times = dict(scatter=prof["t_scatter"] / nprof, gather=prof["t_gather"] / nprof)
     # one right-hand side runs the FP32 instantiations of the FP64 path's kernels
     roof, kern = roofline_from(times, bytes_fp64(st, ndet, nsamp, npix_hit, nbtot, 4),
-                               dict(scatter="k_sr_scatter<float, 3, 4, 1>", gather="k_gather_flat<float, 3, 1>"))
+                               dict(scatter="k_sr_scatter<float, 3, 4, 1>", gather="k_gather_flat<float, 3, 1>"),
+                               "cfg3")
     value = N / t_mixed
     line = {
         "metric": METRIC, "value": value, "unit": "samples/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -303,7 +312,8 @@ def run_cfg2(a):
     times = dict(gather=st["t_gather"] / ng, scatter=st["t_scatter"] / ng)
     npix_hit = int(st["npix_hit"]) if st.get("npix_hit") else npix
     roof, kern = roofline_from(times, bytes_fp64(st, ndet, nsamp, npix_hit, nbtot),
-                               dict(gather="k_gather_flat<double, 3, 1>", scatter="k_sr_scatter<double, 3, 4, 1>"))
+                               dict(gather="k_gather_flat<double, 3, 1>", scatter="k_sr_scatter<double, 3, 4, 1>"),
+                               "cfg2")
     lat_s = sorted(lat[1:] or lat)
     return {
         "metric": METRIC, "value": N / (ms_step * 1e-3), "unit": "samples/s", "n_gpus": 1, "steps": a.steps,
@@ -371,10 +381,13 @@ def run_cfg4(a):
             st = m.stats()
             nbt = nd * ((ns // nperiod - 1) // int(round(base * fs)) + 1) * nperiod
             nph = int(st["npix_hit"]) if st.get("npix_hit") else 12 * nside * nside
+            rl = 0 if prof.get("rle_records_gather") else 1   # REC16 for run-length records, else REC12
             roof4, kern4 = roofline_from(dict(scatter=prof["t_scatter"] / nprof, gather=prof["t_gather"] / nprof),
-                                         bytes_batch(st, nd, ns, nph, nbt, R, 4),
-                                         dict(scatter="k_bscatter<float, %d, 1, 3>" % R,
-                                              gather="k_bgather_w<float, %d, 1, 3, %d>" % (R, min(16 * R // 64, 8))))
+                                         bytes_batch(st, nd, ns, nph, nbt, R, 4, prof),
+                                         dict(scatter="k_bscatter<float, %d, %d, 3>" % (R, rl),
+                                              gather="k_bgather_w<float, %d, %d, 3, %d>" % (R, rl, min(16 * R // 64, 8))),
+                                         "cfg4")
+            kern4["rle_records"] = dict(gather=prof.get("rle_records_gather"), scatter=prof.get("rle_records_scatter"))
         log("group %d: %d det x %d samples x %d RHS: step %.3f ms, solve %.2f s, breakdown %s"
             % (gi, nd, ns, R, t_step * 1e3, t_solve, brk))
         if gi == 0:

@@ -108,7 +108,8 @@ int madam_get_batch_baselines(void *ctx, int irhs, double *aa, double *yba);
  * batch, their time, entries per pixel-side work item (autotuned); with
  * madam_set_profiling on, the summed HIP-event times of the steps' pixel
  * side, prior, baseline side, preconditioner and whole step, and the number
- * of steps timed (n <= 16) */
+ * of steps timed; the run-length records of the FP32 operator on the baseline
+ * and pixel sides (0 when it reads the sample records) (n <= 18) */
 int madam_get_batch_info(void *ctx, int irhs, double *out, int n);
 /* one operator of the last madam_solve on host vectors x, y of ndet * nbase
  * baselines: which 0 = noise prior y = C_a^-1 x (cinvmul, src/noise_routines.f90:745-913),

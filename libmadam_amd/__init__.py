@@ -280,7 +280,8 @@ class Madam:
         yba = np.zeros((sz["ndet"], sz["nbase"]))
         _check(lib().madam_get_batch_baselines(self._h, int(irhs), _ptr(aa), _ptr(yba)))
         keys = ["steps", "status", "rr_inner", "passes", "rr_final", "t_setup", "t_solve", "pcg_steps", "t_pcg",
-                "piece", "t_scatter", "t_prior", "t_gather", "t_prec", "t_step", "n_prof"]
+                "piece", "t_scatter", "t_prior", "t_gather", "t_prec", "t_step", "n_prof", "rle_records_gather",
+                "rle_records_scatter"]
         info = np.zeros(len(keys))
         _check(lib().madam_get_batch_info(self._h, int(irhs), _ptr(info), len(keys)))
         return dict(map=m, binmap=b, aa=aa, yba=yba, info=dict(zip(keys, info.tolist())))

@@ -275,7 +275,10 @@ void batch_scatter(hipStream_t s, int R, int rl, int nmap, const BsTables &bt, c
     unsigned g = (unsigned)((bt.nitems + BTPB / 64 - 1) / (BTPB / 64));
     BATCH_DISPATCH_R(R, ({
         if (nmap == 1) {
-            k_bscatter<T, R, REC4, 1><<<g, BTPB, 0, s>>>(bt.items, bt.nitems, ent, pw, cc_c, nzq, m, part);
+            if (rl == REC16)
+                k_bscatter<T, R, REC16, 1><<<g, BTPB, 0, s>>>(bt.items, bt.nitems, ent, pw, cc_c, nzq, m, part);
+            else
+                k_bscatter<T, R, REC4, 1><<<g, BTPB, 0, s>>>(bt.items, bt.nitems, ent, pw, cc_c, nzq, m, part);
             if (bt.nfix > 0)
                 k_bscatter_fix<T, R, 1><<<bgrid(bt.nfix * R, BTPB), BTPB, 0, s>>>(bt.fix, bt.nfix, part, cc_c, nzq, m);
         } else {
@@ -313,6 +316,139 @@ void batch_pixel_keys(hipStream_t s, const int64_t *segstart, int64_t nz, const 
 {
     if (nz <= 0) return;
     k_pixel_first_time<<<bgrid(nz, BTPB), BTPB, 0, s>>>(segstart, nz, ent, rec_b, bord, nbase, key);
+}
+
+// ---------------------------------------------------------------- run-length records
+// Consecutive samples of one baseline on one (cross) pixel, and consecutive
+// CSR entries of one pixel from one baseline slot, contribute
+// (sum of weights) x (map value) and (sum of weights) x (baseline value): one
+// record {index, W0, W1, W2} stands for the run (W0 counts the samples of a
+// temperature-only map, whose projections take no weight).  On a ring scan at
+// Nside 512 a run holds 1.6 samples.  Runs are found and summed in stream
+// order (deterministic); positions come from an exclusive scan of the run
+// starts.
+
+__global__ void k_rle_mark(const int64_t *starts, int64_t n, uint32_t *mask)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = starts[k];
+        atomicOr(mask + (i >> 5), 1u << (i & 31));
+    }
+}
+
+__device__ inline bool rle_bit(const uint32_t *mask, int64_t i) { return (mask[i >> 5] >> (i & 31)) & 1u; }
+
+// baseline side: rec = the samples' full records, detector-major
+__device__ inline int32_t rle_q(const PixW &r, int dsh) { return r.pix >= 0 ? r.pix >> dsh : -1; }
+
+__global__ void k_rle_gather_flags(const PixW *rec, int64_t n, int64_t nsamp, const uint32_t *bmask, int dsh,
+                                   int32_t *flag)
+{
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = g % nsamp;
+        int32_t q = rle_q(rec[g], dsh);
+        bool st = q >= 0;
+        if (st && i > 0 && !rle_bit(bmask, i)) st = rle_q(rec[g - 1], dsh) != q;
+        flag[g] = st ? 1 : 0;
+    }
+}
+
+__global__ void k_rle_gather_write(const PixW *rec, int64_t n, int64_t nsamp, const uint32_t *bmask, int dsh,
+                                   int temperature, const int32_t *flag, const int64_t *pos, PixW *out)
+{
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[g]) continue;
+        int64_t i = g % nsamp;
+        PixW r = rec[g];
+        int32_t q = rle_q(r, dsh);
+        float w0 = temperature ? 1.0f : r.w0, w1 = r.w1, w2 = r.w2;
+        for (int64_t j = i + 1, gj = g + 1; j < nsamp && !rle_bit(bmask, j); ++j, ++gj) {
+            PixW rj = rec[gj];
+            if (rle_q(rj, dsh) != q) break;
+            w0 += temperature ? 1.0f : rj.w0;
+            w1 += rj.w1;
+            w2 += rj.w2;
+        }
+        out[pos[g]] = PixW{q, w0, w1, w2};
+    }
+}
+
+// per detector: its first record, and each baseline's first record relative to it
+__global__ void k_rle_gather_starts(const int64_t *bstart, int64_t nbase, int ndet, int64_t nsamp,
+                                    const int64_t *pos, int64_t *cst, int64_t *dbase)
+{
+    int64_t tot = (int64_t)ndet * (nbase + 1);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t det = t / (nbase + 1), k = t - det * (nbase + 1);
+        int64_t g0 = det * nsamp;
+        cst[t] = pos[g0 + bstart[k]] - pos[g0];
+        if (k == 0) dbase[det] = pos[g0];
+    }
+}
+
+// pixel side: ent = the CSR entries' full records {slot, w0, w1, w2}, pixel segments at smask
+__global__ void k_rle_scatter_flags(const PixW *ent, int64_t n, const uint32_t *smask, int32_t *flag)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        flag[e] = (e == 0 || rle_bit(smask, e) || ent[e].pix != ent[e - 1].pix) ? 1 : 0;
+}
+
+__global__ void k_rle_scatter_write(const PixW *ent, int64_t n, const uint32_t *smask, int temperature,
+                                    const int32_t *flag, const int64_t *pos, PixW *out)
+{
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[e]) continue;
+        PixW r = ent[e];
+        float w0 = temperature ? 1.0f : r.w0, w1 = r.w1, w2 = r.w2;
+        for (int64_t j = e + 1; j < n && !rle_bit(smask, j) && ent[j].pix == r.pix; ++j) {
+            w0 += temperature ? 1.0f : ent[j].w0;
+            w1 += ent[j].w1;
+            w2 += ent[j].w2;
+        }
+        out[pos[e]] = PixW{r.pix, w0, w1, w2};
+    }
+}
+
+__global__ void k_rle_scatter_segs(const int64_t *segstart, int64_t nz, const int64_t *pos, int64_t *cseg)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k <= nz; k += (int64_t)gridDim.x * blockDim.x)
+        cseg[k] = pos[segstart[k]];
+}
+
+void batch_rle_gather(hipStream_t s, const PixW *rec, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
+                      int dsh, int temperature, uint32_t *bmask, int32_t *flag, int64_t *pos, int64_t *scan_tmp,
+                      PixW *out, int64_t *cst, int64_t *dbase, bool count_only)
+{
+    int64_t n = (int64_t)ndet * nsamp;
+    if (n <= 0) return;
+    if (count_only) {
+        MADAM_CHECK(hipMemsetAsync(bmask, 0, sizeof(uint32_t) * ((nsamp + 31) / 32 + 1), s));
+        k_rle_mark<<<bgrid(nbase, BTPB), BTPB, 0, s>>>(bstart, nbase, bmask);
+        k_rle_gather_flags<<<bgrid(n, BTPB), BTPB, 0, s>>>(rec, n, nsamp, bmask, dsh, flag);
+        int64_t nchunks = (n + 1 + 4095) / 4096;
+        launch_exclusive_scan_i32_to_i64(s, flag, n, pos, scan_tmp, nchunks);
+        return;
+    }
+    k_rle_gather_write<<<bgrid(n, BTPB), BTPB, 0, s>>>(rec, n, nsamp, bmask, dsh, temperature, flag, pos, out);
+    k_rle_gather_starts<<<bgrid((int64_t)ndet * (nbase + 1), BTPB), BTPB, 0, s>>>(bstart, nbase, ndet, nsamp, pos, cst,
+                                                                                 dbase);
+}
+
+void batch_rle_scatter(hipStream_t s, const PixW *ent, int64_t nent, const int64_t *segstart, int64_t nz,
+                       int temperature, uint32_t *smask, int32_t *flag, int64_t *pos, int64_t *scan_tmp, PixW *out,
+                       int64_t *cseg, bool count_only)
+{
+    if (nent <= 0) return;
+    if (count_only) {
+        MADAM_CHECK(hipMemsetAsync(smask, 0, sizeof(uint32_t) * ((nent + 31) / 32 + 1), s));
+        k_rle_mark<<<bgrid(nz, BTPB), BTPB, 0, s>>>(segstart, nz, smask);
+        k_rle_scatter_flags<<<bgrid(nent, BTPB), BTPB, 0, s>>>(ent, nent, smask, flag);
+        int64_t nchunks = (nent + 1 + 4095) / 4096;
+        launch_exclusive_scan_i32_to_i64(s, flag, nent, pos, scan_tmp, nchunks);
+        return;
+    }
+    k_rle_scatter_write<<<bgrid(nent, BTPB), BTPB, 0, s>>>(ent, nent, smask, temperature, flag, pos, out);
+    k_rle_scatter_segs<<<bgrid(nz + 1, BTPB), BTPB, 0, s>>>(segstart, nz, pos, cseg);
 }
 
 template <typename T>
@@ -416,7 +552,8 @@ __global__ void __launch_bounds__(BTPB) k_bgather(int ndet, int64_t nsamp, int64
 // reads p, ap, nna, dwb and writes ap as R contiguous row segments of the run.
 template <typename T, int R, int RL, int NMAP, int U>
 __global__ void __launch_bounds__(BTPB) k_bgather_w(int ndet, int64_t nsamp, int64_t nbase, int run_len, int ndg,
-                                                    const int64_t *__restrict__ bstart, const void *__restrict__ rec,
+                                                    const int64_t *__restrict__ bstart, int64_t bstride,
+                                                    const int64_t *__restrict__ dbase, const void *__restrict__ rec,
                                                     const T *__restrict__ m, int dsh, const T *dwb, const T *nna,
                                                     const T *p, T *ap, const int *active)
 {
@@ -450,9 +587,10 @@ __global__ void __launch_bounds__(BTPB) k_bgather_w(int ndet, int64_t nsamp, int
     int64_t k0 = run * run_len;
     int nk = (int)(k0 + run_len < nbase ? run_len : nbase - k0);
     // sample indices inside the detector's stream fit 32 bits (nsamp < 2^31)
-    int32_t bsl = (int32_t)bstart[k0 + (lane < nk ? lane : nk)];   // run_len <= 63: lane nk holds the end
+    // (run-length records: per-detector baseline starts and record bases)
+    int32_t bsl = (int32_t)bstart[det * bstride + k0 + (lane < nk ? lane : nk)];   // run_len <= 63: lane nk holds the end
     int64_t nbtot = (int64_t)ndet * nbase;
-    const char *recd = (const char *)rec + (size_t)det * nsamp * RB;
+    const char *recd = (const char *)rec + (size_t)(dbase ? dbase[det] : det * nsamp) * RB;
     // the baseline bounds are read with readlane: the chunk control is scalar
     int kc = 0;
     int32_t ib = __builtin_amdgcn_readlane(bsl, 0), ie = __builtin_amdgcn_readlane(bsl, 1);
@@ -558,7 +696,7 @@ static bool bg_xcd_split()
 template <typename T>
 void batch_gather(hipStream_t s, int R, int rl, int nmap, int ndet, int64_t nsamp, int64_t nbase,
                   const int64_t *bstart, const void *rec, const T *m, int64_t ndeg, const T *dwb, const T *nna,
-                  const T *p, T *ap, const int *active)
+                  const T *p, T *ap, const int *active, int64_t bstride, const int64_t *dbase)
 {
     if (ndet <= 0 || nbase <= 0) return;
     int dsh = 0;
@@ -576,18 +714,18 @@ void batch_gather(hipStream_t s, int R, int rl, int nmap, int ndet, int64_t nsam
             int64_t w = (int64_t)((ndet + ndg - 1) / ndg) * ((nruns + ntg - 1) / ntg);
             g = (unsigned)(8 * ((w + BTPB / 64 - 1) / (BTPB / 64)));
         }
+#define BGW_ARGS ndet, nsamp, nbase, run_len, ndg, bstart, bstride, dbase, rec, m, dsh, dwb, nna, p, ap, active
 #define BGW_LAUNCH(RR, UU)                                                                                           \
     do {                                                                                                             \
         size_t lds = sizeof(T) * (BTPB / 64) * run_len * (RR + 1) + (BTPB / 64) * 16 * sizeof(int4);                 \
-        if (nmap == 1)                                                                                               \
-            k_bgather_w<T, RR, REC4, 1, UU><<<g, BTPB, lds, s>>>(ndet, nsamp, nbase, run_len, ndg, bstart, rec, m, dsh, \
-                                                              dwb, nna, p, ap, active);                                   \
+        if (nmap == 1 && rl == REC16)                                                                                \
+            k_bgather_w<T, RR, REC16, 1, UU><<<g, BTPB, lds, s>>>(BGW_ARGS);                                         \
+        else if (nmap == 1)                                                                                          \
+            k_bgather_w<T, RR, REC4, 1, UU><<<g, BTPB, lds, s>>>(BGW_ARGS);                                          \
         else if (rl == REC12)                                                                                        \
-            k_bgather_w<T, RR, REC12, 3, UU><<<g, BTPB, lds, s>>>(ndet, nsamp, nbase, run_len, ndg, bstart, rec, m,     \
-                                                               dsh, dwb, nna, p, ap, active);                             \
+            k_bgather_w<T, RR, REC12, 3, UU><<<g, BTPB, lds, s>>>(BGW_ARGS);                                         \
         else                                                                                                         \
-            k_bgather_w<T, RR, REC16, 3, UU><<<g, BTPB, lds, s>>>(ndet, nsamp, nbase, run_len, ndg, bstart, rec, m,     \
-                                                               dsh, dwb, nna, p, ap, active);                             \
+            k_bgather_w<T, RR, REC16, 3, UU><<<g, BTPB, lds, s>>>(BGW_ARGS);                                         \
     } while (0)
         switch (R) {   // 16 records per chunk (8 at R = 64)
         case 8: BGW_LAUNCH(8, 2); break;
@@ -596,8 +734,11 @@ void batch_gather(hipStream_t s, int R, int rl, int nmap, int ndet, int64_t nsam
         default: BGW_LAUNCH(64, 8); break;
         }
 #undef BGW_LAUNCH
+#undef BGW_ARGS
         return;
     }
+    if (dbase)
+        throw HipError(hipErrorInvalidValue, "batch_gather: run-length records need R >= 8", __FILE__, __LINE__);
     int64_t nw = (int64_t)ndet * ((nbase + BG_RUN - 1) / BG_RUN);
     unsigned g = (unsigned)((nw + BTPB / 64 - 1) / (BTPB / 64));
     BATCH_DISPATCH_R(R, ({
@@ -927,7 +1068,8 @@ __global__ void k_bmap_unpack(const T *m, int64_t npix, int nmap, int R, int r, 
     template void batch_pw<T>(hipStream_t, int, const T *, int64_t, const int32_t *, const int32_t *, const T *,   \
                                const T *, const int *, T *); \
     template void batch_gather<T>(hipStream_t, int, int, int, int, int64_t, int64_t, const int64_t *, const void *, \
-                                  const T *, int64_t, const T *, const T *, const T *, T *, const int *);          \
+                                  const T *, int64_t, const T *, const T *, const T *, T *, const int *, int64_t,      \
+                                  const int64_t *);          \
     template void batch_dots<T>(hipStream_t, int, int, const T *, const T *, const T *, const int8_t *, int64_t,    \
                                 double *);                                                                         \
     template void batch_update_r<T>(hipStream_t, int, const BatchScal *, const int *, T *, T *, T *, const T *,     \

@@ -48,13 +48,26 @@ void batch_pw(hipStream_t s, int R, const T *p, int64_t nbtot, const int32_t *bo
 // the earliest baseline (index in its detector) among each non-empty pixel's entries
 void batch_pixel_keys(hipStream_t s, const int64_t *segstart, int64_t nz, const void *ent, int rec_b,
                       const int32_t *bord, int64_t nbase, int64_t *key);
+// run-length records of the two projections (FP32 path): count_only marks the
+// run starts and scans them into pos[n + 1]; the second call writes the
+// records, and per detector the baselines' first records (cst[ndet][nbase + 1],
+// relative to dbase[det]) or the pixels' first entries (cseg[nz + 1])
+void batch_rle_gather(hipStream_t s, const PixW *rec, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
+                      int dsh, int temperature, uint32_t *bmask, int32_t *flag, int64_t *pos, int64_t *scan_tmp,
+                      PixW *out, int64_t *cst, int64_t *dbase, bool count_only);
+void batch_rle_scatter(hipStream_t s, const PixW *ent, int64_t nent, const int64_t *segstart, int64_t nz,
+                       int temperature, uint32_t *smask, int32_t *flag, int64_t *pos, int64_t *scan_tmp, PixW *out,
+                       int64_t *cseg, bool count_only);
 template <typename T>
 void batch_scatter(hipStream_t s, int R, int rl, int nmap, const BsTables &bt, const void *ent, const T *pw,
                    const T *cc_c, const int32_t *nzq, T *m, T *part);
 template <typename T>
+// bstride, dbase: 0, null for the sample records (bstart shared by the
+// detectors); nbase + 1 and the per-detector record bases for run-length
+// records (bstart[det][nbase + 1] relative to dbase[det]; R >= 8, ndeg 1)
 void batch_gather(hipStream_t s, int R, int rl, int nmap, int ndet, int64_t nsamp, int64_t nbase,
                   const int64_t *bstart, const void *rec, const T *m, int64_t ndeg, const T *dwb, const T *nna,
-                  const T *p, T *ap, const int *active);
+                  const T *p, T *ap, const int *active, int64_t bstride = 0, const int64_t *dbase = nullptr);
 template <typename T>
 void batch_dots(hipStream_t s, int which, int R, const T *a, const T *b, const T *c, const int8_t *mask, int64_t n,
                 double *partials);

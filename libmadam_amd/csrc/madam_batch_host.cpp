@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "madam_context.hpp"
@@ -41,6 +42,14 @@ struct BatchState {
     const void *tables_ent = nullptr;   // the CSR the tables were built for
     std::vector<int64_t> item_order;    // pixels in first-hit order (build_items)
     const void *order_ent = nullptr;
+    // run-length records of the FP32 operator (build_rle)
+    bool rle_ok = false;
+    DBuf<PixW> rg_rec, rs_rec;
+    DBuf<int64_t> rg_st, rg_base, rs_seg;
+    DBuf<BsItem> items_c;
+    DBuf<BsFix> fix_c;
+    BsTables bt_c;
+    int64_t n_rg = 0, n_rs = 0;
     // FP32 copies of the shared per-baseline / per-pixel terms
     DBuf<float> dwb_s32, dwb32, nna32, cc_c32;
     DBuf<float> m4f;   // the one-RHS FP32 map, 4 per pixel (the FP64 one is the context's m4x)
@@ -101,6 +110,7 @@ void batch_invalidate(Context &c)
 {
     if (!c.batch) return;
     c.batch->tables_ok = false;
+    c.batch->rle_ok = false;
     c.batch->item_order.clear();
     c.batch->drop_graphs();
     c.batch->nrhs = 0;
@@ -123,11 +133,10 @@ static const void *ent_records(const Context &c);
 // pixels (the earliest baseline among a pixel's entries, ties by pixel): the
 // waves that run together then read the baseline slots of one stretch of the
 // scan, which the L2 holds.  The order of the items changes no sum.
-static void build_items(Context &c, BatchState &B, int piece)
+// item order (first-hit) of the non-empty cross pixels, kept per pointing
+static const std::vector<int64_t> &item_order(Context &c, BatchState &B)
 {
     int64_t nz = c.nzpix;
-    std::vector<int64_t> st(nz + 1);
-    d2h(st.data(), c.d_segstart_s.p, nz + 1, c.stream);
     std::vector<int64_t> &order = B.item_order;
     if ((int64_t)order.size() != nz || B.order_ent != ent_records(c)) {
         order.resize(nz);
@@ -146,6 +155,17 @@ static void build_items(Context &c, BatchState &B, int piece)
         }
         B.order_ent = ent_records(c);
     }
+    return order;
+}
+
+// the work items over the pixel segments seg_dev[nz + 1] (entries or run-length records)
+static void build_items_into(Context &c, BatchState &B, int piece, const int64_t *seg_dev, DBuf<BsItem> &ditems,
+                             DBuf<BsFix> &dfix, BsTables &bt)
+{
+    int64_t nz = c.nzpix;
+    std::vector<int64_t> st(nz + 1);
+    d2h(st.data(), seg_dev, nz + 1, c.stream);
+    const std::vector<int64_t> &order = item_order(c, B);
     MADAM_CHECK(hipStreamSynchronize(c.stream));
     std::vector<BsItem> items;
     std::vector<BsFix> fix;
@@ -165,17 +185,75 @@ static void build_items(Context &c, BatchState &B, int piece)
         }
         fix.push_back(f);
     }
-    B.items.alloc(std::max<size_t>(items.size(), 1));
-    B.fix.alloc(std::max<size_t>(fix.size(), 1));
-    h2d(B.items.p, items.data(), items.size(), c.stream);
-    h2d(B.fix.p, fix.data(), fix.size(), c.stream);
+    ditems.alloc(std::max<size_t>(items.size(), 1));
+    dfix.alloc(std::max<size_t>(fix.size(), 1));
+    h2d(ditems.p, items.data(), items.size(), c.stream);
+    h2d(dfix.p, fix.data(), fix.size(), c.stream);
     MADAM_CHECK(hipStreamSynchronize(c.stream));
-    B.bt.items = B.items.p;
-    B.bt.nitems = (int64_t)items.size();
-    B.bt.fix = B.fix.p;
-    B.bt.nfix = (int64_t)fix.size();
-    B.bt.nparts = nparts;
+    bt.items = ditems.p;
+    bt.nitems = (int64_t)items.size();
+    bt.fix = dfix.p;
+    bt.nfix = (int64_t)fix.size();
+    bt.nparts = nparts;
+}
+
+static void build_items(Context &c, BatchState &B, int piece)
+{
+    build_items_into(c, B, piece, c.d_segstart_s.p, B.items, B.fix, B.bt);
     B.piece = piece;
+    B.rle_ok = false;
+}
+
+// Run-length records of the FP32 operator's two projections (R >= 8): one
+// record per run of samples of one baseline on one cross pixel, and per run of
+// CSR entries of one pixel from one baseline slot, weights summed in stream
+// order; the pixel side's items are rebuilt over them with the same cut.
+// MADAM_RLE=0 keeps the sample records.
+static void build_rle(Context &c, BatchState &B)
+{
+    hipStream_t s = c.stream;
+    B.rle_ok = false;
+    const char *er = getenv("MADAM_RLE");
+    if (er && atoi(er) == 0) return;
+    int dsh = 0;
+    while ((1ll << dsh) < c.ndeg) ++dsh;
+    int temperature = c.nmap == 1 ? 1 : 0;
+    int64_t n = (int64_t)c.ndet * c.nsamp, nent = c.nactive;
+    int64_t nmax = std::max(n, nent);
+    DBuf<int32_t> flag;
+    DBuf<int64_t> pos, tmp;
+    DBuf<uint32_t> mask;
+    flag.alloc(std::max<int64_t>(nmax, 1));
+    pos.alloc(nmax + 1);
+    tmp.alloc((nmax + 1 + 4095) / 4096 + 1);
+    mask.alloc((std::max<int64_t>(c.nsamp, nent) + 31) / 32 + 1);
+    // baseline side
+    int64_t tot = 0;
+    batch_rle_gather(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, dsh, temperature, mask.p, flag.p, pos.p,
+                     tmp.p, nullptr, nullptr, nullptr, true);
+    d2h(&tot, pos.p + n, 1, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    B.rg_rec.alloc(tot + REC_PAD);
+    MADAM_CHECK(hipMemsetAsync(B.rg_rec.p + tot, 0, sizeof(PixW) * REC_PAD, s));
+    B.rg_st.alloc((size_t)c.ndet * (c.nbase + 1));
+    B.rg_base.alloc(std::max(c.ndet, 1));
+    batch_rle_gather(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, dsh, temperature, mask.p, flag.p, pos.p,
+                     tmp.p, B.rg_rec.p, B.rg_st.p, B.rg_base.p, false);
+    B.n_rg = tot;
+    // pixel side
+    batch_rle_scatter(s, (const PixW *)c.d_ent.p, nent, c.d_segstart_s.p, c.nzpix, temperature, mask.p, flag.p, pos.p,
+                      tmp.p, nullptr, nullptr, true);
+    d2h(&tot, pos.p + nent, 1, s);
+    MADAM_CHECK(hipStreamSynchronize(s));
+    B.rs_rec.alloc(tot + REC_PAD);
+    MADAM_CHECK(hipMemsetAsync(B.rs_rec.p + tot, 0, sizeof(PixW) * REC_PAD, s));
+    B.rs_seg.alloc(c.nzpix + 1);
+    batch_rle_scatter(s, (const PixW *)c.d_ent.p, nent, c.d_segstart_s.p, c.nzpix, temperature, mask.p, flag.p,
+                      pos.p, tmp.p, B.rs_rec.p, B.rs_seg.p, false);
+    MADAM_CHECK(hipGetLastError());
+    B.n_rs = tot;
+    build_items_into(c, B, B.piece, B.rs_seg.p, B.items_c, B.fix_c, B.bt_c);
+    B.rle_ok = true;
 }
 
 static const void *ent_records(const Context &c)
@@ -233,10 +311,14 @@ static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T
     bool one = R == 1;
     batch_pw<T>(s, R, x, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), active, pw);
     if (tm) tm->sc.start(s);
+    // the FP32 operator at R >= 8 reads the run-length records
+    bool rle = std::is_same<T, float>::value && B.rle_ok && R >= 8;
     if (one)
         launch_scatter_seg<T>(s, c.nmap, c.nactive, ent_records(c), c.rec_layout, pw, c.sr_s.heads.p, c.sr_s.k0.p,
                               c.sr_s.fix_end.p, c.sr_s.fix_ord.p, c.d_nzq.p, c.nzpix, Ops<T>::cc(c, B),
                               Ops<T>::m4(c, B), c.d_sr_part.p);
+    else if (rle)
+        batch_scatter<T>(s, R, REC16, c.nmap, B.bt_c, B.rs_rec.p, pw, Ops<T>::cc(c, B), c.d_nzq.p, m, part);
     else
         batch_scatter<T>(s, R, c.rec_layout, c.nmap, B.bt, ent_records(c), pw, Ops<T>::cc(c, B), c.d_nzq.p, m, part);
     if (tm) {
@@ -259,6 +341,9 @@ static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T
     if (one)
         launch_gather_flat<T>(s, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, sample_records(c), c.rec_layout,
                               Ops<T>::m4(c, B), c.ndeg, Ops<T>::dwb(c, B), Ops<T>::nna(c, B), x, y);
+    else if (rle)
+        batch_gather<T>(s, R, REC16, c.nmap, c.ndet, c.nsamp, c.nbase, B.rg_st.p, B.rg_rec.p, m, 1,
+                        Ops<T>::dwb(c, B), Ops<T>::nna(c, B), x, y, active, c.nbase + 1, B.rg_base.p);
     else
         batch_gather<T>(s, R, c.rec_layout, c.nmap, c.ndet, c.nsamp, c.nbase, c.d_bstart.p, sample_records(c), m,
                         c.ndeg, Ops<T>::dwb(c, B), Ops<T>::nna(c, B), x, y, active);
@@ -622,7 +707,12 @@ static void solve_batch_t(Context &c, int nrhs, const TT *signals, int on_device
             B.tables_ent = ent_records(c);
             B.drop_graphs();
         }
-        B.part.alloc((size_t)std::max<int64_t>(B.bt.nparts, 1) * c.nmap * R * sizeof(float));
+        if (R >= 8 && !B.rle_ok) {
+            build_rle(c, B);
+            B.drop_graphs();
+        }
+        int64_t nparts = std::max<int64_t>(B.bt.nparts, B.rle_ok ? B.bt_c.nparts : 0);
+        B.part.alloc((size_t)std::max<int64_t>(nparts, 1) * c.nmap * R * sizeof(float));
         v.part = tp<float>(B.part);
         // the FP64 operator's own buffers (its pieces in the same tables)
         B.pw64.alloc((size_t)std::max<int64_t>(nb, 1) * R);
@@ -805,7 +895,7 @@ void batch_get_info(Context &c, int irhs, double *out, int n)
 {
     BatchState &B = batch_of(c, irhs);
     const BatchScal &S = B.hsc[irhs];
-    double v[16];
+    double v[18];
     v[0] = S.steps;
     v[1] = S.status;
     v[2] = S.rr0 > 0 ? S.rr / S.rr0 : 0;
@@ -828,7 +918,10 @@ void batch_get_info(Context &c, int irhs, double *out, int n)
     v[13] = B.t_prec;
     v[14] = B.t_step;
     v[15] = (double)B.n_prof;
-    for (int i = 0; i < n && i < 16; ++i) out[i] = v[i];
+    // run-length records of the FP32 operator (0: sample records)
+    v[16] = B.rle_ok ? (double)B.n_rg : 0;
+    v[17] = B.rle_ok ? (double)B.n_rs : 0;
+    for (int i = 0; i < n && i < 18; ++i) out[i] = v[i];
 }
 
 }  // namespace madam

@@ -180,3 +180,54 @@ def test_mixed_precision_single_rhs_default_tolerance():
           "rr_final", r["info"]["rr_final"], "map", rel_rms(mg, m1), "aa", rel_rms(ag, a1))
     assert r["info"]["rr_final"] < 1e-12
     assert rel_rms(mg, m1) <= 1e-4
+
+
+@pytest.mark.parametrize("nnz", [3, 1])
+def test_mixed_precision_run_length_records(nnz):
+    """Six right-hand sides (kernel width 8), so that the FP32 operator reads
+    run-length records: one record per run of samples of one baseline on one
+    pixel, and per run of a pixel's CSR entries from one baseline, weights
+    summed.  With flagged bursts, against the same mixed solve on the sample
+    records (MADAM_RLE=0) and against the FP64 oracle for two right-hand
+    sides, all converged to cglimit 1e-22, less the null-space offset."""
+    case = ring(prior=True, nsamp=40000, nperiod=2, nnz=nnz, extra=dict(cglimit=1e-22, iter_max=2000))
+    c = mc.copy_case(case)
+    ndet = len(c["dets"])
+    nsamp = c["timestamps"].size
+    pix = c["pixels"].reshape(ndet, nsamp)
+    pix[:, 1000:1010] = -1
+    pix[1, 5000:5100] = -1
+    sig = signals_for(c, 6)
+    res, _ = batch(c, sig, 1)
+    nrec = res[0]["info"]["rle_records_gather"]
+    print("run-length records: baseline side", nrec, "pixel side", res[0]["info"]["rle_records_scatter"],
+          "samples", ndet * nsamp)
+    assert 0 < nrec < ndet * nsamp
+    old = os.environ.get("MADAM_RLE")
+    try:
+        os.environ["MADAM_RLE"] = "0"
+        res0, _ = batch(c, sig, 1)
+    finally:
+        if old is None:
+            os.environ.pop("MADAM_RLE", None)
+        else:
+            os.environ["MADAM_RLE"] = old
+    assert res0[0]["info"]["rle_records_gather"] == 0
+    for k in range(6):
+        good = res0[k]["binmap"][:, 0] != 0
+        a1, m1 = offset_free(res[k]["aa"], res[k]["map"], good)
+        a0, m0 = offset_free(res0[k]["aa"], res0[k]["map"], good)
+        print("rhs", k, "passes", res[k]["info"]["passes"], res0[k]["info"]["passes"], "map", rel_rms(m1, m0),
+              "baselines", rel_rms(a1, a0))
+        assert res[k]["info"]["rr_final"] < 1e-22
+        assert rel_rms(m1, m0) <= 1e-6 and rel_rms(a1, a0) <= 1e-6
+        assert np.array_equal(res[k]["binmap"], res0[k]["binmap"])
+    for k in (0, 5):
+        ck = mc.copy_case(c)
+        ck["signal"] = sig[k].reshape(-1).copy()
+        o = oracle.destripe(*mc.call_args(ck))
+        good = o["mask"] > 0
+        ag, mg = offset_free(res[k]["aa"], res[k]["map"], good)
+        ao, mo = offset_free(o["aa"], o["map"], good)
+        print("rhs", k, "vs oracle: map", rel_rms(mg, mo), "baselines", rel_rms(ag, ao))
+        assert rel_rms(mg, mo) <= 1e-6 and rel_rms(ag, ao) <= 1e-6

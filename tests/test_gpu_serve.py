@@ -2,7 +2,8 @@
 precisions submitted together, against each job solved alone.
 
 Bars: FP64 jobs equal their solve alone to 1e-10 relative rms (maps,
-baselines), mixed-precision jobs to 1e-6; hits bit-exact.  The loop must
+baselines), mixed-precision jobs to 1e-6 less the null-space offset; hits
+bit-exact.  The loop must
 have batched same-scan jobs, reused resident scans and run several jobs at
 once, and admission must keep the estimated device memory within the budget.
 """
@@ -27,10 +28,14 @@ def _torch_gpu_first():
 
 
 def scans():
-    """three scans: (case, scan key)"""
-    a = ring(nside=8, ndet=4, nsamp=20000, prior=True, nperiod=2, seed=1)
-    b = ring(nside=16, ndet=2, nsamp=30000, prior=True, nperiod=2, seed=2)
-    c = ring(nside=8, ndet=4, nsamp=12000, prior=False, nperiod=1, seed=3)
+    """three scans: (case, scan key).  The solves converge to cglimit 1e-20:
+    at the default 1e-12, two mixed-precision solves that take different
+    rounding paths (a batch of two against one alone) differ by up to about
+    1e-5, the spread that tolerance leaves."""
+    tight = dict(cglimit=1e-20, iter_max=2000)
+    a = ring(nside=8, ndet=4, nsamp=20000, prior=True, nperiod=2, seed=1, extra=tight)
+    b = ring(nside=16, ndet=2, nsamp=30000, prior=True, nperiod=2, seed=2, extra=tight)
+    c = ring(nside=8, ndet=4, nsamp=12000, prior=False, nperiod=1, seed=3, extra=tight)
     return [(a, 101), (b, 202), (c, 303)]
 
 
@@ -92,8 +97,15 @@ def test_jobs_alone_and_in_the_mix():
         r = mix[k]
         tol = 1e-10 if prec == 0 else 1e-6
         good = alone["hits"] > 0
-        errs = dict(map=rel_rms(r["map"][good], alone["map"][good]), aa=rel_rms(r["aa"], alone["aa"]),
-                    binmap=rel_rms(r["binmap"][good], alone["binmap"][good]))
+        m1, m0 = r["map"][good].copy(), alone["map"][good].copy()
+        a1, a0 = r["aa"].copy(), alone["aa"].copy()
+        if prec == 1:
+            # less the null-space offset traded between the baselines and the I map
+            m1[:, 0] -= m1[:, 0].mean()
+            m0[:, 0] -= m0[:, 0].mean()
+            a1 -= a1.mean()
+            a0 -= a0.mean()
+        errs = dict(map=rel_rms(m1, m0), aa=rel_rms(a1, a0), binmap=rel_rms(r["binmap"][good], alone["binmap"][good]))
         print("job", k, "scan", si, "precision", prec, "batch", r["batch"], "resident", r["resident"], errs)
         assert np.array_equal(r["hits"], alone["hits"])
         for name, e in errs.items():
