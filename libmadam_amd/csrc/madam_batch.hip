@@ -804,6 +804,9 @@ __global__ void __launch_bounds__(BTPB) k_bdots(const T *a, const T *b, const T 
 // more than iter_min steps or rz = 0 converged, iter_max steps reached.  A
 // right-hand side that stops is marked inactive before its p update.
 // stage 2 (start of a solve): rz, rr0 = rr; active if rz >= 1e-30 and not NaN.
+// stage 3: rr only (the refinement's residual test).  stage 4: stage 1 with
+// rz2 = ap . z in place of r_old . z (beta = -(ap . z) / pap, the same
+// Polak-Ribiere beta, since r_old = r + alpha ap).
 __global__ void __launch_bounds__(BTPB) k_bfinal(const double *partials, int nblk, int stage, BatchScal *sc,
                                                  int *active, double lim, int iter_min, int iter_max,
                                                  const int *allow, int stall)
@@ -856,7 +859,10 @@ __global__ void __launch_bounds__(BTPB) k_bfinal(const double *partials, int nbl
         return;
     }
     double rzn = sh[0][0], rz2 = sh[1][0], rr = sh[2][0];
-    S.beta = (rzn - rz2) / S.rz;
+    // stage 1: rz2 = r_old . z; stage 4: rz2 = ap . z, and r_old = r + alpha ap
+    // gives r_old . z = rzn + alpha (ap . z): the same Polak-Ribiere beta
+    // without keeping r_old
+    S.beta = stage == 4 ? -rz2 / S.pap : (rzn - rz2) / S.rz;
     S.rz = rzn;
     S.rr = rr;
     S.steps += 1;
@@ -907,7 +913,7 @@ __global__ void k_bupdate_r(const BatchScal *sc, const int *active, T *x, T *r, 
     for (int64_t i = off + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < off + n;
          i += (int64_t)gridDim.x * blockDim.x) {
         T ri = r[i];
-        ro[i] = ri;
+        if (ro) ro[i] = ri;
         x[i] = x[i] + alpha * p[i];
         r[i] = ri - alpha * ap[i];
     }
@@ -924,6 +930,47 @@ __global__ void k_bupdate_p(const BatchScal *sc, const int *active, T *p, const 
     for (int64_t i = off + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < off + n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = z[i] + beta * p[i];
+}
+
+// p = z + beta p on a tile of BPW_TB baselines x R (RHS-major rows), and the
+// next operator's pw[bslot[b] * R + r] = p[r][b] * dwb[b] from the same tile
+// (0 for finished right-hand sides): k_bupdate_p and k_bpw in one pass
+template <typename T, int R>
+__global__ void __launch_bounds__(BTPB) k_bupdate_p_pw(const BatchScal *sc, const int *active, T *p, const T *z,
+                                                       int64_t nbtot, const int32_t *bslot, const T *dwb, T *pw)
+{
+    __shared__ T tile[R][BPW_TB + 1];
+    int64_t b0 = (int64_t)blockIdx.x * BPW_TB;
+    int nbt = (int)(b0 + BPW_TB <= nbtot ? BPW_TB : nbtot - b0);
+    for (int t = threadIdx.x; t < R * BPW_TB; t += BTPB) {
+        int r = t / BPW_TB, j = t - r * BPW_TB;
+        if (j < nbt) {
+            int64_t i = (int64_t)r * nbtot + b0 + j;
+            T v = p[i];
+            if (active[r]) {
+                v = z[i] + (T)sc[r].beta * v;
+                p[i] = v;
+            }
+            tile[r][j] = v;
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < R * BPW_TB; t += BTPB) {
+        int j = t / R, r = t - j * R;
+        if (j < nbt) {
+            int64_t b = b0 + j;
+            pw[(int64_t)bslot[b] * R + r] = active[r] ? tile[r][j] * dwb[b] : (T)0;
+        }
+    }
+}
+
+template <typename T>
+void batch_update_p_pw(hipStream_t s, int R, const BatchScal *sc, const int *active, T *p, const T *z, int64_t nbtot,
+                       const int32_t *bslot, const T *dwb, T *pw)
+{
+    if (nbtot <= 0) return;
+    unsigned g = (unsigned)((nbtot + BPW_TB - 1) / BPW_TB);
+    BATCH_DISPATCH_R(R, (k_bupdate_p_pw<T, R><<<g, BTPB, 0, s>>>(sc, active, p, z, nbtot, bslot, dwb, pw)));
 }
 
 template <typename T>
@@ -1065,6 +1112,8 @@ __global__ void k_bmap_unpack(const T *m, int64_t npix, int nmap, int R, int r, 
 #define BATCH_INST(T)                                                                                              \
     template void batch_scatter<T>(hipStream_t, int, int, int, const BsTables &, const void *, const T *, const T *, \
                                    const int32_t *, T *, T *);                                                     \
+    template void batch_update_p_pw<T>(hipStream_t, int, const BatchScal *, const int *, T *, const T *, int64_t,  \
+                                       const int32_t *, const T *, T *);                                          \
     template void batch_pw<T>(hipStream_t, int, const T *, int64_t, const int32_t *, const int32_t *, const T *,   \
                                const T *, const int *, T *); \
     template void batch_gather<T>(hipStream_t, int, int, int, int, int64_t, int64_t, const int64_t *, const void *, \

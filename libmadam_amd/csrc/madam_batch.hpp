@@ -42,6 +42,10 @@ struct BatchScal {
 // the power-of-two kernel width for nrhs right-hand sides (-1 above 64)
 int batch_width(int nrhs);
 
+// p = z + beta p and the next operator's pw (batch_pw) in one pass
+template <typename T>
+void batch_update_p_pw(hipStream_t s, int R, const BatchScal *sc, const int *active, T *p, const T *z, int64_t nbtot,
+                       const int32_t *bslot, const T *dwb, T *pw);
 template <typename T>
 void batch_pw(hipStream_t s, int R, const T *p, int64_t nbtot, const int32_t *bord, const int32_t *bslot,
               const T *dwb_s, const T *dwb, const int *active, T *pw);

@@ -67,6 +67,7 @@ struct BatchState {
     DBuf<Segment> d_segR;
     DBuf<double> fftwork;
     int64_t fft_nseg = 0;
+    int64_t fft_batch32 = 1;   // segments per FFT batch of the FP32 convolution
     // per right-hand side results
     std::vector<BatchScal> hsc, hsc64;
     std::vector<int> inner_steps;
@@ -301,7 +302,7 @@ struct StepTimers {
 // side with the ccmultiply, prior, baseline side
 template <typename T>
 static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T *m, T *part, const int *active,
-                           StepTimers *tm = nullptr)
+                           StepTimers *tm = nullptr, bool pw_ready = false)
 {
     hipStream_t s = c.stream;
     int R = B.R;
@@ -309,7 +310,7 @@ static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T
     // one right-hand side: the single solve's segmented-tile pixel side and
     // flat baseline side in the operator's type; several: the batched kernels
     bool one = R == 1;
-    batch_pw<T>(s, R, x, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), active, pw);
+    if (!pw_ready) batch_pw<T>(s, R, x, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), active, pw);
     if (tm) tm->sc.start(s);
     // the FP32 operator at R >= 8 reads the run-length records
     bool rle = std::is_same<T, float>::value && B.rle_ok && R >= 8;
@@ -329,8 +330,8 @@ static void batch_operator(Context &c, BatchState &B, const T *x, T *y, T *pw, T
         if (!c.prior_covers_all) batch_copy<T>(s, y, x, nb * R);
         if (!B.segR.empty())
             fft_convolve_segments<T>(s, c.fft, B.d_segR.p, (int64_t)B.segR.size(), x, y, c.d_fcov.p, c.d_fcov_rows.p,
-                                     c.nof / 2 + 1, B.fftwork.p, c.fft_batch, B.fft_nseg, B.segR.data(),
-                                     Ops<T>::fcov32(B));
+                                     c.nof / 2 + 1, B.fftwork.p, Ops<T>::fcov32(B) ? B.fft_batch32 : c.fft_batch,
+                                     B.fft_nseg, B.segR.data(), Ops<T>::fcov32(B));
     } else {
         batch_fill<T>(s, y, (T)0, nb * R);
     }
@@ -364,7 +365,8 @@ static void batch_precond(Context &c, BatchState &B, const T *r, T *z)
         batch_copy<T>(s, z, r, nb * R);
         if (!B.segR.empty())
             fft_convolve_segments<T>(s, c.fft, B.d_segR.p, (int64_t)B.segR.size(), r, z, c.d_fprec.p,
-                                     c.d_fprec_rows.p, c.nof / 2 + 1, B.fftwork.p, c.fft_batch, B.fft_nseg,
+                                     c.d_fprec_rows.p, c.nof / 2 + 1, B.fftwork.p,
+                                     Ops<T>::fprec32(B) ? B.fft_batch32 : c.fft_batch, B.fft_nseg,
                                      B.segR.data(),
                                      Ops<T>::fprec32(B));
         break;
@@ -403,16 +405,22 @@ static void pcg_step(Context &c, BatchState &B, const Vec<T> &v, double lim, int
     int R = B.R;
     int64_t nb = c.nbtot;
     if (tm) tm->st.start(s);
-    batch_operator<T>(c, B, v.p, v.ap, v.pw, v.m, v.part, B.active.p, tm);
+    // R > 1: the previous step's p update (or the start) left pw ready; the
+    // FP32 inner solve keeps no r_old (beta from ap . z, stage 4)
+    const bool fuse = R > 1, lean = std::is_same<T, float>::value;
+    batch_operator<T>(c, B, v.p, v.ap, v.pw, v.m, v.part, B.active.p, tm, fuse);
     batch_dots<T>(s, 0, R, v.p, v.ap, nullptr, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 0, B.sc.p, B.active.p, lim, iter_min, iter_max);
-    batch_update_r<T>(s, R, B.sc.p, B.active.p, v.x, v.r, v.ro, v.p, v.ap, nb);
+    batch_update_r<T>(s, R, B.sc.p, B.active.p, v.x, v.r, lean ? nullptr : v.ro, v.p, v.ap, nb);
     if (tm) tm->pc.start(s);
     batch_precond<T>(c, B, v.r, v.z);
     if (tm) tm->pc.stop(s);
-    batch_dots<T>(s, 1, R, v.r, v.z, v.ro, c.d_rmask.p, nb, B.dots.p);
-    batch_final(s, R, B.dots.p, 1, B.sc.p, B.active.p, lim, iter_min, iter_max, nullptr, stall);
-    batch_update_p<T>(s, R, B.sc.p, B.active.p, v.p, v.z, nb);
+    batch_dots<T>(s, 1, R, v.r, v.z, lean ? v.ap : v.ro, c.d_rmask.p, nb, B.dots.p);
+    batch_final(s, R, B.dots.p, lean ? 4 : 1, B.sc.p, B.active.p, lim, iter_min, iter_max, nullptr, stall);
+    if (fuse)
+        batch_update_p_pw<T>(s, R, B.sc.p, B.active.p, v.p, v.z, nb, c.d_bslot.p, Ops<T>::dwb(c, B), v.pw);
+    else
+        batch_update_p<T>(s, R, B.sc.p, B.active.p, v.p, v.z, nb);
     if (tm) {
         tm->st.stop(s);
         B.t_scatter += tm->sc.ms() * 1e-3;
@@ -449,6 +457,8 @@ static int64_t batch_pcg(Context &c, BatchState &B, const Vec<T> &v, const T *b,
     batch_copy<T>(s, v.p, v.z, nb * R);
     batch_dots<T>(s, 1, R, v.r, v.z, v.r, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 2, B.sc.p, B.active.p, lim, iter_min, iter_max, allow);
+    // the first step's pixel-side input (the steps leave it for the next one)
+    if (R > 1) batch_pw<T>(s, R, v.p, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), B.active.p, v.pw);
     int gi = std::is_same<T, float>::value ? 1 : 0;
     // profiling times every part with events, so the steps run as launched
     bool graph = B.use_graph && c.segcg.empty() && !c.profile;
@@ -557,7 +567,12 @@ static void make_segments(Context &c, BatchState &B)
         h2d(B.d_segR.p, B.segR.data(), B.segR.size(), c.stream);
     }
     B.fft_nseg = std::max<int64_t>((int64_t)B.segR.size(), 1);
-    if (c.p.kfilter) B.fftwork.alloc(fft_convolve_work_doubles(c.fft, c.fft_batch, B.fft_nseg));
+    // the FP32 work buffer holds twice the segments of the FP64 one in the
+    // same bytes (the batch stays resident in the last-level cache)
+    B.fft_batch32 = std::min<int64_t>(std::min<int64_t>(2 * c.fft_batch, 64), B.fft_nseg);
+    if (const char *e = getenv("MADAM_FFT_BATCH32")) B.fft_batch32 = std::max<int64_t>(std::min<int64_t>(atoi(e), 64), 1);
+    if (c.p.kfilter)
+        B.fftwork.alloc(fft_convolve_work_doubles(c.fft, std::max<int64_t>(c.fft_batch, B.fft_batch32), B.fft_nseg));
 }
 
 // binned map and right-hand side of one signal (src/smadam_routines.f90:252-671),
@@ -810,6 +825,7 @@ static void iterate_t(Context &c, BatchState &B, const T *b, int niter)
     batch_copy<T>(s, v.p, v.z, nb * R);
     batch_dots<T>(s, 1, R, v.r, v.z, v.r, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 2, B.sc.p, B.active.p, 0.0, 0, 1 << 30, B.ones.p);
+    if (R > 1) batch_pw<T>(s, R, v.p, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), B.active.p, v.pw);
     const int big = 1 << 30;
     StepTimers tmr;
     bool graph = B.use_graph && c.segcg.empty() && !c.profile;
