@@ -477,9 +477,13 @@ def run_cfg5(a):
     ids = [None] * njobs
     t0 = time.perf_counter()
 
+    stop = threading.Event()
+
     def feeder():
         t = 0.0
         for k, si in enumerate(specs):
+            if stop.is_set():
+                return
             t += rng.exponential(1.0 / rate)
             dt = t0 + t - time.perf_counter()
             if dt > 0:
@@ -488,14 +492,26 @@ def run_cfg5(a):
             ids[k] = srv.submit(d["pars"], np.full(sc["ndet"], 1.0 / SIGMA ** 2), d["ts"], d["pix"], d["w"], sigs[k],
                                 d["per"], np.ones(sc["ndet"], np.int64), np.zeros(sc["ndet"]), freqs,
                                 np.tile(psd, sc["ndet"]), nnz=3, scan_key=sc["key"], precision=1)
+            if (k + 1) % 250 == 0:
+                log("%d jobs submitted at %.1f s" % (k + 1, time.perf_counter() - t0))
 
     th = threading.Thread(target=feeder)
     th.start()
-    th.join()
+    # results are collected (and released) while the stream runs, so that the
+    # finished jobs' maps do not pile up in host memory
     lat, samples, steps, batch = [], 0, 0, []
-    for k, jid in enumerate(ids):
-        srv.wait(jid)
-        r = srv.result(jid)
+    for k in range(njobs):
+        while ids[k] is None:
+            time.sleep(0.001)
+        jid = ids[k]
+        try:
+            srv.wait(jid)
+        except Exception:
+            log("job %d failed; server stats %s" % (k, srv.stats()))
+            stop.set()
+            th.join()
+            raise
+        r = srv.result(jid, maps=False)
         lat.append(r["latency"])
         sc = scans[specs[k]]
         samples += sc["ndet"] * sc["nsamp"]
@@ -504,6 +520,7 @@ def run_cfg5(a):
         srv.release(jid)
         if (k + 1) % 500 == 0:
             log("%d jobs done" % (k + 1))
+    th.join()
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
@@ -529,8 +546,10 @@ def run_cfg5(a):
 
 
 def cpu_baseline(ndet, nsamp, nside, fs, base, nperiod, niter=3, seed=99):
-    """The FP64 CPU oracle (a port of the reference algorithm) on one detector
-    of the timed workload's shape, timed on this host's cores."""
+    """The FP64 CPU oracle (a port of the reference algorithm) on a sample of
+    the timed workload (ndet detectors of one horn, the workload's Nside,
+    baselines, periods and noise prior), timed on this host's cores; raises if
+    the sample has nothing to solve."""
     import oracle
     from tests import madam_cases as mc
 
@@ -553,7 +572,9 @@ def cpu_baseline(ndet, nsamp, nside, fs, base, nperiod, niter=3, seed=99):
                         np.arange(nsamp) / fs, pix.reshape(-1), w.reshape(-1), sig.reshape(-1), periods,
                         np.ones(ndet, np.int64), np.zeros(ndet), freqs, np.tile(psd, ndet))
     wall = time.perf_counter() - t0
-    t_it = r["t_iter"] / max(r["niter"], 1)
+    if r["niter"] < 1:
+        raise RuntimeError("CPU baseline sample ran no PCG iteration")
+    t_it = r["t_iter"] / r["niter"]
     return dict(value=ndet * nsamp / t_it, unit="samples/s", cores=int(r["nthreads"]), kind="port",
                 sample="%d det x 2^%d samples in %d periods (nof %d), Nside %d, %.2f s baselines, noise prior, band "
                        "preconditioner, FP64; %d PCG iterations timed (%.2f s per iteration), solve wall %.1f s"
@@ -599,12 +620,14 @@ def main():
         del src, dst
     if not a.no_cpu and a.workload != "cfg5":
         print("bench: cpu baseline", file=sys.stderr, flush=True)
+        # one horn of four polarisation angles: a single detector leaves every
+        # pixel's 3x3 matrix singular, so there would be nothing to solve
         if a.workload == "cfg3" and not a.small:
-            line["cpu_baseline"] = cpu_baseline(1, 2 ** 25, 1024, 100.0, 0.25, 4)
+            line["cpu_baseline"] = cpu_baseline(4, 2 ** 23, 1024, 100.0, 0.25, 4)
         elif not a.small:
-            line["cpu_baseline"] = cpu_baseline(1, 2 ** 24, 512, 100.0, 0.25, 2)
+            line["cpu_baseline"] = cpu_baseline(4, 2 ** 22, 512, 100.0, 0.25, 2)
         else:
-            line["cpu_baseline"] = cpu_baseline(1, 2 ** 20, 256, 100.0, 0.25, 2)
+            line["cpu_baseline"] = cpu_baseline(4, 2 ** 18, 256, 100.0, 0.25, 2)
     print(json.dumps(line), flush=True)
 
 

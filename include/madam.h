@@ -170,7 +170,9 @@ int madam_server_result(void *srv, long id, double *map, double *binmap, int32_t
                         double *times4);
 int madam_server_release(void *srv, long id);
 /* out = {done, failed, batches, jobs in batches, jobs on resident scans,
- * evictions, peak estimated bytes, most jobs in flight, resident scans} */
+ * evictions, peak estimated bytes, most jobs in flight, resident scans,
+ * evictions forced by the measured free device memory, least free device
+ * memory seen (bytes)}; n <= 11 */
 int madam_server_stats(void *srv, double *out, int n);
 const char *madam_last_error(void);
 /* status of the last destripe / destripe_with_cache call (0 ok); only

@@ -308,7 +308,7 @@ class Server:
     resident set-up and are batched when they wait together."""
 
     STAT_NAMES = ["done", "failed", "batches", "batched_jobs", "resident_jobs", "evictions", "peak_bytes",
-                  "max_inflight", "resident_scans"]
+                  "max_inflight", "resident_scans", "memory_evictions", "min_free_bytes"]
 
     def __init__(self, device=0, hbm_budget=200e9, max_inflight=4):
         self._h = lib().madam_server_create(int(device), float(hbm_budget), int(max_inflight))
@@ -358,10 +358,14 @@ class Server:
             raise MadamError(lib().madam_server_error(self._h, int(jid)).decode())
         return rc == 0
 
-    def result(self, jid):
+    def result(self, jid, maps=True):
+        """maps=False: sizes, batch, resident flag and times only (no map copies)"""
         sz = np.zeros(6, np.int64)
         tm = np.zeros(4)
         _check(lib().madam_server_result(self._h, int(jid), None, None, None, None, _ptr(sz), _ptr(tm)))
+        if not maps:
+            return dict(batch=int(sz[4]), resident=int(sz[5]), t_submit=tm[0], t_start=tm[1], t_done=tm[2],
+                        latency=tm[2] - tm[0], steps=tm[3])
         npix, nmap, nbase, ndet = int(sz[0]), int(sz[1]), int(sz[2]), int(sz[3])
         m = np.zeros((npix, nmap))
         b = np.zeros((npix, nmap))

@@ -331,7 +331,7 @@ static int ilog2(int64_t v)
     return r;
 }
 
-void fft_plan_init(FftPlan &p, int64_t nof)
+void fft_plan_init(FftPlan &p, int64_t nof, hipStream_t s)
 {
     p.N = nof / 2;
     int logN = ilog2(p.N);
@@ -364,9 +364,10 @@ void fft_plan_init(FftPlan &p, int64_t nof)
     std::vector<float> hf(h.size());
     for (size_t i = 0; i < h.size(); ++i) hf[i] = (float)h[i];
     MADAM_CHECK(hipMalloc(&p.tw_mem, sizeof(double) * hd.size()));
-    MADAM_CHECK(hipMemcpy(p.tw_mem, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice));
+    MADAM_CHECK(hipMemcpyAsync(p.tw_mem, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice, s));
     MADAM_CHECK(hipMalloc(&p.tw32_mem, sizeof(float) * hf.size()));
-    MADAM_CHECK(hipMemcpy(p.tw32_mem, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice));
+    MADAM_CHECK(hipMemcpyAsync(p.tw32_mem, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, s));
+    MADAM_CHECK(hipStreamSynchronize(s));   // hd and hf are freed on return
     const double2 *b = (const double2 *)p.tw_mem;
     p.sub1 = b;
     p.sub2 = p.sub1 + n1;

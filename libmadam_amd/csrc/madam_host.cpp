@@ -635,7 +635,7 @@ static void build_filters(Context &c)
     if (hbad) fail("non-positive PSD values are not supported by the noise filter");
     if (!c.fft_ready || c.fft.N != c.nof / 2) {
         if (c.fft_ready) fft_plan_free(c.fft);
-        fft_plan_init(c.fft, c.nof);
+        fft_plan_init(c.fft, c.nof, s);
         c.fft_ready = true;
     }
     c.d_fcov_rows.alloc((size_t)nh * c.npsdtot);

@@ -259,7 +259,7 @@ struct FftPlan {
     const float2 *sub1f, *sub2f, *naf, *nbf, *faf, *fbf;
 };
 
-void fft_plan_init(FftPlan &p, int64_t nof);
+void fft_plan_init(FftPlan &p, int64_t nof, hipStream_t s);
 void fft_plan_free(FftPlan &p);
 // convolve every segment: y = x outside segments is the caller's job (copy first)
 // fc: filter spectra, nrow x fc_stride; fc_rows: the same in row order (fft_spectrum_rows)

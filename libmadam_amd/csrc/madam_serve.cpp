@@ -8,6 +8,11 @@
 //     reuse (the least recently used resident scans are evicted to make room);
 //     jobs are admitted first-fit in arrival order, so a small job behind a
 //     large one that does not fit yet goes first;
+//   * device memory is checked as well as estimated: before a group starts,
+//     the free HBM (hipMemGetInfo) less the estimates of the groups in flight
+//     must hold the group's estimate and a reserve, or idle resident scans
+//     are evicted first (the estimates can fall short of what a context holds
+//     after a batched solve, and other users of the device are not counted);
 //   * resident scans: a job whose scan key matches a kept context skips the
 //     configuration and pointing set-up (pixel grouping, projection tables)
 //     and, nna being bitwise the same, the filters and preconditioner;
@@ -136,6 +141,19 @@ struct Server {
     long n_done = 0, n_failed = 0, n_batches = 0, n_batched_jobs = 0, n_resident_hits = 0, n_evicted = 0;
     size_t peak_bytes = 0;
     int max_concurrent = 0, running = 0;
+    long n_mem_evicted = 0;
+    size_t min_free = SIZE_MAX;
+
+    // HBM left free beside the device's own runtime allocations
+    static constexpr size_t RESERVE = 6ull << 30;
+
+    size_t device_free()
+    {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return SIZE_MAX;   // unknown: the estimate alone decides
+        min_free = std::min(min_free, fr);
+        return fr;
+    }
 
     size_t resident_bytes() const
     {
@@ -144,19 +162,32 @@ struct Server {
         return b;
     }
 
+    bool evict_one()
+    {
+        auto victim = resident.end();
+        for (auto it = resident.begin(); it != resident.end(); ++it)
+            if (!it->busy && (victim == resident.end() || it->last_use < victim->last_use)) victim = it;
+        if (victim == resident.end()) return false;
+        madam_destroy(victim->ctx);
+        resident.erase(victim);
+        n_evicted++;
+        return true;
+    }
+
     // evict idle resident scans (least recently used first) until `need` fits
+    // the budget and, measured, the device
     bool make_room(size_t need)
     {
         for (;;) {
             size_t used = inflight_bytes + resident_bytes();
-            if (used + need <= budget) return true;
-            auto victim = resident.end();
-            for (auto it = resident.begin(); it != resident.end(); ++it)
-                if (!it->busy && (victim == resident.end() || it->last_use < victim->last_use)) victim = it;
-            if (victim == resident.end()) return false;
-            madam_destroy(victim->ctx);
-            resident.erase(victim);
-            n_evicted++;
+            if (used + need <= budget) break;
+            if (!evict_one()) return false;
+        }
+        for (;;) {
+            size_t fr = device_free();
+            if (fr == SIZE_MAX || fr >= need + inflight_bytes + RESERVE) return true;
+            if (!evict_one()) return running == 0;   // nothing runs that could free memory: try anyway
+            n_mem_evicted++;
         }
     }
 
@@ -184,11 +215,14 @@ struct Server {
             size_t need = 0;
             for (auto &j : grp) need += j->est_bytes;
             if (res) need = need > res->bytes ? need - res->bytes : 0;   // the set-up is already resident
-            if (!make_room(need)) continue;
+            if (res) res->busy = true;   // not a candidate for eviction while room is made
+            if (!make_room(need)) {
+                if (res) res->busy = false;
+                continue;
+            }
             for (auto &j : grp) pending.remove(j);
             for (auto &j : grp) j->state = 1;
             inflight_bytes += need;
-            if (res) res->busy = true;
             if (head->scan_key) busy_keys.insert(head->scan_key);
             *res_out = res;
             size_t used = inflight_bytes + resident_bytes();
@@ -198,11 +232,19 @@ struct Server {
         return {};
     }
 
-    void run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res);
+    // a worker's own copy stream and staging buffer for the jobs' signals
+    struct Lane {
+        hipStream_t stream = nullptr;
+        void *stage = nullptr;
+        size_t cap = 0;
+    };
+    static constexpr size_t KEEP_STAGE = 256ull << 20;   // larger staging buffers are freed after the group
+
+    void run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res, Lane &lane);
     void worker();
 };
 
-void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res)
+void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res, Lane &lane)
 {
     Job &h = *grp[0];
     double t0 = wall();
@@ -216,7 +258,6 @@ void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res)
     auto check = [&](int rc) {
         if (rc != 0) throw std::runtime_error(madam_last_error());
     };
-    void *stage = nullptr;
     try {
         if (!ctx) {
             ctx = madam_create(device);
@@ -238,11 +279,36 @@ void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res)
             check(madam_solve(ctx, tod.data(), 0));
         } else {
             size_t es = h.tod_f32 ? 4 : 8;
-            if (hipMalloc(&stage, es * n * R) != hipSuccess) throw std::runtime_error("staging allocation failed");
-            for (int k = 0; k < R; ++k)
-                if (hipMemcpy((char *)stage + es * n * k, grp[k]->signal, es * n, hipMemcpyHostToDevice) != hipSuccess)
-                    throw std::runtime_error("staging copy failed");
-            check(madam_solve_batch(ctx, R, stage, h.tod_f32, 1, h.precision));
+            auto fail = [&](const char *what, hipError_t e) {
+                size_t fr = 0, tot = 0;
+                (void)hipMemGetInfo(&fr, &tot);
+                throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e) + " (" +
+                                         std::to_string(es * n) + " bytes per job, " + std::to_string(fr >> 20) +
+                                         " MiB of device memory free)");
+            };
+            size_t bytes = es * n * R;
+            hipError_t e = hipSuccess;
+            if (!lane.stream) {
+                e = hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking);
+                if (e != hipSuccess) fail("stream creation failed", e);
+            }
+            if (bytes > lane.cap) {
+                if (lane.stage) (void)hipFree(lane.stage);
+                lane.stage = nullptr;
+                lane.cap = 0;
+                e = hipMalloc(&lane.stage, bytes);
+                if (e != hipSuccess) fail("staging allocation failed", e);
+                lane.cap = bytes;
+            }
+            // copies on the worker's own stream: the legacy stream would have
+            // to wait for every blocking stream, which fails while another
+            // worker captures its PCG step into a graph
+            for (int k = 0; k < R && e == hipSuccess; ++k)
+                e = hipMemcpyAsync((char *)lane.stage + es * n * k, grp[k]->signal, es * n, hipMemcpyHostToDevice,
+                                   lane.stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(lane.stream);
+            if (e != hipSuccess) fail("staging copy failed", e);
+            check(madam_solve_batch(ctx, R, lane.stage, h.tod_f32, 1, h.precision));
         }
         check(madam_get_sizes(ctx, sz));
         for (int k = 0; k < R; ++k) {
@@ -272,7 +338,11 @@ void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res)
     } catch (const std::exception &e) {
         err = e.what();
     }
-    if (stage) (void)hipFree(stage);
+    if (lane.cap > KEEP_STAGE) {
+        (void)hipFree(lane.stage);
+        lane.stage = nullptr;
+        lane.cap = 0;
+    }
     std::lock_guard<std::mutex> lk(mu);
     double t1 = wall();
     size_t used = 0;
@@ -326,6 +396,7 @@ void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res)
 void Server::worker()
 {
     (void)hipSetDevice(device);
+    Lane lane;
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         Resident *res = nullptr;
@@ -335,11 +406,15 @@ void Server::worker()
             grp = take_group(&res);
             return !grp.empty();
         });
-        if (stop && grp.empty()) return;
+        if (stop && grp.empty()) {
+            if (lane.stage) (void)hipFree(lane.stage);
+            if (lane.stream) (void)hipStreamDestroy(lane.stream);
+            return;
+        }
         running++;
         max_concurrent = std::max(max_concurrent, running);
         lk.unlock();
-        run_group(grp, res);
+        run_group(grp, res, lane);
         lk.lock();
     }
 }
@@ -481,14 +556,16 @@ MADAM_API int madam_server_release(void *srv, long id)
 }
 
 // out = {done, failed, batches, jobs in batches, resident-scan jobs, evictions,
-// peak estimated bytes, most jobs in flight at once, resident scans now}
+// peak estimated bytes, most jobs in flight at once, resident scans now,
+// evictions forced by the measured free memory, least free device memory seen}
 MADAM_API int madam_server_stats(void *srv, double *out, int n)
 {
     Server *s = (Server *)srv;
     std::lock_guard<std::mutex> lk(s->mu);
-    double v[9] = {(double)s->n_done, (double)s->n_failed, (double)s->n_batches, (double)s->n_batched_jobs,
-                   (double)s->n_resident_hits, (double)s->n_evicted, (double)s->peak_bytes,
-                   (double)s->max_concurrent, (double)s->resident.size()};
-    for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
+    double v[11] = {(double)s->n_done, (double)s->n_failed, (double)s->n_batches, (double)s->n_batched_jobs,
+                    (double)s->n_resident_hits, (double)s->n_evicted, (double)s->peak_bytes,
+                    (double)s->max_concurrent, (double)s->resident.size(), (double)s->n_mem_evicted,
+                    s->min_free == SIZE_MAX ? -1.0 : (double)s->min_free};
+    for (int i = 0; i < n && i < 11; ++i) out[i] = v[i];
     return 0;
 }

@@ -231,3 +231,30 @@ def test_mixed_precision_run_length_records(nnz):
         ao, mo = offset_free(o["aa"], o["map"], good)
         print("rhs", k, "vs oracle: map", rel_rms(mg, mo), "baselines", rel_rms(ag, ao))
         assert rel_rms(mg, mo) <= 1e-6 and rel_rms(ag, ao) <= 1e-6
+
+
+def test_wide_batch_64():
+    """Forty right-hand sides, kernel width 64 (24 padding lanes): FP64 batch
+    against single solves for seven of them (1e-10, same iteration counts
+    within one), and the mixed solve of the same batch (run-length records)
+    against the FP64 batch to 1e-6 less the null-space offset, both at
+    cglimit 1e-20."""
+    case = ring(prior=True, nsamp=24000, nperiod=2, extra=dict(cglimit=1e-20, iter_max=2000))
+    sig = signals_for(case, 40)
+    res, _ = batch(case, sig, 0)
+    for k in (0, 1, 7, 8, 31, 32, 39):
+        s1 = single(case, sig[k])
+        r = res[k]
+        errs = dict(map=rel_rms(r["map"], s1["map"]), aa=rel_rms(r["aa"], s1["aa"]), yba=rel_rms(r["yba"], s1["yba"]))
+        print("rhs", k, "steps", r["info"]["steps"], "single", s1["niter"], errs)
+        assert abs(r["info"]["steps"] - s1["niter"]) <= 1
+        for name, e in errs.items():
+            assert e <= 1e-10, (k, name, e)
+    mix, _ = batch(case, sig, 1)
+    assert mix[0]["info"]["rle_records_gather"] > 0
+    for k in range(0, 40, 3):
+        good = res[k]["binmap"][:, 0] != 0
+        a1, m1 = offset_free(mix[k]["aa"], mix[k]["map"], good)
+        a0, m0 = offset_free(res[k]["aa"], res[k]["map"], good)
+        assert mix[k]["info"]["rr_final"] < 1e-20
+        assert rel_rms(m1, m0) <= 1e-6 and rel_rms(a1, a0) <= 1e-6, (k, rel_rms(m1, m0), rel_rms(a1, a0))
