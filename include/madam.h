@@ -67,6 +67,14 @@ int madam_get_maps(void *ctx, double *map, double *binmap, int32_t *hits, double
 /* hit counts [npix] of the pointing set with madam_set_pointing (count_hits,
  * src/smadam_routines.f90:349-422) */
 int madam_get_hits(void *ctx, int32_t *hits);
+/* The samples grouped by map pixel, which fixes the order of every per-pixel
+ * sum (the reference visits a pixel's samples in time order inside each
+ * thread's pixel range, src/smadam_routines.f90:111-422; no entry point of its
+ * own): off [npix + 1] the first entry of each pixel, src [nactive] the sample
+ * index (det * nsamp + t) of each entry, ascending inside a pixel, bord
+ * [ndet * nbase] the baseline (det * nbase + k) held by each slot of the
+ * projections' baseline order.  NULL buffers are skipped. */
+int madam_get_csr(void *ctx, int64_t *off, uint32_t *src, int32_t *bord);
 /* baselines [ndet][nbase] */
 int madam_get_baselines(void *ctx, double *aa, double *yba, double *nna);
 /* run statistics, out[0..n): niter, rrinit, rzinit, t_pointing, t_bin, t_init,

@@ -76,6 +76,8 @@ def lib():
         L.madam_get_batch_info.argtypes = [vp, i32, vp, i32]
         L.madam_get_hits.restype = i32
         L.madam_get_hits.argtypes = [vp, vp]
+        L.madam_get_csr.restype = i32
+        L.madam_get_csr.argtypes = [vp, vp, vp, vp]
         L.madam_server_create.restype = vp
         L.madam_server_create.argtypes = [i32, ct.c_double, i32]
         L.madam_server_destroy.argtypes = [vp]
@@ -232,6 +234,17 @@ class Madam:
         out = np.zeros(len(STAT_NAMES))
         _check(lib().madam_get_stats(self._h, _ptr(out), len(STAT_NAMES)))
         return dict(zip(STAT_NAMES, out.tolist()))
+
+    def csr(self):
+        """The samples grouped by map pixel: off [npix + 1], src [nactive]
+        (sample index det * nsamp + t of each entry) and bord [ndet * nbase]
+        (the baseline of each slot of the projections' baseline order)."""
+        sz = self.sizes()
+        off = np.zeros(sz["npix"] + 1, np.int64)
+        src = np.zeros(int(self.stats()["nactive"]), np.uint32)
+        bord = np.zeros(sz["ndet"] * sz["nbase"], np.int32)
+        _check(lib().madam_get_csr(self._h, _ptr(off), _ptr(src), _ptr(bord)))
+        return dict(off=off, src=src, bord=bord)
 
     def rr_history(self):
         n = self.sizes()["niter"]

@@ -84,6 +84,29 @@ struct DBuf {
     ~DBuf() { release(); }
 };
 
+// The samples sorted by map pixel (madam_csr.hip): keys (pixel, or npix_map
+// for flagged samples, which sort last) and sample indices, ascending in
+// sample index inside each pixel; the sorted pairs are k[cur], v[cur].
+struct CsrSort {
+    DBuf<uint32_t> k[2], v[2];
+    int cur = 0;
+    int64_t n = 0, npix_map = 0;
+};
+// buffers for n samples; the caller writes the keys of sample g to k[0][g]
+void csr_alloc(CsrSort &cs, int64_t n, int64_t npix_map);
+// sorts the samples by key
+void csr_sort(hipStream_t s, CsrSort &cs);
+// hits[p] for p < npix_map: the number of unflagged samples of pixel p
+void csr_hits(hipStream_t s, const CsrSort &cs, int32_t *hits);
+// the first `total` (unflagged) sorted samples as CSR entries: the record with
+// its baseline's slot (bslot may be null: the baseline index), the sample
+// index, and for rl != REC16 the compact copy of the entries in layout rl
+void csr_fill(hipStream_t s, const CsrSort &cs, const PixW *pixw, int64_t nsamp, const int64_t *bstart, int64_t nbase,
+              int64_t total, const int32_t *bslot, int rl, CsrEnt *ent, uint32_t *src, void *entc);
+// locality order of the baselines: bord[slot] = baseline, bslot[baseline] = slot
+void baseline_slots(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
+                    int64_t npix_map, int32_t *bord, int32_t *bslot);
+
 enum { ST_NITER, ST_RRINIT, ST_RZINIT, ST_T_POINTING, ST_T_BIN, ST_T_INIT, ST_T_PRECOND, ST_T_ITER, ST_T_FINAL,
        ST_T_SCATTER, ST_T_GATHER, ST_T_PRIOR, ST_T_APPLY_PREC, ST_N_SCATTER, ST_N_GATHER, ST_PRECOND_KIND,
        ST_NBAND, ST_NSEG, ST_NOF, ST_RR_FINAL, ST_NPIX_HIT, ST_NACTIVE, ST_T_TOTAL, ST_T_STEP, ST_NCG,

@@ -710,11 +710,35 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device, bool
     // projections read whole chunks of records without clamping their index
     c.d_pixw.alloc(n + REC_PAD);
     MADAM_CHECK(hipMemsetAsync(c.d_pixw.p + n, 0, sizeof(PixW) * REC_PAD, s));
-    launch_pack(s, c.d_pix, c.d_w, c.nnz, c.nmap, c.ndet, c.nsamp, c.d_ivs.p, c.nival, c.d_dw_ival.p, c.p.nsubchunk,
-                c.pass, c.d_pixw.p);
+    // compact records for the projections: temperature-only projections use no
+    // weight (src/smadam_routines.f90:1038-1070), polarised ones drop w0 when
+    // it is 1 for every active sample; the packing pass writes them, and the
+    // CSR sort keys, beside the full records
+    int rl = c.nmap == 1 ? REC4 : REC12;
+    c.d_pixc.alloc((size_t)(n + REC_PAD) * rec_bytes(rl));
+    MADAM_CHECK(hipMemsetAsync(c.d_pixc.p + (size_t)n * rec_bytes(rl), 0, (size_t)REC_PAD * rec_bytes(rl), s));
+    CsrSort cs;
+    csr_alloc(cs, n, c.npix_map);
+    {
+        DBuf<unsigned long long> cnt;
+        cnt.alloc(1);
+        MADAM_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
+        launch_pack(s, c.d_pix, c.d_w, c.nnz, c.nmap, c.ndet, c.nsamp, c.d_ivs.p, c.nival, c.d_dw_ival.p,
+                    c.p.nsubchunk, c.pass, c.d_pixw.p, cs.k[0].p, (uint32_t)c.npix_map, c.d_pixc.p, cnt.p);
+        MADAM_CHECK(hipGetLastError());
+        unsigned long long h = 0;
+        d2h(&h, cnt.p, 1, s);
+        MADAM_CHECK(hipStreamSynchronize(s));
+        if (c.nmap == 3 && h != 0) rl = REC16;
+    }
+    c.rec_layout = rl;
+    c.stats[ST_REC_BYTES] = rec_bytes(rl);
+    if (rl == REC16) c.d_pixc.release();
+    // the CSR by map pixel: a stable sort of the samples by pixel, the hit
+    // counts from its runs, the offsets from the counts
+    csr_sort(s, cs);
     c.d_hits.alloc(c.npix_map);
-    MADAM_CHECK(hipMemsetAsync(c.d_hits.p, 0, sizeof(int32_t) * c.npix_map, s));
-    launch_hist(s, c.d_pixw.p, (uint64_t)n, c.d_hits.p);
+    csr_hits(s, cs, c.d_hits.p);
     c.d_off.alloc(c.npix_map + 1);
     int64_t nchunks = (c.npix_map + 1 + 4095) / 4096;
     c.d_scan_tmp.alloc(nchunks);
@@ -723,45 +747,24 @@ static void set_pointing(Context &c, int32_t *pix, float *w, int on_device, bool
     d2h(&total, c.d_off.p + c.npix_map, 1, s);
     MADAM_CHECK(hipStreamSynchronize(s));
     c.nactive = total;
+    // the fill reads only the sorted sample indices
+    cs.k[0].release();
+    cs.k[1].release();
+    cs.v[cs.cur ^ 1].release();
     c.d_ent.alloc(total + REC_PAD);
     MADAM_CHECK(hipMemsetAsync(c.d_ent.p + total, 0, sizeof(c.d_ent.p[0]) * REC_PAD, s));
     c.d_src.alloc(total > 0 ? total : 1);
     c.d_bord.alloc(std::max<int64_t>(c.nbtot, 1));
     c.d_bslot.alloc(std::max<int64_t>(c.nbtot, 1));
-    baseline_slots(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.d_bord.p, c.d_bslot.p);
-    csr_build_sorted(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.npix_map, total, c.d_bslot.p,
-                     c.d_ent.p, c.d_src.p);
-    // compact records for the projections: temperature-only projections use no
-    // weight (src/smadam_routines.f90:1038-1070), polarised ones drop w0 when
-    // it is 1 for every active sample
-    {
-        int rl = REC16;
-        if (c.nmap == 1) rl = REC4;
-        else {
-            DBuf<unsigned long long> cnt;
-            cnt.alloc(1);
-            MADAM_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
-            launch_count_nonunit_w0(s, c.d_pixw.p, n, cnt.p);
-            unsigned long long h = 0;
-            d2h(&h, cnt.p, 1, s);
-            MADAM_CHECK(hipStreamSynchronize(s));
-            if (h == 0) rl = REC12;
-        }
-        c.rec_layout = rl;
-        c.stats[ST_REC_BYTES] = rec_bytes(rl);
-        if (rl != REC16) {
-            c.d_pixc.alloc((size_t)(n + REC_PAD) * rec_bytes(rl));
-            c.d_entc.alloc((size_t)(total + REC_PAD) * rec_bytes(rl));
-            MADAM_CHECK(hipMemsetAsync(c.d_pixc.p + (size_t)n * rec_bytes(rl), 0, (size_t)REC_PAD * rec_bytes(rl), s));
-            MADAM_CHECK(hipMemsetAsync(c.d_entc.p + (size_t)total * rec_bytes(rl), 0, (size_t)REC_PAD * rec_bytes(rl), s));
-            launch_compact_records(s, c.d_pixw.p, n, rl, c.d_pixc.p);
-            launch_compact_records(s, (const PixW *)c.d_ent.p, total, rl, c.d_entc.p);
-            MADAM_CHECK(hipGetLastError());
-        } else {
-            c.d_pixc.release();
-            c.d_entc.release();
-        }
+    baseline_slots(s, c.d_pixw.p, c.ndet, c.nsamp, c.d_bstart.p, c.nbase, c.npix_map, c.d_bord.p, c.d_bslot.p);
+    if (rl != REC16) {
+        c.d_entc.alloc((size_t)(total + REC_PAD) * rec_bytes(rl));
+        MADAM_CHECK(hipMemsetAsync(c.d_entc.p + (size_t)total * rec_bytes(rl), 0, (size_t)REC_PAD * rec_bytes(rl), s));
+    } else {
+        c.d_entc.release();
     }
+    csr_fill(s, cs, c.d_pixw.p, c.nsamp, c.d_bstart.p, c.nbase, total, c.d_bslot.p, rl, c.d_ent.p, c.d_src.p,
+             rl != REC16 ? (void *)c.d_entc.p : nullptr);
     MADAM_CHECK(hipGetLastError());
     MADAM_CHECK(hipStreamSynchronize(s));
     c.stats[ST_T_POINTING] = now() - t0;
@@ -2339,6 +2342,18 @@ MADAM_API int madam_get_hits(void *ctx, int32_t *hits)
         Context &c = *(Context *)ctx;
         if (!c.pointed) fail("madam_get_hits needs the pointing");
         d2h(hits, c.d_hits.p, c.npix_map, c.stream);
+        MADAM_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+MADAM_API int madam_get_csr(void *ctx, int64_t *off, uint32_t *src, int32_t *bord)
+{
+    return guarded([&] {
+        Context &c = *(Context *)ctx;
+        if (!c.pointed) fail("madam_get_csr needs the pointing");
+        if (off) d2h(off, c.d_off.p, c.npix_map + 1, c.stream);
+        if (src) d2h(src, c.d_src.p, c.nactive, c.stream);
+        if (bord) d2h(bord, c.d_bord.p, c.nbtot, c.stream);
         MADAM_CHECK(hipStreamSynchronize(c.stream));
     });
 }

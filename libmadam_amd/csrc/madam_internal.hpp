@@ -36,12 +36,20 @@ struct Rec12 {
     float w1, w2;
 };
 inline int rec_bytes(int rl) { return rl == REC16 ? 16 : (rl == REC12 ? 12 : 4); }
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Remap so that XCD x owns a contiguous range of logical blocks: consecutive
+// work (the same ring of the scan, or neighbouring nested pixels) then shares
+// one L2 instead of being spread over eight.
+__device__ inline int64_t xcd_block(int64_t bid, int64_t nblk)
+{
+    const int64_t NX = 8;
+    int64_t per = nblk / NX;
+    if (bid >= per * NX) return bid;
+    return (bid % NX) * per + bid / NX;
+}
+
 // zero records after the end of every record array (pixel 0, weights 0)
 static constexpr int REC_PAD = 128;
-// number of active records (index >= 0) whose w0 is not exactly 1; *count must start at 0
-void launch_count_nonunit_w0(hipStream_t s, const PixW *rec, int64_t n, unsigned long long *count);
-// compact copy of 16-byte records (PixW or CsrEnt) in layout rl
-void launch_compact_records(hipStream_t s, const PixW *rec, int64_t n, int rl, void *out);
 
 // record i of a record array in layout RL, as a full record (w0 = 1 for the compact layouts)
 template <int RL>
@@ -116,10 +124,14 @@ void launch_flag_and_count(hipStream_t s, int32_t *pix, const float *w, int nnz,
                            unsigned long long *nzero);
 void launch_flag_intervals(hipStream_t s, int32_t *pix, int64_t nsamp, const int64_t *ival_start,
                            int ninterval, int ndet, const int8_t *flag_ival);
+// the packed records of the samples (pixel -1 for inactive ones) and, in the
+// same pass, the CSR sort keys (the pixel, flag_key for inactive samples), the
+// compact records (REC12 for nmap 3, REC4 for nmap 1; those of REC12 are only
+// valid when *nonunit stays 0) and *nonunit += the number of active samples
+// whose w0 is not exactly 1 (*nonunit must start at 0)
 void launch_pack(hipStream_t s, const int32_t *pix, const float *w, int nnz, int nmap, int ndet, int64_t nsamp,
                  const int64_t *ival_start, int ninterval, const double *dw_ival, int nsubchunk, int pass,
-                 PixW *pixw);
-void launch_hist(hipStream_t s, const PixW *pixw, uint64_t n, int32_t *counts);
+                 PixW *pixw, uint32_t *key, uint32_t flag_key, void *pixc, unsigned long long *nonunit);
 void launch_exclusive_scan_i32_to_i64(hipStream_t s, const int32_t *in, int64_t n, int64_t *out, int64_t *tmp,
                                       int64_t tmp_len);
 // TT: the TOD's type (double, or float for the mixed-precision solves)
@@ -180,13 +192,6 @@ void segred_compact_cc(hipStream_t s, const int32_t *nzq, int64_t nseg, const do
 void launch_gather_seg(hipStream_t s, int nmap, int64_t nent, const PixW *pixw, const double *m4, int64_t ndeg,
                        const uint8_t *heads, const int64_t *tile_k0, const int64_t *fix_end, const int64_t *fix_ord,
                        const double *dwb, const double *nna, const double *p, double *ap, double4 *part);
-// stable regrouping of the samples by map pixel (off from the hit histogram)
-// bslot (may be null): slot number of each baseline, stored in the entries
-void csr_build_sorted(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
-                      int64_t npix_map, int64_t total, const int32_t *bslot, CsrEnt *ent, uint32_t *src);
-// locality order of the baselines: bord[slot] = baseline, bslot[baseline] = slot
-void baseline_slots(hipStream_t s, const PixW *pixw, int ndet, int64_t nsamp, const int64_t *bstart, int64_t nbase,
-                    int32_t *bord, int32_t *bslot);
 // out[i] = in[ord[i]]
 void launch_gather_order(hipStream_t s, const double *in, const int32_t *ord, double *out, int64_t n);
 // pw[i] = p[ord[i]] * dwb_s[i]: the pixel-side input in slot order

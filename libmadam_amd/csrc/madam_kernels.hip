@@ -64,18 +64,6 @@ __device__ inline int64_t find_baseline(const int64_t *bstart, int64_t nbase, in
     return lo;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
-// Remap so that XCD x owns a contiguous range of logical blocks: consecutive
-// work (the same ring of the scan, or neighbouring nested pixels) then shares
-// one L2 instead of being spread over eight.
-__device__ inline int64_t xcd_block(int64_t bid, int64_t nblk)
-{
-    const int64_t NX = 8;
-    int64_t per = nblk / NX;
-    if (bid >= per * NX) return bid;
-    return (bid % NX) * per + bid / NX;
-}
-
 static constexpr int ITEMS_PER_BLOCK = 64;
 
 template <typename T>
@@ -167,9 +155,11 @@ void launch_flag_intervals(hipStream_t s, int32_t *pix, int64_t nsamp, const int
 // ---------------------------------------------------------------- packing
 
 __global__ void k_pack(const int32_t *pix, const float *w, int nnz, int nmap, int ndet, int64_t nsamp,
-                       const int64_t *ivs, int nival, const double *dw_ival, int nsubchunk, int pass, PixW *out)
+                       const int64_t *ivs, int nival, const double *dw_ival, int nsubchunk, int pass, PixW *out,
+                       uint32_t *key, uint32_t flag_key, void *pixc, unsigned long long *nonunit)
 {
     int64_t n = (int64_t)ndet * nsamp;
+    unsigned long long nu = 0;
     for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
         int det = (int)(g / nsamp);
         int64_t i = g % nsamp;
@@ -195,30 +185,32 @@ __global__ void k_pack(const int32_t *pix, const float *w, int nnz, int nmap, in
             r.w2 = 0.0f;
         }
         out[g] = r;
+        key[g] = r.pix >= 0 ? (uint32_t)r.pix : flag_key;
+        nu += (r.pix >= 0 && r.w0 != 1.0f) ? 1 : 0;
+        if (nmap == 3) {
+            Rec12 c;
+            c.i = r.pix;
+            c.w1 = r.w1;
+            c.w2 = r.w2;
+            ((Rec12 *)pixc)[g] = c;
+        } else {
+            ((int32_t *)pixc)[g] = r.pix;
+        }
     }
+    for (int o = 32; o > 0; o >>= 1) nu += __shfl_xor(nu, o, 64);
+    if ((threadIdx.x & 63) == 0 && nu) atomicAdd(nonunit, nu);
 }
 
 void launch_pack(hipStream_t s, const int32_t *pix, const float *w, int nnz, int nmap, int ndet, int64_t nsamp,
-                 const int64_t *ivs, int nival, const double *dw_ival, int nsubchunk, int pass, PixW *pixw)
+                 const int64_t *ivs, int nival, const double *dw_ival, int nsubchunk, int pass, PixW *pixw,
+                 uint32_t *key, uint32_t flag_key, void *pixc, unsigned long long *nonunit)
 {
     k_pack<<<grid_for((int64_t)ndet * nsamp, TPB), TPB, 0, s>>>(pix, w, nnz, nmap, ndet, nsamp, ivs, nival,
-                                                                dw_ival, nsubchunk, pass, pixw);
+                                                                dw_ival, nsubchunk, pass, pixw, key, flag_key, pixc,
+                                                                nonunit);
 }
 
 // ---------------------------------------------------------------- CSR by pixel
-
-__global__ void k_hist(const PixW *pixw, uint64_t n, int32_t *counts)
-{
-    for (uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x) {
-        int32_t p = pixw[g].pix;
-        if (p >= 0) atomicAdd(&counts[p], 1);
-    }
-}
-
-void launch_hist(hipStream_t s, const PixW *pixw, uint64_t n, int32_t *counts)
-{
-    k_hist<<<grid_for((int64_t)n, TPB), TPB, 0, s>>>(pixw, n, counts);
-}
 
 static constexpr int SCAN_CHUNK = 4096;
 
@@ -855,45 +847,6 @@ void launch_scatter_pixels(hipStream_t s, int nmap, int64_t npix_cross, int64_t 
 {
     int grid = (int)((npix_cross + ITEMS_PER_BLOCK - 1) / ITEMS_PER_BLOCK);
     k_scatter_pixels<<<grid, TPB, 0, s>>>(nmap, npix_cross, ndeg, off, ent, pw, ccinv6, m4);
-}
-
-// ---------------------------------------------------------------- pointing records
-
-__global__ void k_count_nonunit_w0(const PixW *rec, int64_t n, unsigned long long *count)
-{
-    unsigned long long c = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        PixW r = rec[i];
-        c += (r.pix >= 0 && r.w0 != 1.0f) ? 1 : 0;
-    }
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
-}
-
-void launch_count_nonunit_w0(hipStream_t s, const PixW *rec, int64_t n, unsigned long long *count)
-{
-    if (n > 0) k_count_nonunit_w0<<<grid_for(n, TPB), TPB, 0, s>>>(rec, n, count);
-}
-
-__global__ void k_compact_records(const PixW *rec, int64_t n, int rl, void *out)
-{
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        PixW r = rec[i];
-        if (rl == REC12) {
-            Rec12 c;
-            c.i = r.pix;
-            c.w1 = r.w1;
-            c.w2 = r.w2;
-            ((Rec12 *)out)[i] = c;
-        } else {
-            ((int32_t *)out)[i] = r.pix;
-        }
-    }
-}
-
-void launch_compact_records(hipStream_t s, const PixW *rec, int64_t n, int rl, void *out)
-{
-    if (n > 0) k_compact_records<<<grid_for(n, TPB), TPB, 0, s>>>(rec, n, rl, out);
 }
 
 // ---------------------------------------------------------------- flat tiles
