@@ -758,10 +758,24 @@ void batch_gather(hipStream_t s, int R, int rl, int nmap, int ndet, int64_t nsam
 
 static constexpr int BDOT_BLK = 512;   // blocks per right-hand side for the dot products
 
+// four consecutive vector elements in one load (rows of length n with
+// n % 4 == 0 start on 4-element boundaries)
+template <typename T>
+struct Quad;
+template <>
+struct Quad<float> {
+    using V = float4;
+};
+template <>
+struct Quad<double> {
+    using V = double4;
+};
+
 // masked dot products of one right-hand side per grid row, FP64 sums:
 // which 0: s0 = a.b;  which 1: s0 = a.b, s1 = c.b, s2 = a.a
 template <typename T, int WHICH>
-__global__ void __launch_bounds__(BTPB) k_bdots(const T *a, const T *b, const T *c, const int8_t *mask, int64_t n,
+__global__ void __launch_bounds__(BTPB) k_bdots(const T *__restrict__ a, const T *__restrict__ b,
+                                                const T *__restrict__ c, const int8_t *__restrict__ mask, int64_t n,
                                                 double *partials)
 {
     int64_t off = (int64_t)blockIdx.y * n;
@@ -769,14 +783,32 @@ __global__ void __launch_bounds__(BTPB) k_bdots(const T *a, const T *b, const T 
     b += off;
     if (WHICH == 1) c += off;
     double s0 = 0, s1 = 0, s2 = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (!mask[i]) continue;
-        double ai = (double)a[i], bi = (double)b[i];
+    int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    auto acc = [&](int8_t m, T av, T bv, T cv) {
+        if (!m) return;
+        double ai = (double)av, bi = (double)bv;
         s0 += ai * bi;
         if (WHICH == 1) {
-            s1 += (double)c[i] * bi;
+            s1 += (double)cv * bi;
             s2 += ai * ai;
         }
+    };
+    if ((n & 3) == 0) {
+        // four elements per load: loads only in the loop body, so they pipeline
+        using V = typename Quad<T>::V;
+        const V *a4 = (const V *)a, *b4 = (const V *)b, *c4 = (const V *)c;
+        const char4 *m4 = (const char4 *)mask;
+        for (int64_t q = i0; q < (n >> 2); q += stride) {
+            char4 m = m4[q];
+            V av = a4[q], bv = b4[q], cv = av;
+            if (WHICH == 1) cv = c4[q];
+            acc(m.x, av.x, bv.x, cv.x);
+            acc(m.y, av.y, bv.y, cv.y);
+            acc(m.z, av.z, bv.z, cv.z);
+            acc(m.w, av.w, bv.w, cv.w);
+        }
+    } else {
+        for (int64_t i = i0; i < n; i += stride) acc(mask[i], a[i], b[i], WHICH == 1 ? c[i] : a[i]);
     }
     for (int o = 32; o > 0; o >>= 1) {
         s0 += __shfl_xor(s0, o, 64);
@@ -903,19 +935,39 @@ void batch_inner_limits(hipStream_t s, int R, const BatchScal *outer, BatchScal 
 
 // alpha = rz / pap; ro = r; x += alpha p; r -= alpha ap (active right-hand sides)
 template <typename T>
-__global__ void k_bupdate_r(const BatchScal *sc, const int *active, T *x, T *r, T *ro, const T *p, const T *ap,
-                            int64_t n)
+__global__ void k_bupdate_r(const BatchScal *sc, const int *active, T *__restrict__ x, T *__restrict__ r,
+                            T *__restrict__ ro, const T *__restrict__ p, const T *__restrict__ ap, int64_t n)
 {
     int y = blockIdx.y;
     if (!active[y]) return;
     T alpha = (T)(sc[y].rz / sc[y].pap);
     int64_t off = (int64_t)y * n;
-    for (int64_t i = off + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < off + n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        T ri = r[i];
-        if (ro) ro[i] = ri;
-        x[i] = x[i] + alpha * p[i];
-        r[i] = ri - alpha * ap[i];
+    int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    if ((n & 3) == 0) {
+        using V = typename Quad<T>::V;
+        V *x4 = (V *)(x + off), *r4 = (V *)(r + off), *ro4 = ro ? (V *)(ro + off) : nullptr;
+        const V *p4 = (const V *)(p + off), *ap4 = (const V *)(ap + off);
+        for (int64_t q = i0; q < (n >> 2); q += stride) {
+            V ri = r4[q], xi = x4[q], pi = p4[q], ai = ap4[q];
+            if (ro4) ro4[q] = ri;
+            xi.x += alpha * pi.x;
+            xi.y += alpha * pi.y;
+            xi.z += alpha * pi.z;
+            xi.w += alpha * pi.w;
+            ri.x -= alpha * ai.x;
+            ri.y -= alpha * ai.y;
+            ri.z -= alpha * ai.z;
+            ri.w -= alpha * ai.w;
+            x4[q] = xi;
+            r4[q] = ri;
+        }
+    } else {
+        for (int64_t i = off + i0; i < off + n; i += stride) {
+            T ri = r[i];
+            if (ro) ro[i] = ri;
+            x[i] = x[i] + alpha * p[i];
+            r[i] = ri - alpha * ap[i];
+        }
     }
 }
 
@@ -936,31 +988,86 @@ __global__ void k_bupdate_p(const BatchScal *sc, const int *active, T *p, const 
 // next operator's pw[bslot[b] * R + r] = p[r][b] * dwb[b] from the same tile
 // (0 for finished right-hand sides): k_bupdate_p and k_bpw in one pass
 template <typename T, int R>
-__global__ void __launch_bounds__(BTPB) k_bupdate_p_pw(const BatchScal *sc, const int *active, T *p, const T *z,
-                                                       int64_t nbtot, const int32_t *bslot, const T *dwb, T *pw)
+__global__ void __launch_bounds__(BTPB) k_bupdate_p_pw(const BatchScal *sc, const int *active, T *__restrict__ p,
+                                                       const T *__restrict__ z, int64_t nbtot,
+                                                       const int32_t *__restrict__ bslot, const T *__restrict__ dwb,
+                                                       T *__restrict__ pw)
 {
     __shared__ T tile[R][BPW_TB + 1];
     int64_t b0 = (int64_t)blockIdx.x * BPW_TB;
     int nbt = (int)(b0 + BPW_TB <= nbtot ? BPW_TB : nbtot - b0);
-    for (int t = threadIdx.x; t < R * BPW_TB; t += BTPB) {
-        int r = t / BPW_TB, j = t - r * BPW_TB;
-        if (j < nbt) {
-            int64_t i = (int64_t)r * nbtot + b0 + j;
-            T v = p[i];
-            if (active[r]) {
-                v = z[i] + (T)sc[r].beta * v;
-                p[i] = v;
+    if ((nbtot & 3) == 0 && nbt == BPW_TB) {
+        // whole tile, rows on 4-element boundaries: every load of the thread
+        // is issued before the first store
+        using V = typename Quad<T>::V;
+        constexpr int QR = BPW_TB / 4;                          // quads per row
+        constexpr int QPT = (R * QR + BTPB - 1) / BTPB;         // quads per thread
+        V pv[QPT], zv[QPT];
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) {
+            int qd = threadIdx.x + k * BTPB;
+            if (qd < R * QR) {
+                int r = qd / QR, j = (qd - r * QR) * 4;
+                int64_t i = (int64_t)r * nbtot + b0 + j;
+                pv[k] = *(const V *)(p + i);
+                zv[k] = *(const V *)(z + i);
             }
-            tile[r][j] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) {
+            int qd = threadIdx.x + k * BTPB;
+            if (qd < R * QR) {
+                int r = qd / QR, j = (qd - r * QR) * 4;
+                V v = pv[k];
+                if (active[r]) {
+                    T beta = (T)sc[r].beta;
+                    v.x = zv[k].x + beta * v.x;
+                    v.y = zv[k].y + beta * v.y;
+                    v.z = zv[k].z + beta * v.z;
+                    v.w = zv[k].w + beta * v.w;
+                    *(V *)(p + (int64_t)r * nbtot + b0 + j) = v;
+                }
+                tile[r][j] = v.x;
+                tile[r][j + 1] = v.y;
+                tile[r][j + 2] = v.z;
+                tile[r][j + 3] = v.w;
+            }
+        }
+    } else {
+        for (int t = threadIdx.x; t < R * BPW_TB; t += BTPB) {
+            int r = t / BPW_TB, j = t - r * BPW_TB;
+            if (j < nbt) {
+                int64_t i = (int64_t)r * nbtot + b0 + j;
+                T v = p[i];
+                if (active[r]) {
+                    v = z[i] + (T)sc[r].beta * v;
+                    p[i] = v;
+                }
+                tile[r][j] = v;
+            }
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < R * BPW_TB; t += BTPB) {
-        int j = t / R, r = t - j * R;
-        if (j < nbt) {
-            int64_t b = b0 + j;
-            pw[(int64_t)bslot[b] * R + r] = active[r] ? tile[r][j] * dwb[b] : (T)0;
+    // pw rows: a thread keeps one right-hand side (BTPB is a multiple of R),
+    // its slots and weights loaded before the stores
+    constexpr int NP = (BPW_TB * R + BTPB - 1) / BTPB;               // passes
+    int32_t sl[NP];
+    T dw[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        int t = threadIdx.x + k * BTPB;
+        int j = t / R;
+        sl[k] = -1;
+        if (t < R * BPW_TB && j < nbt) {
+            sl[k] = bslot[b0 + j];
+            dw[k] = dwb[b0 + j];
         }
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        int t = threadIdx.x + k * BTPB;
+        int j = t / R, r = t - j * R;
+        if (sl[k] >= 0) pw[(int64_t)sl[k] * R + r] = active[r] ? tile[r][j] * dw[k] : (T)0;
     }
 }
 

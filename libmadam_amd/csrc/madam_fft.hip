@@ -593,7 +593,10 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const T *x, const
     C *s = reinterpret_cast<C *>(conv_lds);
     const int ncol = CONV_NCOL;
     int64_t b = blockIdx.y;
-    int c0 = blockIdx.x * ncol;
+    // neighbouring column groups share the cache lines of x and work: keep
+    // them on one XCD (gridDim.x is a multiple of 8, so every row of the grid
+    // starts on XCD 0)
+    int c0 = (int)xcd_block(blockIdx.x, gridDim.x) * ncol;
     Segment sg = seg[s0 + b];
     // the mean from the segment's SEG_NB partials: one per lane of the first
     // wave, a butterfly over the lanes (the same order in every block)
@@ -741,7 +744,7 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const C *work, co
     C *s = reinterpret_cast<C *>(conv_lds);
     const int ncol = CONV_NCOL;
     int64_t b = blockIdx.y;
-    int c0 = blockIdx.x * ncol;
+    int c0 = (int)xcd_block(blockIdx.x, gridDim.x) * ncol;   // as in k_conv_col_fwd
     Segment sg = seg[s0 + b];
     int64_t noba = sg.noba;
     const C *d = work + b * N;
@@ -783,7 +786,7 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const C *work, co
     if (threadIdx.x == 0) {
         double t = 0;
         for (int q = 0; q < CONV_COL_TPB / 64; ++q) t += sh[q];
-        partial_out[b * gridDim.x + blockIdx.x] = t;
+        partial_out[b * gridDim.x + c0 / ncol] = t;
     }
 }
 
