@@ -431,11 +431,14 @@ def run_cfg5(a):
     """Serving stream: Poisson arrivals of destripe jobs through the request
     loop (madam_server_*).  Per job: Nside uniform in {128, 256, 512, 1024},
     detectors in {1, 2, 4, 8, 16, 32}, samples per detector log-uniform in
-    [2^lo, 2^hi] at 100 Hz, baseline length uniform in 0.25..1 s, noise prior
-    on half of the jobs, FP32 TOD, mixed precision; 30 % of the jobs reuse a
-    scan already seen (same pointing and parameters, a new signal).  The
-    sample range is scaled down from the 2^18..2^25 of the full stream (see
-    the config line) so that the stream runs in about a minute."""
+    [2^lo, 2^hi] at 100 Hz (2^18..2^25 by default), baseline length uniform in
+    0.25..1 s, noise prior on half of the jobs, FP32 TOD, mixed precision; 30 %
+    of the jobs reuse a scan already seen (same pointing and parameters, a new
+    signal).  A feeder thread generates each job's pointing and signal on the
+    GPU at its arrival time (on a stream of its own) and submits the device
+    buffers; a job's inputs are freed when its result has been read.  The
+    feeder holds back while the inputs of the jobs in the system exceed
+    --input-cap GB (the time held back is reported)."""
     import threading
     import torch
     import libmadam_amd
@@ -457,69 +460,96 @@ def run_cfg5(a):
             scans.append(dict(nside=nside, ndet=ndet, nsamp=nsamp, base=base, prior=prior, key=len(scans) + 1))
             si = len(scans) - 1
         specs.append(si)
-    log("cfg5: %d jobs over %d scans, samples/det 2^%g..2^%g, rate %g/s" % (njobs, len(scans), lo, hi, rate))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, njobs))
+    total_samples = sum(scans[si]["ndet"] * scans[si]["nsamp"] for si in specs)
+    log("cfg5: %d jobs over %d scans, samples/det 2^%g..2^%g, %.3g samples in all, rate %g/s"
+        % (njobs, len(scans), lo, hi, total_samples, rate))
     freqs, psd = psd_table(fs)
-    data = []
-    for sc in scans:
-        pix, w = libmadam_amd.expand_pointing(SCAN | dict(fsample=fs), detectors(sc["ndet"]), sc["nside"], 0,
-                                              sc["nsamp"])
-        pars = dict(params(sc["nside"], fs, sc["base"]), kfilter=sc["prior"], nside_submap=min(16, sc["nside"]))
-        data.append(dict(pix=pix.reshape(-1), w=w.reshape(-1), pars=pars,
-                         ts=np.arange(sc["nsamp"]) / fs, per=np.array([0], np.int64)))
-    sigs = []
-    for k, si in enumerate(specs):
-        sc = scans[si]
-        r = np.random.default_rng(k)
-        n = sc["ndet"] * sc["nsamp"]
-        noise = np.cumsum(r.standard_normal(n).astype(np.float32) * 2e-5) + r.standard_normal(n).astype(np.float32) * 2e-4
-        sigs.append(noise.astype(np.float32).reshape(sc["ndet"], sc["nsamp"]))
+    torch.cuda.init()   # torch's runtime first, then the library's
     srv = libmadam_amd.Server(0, hbm_budget=200e9, max_inflight=a.inflight)
+    cap = a.input_cap * 1e9
+    lock = threading.Lock()
     ids = [None] * njobs
+    keep = {}
+    state = dict(pending=0.0, held=0.0, failed=None)
+    stop = threading.Event()
+    gen_stream = torch.cuda.Stream()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2024)
     t0 = time.perf_counter()
 
-    stop = threading.Event()
-
     def feeder():
-        t = 0.0
-        for k, si in enumerate(specs):
-            if stop.is_set():
-                return
-            t += rng.exponential(1.0 / rate)
-            dt = t0 + t - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
-            sc, d = scans[si], data[si]
-            ids[k] = srv.submit(d["pars"], np.full(sc["ndet"], 1.0 / SIGMA ** 2), d["ts"], d["pix"], d["w"], sigs[k],
-                                d["per"], np.ones(sc["ndet"], np.int64), np.zeros(sc["ndet"]), freqs,
-                                np.tile(psd, sc["ndet"]), nnz=3, scan_key=sc["key"], precision=1)
-            if (k + 1) % 250 == 0:
-                log("%d jobs submitted at %.1f s" % (k + 1, time.perf_counter() - t0))
+        with torch.cuda.stream(gen_stream):
+            for k, si in enumerate(specs):
+                if stop.is_set():
+                    return
+                dt = t0 + arrivals[k] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                sc = scans[si]
+                nbytes = 20.0 * sc["ndet"] * sc["nsamp"]
+                th0 = time.perf_counter()
+                while not stop.is_set():
+                    with lock:
+                        if state["pending"] + nbytes <= cap or state["pending"] == 0:
+                            state["pending"] += nbytes
+                            break
+                    time.sleep(0.002)
+                state["held"] += time.perf_counter() - th0
+                nd, ns = sc["ndet"], sc["nsamp"]
+                pix = torch.empty(nd, ns, dtype=torch.int32, device="cuda")
+                w = torch.empty(nd, ns, 3, dtype=torch.float32, device="cuda")
+                libmadam_amd.expand_pointing_device(SCAN | dict(fsample=fs), detectors(nd), sc["nside"], 0, ns, pix, w)
+                # white noise plus a random walk (1/f-like), in place
+                sig = torch.randn(nd, ns, generator=g, device="cuda").mul_(2e-4)
+                sig.add_(torch.cumsum(torch.randn(nd, ns, generator=g, device="cuda").mul_(2e-5), dim=1))
+                gen_stream.synchronize()
+                pars = dict(params(sc["nside"], fs, sc["base"]), kfilter=sc["prior"], nside_submap=min(16, sc["nside"]))
+                jid = srv.submit(pars, np.full(nd, 1.0 / SIGMA ** 2), np.arange(ns) / fs, pix.view(-1), w.view(-1),
+                                 sig, np.array([0], np.int64), np.ones(nd, np.int64), np.zeros(nd), freqs,
+                                 np.tile(psd, nd), nnz=3, scan_key=sc["key"], precision=1, on_device=True)
+                with lock:
+                    keep[jid] = (pix, w, sig, nbytes)
+                ids[k] = jid
+                if (k + 1) % 100 == 0:
+                    log("%d jobs submitted at %.1f s" % (k + 1, time.perf_counter() - t0))
 
     th = threading.Thread(target=feeder)
     th.start()
-    # results are collected (and released) while the stream runs, so that the
-    # finished jobs' maps do not pile up in host memory
-    lat, samples, steps, batch = [], 0, 0, []
-    for k in range(njobs):
-        while ids[k] is None:
-            time.sleep(0.001)
-        jid = ids[k]
-        try:
-            srv.wait(jid)
-        except Exception:
-            log("job %d failed; server stats %s" % (k, srv.stats()))
-            stop.set()
-            th.join()
-            raise
-        r = srv.result(jid, maps=False)
-        lat.append(r["latency"])
-        sc = scans[specs[k]]
-        samples += sc["ndet"] * sc["nsamp"]
-        steps += r["steps"]
-        batch.append(r["batch"])
-        srv.release(jid)
-        if (k + 1) % 500 == 0:
-            log("%d jobs done" % (k + 1))
+    # results are collected (and the jobs' inputs released) as the jobs finish
+    lat, samples, steps, batch = [0.0] * njobs, 0, 0, [0] * njobs
+    open_jobs, nxt, ndone = [], 0, 0
+    try:
+        while ndone < njobs:
+            while nxt < njobs and ids[nxt] is not None:
+                open_jobs.append(nxt)
+                nxt += 1
+            progressed = False
+            for k in list(open_jobs):
+                jid = ids[k]
+                if not srv.wait(jid, 0.0):
+                    continue
+                r = srv.result(jid, maps=False)
+                lat[k] = r["latency"]
+                sc = scans[specs[k]]
+                samples += sc["ndet"] * sc["nsamp"]
+                steps += r["steps"]
+                batch[k] = r["batch"]
+                srv.release(jid)
+                with lock:
+                    state["pending"] -= keep.pop(jid)[3]
+                open_jobs.remove(k)
+                ndone += 1
+                progressed = True
+                if ndone % 100 == 0:
+                    log("%d jobs done at %.1f s" % (ndone, time.perf_counter() - t0))
+            if not progressed:
+                time.sleep(0.002)
+    except Exception:
+        log("a job failed; server stats %s" % srv.stats())
+        stop.set()
+        th.join()
+        raise
     th.join()
     wall = time.perf_counter() - t0
     st = srv.stats()
@@ -531,14 +561,14 @@ def run_cfg5(a):
         "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 operator / fp64 refinement, dots and pixel matrices", "data": "synthetic",
         "config": {"workload": "serving stream: %d Poisson jobs at %g/s, Nside {128..1024}, det {1..32}, samples/det "
-                               "2^%g..2^%g (the full stream's 2^18..2^25 scaled down), baselines 0.25..1 s, prior on "
-                               "half, 30%% resident-scan reuse, 200 GB HBM budget, %d workers"
-                               % (njobs, rate, lo, hi, a.inflight),
+                               "log-uniform 2^%g..2^%g at 100 Hz, baselines 0.25..1 s, prior on half, FP32 TOD, 30%% "
+                               "resident-scan reuse, 200 GB HBM budget, %d workers; inputs generated on the GPU at "
+                               "arrival" % (njobs, rate, lo, hi, a.inflight),
                    "parallelism": "1 GPU"},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                      "mean": float(lat.mean()), "n": int(lat.size)},
-        "jobs_per_s": njobs / wall, "wall_s": wall, "server": st,
-        "mean_batch": float(np.mean(batch)),
+                      "mean": float(lat.mean()), "max": float(lat.max()), "n": int(lat.size)},
+        "jobs_per_s": njobs / wall, "wall_s": wall, "samples": samples, "pcg_steps": steps,
+        "feeder_held_s": state["held"], "server": st, "mean_batch": float(np.mean(batch)),
     }
 
 
@@ -590,9 +620,10 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=5)
     ap.add_argument("--workload", default="cfg3", choices=["cfg3", "cfg2", "cfg4", "cfg5"])
     ap.add_argument("--jobs", type=int, default=2000)
-    ap.add_argument("--log2min", type=float, default=12.0)
-    ap.add_argument("--log2max", type=float, default=16.0)
-    ap.add_argument("--rate", type=float, default=12.0)
+    ap.add_argument("--log2min", type=float, default=18.0)
+    ap.add_argument("--log2max", type=float, default=25.0)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--input-cap", type=float, default=40.0, help="cfg5: GB of job inputs held on the GPU at most")
     ap.add_argument("--inflight", type=int, default=4)
     ap.add_argument("--groups", type=int, default=0, help="cfg4: run only the first N detector subsets")
     ap.add_argument("--distinct", action="store_true", help="every detector on its own boresight")

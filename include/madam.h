@@ -158,8 +158,10 @@ int madam_expand_pointing_noalloc(const double *scan5, const double *dets3, int 
  * with equal non-zero scan_key share one resident set-up (pointing, pixel
  * matrices, filters, preconditioner: the key must identify the pointing and
  * the parameters), and those waiting together are solved as one batch of
- * right-hand sides.  The job's host buffers are read in place until it is
- * done.  precision as madam_solve_batch; signal float64 or float32 (tod_f32). */
+ * right-hand sides.  The job's buffers are read in place until it is done;
+ * on_device: pix, pixweights and signal are device memory of the server's
+ * GPU (the other arrays are host memory).  precision as madam_solve_batch;
+ * signal float64 or float32 (tod_f32). */
 void *madam_server_create(int device, double hbm_budget_bytes, int max_inflight);
 void madam_server_destroy(void *srv);
 /* returns the job id, or -1 when the job alone exceeds the budget */
@@ -167,7 +169,7 @@ long madam_server_submit(void *srv, const char *parstring, long ndet, const doub
                          const double *timestamps, const int32_t *pix, const float *pixweights, const void *signal,
                          int tod_f32, long nperiod, const long *periods, const long *npsd, long npsdtot,
                          const double *psdstarts, long npsdbin, const double *psdfreqs, long npsdval,
-                         const double *psdvals, unsigned long long scan_key, int precision);
+                         const double *psdvals, unsigned long long scan_key, int precision, int on_device);
 /* 0 done, 1 failed (madam_server_error), 2 not finished within timeout_s (< 0: wait) */
 int madam_server_wait(void *srv, long id, double timeout_s);
 const char *madam_server_error(void *srv, long id);

@@ -83,7 +83,7 @@ def lib():
         L.madam_server_destroy.argtypes = [vp]
         L.madam_server_submit.restype = lg
         L.madam_server_submit.argtypes = [vp, cp, lg, vp, lg, lg, vp, vp, vp, vp, i32, lg, vp, vp, lg, vp, lg, vp,
-                                          lg, vp, ct.c_ulonglong, i32]
+                                          lg, vp, ct.c_ulonglong, i32, i32]
         L.madam_server_wait.restype = i32
         L.madam_server_wait.argtypes = [vp, lg, ct.c_double]
         L.madam_server_error.restype = cp
@@ -340,14 +340,23 @@ class Server:
             pass
 
     def submit(self, pars, weights, timestamps, pixels, pixweights, signal, periods, npsd, psdstarts, psdfreqs,
-               psdvals, nnz=3, scan_key=0, precision=0):
-        """signal [ndet][nsamp] float64 or float32; the arrays are kept until the job is released."""
+               psdvals, nnz=3, scan_key=0, precision=0, on_device=False):
+        """signal [ndet][nsamp] float64 or float32; the arrays are kept until the job is released.
+        on_device: pixels (int32), pixweights (float32) and signal are contiguous device tensors."""
         w = np.ascontiguousarray(weights, np.float64)
         ts = np.ascontiguousarray(timestamps, np.float64)
-        pix = np.ascontiguousarray(pixels, np.int32)
-        pw = np.ascontiguousarray(pixweights, np.float32)
-        sig = np.ascontiguousarray(signal)
-        assert sig.dtype in (np.float64, np.float32)
+        if on_device:
+            pix, pw, sig = pixels, pixweights, signal
+            assert pix.is_contiguous() and pw.is_contiguous() and sig.is_contiguous()
+            assert str(pix.dtype) == "torch.int32" and str(pw.dtype) == "torch.float32"
+            f32 = str(sig.dtype) == "torch.float32"
+            assert f32 or str(sig.dtype) == "torch.float64"
+        else:
+            pix = np.ascontiguousarray(pixels, np.int32)
+            pw = np.ascontiguousarray(pixweights, np.float32)
+            sig = np.ascontiguousarray(signal)
+            assert sig.dtype in (np.float64, np.float32)
+            f32 = sig.dtype == np.float32
         per = np.ascontiguousarray(periods, np.int64)
         nps = np.ascontiguousarray(npsd, np.int64)
         pst = np.ascontiguousarray(psdstarts, np.float64)
@@ -357,9 +366,10 @@ class Server:
         ndet, nsamp = w.size, ts.size
         npsdbin = pfr.size
         jid = lib().madam_server_submit(self._h, dict2parstring(pars), ndet, _ptr(w), nsamp, int(nnz), _ptr(ts),
-                                        _ptr(pix), _ptr(pw), _ptr(sig), int(sig.dtype == np.float32), per.size,
+                                        _ptr(pix), _ptr(pw), _ptr(sig), int(f32), per.size,
                                         _ptr(per), _ptr(nps), pv.size // max(npsdbin, 1), _ptr(pst), npsdbin,
-                                        _ptr(pfr), pv.size, _ptr(pv), int(scan_key), int(precision))
+                                        _ptr(pfr), pv.size, _ptr(pv), int(scan_key), int(precision),
+                                        int(bool(on_device)))
         if jid < 0:
             raise MadamError("job exceeds the server's HBM budget")
         self._jobs[jid] = (keep, ndet)

@@ -22,8 +22,8 @@
 //     streams from their own worker threads, so their work overlaps on the
 //     device; a job arriving while others run is admitted as soon as a worker
 //     and memory are free, without waiting for a batch to drain.
-// The jobs' host buffers are read in place and must stay valid until the job
-// is done.
+// The jobs' buffers are read in place and must stay valid until the job is
+// done; the pointing and the signal may be host or device memory (on_device).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,7 +60,7 @@ struct Job {
     const int32_t *pix = nullptr;
     const float *w = nullptr;
     const void *signal = nullptr;
-    int tod_f32 = 0, precision = 0;
+    int tod_f32 = 0, precision = 0, on_device = 0;
     unsigned long long scan_key = 0;
     size_t est_bytes = 0;
     // state
@@ -209,7 +209,7 @@ struct Server {
                 for (auto jt = std::next(it); jt != pending.end() && grp.size() < 64; ++jt) {
                     const Job &o = **jt;
                     if (o.scan_key == head->scan_key && o.precision == head->precision &&
-                        o.tod_f32 == head->tod_f32 && o.pars == head->pars)
+                        o.tod_f32 == head->tod_f32 && o.on_device == head->on_device && o.pars == head->pars)
                         grp.push_back(*jt);
                 }
             size_t need = 0;
@@ -266,19 +266,22 @@ void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res, La
                                   h.nperiod, h.periods, h.npsd, h.npsdtot, h.psdstarts, h.npsdbin, h.psdfreqs,
                                   h.npsdval, h.psdvals, 0));
             // the context keeps its own device copies of the pointing
-            check(madam_set_pointing(ctx, (int32_t *)h.pix, (float *)h.w, 0));
+            check(madam_set_pointing(ctx, (int32_t *)h.pix, (float *)h.w, h.on_device));
         }
         long sz[6];
         check(madam_get_sizes(ctx, sz));
         long n = h.ndet * h.nsamp;
         int R = (int)grp.size();
-        bool single64 = R == 1 && h.precision == 0;
-        if (single64 && !h.tod_f32) {
+        bool single64 = R == 1 && h.precision == 0 && !h.tod_f32;
+        size_t es = h.tod_f32 ? 4 : 8;
+        if (single64 && !h.on_device) {
             // the FP64 path of destripe() (it reads a host signal and leaves it unchanged here)
             std::vector<double> tod((const double *)h.signal, (const double *)h.signal + n);
             check(madam_solve(ctx, tod.data(), 0));
+        } else if (R == 1 && h.on_device && !single64) {
+            // one device signal: the batched solve reads it in place
+            check(madam_solve_batch(ctx, 1, h.signal, h.tod_f32, 1, h.precision));
         } else {
-            size_t es = h.tod_f32 ? 4 : 8;
             auto fail = [&](const char *what, hipError_t e) {
                 size_t fr = 0, tot = 0;
                 (void)hipMemGetInfo(&fr, &tot);
@@ -303,12 +306,14 @@ void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res, La
             // copies on the worker's own stream: the legacy stream would have
             // to wait for every blocking stream, which fails while another
             // worker captures its PCG step into a graph
+            hipMemcpyKind kind = h.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
             for (int k = 0; k < R && e == hipSuccess; ++k)
-                e = hipMemcpyAsync((char *)lane.stage + es * n * k, grp[k]->signal, es * n, hipMemcpyHostToDevice,
-                                   lane.stream);
+                e = hipMemcpyAsync((char *)lane.stage + es * n * k, grp[k]->signal, es * n, kind, lane.stream);
             if (e == hipSuccess) e = hipStreamSynchronize(lane.stream);
             if (e != hipSuccess) fail("staging copy failed", e);
-            check(madam_solve_batch(ctx, R, lane.stage, h.tod_f32, 1, h.precision));
+            // (the FP64 path works on a copy of the signal: it may write it)
+            if (single64) check(madam_solve(ctx, (double *)lane.stage, 1));
+            else check(madam_solve_batch(ctx, R, lane.stage, h.tod_f32, 1, h.precision));
         }
         check(madam_get_sizes(ctx, sz));
         for (int k = 0; k < R; ++k) {
@@ -320,7 +325,7 @@ void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res, La
             j.binmap.assign((size_t)j.npix * j.nmap, 0.0);
             j.hits.assign(j.npix, 0);
             j.aa.assign((size_t)j.ndet * j.nbase, 0.0);
-            if (single64 && !h.tod_f32) {
+            if (single64) {
                 check(madam_get_maps(ctx, j.map.data(), j.binmap.data(), j.hits.data(), nullptr, nullptr, nullptr));
                 check(madam_get_baselines(ctx, j.aa.data(), nullptr, nullptr));
                 double st[1];
@@ -453,7 +458,7 @@ MADAM_API long madam_server_submit(void *srv, const char *parstring, long ndet, 
                                    const void *signal, int tod_f32, long nperiod, const long *periods,
                                    const long *npsd, long npsdtot, const double *psdstarts, long npsdbin,
                                    const double *psdfreqs, long npsdval, const double *psdvals,
-                                   unsigned long long scan_key, int precision)
+                                   unsigned long long scan_key, int precision, int on_device)
 {
     Server *s = (Server *)srv;
     auto j = std::make_shared<Job>();
@@ -478,6 +483,7 @@ MADAM_API long madam_server_submit(void *srv, const char *parstring, long ndet, 
     j->psdvals = psdvals;
     j->scan_key = scan_key;
     j->precision = precision;
+    j->on_device = on_device;
     long nside = std::max(parse_long(j->pars, "nside_map", 512), parse_long(j->pars, "nside_cross", 0));
     double base = parse_double(j->pars, "base_first", 1.0) * parse_double(j->pars, "fsample", 1.0);
     long nbase = (long)(nsamp / std::max(1.0, base)) + nperiod + 1;
