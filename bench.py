@@ -541,7 +541,7 @@ def run_cfg5(a):
                 open_jobs.remove(k)
                 ndone += 1
                 progressed = True
-                if ndone % 100 == 0:
+                if ndone % 200 == 0:
                     log("%d jobs done at %.1f s" % (ndone, time.perf_counter() - t0))
             if not progressed:
                 time.sleep(0.002)
