@@ -1071,11 +1071,38 @@ __global__ void __launch_bounds__(BTPB) k_bupdate_p_pw(const BatchScal *sc, cons
     }
 }
 
+// one right-hand side: p = z + beta p in baseline order (coalesced) and pw
+// written scattered to the slots, in place of a pass gathering p through the
+// slot order
+template <typename T>
+__global__ void k_bupdate_p_pw1(const BatchScal *sc, const int *active, T *__restrict__ p, const T *__restrict__ z,
+                                int64_t n, const int32_t *__restrict__ bslot, const T *__restrict__ dwb,
+                                T *__restrict__ pw)
+{
+    bool on = active[0] != 0;
+    T beta = (T)sc[0].beta;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        T v = p[i];
+        T w = dwb[i];
+        int32_t sl = bslot[i];
+        if (on) {
+            v = z[i] + beta * v;
+            p[i] = v;
+        }
+        pw[sl] = on ? v * w : (T)0;
+    }
+}
+
 template <typename T>
 void batch_update_p_pw(hipStream_t s, int R, const BatchScal *sc, const int *active, T *p, const T *z, int64_t nbtot,
                        const int32_t *bslot, const T *dwb, T *pw)
 {
     if (nbtot <= 0) return;
+    if (R == 1) {
+        k_bupdate_p_pw1<T><<<(unsigned)bgrid(nbtot, BTPB, 4096 * 4), BTPB, 0, s>>>(sc, active, p, z, nbtot, bslot, dwb,
+                                                                                  pw);
+        return;
+    }
     unsigned g = (unsigned)((nbtot + BPW_TB - 1) / BPW_TB);
     BATCH_DISPATCH_R(R, (k_bupdate_p_pw<T, R><<<g, BTPB, 0, s>>>(sc, active, p, z, nbtot, bslot, dwb, pw)));
 }

@@ -405,9 +405,9 @@ static void pcg_step(Context &c, BatchState &B, const Vec<T> &v, double lim, int
     int R = B.R;
     int64_t nb = c.nbtot;
     if (tm) tm->st.start(s);
-    // R > 1: the previous step's p update (or the start) left pw ready; the
-    // FP32 inner solve keeps no r_old (beta from ap . z, stage 4)
-    const bool fuse = R > 1, lean = std::is_same<T, float>::value;
+    // the previous step's p update (or the start) left pw ready; the FP32
+    // inner solve keeps no r_old (beta from ap . z, stage 4)
+    const bool fuse = true, lean = std::is_same<T, float>::value;
     batch_operator<T>(c, B, v.p, v.ap, v.pw, v.m, v.part, B.active.p, tm, fuse);
     batch_dots<T>(s, 0, R, v.p, v.ap, nullptr, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 0, B.sc.p, B.active.p, lim, iter_min, iter_max);
@@ -417,10 +417,7 @@ static void pcg_step(Context &c, BatchState &B, const Vec<T> &v, double lim, int
     if (tm) tm->pc.stop(s);
     batch_dots<T>(s, 1, R, v.r, v.z, lean ? v.ap : v.ro, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, lean ? 4 : 1, B.sc.p, B.active.p, lim, iter_min, iter_max, nullptr, stall);
-    if (fuse)
-        batch_update_p_pw<T>(s, R, B.sc.p, B.active.p, v.p, v.z, nb, c.d_bslot.p, Ops<T>::dwb(c, B), v.pw);
-    else
-        batch_update_p<T>(s, R, B.sc.p, B.active.p, v.p, v.z, nb);
+    batch_update_p_pw<T>(s, R, B.sc.p, B.active.p, v.p, v.z, nb, c.d_bslot.p, Ops<T>::dwb(c, B), v.pw);
     if (tm) {
         tm->st.stop(s);
         B.t_scatter += tm->sc.ms() * 1e-3;
@@ -458,7 +455,7 @@ static int64_t batch_pcg(Context &c, BatchState &B, const Vec<T> &v, const T *b,
     batch_dots<T>(s, 1, R, v.r, v.z, v.r, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 2, B.sc.p, B.active.p, lim, iter_min, iter_max, allow);
     // the first step's pixel-side input (the steps leave it for the next one)
-    if (R > 1) batch_pw<T>(s, R, v.p, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), B.active.p, v.pw);
+    batch_pw<T>(s, R, v.p, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), B.active.p, v.pw);
     int gi = std::is_same<T, float>::value ? 1 : 0;
     // profiling times every part with events, so the steps run as launched
     bool graph = B.use_graph && c.segcg.empty() && !c.profile;
@@ -825,7 +822,7 @@ static void iterate_t(Context &c, BatchState &B, const T *b, int niter)
     batch_copy<T>(s, v.p, v.z, nb * R);
     batch_dots<T>(s, 1, R, v.r, v.z, v.r, c.d_rmask.p, nb, B.dots.p);
     batch_final(s, R, B.dots.p, 2, B.sc.p, B.active.p, 0.0, 0, 1 << 30, B.ones.p);
-    if (R > 1) batch_pw<T>(s, R, v.p, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), B.active.p, v.pw);
+    batch_pw<T>(s, R, v.p, nb, c.d_bord.p, c.d_bslot.p, Ops<T>::dwb_s(c, B), Ops<T>::dwb(c, B), B.active.p, v.pw);
     const int big = 1 << 30;
     StepTimers tmr;
     bool graph = B.use_graph && c.segcg.empty() && !c.profile;

@@ -1233,6 +1233,8 @@ static int pcg_steps(Context &c, int nmax, bool converge)
             if (rz == 0) last = true;
         }
         if (!last) {
+            // (FP64: the slot-order gather of p measured faster than writing
+            // pw scattered from the p update, which the FP32 path does)
             launch_pcg_update_p(s, c.d_scal.p, c.d_p.p, c.d_z.p, nb);
             launch_pw_slots(s, c.d_p.p, c.d_bord.p, c.d_dwb_s.p, c.d_pw.p, nb);
         }
