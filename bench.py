@@ -624,7 +624,7 @@ def main():
     ap.add_argument("--log2max", type=float, default=25.0)
     ap.add_argument("--rate", type=float, default=4.0)
     ap.add_argument("--input-cap", type=float, default=40.0, help="cfg5: GB of job inputs held on the GPU at most")
-    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--inflight", type=int, default=6)
     ap.add_argument("--groups", type=int, default=0, help="cfg4: run only the first N detector subsets")
     ap.add_argument("--distinct", action="store_true", help="every detector on its own boresight")
     ap.add_argument("--small", action="store_true", help="a reduced shape for quick checks (not a bench number)")

@@ -16,6 +16,9 @@
 //   * resident scans: a job whose scan key matches a kept context skips the
 //     configuration and pointing set-up (pixel grouping, projection tables)
 //     and, nna being bitwise the same, the filters and preconditioner;
+//   * large jobs (an estimate above an eighth of the budget) take at most
+//     half of the workers, so that a few long solves cannot hold every worker
+//     while short jobs queue behind them;
 //   * continuous batching: jobs waiting on the same scan with the same
 //     settings are solved together as one batched PCG over their right-hand
 //     sides (madam_solve_batch), and jobs of different scans run on their own
@@ -140,7 +143,10 @@ struct Server {
     // statistics
     long n_done = 0, n_failed = 0, n_batches = 0, n_batched_jobs = 0, n_resident_hits = 0, n_evicted = 0;
     size_t peak_bytes = 0;
-    int max_concurrent = 0, running = 0;
+    int max_concurrent = 0, running = 0, running_large = 0;
+
+    bool is_large(size_t est) const { return est > budget / 8; }
+    int max_large() const { return std::max(1, workers / 2); }
     long n_mem_evicted = 0;
     size_t min_free = SIZE_MAX;
 
@@ -214,6 +220,7 @@ struct Server {
                 }
             size_t need = 0;
             for (auto &j : grp) need += j->est_bytes;
+            if (is_large(head->est_bytes) && running_large >= max_large()) continue;
             if (res) need = need > res->bytes ? need - res->bytes : 0;   // the set-up is already resident
             if (res) res->busy = true;   // not a candidate for eviction while room is made
             if (!make_room(need)) {
@@ -222,6 +229,7 @@ struct Server {
             }
             for (auto &j : grp) pending.remove(j);
             for (auto &j : grp) j->state = 1;
+            if (is_large(head->est_bytes)) running_large++;
             inflight_bytes += need;
             if (head->scan_key) busy_keys.insert(head->scan_key);
             *res_out = res;
@@ -394,6 +402,7 @@ void Server::run_group(std::vector<std::shared_ptr<Job>> &grp, Resident *res, La
     }
     if (h.scan_key) busy_keys.erase(h.scan_key);
     running--;
+    if (is_large(h.est_bytes)) running_large--;
     cv_done.notify_all();
     cv_work.notify_all();
 }
