@@ -358,10 +358,12 @@ def run_cfg4(a):
         for r in range(R):
             sig[r] = make_tod_gpu(torch, pix, w, nside, nd, ns, fs, 1000 + 97 * gi + r, torch.float32, sky=False)
         torch.cuda.synchronize()
+        log("group %d: %d realisations generated" % (gi, R))
         t0 = time.perf_counter()
         m.solve_batch(sig, precision=1, on_device=True)
         torch.cuda.synchronize()
         t_solve = time.perf_counter() - t0
+        log("group %d: batched solve done in %.2f s" % (gi, t_solve))
         info = m.batch_result(0)["info"]
         m.iterate_batch(a.warmup)
         torch.cuda.synchronize()
