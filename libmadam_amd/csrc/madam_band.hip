@@ -323,12 +323,24 @@ __global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_
         ns[threadIdx.x] = c < nch ? ch[c].n : 0;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
+    // (256 threads: every load of a thread is issued before its LDS stores)
+    constexpr int PT = 64 * BAND_CH / 256;
+    V in[PT];
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+        int t = threadIdx.x + k * 256;
         int j = t / BAND_CH, i = t % BAND_CH;   // chunk j of the block, row i
-        tile[j * BAND_TP + i] = i < ns[j] ? (V)v[b0s[j] + i] : (V)0;
+        in[k] = i < ns[j] ? (V)v[b0s[j] + i] : (V)0;
+    }
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+        int t = threadIdx.x + k * 256;
+        tile[(t / BAND_CH) * BAND_TP + t % BAND_CH] = in[k];
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+        int t = threadIdx.x + k * 256;
         int i = t / 64, j = t % 64;
         vi[(w * BAND_CH + i) * 64 + j] = tile[j * BAND_TP + i];
     }
@@ -349,12 +361,23 @@ __global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int6
         b0s[threadIdx.x] = c < nch ? ch[c].b0 : 0;
         ns[threadIdx.x] = c < nch ? ch[c].n : 0;
     }
-    for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
-        int i = t / 64, j = t % 64;
-        tile[j * BAND_TP + i] = vi[(w * BAND_CH + i) * 64 + j];
+    // (256 threads: every load of a thread is issued before its LDS stores)
+    constexpr int PT = 64 * BAND_CH / 256;
+    V in[PT];
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+        int t = threadIdx.x + k * 256;
+        in[k] = vi[(w * BAND_CH + t / 64) * 64 + t % 64];
+    }
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+        int t = threadIdx.x + k * 256;
+        tile[(t % 64) * BAND_TP + t / 64] = in[k];
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < 64 * BAND_CH; t += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+        int t = threadIdx.x + k * 256;
         int j = t / BAND_CH, i = t % BAND_CH;
         if (i < ns[j]) v[b0s[j] + i] = (T)tile[j * BAND_TP + i];
     }
