@@ -619,13 +619,31 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_fwd(const T *x, const
     // the stage twiddles, read from LDS by every round of the FFT
     C *tw = s + conv_lds_elems(tot);
     for (int i = threadIdx.x; i < (L1 > 1 ? L1 / 2 : 1); i += blockDim.x) tw[i] = CxT<C>::sub1(P)[i];
-    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-        int col = t % ncol, n1 = t / ncol;
-        int64_t jj = 2 * ((int64_t)n1 * L2 + c0 + col);
-        C v;
-        v.x = jj < noba ? (S)((double)xs[jj] - mean) : (S)0;
-        v.y = jj + 1 < noba ? (S)((double)xs[jj + 1] - mean) : (S)0;
-        s[lp(bitrev(n1, logL1) * ncol + col)] = v;
+    // (FP32 work: eight points per thread at a time, every load issued before
+    // the LDS stores; FP64 keeps one at a time, the registers of eight cost
+    // it more occupancy than the loads gain)
+    constexpr int NB = sizeof(C) == 8 ? 8 : 1;
+    for (int t0 = threadIdx.x; t0 < tot; t0 += NB * blockDim.x) {
+        T a[NB], b2[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            int t = t0 + k * blockDim.x;
+            int64_t jj = 2 * ((int64_t)(t / ncol) * L2 + c0 + t % ncol);
+            a[k] = t < tot && jj < noba ? xs[jj] : (T)0;
+            b2[k] = t < tot && jj + 1 < noba ? xs[jj + 1] : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            int t = t0 + k * blockDim.x;
+            if (t < tot) {
+                int col = t % ncol, n1 = t / ncol;
+                int64_t jj = 2 * ((int64_t)n1 * L2 + c0 + col);
+                C v;
+                v.x = jj < noba ? (S)((double)a[k] - mean) : (S)0;
+                v.y = jj + 1 < noba ? (S)((double)b2[k] - mean) : (S)0;
+                s[lp(bitrev(n1, logL1) * ncol + col)] = v;
+            }
+        }
     }
     __syncthreads();
     lds_fft8<C>(s, L1, logL1, 3, tw, false);   // CONV_NCOL = 8 columns
@@ -696,10 +714,19 @@ __global__ void k_conv_rows(C *work, int64_t N, int L1, int L2, int logL2, FftPl
     // natural order in and out of LDS (no bank conflicts): the forward row
     // FFTs are DIF (bit-reversed output), the inverse ones DIT (bit-reversed
     // input), and the spectral step addresses the bit-reversed positions
-    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-        int r = t >> logL2, n = t & (L2 - 1);
-        int row = r == 0 ? ra : rb;
-        s[lp(n * 2 + r)] = d[(int64_t)row * L2 + n];
+    constexpr int NB = sizeof(C) == 8 ? 8 : 1;   // as in k_conv_col_fwd
+    for (int t0 = threadIdx.x; t0 < tot; t0 += NB * blockDim.x) {
+        C in[NB];   // every load issued before the LDS stores
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            int t = t0 + k * blockDim.x;
+            if (t < tot) in[k] = d[(int64_t)((t >> logL2) == 0 ? ra : rb) * L2 + (t & (L2 - 1))];
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            int t = t0 + k * blockDim.x;
+            if (t < tot) s[lp((t & (L2 - 1)) * 2 + (t >> logL2))] = in[k];
+        }
     }
     __syncthreads();
     lds_fft8_dif<C>(s, L2, logL2, 1, tw, false);   // 2 rows
@@ -765,9 +792,22 @@ __global__ void __launch_bounds__(CONV_COL_TPB) k_conv_col_inv(const C *work, co
         int64_t n2 = c0 + col;
         C w = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * (int64_t)(threadIdx.x / ncol)) & (N - 1));
         C ws = tw2(CxT<C>::na(P), CxT<C>::nb(P), (n2 * kst) & (N - 1));   /* N is a power of two */
-        for (int k1 = threadIdx.x / ncol; k1 < L1; k1 += kst) {
-            s[lp(bitrev(k1, logL1) * ncol + col)] = cmulc(d[(int64_t)k1 * L2 + n2], w);
-            w = cmul(w, ws);
+        constexpr int NB = sizeof(C) == 8 ? 8 : 1;   // as in k_conv_col_fwd
+        for (int k10 = threadIdx.x / ncol; k10 < L1; k10 += NB * kst) {
+            C in[NB];   // every load issued before the twiddle products and LDS stores
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                int k1 = k10 + k * kst;
+                if (k1 < L1) in[k] = d[(int64_t)k1 * L2 + n2];
+            }
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                int k1 = k10 + k * kst;
+                if (k1 < L1) {
+                    s[lp(bitrev(k1, logL1) * ncol + col)] = cmulc(in[k], w);
+                    w = cmul(w, ws);
+                }
+            }
         }
     }
     __syncthreads();
