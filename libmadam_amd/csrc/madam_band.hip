@@ -605,7 +605,20 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
     int64_t c0 = gfirst[g];
     int n = gn[g];
     double st = (Sg && on) ? Sg[g * KD + lane] : 0.0;
-    for (int t = lane; t < n * KD; t += 64) s0l[t] = s0[c0 * KD + t];
+    // eight loads in flight before their LDS stores (n <= BAND_G)
+    for (int k0 = 0; k0 < KD; k0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            int t = lane + 64 * (k0 + k);
+            v[k] = k0 + k < KD && t < n * KD ? s0[c0 * KD + t] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            int t = lane + 64 * (k0 + k);
+            if (k0 + k < KD && t < n * KD) s0l[t] = v[k];
+        }
+    }
     // lane t holds the transfer-matrix index of the group's chunk t (n <= 64)
     int myct = lane < n ? (cmap ? cmap[c0 + lane] : (int32_t)(c0 + lane)) : 0;
     __syncthreads();
@@ -655,7 +668,20 @@ __global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, in
         // window of chain positions t0 .. t0 + m - 1: groups lg0 .. lg0 + m - 1
         int64_t lg0 = dir == 0 ? t0 : n - t0 - m;
         __syncthreads();
-        for (int64_t t = lane; t < m * KD; t += 64) cgl[t] = cg[(g0 + lg0) * KD + t];
+        // eight loads in flight before their LDS stores (m <= W)
+        for (int k0 = 0; k0 < KD; k0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                int64_t t = lane + 64 * (k0 + k);
+                v[k] = k0 + k < KD && t < m * KD ? cg[(g0 + lg0) * KD + t] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                int64_t t = lane + 64 * (k0 + k);
+                if (k0 + k < KD && t < m * KD) cgl[t] = v[k];
+            }
+        }
         // lane t holds the group-matrix index of the window's group t
         int mygt = lane < m ? (gmap ? gmap[g0 + lg0 + lane] : (int32_t)(g0 + lg0 + lane)) : 0;
         __syncthreads();
