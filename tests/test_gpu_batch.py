@@ -258,3 +258,24 @@ def test_wide_batch_64():
         a0, m0 = offset_free(res[k]["aa"], res[k]["map"], good)
         assert mix[k]["info"]["rr_final"] < 1e-20
         assert rel_rms(m1, m0) <= 1e-6 and rel_rms(a1, a0) <= 1e-6, (k, rel_rms(m1, m0), rel_rms(a1, a0))
+
+
+@pytest.mark.parametrize("width", [20, 30])
+def test_fp64_batch_wide_band(width):
+    """Three right-hand sides with band widths of the wider kernel classes
+    (kd 19 and 29: the carry chains stage more than 64 terms per lane group)
+    against three single solves."""
+    case = ring(prior=True, ndet=4, nsamp=60000, nperiod=2,
+                extra=dict(precond_width_min=width, precond_width_max=width))
+    sig = signals_for(case, 3, seed=300)
+    res, st = batch(case, sig, 0)
+    assert st["precond_kind"] == 3
+    for k in range(3):
+        s1 = single(case, sig[k])
+        assert s1["stats"]["nband"] == width
+        r = res[k]
+        errs = dict(map=rel_rms(r["map"], s1["map"]), aa=rel_rms(r["aa"], s1["aa"]))
+        print("width", width, "rhs", k, "steps", r["info"]["steps"], "single", s1["niter"], errs)
+        assert abs(r["info"]["steps"] - s1["niter"]) <= 1
+        for name, e in errs.items():
+            assert e <= 1e-10, (name, e)
