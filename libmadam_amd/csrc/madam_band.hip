@@ -290,97 +290,53 @@ __device__ inline const double *li_chunk(const double *Li, const int32_t *bmap, 
     return Li + w * (int64_t)BAND_CH * K * 64 + lane;
 }
 
-// Vectors of the sweeps in the factor's interleaved layout: row i of chunk c
-// at V[((c / 64) * CH + i) * 64 + c % 64].
-template <typename V>
-__device__ inline V *vi_chunk(V *v, int64_t c)
-{
-    return v + (c >> 6) * (int64_t)BAND_CH * 64 + (c & 63);
-}
-
-// To and from the interleaved layout, one 64-chunk block per workgroup through
-// a padded LDS tile: both sides access whole 512-byte rows.
+// The sweeps read r and write z in the solve's own layout (baseline order,
+// one right-hand side after another) through one LDS tile per 64-chunk block:
+// load k of the wave reads rows 0..63 of chunk k of the block (one contiguous
+// piece of the vector), stored at tile[k * BAND_TP + row]; each lane then walks
+// its own chunk's rows at tile[lane * BAND_TP + i] (row stride 1, no bank
+// conflicts with the padding).  Rows past a chunk's end read as 0 and are
+// never written back.
 static constexpr int BAND_TP = BAND_CH + 1;
+static_assert(BAND_CH == 64, "the tile maps one row of a chunk to each lane of the wave");
 // Strides of one right-hand side (grid dimension y) in the solve's buffers:
-// v = interleaved vectors (ri, zi), s = per-chunk carry states, g = per-group states
+// s = per-chunk carry states, g = per-group states
 struct BandRhs {
-    int64_t v, s, g;
+    int64_t s, g;
 };
 
-template <typename T, typename V>
-__global__ void __launch_bounds__(256) k_band_to_int(const ChunkInfo *ch, int64_t nch, const T *v, V *vi,
-                                                     int64_t vstride, BandRhs st)
+// lane j: chunk w * 64 + j of the block; its first row and row count to LDS
+__device__ inline void band_tile_chunks(const ChunkInfo *ch, int64_t nch, int64_t c, int64_t *b0s, int *ns)
 {
-    v += blockIdx.y * vstride;
-    vi += blockIdx.y * st.v;
-    __shared__ V tile[64 * BAND_TP];
-    __shared__ int64_t b0s[64];
-    __shared__ int ns[64];
-    int64_t w = blockIdx.x;
-    if (threadIdx.x < 64) {
-        int64_t c = w * 64 + threadIdx.x;
-        b0s[threadIdx.x] = c < nch ? ch[c].b0 : 0;
-        ns[threadIdx.x] = c < nch ? ch[c].n : 0;
-    }
+    b0s[threadIdx.x] = c < nch ? ch[c].b0 : 0;
+    ns[threadIdx.x] = c < nch ? ch[c].n : 0;
     __syncthreads();
-    // (256 threads: every load of a thread is issued before its LDS stores)
-    constexpr int PT = 64 * BAND_CH / 256;
-    V in[PT];
-#pragma unroll
-    for (int k = 0; k < PT; ++k) {
-        int t = threadIdx.x + k * 256;
-        int j = t / BAND_CH, i = t % BAND_CH;   // chunk j of the block, row i
-        in[k] = i < ns[j] ? (V)v[b0s[j] + i] : (V)0;
-    }
-#pragma unroll
-    for (int k = 0; k < PT; ++k) {
-        int t = threadIdx.x + k * 256;
-        tile[(t / BAND_CH) * BAND_TP + t % BAND_CH] = in[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PT; ++k) {
-        int t = threadIdx.x + k * 256;
-        int i = t / 64, j = t % 64;
-        vi[(w * BAND_CH + i) * 64 + j] = tile[j * BAND_TP + i];
-    }
 }
 
-template <typename T, typename V>
-__global__ void __launch_bounds__(256) k_band_from_int(const ChunkInfo *ch, int64_t nch, const V *vi, T *v,
-                                                       int64_t vstride, BandRhs st)
+template <typename V>
+__device__ inline void band_tile_in(const V *v, const int64_t *b0s, const int *ns, V *tile)
 {
-    v += blockIdx.y * vstride;
-    vi += blockIdx.y * st.v;
-    __shared__ V tile[64 * BAND_TP];
-    __shared__ int64_t b0s[64];
-    __shared__ int ns[64];
-    int64_t w = blockIdx.x;
-    if (threadIdx.x < 64) {
-        int64_t c = w * 64 + threadIdx.x;
-        b0s[threadIdx.x] = c < nch ? ch[c].b0 : 0;
-        ns[threadIdx.x] = c < nch ? ch[c].n : 0;
-    }
-    // (256 threads: every load of a thread is issued before its LDS stores)
-    constexpr int PT = 64 * BAND_CH / 256;
-    V in[PT];
+    const int lane = threadIdx.x;
+    constexpr int PB = 128 / sizeof(V);   // loads in flight per lane before their LDS stores
 #pragma unroll
-    for (int k = 0; k < PT; ++k) {
-        int t = threadIdx.x + k * 256;
-        in[k] = vi[(w * BAND_CH + t / 64) * 64 + t % 64];
-    }
+    for (int k0 = 0; k0 < 64; k0 += PB) {
+        V in[PB];
 #pragma unroll
-    for (int k = 0; k < PT; ++k) {
-        int t = threadIdx.x + k * 256;
-        tile[(t % 64) * BAND_TP + t / 64] = in[k];
+        for (int k = 0; k < PB; ++k) in[k] = lane < ns[k0 + k] ? v[b0s[k0 + k] + lane] : (V)0;
+#pragma unroll
+        for (int k = 0; k < PB; ++k) tile[(k0 + k) * BAND_TP + lane] = in[k];
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PT; ++k) {
-        int t = threadIdx.x + k * 256;
-        int j = t / BAND_CH, i = t % BAND_CH;
-        if (i < ns[j]) v[b0s[j] + i] = (T)tile[j * BAND_TP + i];
-    }
+}
+
+template <typename V>
+__device__ inline void band_tile_out(V *v, const int64_t *b0s, const int *ns, const V *tile)
+{
+    __syncthreads();
+    const int lane = threadIdx.x;
+#pragma unroll 16
+    for (int k = 0; k < 64; ++k)
+        if (lane < ns[k]) v[b0s[k] + lane] = tile[k * BAND_TP + lane];
 }
 
 // compact row-major copy [i][m] of lane 0's column of block reps[u] (a block
@@ -443,9 +399,9 @@ struct BandPF {
 // (a map load inside the row loop would wait for every load in flight), with
 // element m of row i at lic[(i * K + m) * MS]: MS = 64 for the interleaved
 // factor, MS = 1 for a block's compact copy read with scalar loads; r and
-// y are the chunk's columns of vectors in the same interleaved layout (row i
-// at [i * 64]), so the lanes of a wave read and write consecutive addresses.
-template <int KD, int MS = 64, typename V = double>
+// y are the chunk's rows in the block's LDS tile (row i at [i * RS]); y may be
+// r (each row is read before it is written).
+template <int KD, int MS = 64, typename V = double, int RS = 1>
 __device__ inline void fwd_sweep(const double *lic, int n, const V *r, double *acc, V *y, bool store,
                                  bool zero_rhs)
 {
@@ -459,7 +415,7 @@ __device__ inline void fwd_sweep(const double *lic, int n, const V *r, double *a
                 const double *row = lic + (int64_t)i * K * MS;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * MS];
-                rh[u] = zero_rhs ? 0.0 : (double)r[(int64_t)i * 64];
+                rh[u] = zero_rhs ? 0.0 : (double)r[(int64_t)i * RS];
             }
         }
 #pragma unroll
@@ -470,14 +426,15 @@ __device__ inline void fwd_sweep(const double *lic, int n, const V *r, double *a
 #pragma unroll
                 for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - rw[u][m] * v;
                 acc[KD - 1] = -rw[u][KD] * v;
-                if (store) y[(int64_t)i * 64] = (V)v;
+                if (store) y[(int64_t)i * RS] = (V)v;
             }
         }
     }
 }
 
 // backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
-template <int KD, int MS = 64, typename V = double>
+// (x may be y, as for fwd_sweep)
+template <int KD, int MS = 64, typename V = double, int RS = 1>
 __device__ inline void bwd_sweep(const double *lic, int n, const V *y, double *xs, V *x, bool store,
                                  bool zero_rhs)
 {
@@ -491,7 +448,7 @@ __device__ inline void bwd_sweep(const double *lic, int n, const V *y, double *x
                 const double *row = lic + (int64_t)i * K * MS;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * MS];
-                rh[u] = zero_rhs ? 0.0 : (double)y[(int64_t)i * 64];
+                rh[u] = zero_rhs ? 0.0 : (double)y[(int64_t)i * RS];
             }
         }
 #pragma unroll
@@ -505,7 +462,7 @@ __device__ inline void bwd_sweep(const double *lic, int n, const V *y, double *x
 #pragma unroll
                 for (int m = KD - 1; m > 0; --m) xs[m] = xs[m - 1];
                 xs[0] = xi;
-                if (store) x[(int64_t)i * 64] = (V)xi;
+                if (store) x[(int64_t)i * RS] = (V)xi;
             }
         }
     }
@@ -534,26 +491,32 @@ __global__ void __launch_bounds__(64) k_band_response(const ChunkInfo *ch, int64
     for (int m = 0; m < KD; ++m) Tb[(c * KD + m) * KD + j] = st[m];
 }
 
+// zero-carry forward sweep of every chunk: its outgoing partial state s0
 template <int KD, typename V>
 __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                   const double *Lb, const int32_t *uidx,
-                                                  const double *__restrict__ Lu, const V *r, double *s0,
-                                                  BandRhs st)
+                                                  const double *__restrict__ Lu, const V *r, int64_t vstride,
+                                                  double *s0, BandRhs st)
 {
-    r += blockIdx.y * st.v;
+    __shared__ V tile[64 * BAND_TP];
+    __shared__ int64_t b0s[64];
+    __shared__ int ns[64];
+    r += blockIdx.y * vstride;
     s0 += blockIdx.y * st.s;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= nch) return;
-    ChunkInfo ci = ch[c];
+    band_tile_chunks(ch, nch, c, b0s, ns);
+    band_tile_in(r, b0s, ns, tile);
+    if (c >= nch) return;   // past the last barrier
+    const V *tl = tile + threadIdx.x * BAND_TP;
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
     int32_t ui = uidx[blockIdx.x];
     if (ui >= 0)
-        fwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ci.n),
-                            vi_chunk(r, c), acc, (V *)nullptr, false, false);
+        fwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ns[0]), tl, acc,
+                            (V *)nullptr, false, false);
     else
-        fwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), ci.n, vi_chunk(r, c), acc, (V *)nullptr, false, false);
+        fwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), ns[threadIdx.x], tl, acc, (V *)nullptr, false, false);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
 }
@@ -736,64 +699,80 @@ __global__ void __launch_bounds__(64) k_band_group_mats(const int64_t *gfirst, c
     }
 }
 
-// final forward sweep with true carries writes y into z, then the zero-carry
-// backward sweep of the same chunk gives its backward partial state
+// final forward sweep with true carries gives y (written to z, which the last
+// backward sweep reads), then the zero-carry backward sweep of the same chunk,
+// on y in LDS, gives its backward partial state
 template <int KD, typename V>
 __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                        const double *Lb, const int32_t *uidx,
-                                                       const double *__restrict__ Lu, const V *r,
+                                                       const double *__restrict__ Lu, const V *r, int64_t vstride,
                                                        const double *sinf, V *z, double *s0b, BandRhs st)
 {
-    r += blockIdx.y * st.v;
-    z += blockIdx.y * st.v;
+    __shared__ V tile[64 * BAND_TP];
+    __shared__ int64_t b0s[64];
+    __shared__ int ns[64];
+    r += blockIdx.y * vstride;
+    z += blockIdx.y * vstride;
     sinf += blockIdx.y * st.s;
     s0b += blockIdx.y * st.s;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= nch) return;
-    ChunkInfo ci = ch[c];
-    double acc[KD];
+    band_tile_chunks(ch, nch, c, b0s, ns);
+    band_tile_in(r, b0s, ns, tile);
+    if (c < nch) {
+        V *tl = tile + threadIdx.x * BAND_TP;
+        double acc[KD];
 #pragma unroll
-    for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
-    int32_t ui = uidx[blockIdx.x];
-    if (ui >= 0) {
-        const double *lu = Lu + (int64_t)ui * BAND_CH * (KD + 1);
-        int n = __builtin_amdgcn_readfirstlane(ci.n);
-        fwd_sweep<KD, 1, V>(lu, n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
+        for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
+        int32_t ui = uidx[blockIdx.x];
+        if (ui >= 0) {
+            const double *lu = Lu + (int64_t)ui * BAND_CH * (KD + 1);
+            int n = __builtin_amdgcn_readfirstlane(ns[0]);
+            fwd_sweep<KD, 1, V>(lu, n, tl, acc, tl, true, false);
 #pragma unroll
-        for (int m = 0; m < KD; ++m) acc[m] = 0;
-        bwd_sweep<KD, 1, V>(lu, n, (const V *)vi_chunk(z, c), acc, (V *)nullptr, false, false);
-    } else {
-        const double *lic = li_chunk(Lb, bmap, c, KD + 1);
-        fwd_sweep<KD, 64, V>(lic, ci.n, vi_chunk(r, c), acc, vi_chunk(z, c), true, false);
+            for (int m = 0; m < KD; ++m) acc[m] = 0;
+            bwd_sweep<KD, 1, V>(lu, n, tl, acc, (V *)nullptr, false, false);
+        } else {
+            const double *lic = li_chunk(Lb, bmap, c, KD + 1);
+            int n = ns[threadIdx.x];
+            fwd_sweep<KD, 64, V>(lic, n, tl, acc, tl, true, false);
 #pragma unroll
-        for (int m = 0; m < KD; ++m) acc[m] = 0;
-        bwd_sweep<KD, 64, V>(lic, ci.n, (const V *)vi_chunk(z, c), acc, (V *)nullptr, false, false);
+            for (int m = 0; m < KD; ++m) acc[m] = 0;
+            bwd_sweep<KD, 64, V>(lic, n, tl, acc, (V *)nullptr, false, false);
+        }
+#pragma unroll
+        for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
     }
-#pragma unroll
-    for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
+    band_tile_out(z, b0s, ns, tile);
 }
 
+// final backward sweep with true carries: z = x
 template <int KD, typename V>
 __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t nch, const int32_t *bmap,
                                                   const double *Lb, const int32_t *uidx,
                                                   const double *__restrict__ Lu, const double *sinb, V *z,
-                                                  BandRhs st)
+                                                  int64_t vstride, BandRhs st)
 {
+    __shared__ V tile[64 * BAND_TP];
+    __shared__ int64_t b0s[64];
+    __shared__ int ns[64];
     sinb += blockIdx.y * st.s;
-    z += blockIdx.y * st.v;
+    z += blockIdx.y * vstride;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (c >= nch) return;
-    ChunkInfo ci = ch[c];
-    double xs[KD];
+    band_tile_chunks(ch, nch, c, b0s, ns);
+    band_tile_in(z, b0s, ns, tile);
+    if (c < nch) {
+        V *tl = tile + threadIdx.x * BAND_TP;
+        double xs[KD];
 #pragma unroll
-    for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
-    int32_t ui = uidx[blockIdx.x];
-    if (ui >= 0)
-        bwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ci.n),
-                            (const V *)vi_chunk(z, c), xs, vi_chunk(z, c), true, false);
-    else
-        bwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), ci.n, (const V *)vi_chunk(z, c), xs, vi_chunk(z, c), true,
-                             false);
+        for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
+        int32_t ui = uidx[blockIdx.x];
+        if (ui >= 0)
+            bwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ns[0]), tl, xs,
+                                tl, true, false);
+        else
+            bwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), ns[threadIdx.x], tl, xs, tl, true, false);
+    }
+    band_tile_out(z, b0s, ns, tile);
 }
 
 // ---------------------------------------------------------------- host side
@@ -812,14 +791,12 @@ struct BandImpl {
     double *Gf = nullptr, *Gb = nullptr, *cg = nullptr, *Sg = nullptr;
     // runs of bitwise-equal chunks and blocks of the factor (see li_chunk)
     int32_t *cmap = nullptr, *bmap = nullptr;
-    // right-hand side and solution in the interleaved layout (see vi_chunk)
-    double *ri = nullptr, *zi = nullptr;
     // runs of groups with equal chunk sequences (equal group matrices)
     int32_t *gmap = nullptr;
     // uniform blocks: uidx[w] = slot of the block's compact factor copy in Lu, or -1
     int32_t *uidx = nullptr;
     double *Lu = nullptr;
-    int rcap = 1;   // right-hand sides the solve buffers (ri, zi, states) hold
+    int rcap = 1;   // right-hand sides the carry-state buffers hold
 };
 
 static int kd_class(int kd)
@@ -957,8 +934,6 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
     if (bp.nchunk > 0) {
         im->NCP = (bp.nchunk + 63) / 64 * 64;
         MADAM_CHECK(hipMalloc(&im->Li, sizeof(double) * im->NCP * BAND_CH * K));
-        MADAM_CHECK(hipMalloc(&im->ri, sizeof(double) * im->NCP * BAND_CH));
-        MADAM_CHECK(hipMalloc(&im->zi, sizeof(double) * im->NCP * BAND_CH));
         int64_t tot = im->NCP * BAND_CH * K;
         unsigned g = (unsigned)std::min<int64_t>((tot + 255) / 256, 65535 * 16);
         k_band_relayout<<<g, 256, 0, s>>>(im->ch, bp.nchunk, im->NCP, K, BAND_CH, bp.L, im->Li);
@@ -1049,17 +1024,13 @@ int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *
     return 0;
 }
 
-// grow the solve buffers (interleaved vectors and carry states) to nrhs right-hand sides
+// grow the carry-state buffers to nrhs right-hand sides
 void band_reserve(const BandPrec &bp, int nrhs)
 {
     BandImpl *im = (BandImpl *)bp.impl;
     if (!im || bp.nchunk == 0 || nrhs <= im->rcap) return;
     int KD = bp.kd;
     int64_t nc = bp.nchunk, ng = im->ngrp > 0 ? im->ngrp : 1;
-    for (double **b : {&im->ri, &im->zi}) {
-        (void)hipFree(*b);
-        MADAM_CHECK(hipMalloc(b, sizeof(double) * im->NCP * BAND_CH * nrhs));
-    }
     for (double **b : {&im->s0f, &im->s0b, &im->sinf, &im->sinb}) {
         (void)hipFree(*b);
         MADAM_CHECK(hipMalloc(b, sizeof(double) * nc * KD * nrhs));
@@ -1086,28 +1057,27 @@ void band_apply(hipStream_t s, const BandPrec &bp, const T *r, T *z, int nrhs, i
     dim3 g((unsigned)((bp.nchunk + 63) / 64), R);
     dim3 gs((unsigned)im->nseg_used, R);   // one 64-lane wave per segment
     dim3 gg((unsigned)im->ngrp, R);        // one wave per chunk group
-    BandRhs st{im->NCP * BAND_CH, bp.nchunk * KD, (im->ngrp > 0 ? im->ngrp : 1) * KD};
-    // the interleaved vectors in the type of r and z (FP32 halves their
+    BandRhs st{bp.nchunk * KD, (im->ngrp > 0 ? im->ngrp : 1) * KD};
+    // the LDS tiles hold r and z in their own type (FP32 halves their
     // traffic); the sweeps compute in FP64
     using V = T;
-    V *ri = (V *)im->ri, *zi = (V *)im->zi;
     BAND_DISPATCH(KD, ({
-        k_band_to_int<T, V><<<g, 256, 0, s>>>(im->ch, bp.nchunk, r, ri, vstride, st);
-        k_band_fwd0<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, ri, im->s0f, st);
+        k_band_fwd0<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, r, vstride,
+                                            im->s0f, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, nullptr,
                                                  nullptr, im->cg, st);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, im->gmap, 0, im->Sg, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, im->Sg,
                                                  im->sinf, nullptr, st);
-        k_band_fwd1_bwd0<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, ri, im->sinf,
-                                                 zi, im->s0b, st);
+        k_band_fwd1_bwd0<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, r, vstride,
+                                                 im->sinf, z, im->s0b, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, nullptr,
                                                  nullptr, im->cg, st);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, im->gmap, 1, im->Sg, st);
         k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, im->Sg,
                                                  im->sinb, nullptr, st);
-        k_band_bwd1<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->sinb, zi, st);
-        k_band_from_int<T, V><<<g, 256, 0, s>>>(im->ch, bp.nchunk, zi, z, vstride, st);
+        k_band_bwd1<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->sinb, z,
+                                            vstride, st);
     }));
 }
 
@@ -1120,11 +1090,9 @@ void band_free(BandPrec &bp)
     if (im) {
         if (im->Li) (void)hipFree(im->Li);
         if (im->cmap) (void)hipFree(im->cmap);
-        if (im->ri) (void)hipFree(im->ri);
         if (im->gmap) (void)hipFree(im->gmap);
         if (im->uidx) (void)hipFree(im->uidx);
         if (im->Lu) (void)hipFree(im->Lu);
-        if (im->zi) (void)hipFree(im->zi);
         if (im->bmap) (void)hipFree(im->bmap);
         (void)hipFree(im->ch);
         (void)hipFree(im->seg_first);

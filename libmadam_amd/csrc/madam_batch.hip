@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *__restrict__ it
             // records in flight with the current baseline values
             auto load_lane = [&](PixW *e, int jb) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) e[u] = load_rec<RL>(ente, (int64_t)(uint32_t)(jb + u * G + g));
+                for (int u = 0; u < U; ++u) e[u] = load_rec_nt<RL>(ente, (int64_t)(uint32_t)(jb + u * G + g));
             };
             PixW ea[U], eb[U];
             auto step = [&](const PixW *cur, PixW *nxt, int j0) -> bool {
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *__restrict__ it
             __shared__ int4 srec[BTPB / 64][CH];
             int4 *rl = srec[wib];
             PixW rr{0, 0.0f, 0.0f, 0.0f};
-            if (lane < CH) rr = load_rec<RL>(ente, (int64_t)(uint32_t)lane);
+            if (lane < CH) rr = load_rec_nt<RL>(ente, (int64_t)(uint32_t)lane);
             for (int j0 = 0;; j0 += CH) {
                 bool more = j0 + CH < n;
                 if (lane < CH) {
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(BTPB) k_bscatter(const BsItem *__restrict__ it
                 T v[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) v[u] = pw[(int64_t)e[u].pix * R + r];
-                if (more && lane < CH) rr = load_rec<RL>(ente, (int64_t)(uint32_t)(j0 + CH + lane));   // in flight
+                if (more && lane < CH) rr = load_rec_nt<RL>(ente, (int64_t)(uint32_t)(j0 + CH + lane));   // in flight
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     a0 += (T)e[u].w0 * v[u];
@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(BTPB) k_bgather(int ndet, int64_t nsamp, int64
         for (; i + 3 * G < i1; i += 4 * G) {
             PixW e[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) e[u] = load_rec<RL>(rec, rbase + i + u * G);
+            for (int u = 0; u < 4; ++u) e[u] = load_rec_nt<RL>(rec, rbase + i + u * G);
             T mv[4][NMAP];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(BTPB) k_bgather(int ndet, int64_t nsamp, int64
             }
         }
         for (; i < i1; i += G) {
-            PixW e = load_rec<RL>(rec, rbase + i);
+            PixW e = load_rec_nt<RL>(rec, rbase + i);
             if (e.pix >= 0) {
                 const T *mq = m + (int64_t)(e.pix >> dsh) * NMAP * R + r;
                 acc += NMAP == 1 ? mq[0] : (T)e.w0 * mq[0] + (T)e.w1 * mq[R] + (T)e.w2 * mq[2 * R];
@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(BTPB) k_bgather_w(int ndet, int64_t nsamp, int
     // masks stay off the VALU.
     int4 *rl = reinterpret_cast<int4 *>(reinterpret_cast<T *>(bg_lds) + (BTPB / 64) * run_len * (R + 1)) + wib * CH;
     PixW rr{0, 0.0f, 0.0f, 0.0f};
-    if (lane < CH) rr = load_rec<RL>(recd, (int64_t)(uint32_t)(i + lane));
+    if (lane < CH) rr = load_rec_nt<RL>(recd, (int64_t)(uint32_t)(i + lane));
     for (;;) {
         bool last = i + CH >= ie;
         int nkc = kc;
@@ -639,7 +639,7 @@ __global__ void __launch_bounds__(BTPB) k_bgather_w(int ndet, int64_t nsamp, int
             for (int c = 0; c < NMAP; ++c) mv[u][c] = mq[c * R];
         }
         // the next chunk's records, in flight with the map values
-        if (more && lane < CH) rr = load_rec<RL>(recd, (int64_t)(uint32_t)(ni + lane));
+        if (more && lane < CH) rr = load_rec_nt<RL>(recd, (int64_t)(uint32_t)(ni + lane));
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (NMAP == 1) {

@@ -72,6 +72,34 @@ __device__ inline PixW load_rec(const void *base, int64_t i)
     return r;
 }
 
+// load_rec with the non-temporal hint: the projections stream their records
+// once per pass, so the records should not displace the map and baseline
+// rows that the same kernel reads many times from L2
+typedef int rec_v4i __attribute__((ext_vector_type(4)));
+template <int RL>
+__device__ inline PixW load_rec_nt(const void *base, int64_t i)
+{
+    PixW r;
+    if constexpr (RL == REC16) {
+        rec_v4i v = __builtin_nontemporal_load((const rec_v4i *)base + i);
+        r.pix = v.x;
+        r.w0 = __int_as_float(v.y);
+        r.w1 = __int_as_float(v.z);
+        r.w2 = __int_as_float(v.w);
+    } else if constexpr (RL == REC12) {
+        const int32_t *q = (const int32_t *)base + 3 * i;
+        r.pix = __builtin_nontemporal_load(q);
+        r.w0 = 1.0f;
+        r.w1 = __int_as_float(__builtin_nontemporal_load(q + 1));
+        r.w2 = __int_as_float(__builtin_nontemporal_load(q + 2));
+    } else {
+        r.pix = __builtin_nontemporal_load((const int32_t *)base + i);
+        r.w0 = 1.0f;
+        r.w1 = r.w2 = 0.0f;
+    }
+    return r;
+}
+
 // two- and four-wide vector types of the projections' element type
 template <typename T>
 struct VecT;

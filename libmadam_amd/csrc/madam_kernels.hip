@@ -922,7 +922,7 @@ __global__ void __launch_bounds__(256) k_gather_flat(int ndet, int64_t nsamp, in
             PixW r[FG_D];
 #pragma unroll
             for (int d = 0; d < FG_D; ++d)
-                r[d] = load_rec<RL>(pixw, (int64_t)(d0 + (d < nd ? d : 0)) * nsamp + t0 + tl);
+                r[d] = load_rec_nt<RL>(pixw, (int64_t)(d0 + (d < nd ? d : 0)) * nsamp + t0 + tl);
             int q[FG_D];
             bool ld[FG_D];
 #pragma unroll

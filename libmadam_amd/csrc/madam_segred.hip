@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
             int64_t idx = t0 + u * 64 + lane;
-            e[u] = load_rec<RL>(ent, idx < nent ? idx : nent - 1);
+            e[u] = load_rec_nt<RL>(ent, idx < nent ? idx : nent - 1);
         }
 #pragma unroll
         for (int u = 0; u < SR_PER_LANE; ++u) {
