@@ -589,19 +589,28 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
     double tr[KD];
 #pragma unroll
     for (int q = 0; q < KD; ++q) tr[q] = 0.0;
+    // Each chunk's incoming state replaces its additive term in LDS (the slot
+    // is read first) and the states go out in one coalesced pass after the
+    // chain: a global store inside the loop would make every step wait for
+    // the previous step's store (the wait before the step's arithmetic counts
+    // stores and loads alike).
     for (int t = 0; t < n; ++t) {
         int lc = dir == 0 ? t : n - 1 - t;
-        int64_t c = c0 + lc;
         int64_t ct = __builtin_amdgcn_readlane(myct, lc);
         if (ct != cur) {   // uniform over the wave
             cur = ct;
 #pragma unroll
             for (int q = 0; q < KD; ++q) tr[q] = on ? T[(ct * KD + lane) * KD + q] : 0.0;
         }
-        if (sin && on) sin[c * KD + lane] = st;
-        st = chain_step<KD>(on ? s0l[lc * KD + lane] : 0.0, tr, st);
+        double v = on ? s0l[lc * KD + lane] : 0.0;
+        if (sin && on) s0l[lc * KD + lane] = st;
+        st = chain_step<KD>(v, tr, st);
     }
     if (cg && on) cg[g * KD + lane] = st;
+    if (sin) {
+        __syncthreads();
+        for (int t = lane; t < n * KD; t += 64) sin[c0 * KD + t] = s0l[t];
+    }
 }
 
 // wave per segment: S_{g+1} = cg[g] + Gm[g] S_g over the segment's groups in
@@ -648,18 +657,21 @@ __global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, in
         // lane t holds the group-matrix index of the window's group t
         int mygt = lane < m ? (gmap ? gmap[g0 + lg0 + lane] : (int32_t)(g0 + lg0 + lane)) : 0;
         __syncthreads();
+        // (the groups' incoming states staged in LDS as in k_band_chain_local)
         for (int64_t t = 0; t < m; ++t) {
             int lw = (int)(dir == 0 ? t : m - 1 - t);   // position in the window
-            int64_t g = g0 + lg0 + lw;
             int64_t gt = __builtin_amdgcn_readlane(mygt, lw);
             if (gt != cur) {
                 cur = gt;
 #pragma unroll
                 for (int q = 0; q < KD; ++q) tr[q] = on ? Gm[(gt * KD + lane) * KD + q] : 0.0;
             }
-            if (on) Sg[g * KD + lane] = st;
-            st = chain_step<KD>(on ? cgl[lw * KD + lane] : 0.0, tr, st);
+            double v = on ? cgl[lw * KD + lane] : 0.0;
+            if (on) cgl[lw * KD + lane] = st;
+            st = chain_step<KD>(v, tr, st);
         }
+        __syncthreads();
+        for (int64_t t = lane; t < m * KD; t += 64) Sg[(g0 + lg0) * KD + t] = cgl[t];
     }
 }
 
