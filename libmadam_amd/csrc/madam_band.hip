@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
 // group's true incoming state Sg[g] (writing every chunk's incoming state sin).
 // Both chain levels run one wave per chain with lane m < KD holding state
 // component m.  The chain's additive terms are staged in LDS before the loop,
-// the state is broadcast with readlane, and the transfer matrix stays in
+// the state is broadcast through LDS, and the transfer matrix stays in
 // registers while consecutive steps share it (runs of equal chunks or groups,
 // through cmap / gmap); it is reloaded only where the run changes.  The sums
 // are taken in the same order on every path.
@@ -538,15 +538,20 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
 // (depth log2 KD instead of a KD-long dependent chain of FMAs on the path
 // from one step's state to the next)
 template <int KD>
-__device__ inline double chain_step(double v, const double (&tr)[KD], double st)
+__device__ inline double chain_step(double v, const double (&tr)[KD], double st, double *sb)
 {
+    // the state goes to the wave's LDS row and comes back as KD broadcast
+    // reads (cheaper per step than 2 KD lane reads through the scalar unit)
+    sb[threadIdx.x & 63] = st;
+    __builtin_amdgcn_wave_barrier();
     double pr[KD];
 #pragma unroll
-    for (int q = 0; q < KD; ++q) pr[q] = tr[q] * readlane_d(st, q);
+    for (int q = 0; q < KD; ++q) pr[q] = tr[q] * sb[q];
 #pragma unroll
     for (int w = 1; w < KD; w *= 2)
 #pragma unroll
         for (int q = 0; q + w < KD; q += 2 * w) pr[q] += pr[q + w];
+    __builtin_amdgcn_wave_barrier();
     return v + pr[0];
 }
 
@@ -557,6 +562,7 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
                                                          BandRhs rs)
 {
     __shared__ double s0l[BAND_G * KD];
+    __shared__ double stb[64];   // the state, broadcast to the lanes each step
     s0 += blockIdx.y * rs.s;
     if (Sg) Sg += blockIdx.y * rs.g;
     if (sin) sin += blockIdx.y * rs.s;
@@ -604,7 +610,7 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
         }
         double v = on ? s0l[lc * KD + lane] : 0.0;
         if (sin && on) s0l[lc * KD + lane] = st;
-        st = chain_step<KD>(v, tr, st);
+        st = chain_step<KD>(v, tr, st, stb);
     }
     if (cg && on) cg[g * KD + lane] = st;
     if (sin) {
@@ -622,6 +628,7 @@ __global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, in
 {
     constexpr int W = 64;
     __shared__ double cgl[W * KD];
+    __shared__ double stb[64];
     cg += blockIdx.y * rs.g;
     Sg += blockIdx.y * rs.g;
     int lane = threadIdx.x & 63;
@@ -668,7 +675,7 @@ __global__ void __launch_bounds__(64) k_band_chain_top(const int64_t *sfirst, in
             }
             double v = on ? cgl[lw * KD + lane] : 0.0;
             if (on) cgl[lw * KD + lane] = st;
-            st = chain_step<KD>(v, tr, st);
+            st = chain_step<KD>(v, tr, st, stb);
         }
         __syncthreads();
         for (int64_t t = lane; t < m * KD; t += 64) Sg[(g0 + lg0) * KD + t] = cgl[t];
