@@ -709,8 +709,15 @@ __global__ void k_conv_rows(C *work, int64_t N, int L1, int L2, int logL2, FftPl
     C *d = work + b * N;
     const S *f = fc + (int64_t)seg[s0 + b].ipsd * fc_stride;
     int tot = 2 * L2;
-    C *tw = s + conv_lds_elems(tot);   // the stage twiddles in LDS
-    for (int i = threadIdx.x; i < (L2 > 1 ? L2 / 2 : 1); i += blockDim.x) tw[i] = CxT<C>::sub2(P)[i];
+    // the stage twiddles: in LDS for FP32; FP64 reads the (cached) table, so
+    // that its 32 KB of points alone set the blocks per CU (5, not 4)
+    constexpr bool TWL = sizeof(C) == 8;
+    const C *tw = CxT<C>::sub2(P);
+    if (TWL) {
+        C *twl = s + conv_lds_elems(tot);
+        for (int i = threadIdx.x; i < (L2 > 1 ? L2 / 2 : 1); i += blockDim.x) twl[i] = tw[i];
+        tw = twl;
+    }
     // natural order in and out of LDS (no bank conflicts): the forward row
     // FFTs are DIF (bit-reversed output), the inverse ones DIT (bit-reversed
     // input), and the spectral step addresses the bit-reversed positions
@@ -922,7 +929,8 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
             dim3 gc((unsigned)ncb, (unsigned)nb), gr((unsigned)(p.L1 / 2), (unsigned)nb);
             // the points, then the stage twiddles (L / 2 of them)
             size_t col_elems = conv_lds_elems((int)(p.L1 * CONV_NCOL)) + std::max<int64_t>(p.L1 / 2, 1),
-                   row_elems = conv_lds_elems((int)(2 * p.L2)) + std::max<int64_t>(p.L2 / 2, 1);
+                   row_elems = conv_lds_elems((int)(2 * p.L2)) + std::max<int64_t>(p.L2 / 2, 1),
+                   row_elems64 = conv_lds_elems((int)(2 * p.L2));   // (FP64 rows: no twiddles in LDS)
             if (f32) {
                 // FP32 work: the same passes on float2, half the bytes and LDS
                 float2 *w32 = (float2 *)work;
@@ -935,7 +943,7 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
             } else {
                 k_conv_col_fwd<T, double2><<<gc, CONV_COL_TPB, sizeof(double2) * col_elems, s>>>(
                     x, dseg, s0, partial + s0 * SEG_NB, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
-                k_conv_rows<double2><<<gr, FFT_TPB, sizeof(double2) * row_elems, s>>>(
+                k_conv_rows<double2><<<gr, FFT_TPB, sizeof(double2) * row_elems64, s>>>(
                     (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
                 k_conv_col_inv<T, double2><<<gc, CONV_COL_TPB, sizeof(double2) * col_elems, s>>>(
                     (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2, p);
