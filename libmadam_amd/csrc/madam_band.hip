@@ -293,11 +293,12 @@ __device__ inline const double *li_chunk(const double *Li, const int32_t *bmap, 
 // The sweeps read r and write z in the solve's own layout (baseline order,
 // one right-hand side after another) through one LDS tile per 64-chunk block:
 // load k of the wave reads rows 0..63 of chunk k of the block (one contiguous
-// piece of the vector), stored at tile[k * BAND_TP + row]; each lane then walks
-// its own chunk's rows at tile[lane * BAND_TP + i] (row stride 1, no bank
-// conflicts with the padding).  Rows past a chunk's end read as 0 and are
-// never written back.
-static constexpr int BAND_TP = BAND_CH + 1;
+// piece of the vector); row i of chunk j sits at tile[j * 64 + (i ^ j)], so
+// that both the loads (one chunk, all rows) and the sweeps (lane j on chunk j,
+// one row) spread over the banks without padding: the FP64 tile is then
+// exactly 32 KB, five blocks per CU (four with a padded row).  Each lane keeps
+// its own chunk's first row and row count, and the loads take them with lane
+// reads.  Rows past a chunk's end read as 0 and are never written back.
 static_assert(BAND_CH == 64, "the tile maps one row of a chunk to each lane of the wave");
 // Strides of one right-hand side (grid dimension y) in the solve's buffers:
 // s = per-chunk carry states, g = per-group states
@@ -305,16 +306,8 @@ struct BandRhs {
     int64_t s, g;
 };
 
-// lane j: chunk w * 64 + j of the block; its first row and row count to LDS
-__device__ inline void band_tile_chunks(const ChunkInfo *ch, int64_t nch, int64_t c, int64_t *b0s, int *ns)
-{
-    b0s[threadIdx.x] = c < nch ? ch[c].b0 : 0;
-    ns[threadIdx.x] = c < nch ? ch[c].n : 0;
-    __syncthreads();
-}
-
 template <typename V>
-__device__ inline void band_tile_in(const V *v, const int64_t *b0s, const int *ns, V *tile)
+__device__ inline void band_tile_in(const V *v, uint32_t b0, int n, V *tile)
 {
     const int lane = threadIdx.x;
     constexpr int PB = 128 / sizeof(V);   // loads in flight per lane before their LDS stores
@@ -322,21 +315,28 @@ __device__ inline void band_tile_in(const V *v, const int64_t *b0s, const int *n
     for (int k0 = 0; k0 < 64; k0 += PB) {
         V in[PB];
 #pragma unroll
-        for (int k = 0; k < PB; ++k) in[k] = lane < ns[k0 + k] ? v[b0s[k0 + k] + lane] : (V)0;
+        for (int k = 0; k < PB; ++k) {
+            const int nk = __builtin_amdgcn_readlane(n, k0 + k);
+            const uint32_t bk = (uint32_t)__builtin_amdgcn_readlane((int)b0, k0 + k);
+            in[k] = lane < nk ? v[bk + (uint32_t)lane] : (V)0;
+        }
 #pragma unroll
-        for (int k = 0; k < PB; ++k) tile[(k0 + k) * BAND_TP + lane] = in[k];
+        for (int k = 0; k < PB; ++k) tile[(k0 + k) * 64 + (lane ^ (k0 + k))] = in[k];
     }
     __syncthreads();
 }
 
 template <typename V>
-__device__ inline void band_tile_out(V *v, const int64_t *b0s, const int *ns, const V *tile)
+__device__ inline void band_tile_out(V *v, uint32_t b0, int n, const V *tile)
 {
     __syncthreads();
     const int lane = threadIdx.x;
 #pragma unroll 16
-    for (int k = 0; k < 64; ++k)
-        if (lane < ns[k]) v[b0s[k] + lane] = tile[k * BAND_TP + lane];
+    for (int k = 0; k < 64; ++k) {
+        const int nk = __builtin_amdgcn_readlane(n, k);
+        const uint32_t bk = (uint32_t)__builtin_amdgcn_readlane((int)b0, k);
+        if (lane < nk) v[bk + (uint32_t)lane] = tile[k * 64 + (lane ^ k)];
+    }
 }
 
 // compact row-major copy [i][m] of lane 0's column of block reps[u] (a block
@@ -399,11 +399,11 @@ struct BandPF {
 // (a map load inside the row loop would wait for every load in flight), with
 // element m of row i at lic[(i * K + m) * MS]: MS = 64 for the interleaved
 // factor, MS = 1 for a block's compact copy read with scalar loads; r and
-// y are the chunk's rows in the block's LDS tile (row i at [i * RS]); y may be
+// y are the chunk's rows in the block's LDS tile (row i at [i ^ key]); y may be
 // r (each row is read before it is written).
-template <int KD, int MS = 64, typename V = double, int RS = 1>
+template <int KD, int MS = 64, typename V = double>
 __device__ inline void fwd_sweep(const double *lic, int n, const V *r, double *acc, V *y, bool store,
-                                 bool zero_rhs)
+                                 bool zero_rhs, int key = 0)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
     for (int i0 = 0; i0 < n; i0 += BAND_PF) {
@@ -415,7 +415,7 @@ __device__ inline void fwd_sweep(const double *lic, int n, const V *r, double *a
                 const double *row = lic + (int64_t)i * K * MS;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * MS];
-                rh[u] = zero_rhs ? 0.0 : (double)r[(int64_t)i * RS];
+                rh[u] = zero_rhs ? 0.0 : (double)r[i ^ key];
             }
         }
 #pragma unroll
@@ -426,7 +426,7 @@ __device__ inline void fwd_sweep(const double *lic, int n, const V *r, double *a
 #pragma unroll
                 for (int m = 1; m < KD; ++m) acc[m - 1] = acc[m] - rw[u][m] * v;
                 acc[KD - 1] = -rw[u][KD] * v;
-                if (store) y[(int64_t)i * RS] = (V)v;
+                if (store) y[i ^ key] = (V)v;
             }
         }
     }
@@ -434,9 +434,9 @@ __device__ inline void fwd_sweep(const double *lic, int n, const V *r, double *a
 
 // backward sweep: L^T x = y, state xs[0..KD) = x of the KD rows following the chunk
 // (x may be y, as for fwd_sweep)
-template <int KD, int MS = 64, typename V = double, int RS = 1>
+template <int KD, int MS = 64, typename V = double>
 __device__ inline void bwd_sweep(const double *lic, int n, const V *y, double *xs, V *x, bool store,
-                                 bool zero_rhs)
+                                 bool zero_rhs, int key = 0)
 {
     constexpr int K = KD + 1, BAND_PF = BandPF<KD>::value;
     for (int i0 = n - 1; i0 >= 0; i0 -= BAND_PF) {
@@ -448,7 +448,7 @@ __device__ inline void bwd_sweep(const double *lic, int n, const V *y, double *x
                 const double *row = lic + (int64_t)i * K * MS;
 #pragma unroll
                 for (int m = 0; m < K; ++m) rw[u][m] = row[m * MS];
-                rh[u] = zero_rhs ? 0.0 : (double)y[(int64_t)i * RS];
+                rh[u] = zero_rhs ? 0.0 : (double)y[i ^ key];
             }
         }
 #pragma unroll
@@ -462,7 +462,7 @@ __device__ inline void bwd_sweep(const double *lic, int n, const V *y, double *x
 #pragma unroll
                 for (int m = KD - 1; m > 0; --m) xs[m] = xs[m - 1];
                 xs[0] = xi;
-                if (store) x[(int64_t)i * RS] = (V)xi;
+                if (store) x[i ^ key] = (V)xi;
             }
         }
     }
@@ -498,25 +498,25 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
                                                   const double *__restrict__ Lu, const V *r, int64_t vstride,
                                                   double *s0, BandRhs st)
 {
-    __shared__ V tile[64 * BAND_TP];
-    __shared__ int64_t b0s[64];
-    __shared__ int ns[64];
+    __shared__ V tile[64 * 64];
     r += blockIdx.y * vstride;
     s0 += blockIdx.y * st.s;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    band_tile_chunks(ch, nch, c, b0s, ns);
-    band_tile_in(r, b0s, ns, tile);
+    const uint32_t b0 = c < nch ? (uint32_t)ch[c].b0 : 0u;   // baseline rows < ndet * nsamp < 2^32
+    const int n = c < nch ? ch[c].n : 0;
+    band_tile_in(r, b0, n, tile);
     if (c >= nch) return;   // past the last barrier
-    const V *tl = tile + threadIdx.x * BAND_TP;
+    const int lane = threadIdx.x;
+    const V *tl = tile + lane * 64;
     double acc[KD];
 #pragma unroll
     for (int m = 0; m < KD; ++m) acc[m] = 0;
     int32_t ui = uidx[blockIdx.x];
     if (ui >= 0)
-        fwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ns[0]), tl, acc,
-                            (V *)nullptr, false, false);
+        fwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(n), tl, acc,
+                            (V *)nullptr, false, false, lane);
     else
-        fwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), ns[threadIdx.x], tl, acc, (V *)nullptr, false, false);
+        fwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), n, tl, acc, (V *)nullptr, false, false, lane);
 #pragma unroll
     for (int m = 0; m < KD; ++m) s0[c * KD + m] = acc[m];
 }
@@ -727,41 +727,40 @@ __global__ void __launch_bounds__(64) k_band_fwd1_bwd0(const ChunkInfo *ch, int6
                                                        const double *__restrict__ Lu, const V *r, int64_t vstride,
                                                        const double *sinf, V *z, double *s0b, BandRhs st)
 {
-    __shared__ V tile[64 * BAND_TP];
-    __shared__ int64_t b0s[64];
-    __shared__ int ns[64];
+    __shared__ V tile[64 * 64];
     r += blockIdx.y * vstride;
     z += blockIdx.y * vstride;
     sinf += blockIdx.y * st.s;
     s0b += blockIdx.y * st.s;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    band_tile_chunks(ch, nch, c, b0s, ns);
-    band_tile_in(r, b0s, ns, tile);
+    const uint32_t b0 = c < nch ? (uint32_t)ch[c].b0 : 0u;
+    const int nc = c < nch ? ch[c].n : 0;
+    band_tile_in(r, b0, nc, tile);
     if (c < nch) {
-        V *tl = tile + threadIdx.x * BAND_TP;
+        const int lane = threadIdx.x;
+        V *tl = tile + lane * 64;
         double acc[KD];
 #pragma unroll
         for (int m = 0; m < KD; ++m) acc[m] = sinf[c * KD + m];
         int32_t ui = uidx[blockIdx.x];
         if (ui >= 0) {
             const double *lu = Lu + (int64_t)ui * BAND_CH * (KD + 1);
-            int n = __builtin_amdgcn_readfirstlane(ns[0]);
-            fwd_sweep<KD, 1, V>(lu, n, tl, acc, tl, true, false);
+            int n = __builtin_amdgcn_readfirstlane(nc);
+            fwd_sweep<KD, 1, V>(lu, n, tl, acc, tl, true, false, lane);
 #pragma unroll
             for (int m = 0; m < KD; ++m) acc[m] = 0;
-            bwd_sweep<KD, 1, V>(lu, n, tl, acc, (V *)nullptr, false, false);
+            bwd_sweep<KD, 1, V>(lu, n, tl, acc, (V *)nullptr, false, false, lane);
         } else {
             const double *lic = li_chunk(Lb, bmap, c, KD + 1);
-            int n = ns[threadIdx.x];
-            fwd_sweep<KD, 64, V>(lic, n, tl, acc, tl, true, false);
+            fwd_sweep<KD, 64, V>(lic, nc, tl, acc, tl, true, false, lane);
 #pragma unroll
             for (int m = 0; m < KD; ++m) acc[m] = 0;
-            bwd_sweep<KD, 64, V>(lic, n, tl, acc, (V *)nullptr, false, false);
+            bwd_sweep<KD, 64, V>(lic, nc, tl, acc, (V *)nullptr, false, false, lane);
         }
 #pragma unroll
         for (int m = 0; m < KD; ++m) s0b[c * KD + m] = acc[m];
     }
-    band_tile_out(z, b0s, ns, tile);
+    band_tile_out(z, b0, nc, tile);
 }
 
 // final backward sweep with true carries: z = x
@@ -771,27 +770,27 @@ __global__ void __launch_bounds__(64) k_band_bwd1(const ChunkInfo *ch, int64_t n
                                                   const double *__restrict__ Lu, const double *sinb, V *z,
                                                   int64_t vstride, BandRhs st)
 {
-    __shared__ V tile[64 * BAND_TP];
-    __shared__ int64_t b0s[64];
-    __shared__ int ns[64];
+    __shared__ V tile[64 * 64];
     sinb += blockIdx.y * st.s;
     z += blockIdx.y * vstride;
     int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    band_tile_chunks(ch, nch, c, b0s, ns);
-    band_tile_in(z, b0s, ns, tile);
+    const uint32_t b0 = c < nch ? (uint32_t)ch[c].b0 : 0u;
+    const int nc = c < nch ? ch[c].n : 0;
+    band_tile_in((const V *)z, b0, nc, tile);
     if (c < nch) {
-        V *tl = tile + threadIdx.x * BAND_TP;
+        const int lane = threadIdx.x;
+        V *tl = tile + lane * 64;
         double xs[KD];
 #pragma unroll
         for (int m = 0; m < KD; ++m) xs[m] = sinb[c * KD + m];
         int32_t ui = uidx[blockIdx.x];
         if (ui >= 0)
-            bwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(ns[0]), tl, xs,
-                                tl, true, false);
+            bwd_sweep<KD, 1, V>(Lu + (int64_t)ui * BAND_CH * (KD + 1), __builtin_amdgcn_readfirstlane(nc), tl, xs,
+                                tl, true, false, lane);
         else
-            bwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), ns[threadIdx.x], tl, xs, tl, true, false);
+            bwd_sweep<KD, 64, V>(li_chunk(Lb, bmap, c, KD + 1), nc, tl, xs, tl, true, false, lane);
     }
-    band_tile_out(z, b0s, ns, tile);
+    band_tile_out(z, b0, nc, tile);
 }
 
 // ---------------------------------------------------------------- host side
