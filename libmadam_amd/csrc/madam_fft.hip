@@ -913,7 +913,7 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
     double *partial = work + batch * nof + (batch + 1) / 2 + 1;    // npart * SEG_NB doubles
     int logL1 = ilog2(p.L1), logL2 = ilog2(p.L2);
     int ncb = (int)(p.L2 / CONV_NCOL);
-    double *partial_out = partial + npart * SEG_NB;                 // batch * ncb doubles
+    double *partial_out = partial + npart * SEG_NB;                 // npart * ncb doubles (every segment's)
     bool fused = conv_fusable(p);
     // the input means of every segment at once (one launch fills the GPU)
     if (fused)
@@ -939,18 +939,15 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
                 k_conv_rows<float2><<<gr, FFT_TPB, sizeof(float2) * row_elems, s>>>(
                     w32, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows32, fc_stride, dseg, s0);
                 k_conv_col_inv<T, float2><<<gc, CONV_COL_TPB, sizeof(float2) * col_elems, s>>>(
-                    w32, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2, p);
+                    w32, dseg, s0, y, partial_out + s0 * ncb, p.N, (int)p.L1, logL1, (int)p.L2, p);
             } else {
                 k_conv_col_fwd<T, double2><<<gc, CONV_COL_TPB, sizeof(double2) * col_elems, s>>>(
                     x, dseg, s0, partial + s0 * SEG_NB, (double2 *)work, p.N, (int)p.L1, logL1, (int)p.L2, p);
                 k_conv_rows<double2><<<gr, FFT_TPB, sizeof(double2) * row_elems64, s>>>(
                     (double2 *)work, p.N, (int)p.L1, (int)p.L2, logL2, p, fc_rows, fc_stride, dseg, s0);
                 k_conv_col_inv<T, double2><<<gc, CONV_COL_TPB, sizeof(double2) * col_elems, s>>>(
-                    (const double2 *)work, dseg, s0, y, partial_out, p.N, (int)p.L1, logL1, (int)p.L2, p);
+                    (const double2 *)work, dseg, s0, y, partial_out + s0 * ncb, p.N, (int)p.L1, logL1, (int)p.L2, p);
             }
-            unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 4 - 1) / (FFT_TPB * 4));
-            if (gl > 1024) gl = 1024;
-            k_conv_sub_mean<T><<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(y, dseg, s0, partial_out, ncb);
             continue;
         }
         k_seg_partial<T><<<gp, FFT_TPB, 0, s>>>(x, 0, 1, dseg, s0, 1.0, partial);
@@ -966,6 +963,16 @@ void fft_convolve_segments(hipStream_t s, const FftPlan &p, const Segment *dseg,
         k_seg_partial<double><<<gp, FFT_TPB, 0, s>>>(work, nof, 0, dseg, s0, 1 / (double)nof, partial);
         k_seg_store<T><<<dim3(gl, (unsigned)nb), FFT_TPB, 0, s>>>(work, dseg, s0, nof, y, partial);
     }
+    // the fused passes' outputs less their segment means, every segment in one
+    // launch (the output y is read by nobody before this returns)
+    if (fused) {
+        unsigned gl = (unsigned)((nof / 2 + FFT_TPB * 16 - 1) / (FFT_TPB * 16));
+        if (gl > 1024) gl = 1024;
+        for (int64_t q0 = 0; q0 < nseg; q0 += 65535) {
+            int64_t nq = std::min<int64_t>(nseg - q0, 65535);
+            k_conv_sub_mean<T><<<dim3(gl, (unsigned)nq), FFT_TPB, 0, s>>>(y, dseg, q0, partial_out + q0 * ncb, ncb);
+        }
+    }
 }
 
 template void fft_convolve_segments<double>(hipStream_t, const FftPlan &, const Segment *, int64_t, const double *,
@@ -978,7 +985,7 @@ template void fft_convolve_segments<float>(hipStream_t, const FftPlan &, const S
 size_t fft_convolve_work_doubles(const FftPlan &p, int64_t batch, int64_t nseg_max)
 {
     return (size_t)batch * 2 * p.N + (size_t)(batch + 1) / 2 + 1 + (size_t)std::max(batch, nseg_max) * SEG_NB +
-           (size_t)batch * (p.L2 / CONV_NCOL + 1) + 64;
+           (size_t)std::max(batch, nseg_max) * (p.L2 / CONV_NCOL + 1) + 64;
 }
 
 __global__ void k_scale(double *x, double f, int64_t n)
