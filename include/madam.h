@@ -131,7 +131,9 @@ int madam_get_filter(void *ctx, int ipsd, double *fcov);
  * side and LDS tiles on the baseline side (default), 1 = wave per pixel / lane
  * group per baseline, 2 = LDS tiles on both sides, 3 = segmented tiles on both
  * sides; + 16 = band factorization computes every row (no periodic
- * continuation of repeating 64-row blocks) */
+ * continuation of repeating 64-row blocks); + 32 / + 64 = band carry chains
+ * always / never several per wave (default: several when there are enough
+ * of them); no variant changes a result bit */
 int madam_set_kernel_variant(void *ctx, int variant);
 /* pointing expansion of the synthetic ring scan: scan5 = {fsample, spin_period,
  * prec_period, opening_turns, lon0_turns}, dets3[ndet][3] = {psi_turns,

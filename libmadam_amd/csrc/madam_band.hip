@@ -538,15 +538,16 @@ __global__ void __launch_bounds__(64) k_band_fwd0(const ChunkInfo *ch, int64_t n
 // (depth log2 KD instead of a KD-long dependent chain of FMAs on the path
 // from one step's state to the next)
 template <int KD>
-__device__ inline double chain_step(double v, const double (&tr)[KD], double st, double *sb)
+__device__ inline double chain_step(double v, const double (&tr)[KD], double st, double *sb, int base = 0)
 {
     // the state goes to the wave's LDS row and comes back as KD broadcast
-    // reads (cheaper per step than 2 KD lane reads through the scalar unit)
+    // reads of the lane's chain (lanes base .. base + KD - 1), cheaper per
+    // step than 2 KD lane reads through the scalar unit
     sb[threadIdx.x & 63] = st;
     __builtin_amdgcn_wave_barrier();
     double pr[KD];
 #pragma unroll
-    for (int q = 0; q < KD; ++q) pr[q] = tr[q] * sb[q];
+    for (int q = 0; q < KD; ++q) pr[q] = tr[q] * sb[base + q];
 #pragma unroll
     for (int w = 1; w < KD; w *= 2)
 #pragma unroll
@@ -617,6 +618,107 @@ __global__ void __launch_bounds__(64) k_band_chain_local(const int64_t *gfirst, 
         __syncthreads();
         for (int t = lane; t < n * KD; t += 64) sin[c0 * KD + t] = s0l[t];
     }
+}
+
+__device__ inline int64_t readlane64(int64_t v, int src)
+{
+    int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffff), src);
+    int hi = __builtin_amdgcn_readlane((int)(v >> 32), src);
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
+// CPW = 64 / KD chains (group, right-hand side) per wave, lanes k * KD + m
+// (m < KD) of chain k, so that the chains' steps share each instruction (one
+// chain per wave leaves 55 of 64 lanes idle in every step; with few chains,
+// too few packed waves to hide the window staging, band_apply keeps
+// k_band_chain_local).  Chain id = rhs *
+// ngrp + g.  The chains' additive terms and transfer-matrix indices are
+// staged in LDS in windows of CW chunks along the chain (a window of one
+// chain is one contiguous piece of s0); each chunk's incoming state replaces
+// its term in LDS (the slot is read first) and leaves with the window, so that
+// no global store sits inside the sequential loop.
+template <int KD>
+__global__ void __launch_bounds__(64) k_band_chain_pack(const int64_t *gfirst, const int32_t *gn, int64_t ngrp,
+                                                        int nrhs, const double *s0, const double *T,
+                                                         const int32_t *cmap, int dir, const double *Sg, double *sin,
+                                                         double *cg, BandRhs rs)
+{
+    constexpr int CPW = 64 / KD, CW = 16;
+    __shared__ double s0l[CPW * CW * KD];
+    __shared__ int ctl[CPW * CW];
+    __shared__ double stb[64];   // the states, broadcast to the lanes each step
+    const int lane = threadIdx.x & 63;
+    const int k = lane / KD, m = lane - k * KD;
+    const int64_t id = (int64_t)blockIdx.x * CPW + k;
+    const bool on = k < CPW && id < ngrp * nrhs;
+    const int64_t rhs = on ? id / ngrp : 0, g = on ? id - rhs * ngrp : 0;
+    const int64_t c0 = on ? gfirst[g] : 0;
+    const int n = on ? gn[g] : 0;
+    double st = (Sg && on) ? Sg[rhs * rs.g + g * KD + m] : 0.0;
+    int nmax = n;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o, 64));
+    const int kb = k < CPW ? k : 0;   // lanes past the last chain use chain 0's slots
+    int cur = -1;
+    double tr[KD];
+#pragma unroll
+    for (int q = 0; q < KD; ++q) tr[q] = 0.0;
+    for (int w0 = 0; w0 < nmax; w0 += CW) {
+        // window w0 .. w0 + CW - 1 of every chain: chunks lo .. lo + cnt - 1
+        __syncthreads();
+        for (int kk = 0; kk < CPW; ++kk) {
+            const int nk = __builtin_amdgcn_readlane(n, kk * KD);
+            const int cnt = min(CW, nk - w0);
+            if (cnt <= 0) continue;   // uniform
+            const int lo = dir == 0 ? w0 : nk - w0 - cnt;
+            const int64_t ck = readlane64(c0, kk * KD) + lo, rk = readlane64(rhs, kk * KD);
+            const double *sp = s0 + rk * rs.s + ck * KD;
+            double v[(CW * KD + 63) / 64];
+#pragma unroll
+            for (int j = 0; j < (CW * KD + 63) / 64; ++j) {
+                int t = lane + 64 * j;
+                v[j] = t < cnt * KD ? sp[t] : 0.0;
+            }
+            if (lane < cnt) ctl[kk * CW + lane] = cmap ? cmap[ck + lane] : (int32_t)(ck + lane);
+#pragma unroll
+            for (int j = 0; j < (CW * KD + 63) / 64; ++j) {
+                int t = lane + 64 * j;
+                if (t < cnt * KD) s0l[kk * CW * KD + t] = v[j];
+            }
+        }
+        __syncthreads();
+        const int cnt = min(CW, n - w0), lo = dir == 0 ? w0 : n - w0 - cnt;
+        const int tend = min(w0 + CW, nmax);
+        for (int t = w0; t < tend; ++t) {
+            const bool act = on && t < n;
+            const int sl = (dir == 0 ? t : n - 1 - t) - lo;   // slot in the window (act)
+            double v = 0.0;
+            if (act) {
+                int ct = ctl[kb * CW + sl];
+                if (ct != cur) {
+                    cur = ct;
+#pragma unroll
+                    for (int q = 0; q < KD; ++q) tr[q] = T[((int64_t)ct * KD + m) * KD + q];
+                }
+                v = s0l[(kb * CW + sl) * KD + m];
+                if (sin) s0l[(kb * CW + sl) * KD + m] = st;
+            }
+            double ns = chain_step<KD>(v, tr, st, stb, kb * KD);
+            if (act) st = ns;
+        }
+        if (sin) {
+            __syncthreads();
+            for (int kk = 0; kk < CPW; ++kk) {
+                const int nk = __builtin_amdgcn_readlane(n, kk * KD);
+                const int ck_cnt = min(CW, nk - w0);
+                if (ck_cnt <= 0) continue;
+                const int klo = dir == 0 ? w0 : nk - w0 - ck_cnt;
+                const int64_t ck = readlane64(c0, kk * KD) + klo, rk = readlane64(rhs, kk * KD);
+                for (int t = lane; t < ck_cnt * KD; t += 64) sin[rk * rs.s + ck * KD + t] = s0l[kk * CW * KD + t];
+            }
+        }
+    }
+    if (cg && on) cg[rhs * rs.g + g * KD + m] = st;
 }
 
 // wave per segment: S_{g+1} = cg[g] + Gm[g] S_g over the segment's groups in
@@ -1074,26 +1176,34 @@ void band_apply(hipStream_t s, const BandPrec &bp, const T *r, T *z, int nrhs, i
     unsigned R = (unsigned)nrhs;
     dim3 g((unsigned)((bp.nchunk + 63) / 64), R);
     dim3 gs((unsigned)im->nseg_used, R);   // one 64-lane wave per segment
-    dim3 gg((unsigned)im->ngrp, R);        // one wave per chunk group
     BandRhs st{bp.nchunk * KD, (im->ngrp > 0 ? im->ngrp : 1) * KD};
     // the LDS tiles hold r and z in their own type (FP32 halves their
     // traffic); the sweeps compute in FP64
     using V = T;
     BAND_DISPATCH(KD, ({
+        // 64 / KD chunk-group chains per wave where that leaves at least 1024
+        // waves (k_band_chain_pack), else one chain per wave
+        const int64_t nchain = im->ngrp * (int64_t)R;
+        const bool pack = bp.chain_pack < 0 ? nchain >= 1024 * (64 / KD) : bp.chain_pack != 0;
+        dim3 gp((unsigned)((nchain + 64 / KD - 1) / (64 / KD))), gg((unsigned)im->ngrp, R);
+        auto chain = [&](const double *s0x, const double *Tx, int dir, const double *Sgx, double *sinx, double *cgx) {
+            if (pack)
+                k_band_chain_pack<KD><<<gp, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, nrhs, s0x, Tx, im->cmap, dir, Sgx,
+                                                        sinx, cgx, st);
+            else
+                k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, s0x, Tx, im->cmap, dir, Sgx,
+                                                         sinx, cgx, st);
+        };
         k_band_fwd0<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, r, vstride,
                                             im->s0f, st);
-        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, nullptr,
-                                                 nullptr, im->cg, st);
+        chain(im->s0f, bp.Tf, 0, nullptr, nullptr, im->cg);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gf, im->gmap, 0, im->Sg, st);
-        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0f, bp.Tf, im->cmap, 0, im->Sg,
-                                                 im->sinf, nullptr, st);
+        chain(im->s0f, bp.Tf, 0, im->Sg, im->sinf, nullptr);
         k_band_fwd1_bwd0<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, r, vstride,
                                                  im->sinf, z, im->s0b, st);
-        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, nullptr,
-                                                 nullptr, im->cg, st);
+        chain(im->s0b, bp.Tb, 1, nullptr, nullptr, im->cg);
         k_band_chain_top<KD><<<gs, 64, 0, s>>>(im->sgfirst, im->nseg_used, im->cg, im->Gb, im->gmap, 1, im->Sg, st);
-        k_band_chain_local<KD><<<gg, 64, 0, s>>>(im->gfirst, im->gn, im->ngrp, im->s0b, bp.Tb, im->cmap, 1, im->Sg,
-                                                 im->sinb, nullptr, st);
+        chain(im->s0b, bp.Tb, 1, im->Sg, im->sinb, nullptr);
         k_band_bwd1<KD, V><<<g, 64, 0, s>>>(im->ch, bp.nchunk, im->bmap, im->Li, im->uidx, im->Lu, im->sinb, z,
                                             vstride, st);
     }));

@@ -372,6 +372,7 @@ static void batch_precond(Context &c, BatchState &B, const T *r, T *z)
         break;
     default:
         if (!c.prec_covers_all) batch_copy<T>(s, z, r, nb * R);
+        c.band.chain_pack = band_chain_pack(c.kernel_variant);
         band_apply<T>(s, c.band, r, z, R, nb);
         if (!c.segcg.empty()) {
             // the inner-CG fallback segments, one right-hand side at a time in FP64

@@ -945,6 +945,7 @@ static void apply_precond(Context &c, const double *r, double *z)
         break;
     default:
         if (!c.prec_covers_all) launch_copy(s, z, r, c.nbtot);
+        c.band.chain_pack = band_chain_pack(c.kernel_variant);
         band_apply(s, c.band, r, z);
         cg_apply(c, r, z);
         break;

@@ -329,7 +329,16 @@ struct BandPrec {
     int64_t nchunk_unique = 0, nblock = 0, nblock_unique = 0;
     int64_t nrows = 0;      // baseline rows the factored segments cover
     int64_t nrows_factored = 0;   // of those, rows the column loop computed (the rest repeat a block)
+    // carry chains: -1 several chains per wave when there are enough of them,
+    // 1 always, 0 never (one chain per wave); the same results either way
+    int chain_pack = -1;
 };
+
+// kernel_variant bits 32 / 64: carry chains always / never packed several per wave
+inline int band_chain_pack(int kernel_variant)
+{
+    return (kernel_variant & 32) ? 1 : ((kernel_variant & 64) ? 0 : -1);
+}
 
 int band_build(hipStream_t s, BandPrec &bp, const Segment *hseg, const Segment *dseg, int64_t nseg,
                const int *nband_seg_host, const double *invcov_dev, int64_t nof, const double *nna, int *info_out,

@@ -279,3 +279,47 @@ def test_fp64_batch_wide_band(width):
         assert abs(r["info"]["steps"] - s1["niter"]) <= 1
         for name, e in errs.items():
             assert e <= 1e-10, (name, e)
+
+
+def test_band_chain_packing_bitwise():
+    """The band preconditioner's carry chains several per wave (kernel_variant
+    + 32, staged in windows of 16 chunks) and one per wave (+ 64) give the same
+    bits: single FP64 solves with two chain groups per segment, and batched
+    FP64 and mixed solves (right-hand sides indexed across the packed lanes)."""
+    import libmadam_amd
+    case = ring(prior=True, ndet=2, nsamp=500000, nperiod=2)
+    g = {v: libmadam_amd.run(*mc.call_args(mc.copy_case(case)), kernel_variant=v) for v in (32, 64)}
+    assert g[32]["stats"]["precond_kind"] == 3
+    assert g[32]["niter"] == g[64]["niter"]
+    for name in ("map", "aa", "binmap"):
+        assert np.array_equal(np.asarray(g[32][name]), np.asarray(g[64][name])), name
+
+    case = ring(prior=True, nsamp=30000, nperiod=2)
+    sig = signals_for(case, 9)
+    a = mc.call_args(mc.copy_case(case))
+    ndet = len(a[1])
+    nnz = a[5].size // (a[3].size * ndet)
+    old = os.environ.get("MADAM_BS_PIECE")
+    os.environ["MADAM_BS_PIECE"] = "256"   # the same pixel-side cut in both contexts
+    try:
+        for precision in (0, 1):
+            res = {}
+            for v in (32, 64):
+                m = libmadam_amd.Madam(0)
+                try:
+                    m.configure(*a[:4], a[7], a[8], a[9], a[10], a[11], nnz, 0)
+                    m.set_kernel_variant(v)
+                    m.set_pointing(a[4], a[5])
+                    m.solve_batch(sig, precision=precision)
+                    res[v] = [m.batch_result(k) for k in range(sig.shape[0])]
+                finally:
+                    m.close()
+            for k in range(sig.shape[0]):
+                assert res[32][k]["info"]["steps"] == res[64][k]["info"]["steps"], (precision, k)
+                for name in ("map", "aa"):
+                    assert np.array_equal(res[32][k][name], res[64][k][name]), (precision, k, name)
+    finally:
+        if old is None:
+            os.environ.pop("MADAM_BS_PIECE", None)
+        else:
+            os.environ["MADAM_BS_PIECE"] = old
